@@ -1,0 +1,541 @@
+"""ctypes bindings of the MI355X semi-naive Datalog engine.
+
+The engine is the C ABI of ``include/gdl_hip.h`` (libgdl_hip.so) and the
+program runners of ``include/gdl_programs.h`` (libgdlog.so, the gdlog C++ API
+and the TC / SG / CSPA programs).  This module only marshals numpy arrays
+through that ABI; it has no compute path of its own.  If the shared libraries
+are missing, ``load()`` raises -- there is no fallback.
+
+Seeded input generators (SplitMix64) come from libgdl_gen.so.
+"""
+import ctypes
+import os
+from ctypes import (POINTER, Structure, c_char_p, c_double, c_float, c_int, c_int32,
+                    c_uint32, c_uint64, c_void_p)
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+REPO = os.path.dirname(HERE)
+INCLUDE_DIR = os.path.join(REPO, "include")
+
+GDL_OK = 0
+STATUS_NAMES = {0: "ok", 1: "invalid argument", 2: "out of device memory", 3: "HIP error",
+                4: "unsupported", 5: "size limit exceeded", 6: "device-side error"}
+
+COL_INNER0, COL_INNER1, COL_OUTER0, COL_OUTER1 = 0, 1, 2, 3
+PRED_NONE, PRED_EQ, PRED_NE, PRED_LT, PRED_LE, PRED_GT, PRED_GE, PRED_FALSE = range(8)
+
+
+class GdlError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{msg}: {STATUS_NAMES.get(status, status)}")
+        self.status = status
+
+
+class Config(Structure):
+    _fields_ = [("device", c_int), ("mem_cap_bytes", c_uint64), ("load_factor", c_float),
+                ("collect_stats", c_int), ("verbose", c_int)]
+
+
+class Index(Structure):
+    _fields_ = [("slots", c_void_p), ("n_slots", c_uint64), ("n_distinct", c_uint64),
+                ("prefix_cols", c_int)]
+
+
+class ColMap(Structure):
+    _fields_ = [("arity", c_int32), ("src", c_int32 * 2)]
+
+
+class Pred(Structure):
+    _fields_ = [("op", c_int32), ("lhs", c_int32), ("rhs", c_int32)]
+
+
+class KernelStat(Structure):
+    _fields_ = [("name", c_char_p), ("launches", c_uint64), ("total_ms", c_double),
+                ("alg_bytes", c_double)]
+
+
+class TraceRec(Structure):
+    _fields_ = [("iteration", c_int32), ("relation", c_int32), ("newt", c_uint64),
+                ("delta", c_uint64), ("full", c_uint64)]
+
+
+class ProgResult(Structure):
+    _fields_ = [("iterations", c_int32), ("init_iterations", c_int32), ("derived", c_uint64),
+                ("n_out", c_uint64), ("sizes", c_uint64 * 4), ("load_ms", c_double),
+                ("fixpoint_ms", c_double), ("n_trace", c_uint64), ("n_trace_init", c_uint64)]
+
+
+class ProgBuffers(Structure):
+    _fields_ = [("out", POINTER(c_uint64)), ("out_cap", c_uint64),
+                ("extra0", POINTER(c_uint64)), ("extra0_cap", c_uint64),
+                ("extra1", POINTER(c_uint64)), ("extra1_cap", c_uint64),
+                ("trace", POINTER(TraceRec)), ("trace_cap", c_uint64),
+                ("trace_init", POINTER(TraceRec)), ("trace_init_cap", c_uint64)]
+
+
+_U64P = POINTER(c_uint64)
+_engine = None
+_programs = None
+_gen = None
+
+
+def lib_path(name):
+    return os.path.join(LIB_DIR, name)
+
+
+def _decl(lib, name, restype, argtypes):
+    f = getattr(lib, name)
+    f.restype = restype
+    f.argtypes = argtypes
+    return f
+
+
+def load():
+    """Loads libgdl_hip.so and libgdlog.so (raises OSError if not built)."""
+    global _engine, _programs
+    if _engine is not None:
+        return _engine, _programs
+    for n in ("libgdl_hip.so", "libgdlog.so"):
+        if not os.path.exists(lib_path(n)):
+            raise OSError(f"{lib_path(n)} is not built (run __graft_entry__.build())")
+    eng = ctypes.CDLL(lib_path("libgdl_hip.so"), mode=ctypes.RTLD_GLOBAL)
+    prog = ctypes.CDLL(lib_path("libgdlog.so"), mode=ctypes.RTLD_GLOBAL)
+    vp = c_void_p
+    _decl(eng, "gdl_config_default", None, [POINTER(Config)])
+    _decl(eng, "gdl_ctx_create", c_int, [POINTER(Config), POINTER(vp)])
+    _decl(eng, "gdl_ctx_destroy", c_int, [vp])
+    _decl(eng, "gdl_status_string", c_char_p, [c_int])
+    _decl(eng, "gdl_last_error", c_char_p, [vp])
+    _decl(eng, "gdl_sync", c_int, [vp])
+    _decl(eng, "gdl_stream", vp, [vp])
+    _decl(eng, "gdl_device", c_int, [vp])
+    _decl(eng, "gdl_set_load_factor", c_int, [vp, c_float])
+    _decl(eng, "gdl_alloc", c_int, [vp, c_uint64, POINTER(vp)])
+    _decl(eng, "gdl_free", c_int, [vp, vp])
+    _decl(eng, "gdl_memcpy_h2d", c_int, [vp, vp, vp, c_uint64])
+    _decl(eng, "gdl_memcpy_d2h", c_int, [vp, vp, vp, c_uint64])
+    _decl(eng, "gdl_memcpy_d2d", c_int, [vp, vp, vp, c_uint64])
+    _decl(eng, "gdl_mem_info", c_int, [vp] + [_U64P] * 5)
+    _decl(eng, "gdl_release_cached", c_int, [vp])
+    _decl(eng, "gdl_set_mem_cap", c_int, [vp, c_uint64])
+    _decl(eng, "gdl_sort_u64", c_int, [vp, vp, c_uint64, c_int, c_int, vp])
+    _decl(eng, "gdl_unique_u64", c_int, [vp, vp, c_uint64, vp, _U64P])
+    _decl(eng, "gdl_key_bits", c_int, [vp, vp, c_uint64, POINTER(c_int), POINTER(c_int)])
+    _decl(eng, "gdl_sort_unique_u64", c_int, [vp, vp, c_uint64, c_int, c_int, vp, _U64P])
+    _decl(eng, "gdl_merge_union_u64", c_int, [vp, vp, c_uint64, vp, c_uint64, vp, _U64P])
+    _decl(eng, "gdl_set_difference_u64", c_int, [vp, vp, c_uint64, vp, c_uint64, vp, _U64P])
+    _decl(eng, "gdl_merge_diff_u64", c_int,
+          [vp, vp, c_uint64, vp, c_uint64, vp, _U64P, vp, _U64P])
+    _decl(eng, "gdl_index_build", c_int, [vp, vp, c_uint64, c_int, POINTER(Index)])
+    _decl(eng, "gdl_index_free", c_int, [vp, POINTER(Index)])
+    _decl(eng, "gdl_index_lookup", c_int,
+          [vp, POINTER(Index), vp, c_uint64, vp, c_uint64, vp, vp])
+    _decl(eng, "gdl_join_count", c_int,
+          [vp, vp, c_uint64, POINTER(Index), vp, c_uint64, vp, vp])
+    _decl(eng, "gdl_join_scan", c_int, [vp, vp, c_uint64, vp, _U64P])
+    _decl(eng, "gdl_join_materialize", c_int,
+          [vp, vp, vp, c_uint64, vp, vp, c_uint64, ColMap, vp])
+    _decl(eng, "gdl_join", c_int,
+          [vp, vp, c_uint64, POINTER(Index), vp, c_uint64, ColMap, Pred, POINTER(vp), _U64P])
+    _decl(eng, "gdl_project_u64", c_int, [vp, vp, c_uint64, ColMap, vp])
+    _decl(eng, "gdl_filter_u64", c_int, [vp, vp, c_uint64, Pred, vp, _U64P])
+    _decl(eng, "gdl_stats_enable", c_int, [vp, c_int])
+    _decl(eng, "gdl_stats_reset", c_int, [vp])
+    _decl(eng, "gdl_stats_get", c_int, [vp, POINTER(KernelStat), c_int, POINTER(c_int)])
+    rp, bp = POINTER(ProgResult), POINTER(ProgBuffers)
+    _decl(prog, "gdl_prog_tc", c_int, [vp, vp, c_uint64, c_int, c_int, rp, bp])
+    _decl(prog, "gdl_prog_sg", c_int, [vp, vp, c_uint64, c_int, rp, bp])
+    _decl(prog, "gdl_prog_cspa", c_int, [vp, vp, c_uint64, vp, c_uint64, c_int, rp, bp])
+    _decl(prog, "gdl_prog_last_error", c_char_p, [])
+    _engine, _programs = eng, prog
+    return eng, prog
+
+
+def gen_lib():
+    """libgdl_gen.so (pure C, no GPU needed)."""
+    global _gen
+    if _gen is None:
+        p = lib_path("libgdl_gen.so")
+        if not os.path.exists(p):
+            raise OSError(f"{p} is not built (run __graft_entry__.build())")
+        g = ctypes.CDLL(p)
+        _decl(g, "gdl_gen_er", c_uint64, [c_uint64, c_uint32, c_uint64, _U64P])
+        _decl(g, "gdl_gen_tree", c_uint64, [c_uint64, c_uint32, c_uint32, _U64P])
+        _decl(g, "gdl_gen_rmat", c_uint64,
+              [c_uint64, c_int, c_uint64, c_double, c_double, c_double, _U64P])
+        _decl(g, "gdl_gen_batched_shape", None,
+              [c_uint64, c_uint64, c_uint64, POINTER(c_int), _U64P, POINTER(c_uint32)])
+        _decl(g, "gdl_gen_batched_instance", c_uint64,
+              [c_uint64, c_uint64, c_uint64, c_uint32, _U64P])
+        _decl(g, "gdl_gen_cspa", c_uint64,
+              [c_uint64, c_uint32, c_double, c_uint64, c_uint64, _U64P, _U64P, _U64P])
+        _gen = g
+    return _gen
+
+
+def _p64(a):
+    return a.ctypes.data_as(_U64P)
+
+
+# --------------------------------------------------------------- generators
+
+def gen_er(seed, n_nodes, m):
+    out = np.zeros(m, dtype=np.uint64)
+    k = gen_lib().gdl_gen_er(seed, n_nodes, m, _p64(out))
+    return out[:k]
+
+
+def gen_tree(seed, n_nodes, window=1000):
+    out = np.zeros(max(n_nodes - 1, 1), dtype=np.uint64)
+    k = gen_lib().gdl_gen_tree(seed, n_nodes, window, _p64(out))
+    return out[:k]
+
+
+def gen_rmat(seed, scale, m, a=0.57, b=0.19, c=0.19):
+    out = np.zeros(m, dtype=np.uint64)
+    k = gen_lib().gdl_gen_rmat(seed, scale, m, a, b, c, _p64(out))
+    return out[:k]
+
+
+def gen_batched_shape(seed, min_edges=1000, max_edges=50000):
+    t, m, n = c_int(), c_uint64(), c_uint32()
+    gen_lib().gdl_gen_batched_shape(seed, min_edges, max_edges, ctypes.byref(t),
+                                    ctypes.byref(m), ctypes.byref(n))
+    return bool(t.value), int(m.value), int(n.value)
+
+
+def gen_batched_instance(seed, min_edges=1000, max_edges=50000, tree_window=1000):
+    _, m, _ = gen_batched_shape(seed, min_edges, max_edges)
+    out = np.zeros(m + 1, dtype=np.uint64)
+    k = gen_lib().gdl_gen_batched_instance(seed, min_edges, max_edges, tree_window, _p64(out))
+    return out[:k]
+
+
+def gen_cspa(seed, n_ids, zipf_s, n_assign, n_deref):
+    a = np.zeros(n_assign, dtype=np.uint64)
+    d = np.zeros(n_deref, dtype=np.uint64)
+    nd = c_uint64()
+    na = gen_lib().gdl_gen_cspa(seed, n_ids, zipf_s, n_assign, n_deref, _p64(a), _p64(d),
+                                ctypes.byref(nd))
+    return a[:na], d[:nd.value]
+
+
+def pack(src, dst):
+    return (np.asarray(src, dtype=np.uint64) << np.uint64(32)) | np.asarray(dst, dtype=np.uint64)
+
+
+def unpack(keys):
+    k = np.asarray(keys, dtype=np.uint64)
+    return (k >> np.uint64(32)).astype(np.int64), (k & np.uint64(0xffffffff)).astype(np.int64)
+
+
+# ------------------------------------------------------------------ engine
+
+class DeviceArray:
+    """A device buffer of u64 (or u32) owned by a Context's allocator."""
+
+    def __init__(self, ctx, ptr, n, dtype=np.uint64):
+        self.ctx, self.ptr, self.n, self.dtype = ctx, ptr, int(n), np.dtype(dtype)
+
+    def numpy(self, n=None):
+        n = self.n if n is None else int(n)
+        out = np.zeros(n, dtype=self.dtype)
+        if n:
+            self.ctx._check(self.ctx.eng.gdl_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr,
+                                                        n * self.dtype.itemsize), "d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.ctx.eng.gdl_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+
+class Context:
+    def __init__(self, device=0, mem_cap_bytes=0, load_factor=0.8, collect_stats=False,
+                 verbose=False):
+        self.eng, self.prog = load()
+        cfg = Config()
+        self.eng.gdl_config_default(ctypes.byref(cfg))
+        cfg.device, cfg.mem_cap_bytes, cfg.load_factor = device, mem_cap_bytes, load_factor
+        cfg.collect_stats, cfg.verbose = int(collect_stats), int(verbose)
+        h = c_void_p()
+        st = self.eng.gdl_ctx_create(ctypes.byref(cfg), ctypes.byref(h))
+        if st != GDL_OK:
+            raise GdlError(st, "gdl_ctx_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.eng.gdl_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, st, what):
+        if st != GDL_OK:
+            detail = self.eng.gdl_last_error(self.h) or b""
+            raise GdlError(st, f"{what} ({detail.decode(errors='replace')})")
+
+    # memory
+    def alloc(self, n, dtype=np.uint64):
+        p = c_void_p()
+        self._check(self.eng.gdl_alloc(self.h, max(int(n), 1) * np.dtype(dtype).itemsize,
+                                       ctypes.byref(p)), "gdl_alloc")
+        return DeviceArray(self, p.value, n, dtype)
+
+    def to_device(self, arr, dtype=np.uint64):
+        a = np.ascontiguousarray(arr, dtype=dtype)
+        d = self.alloc(a.size, dtype)
+        if a.size:
+            self._check(self.eng.gdl_memcpy_h2d(self.h, d.ptr, a.ctypes.data, a.nbytes), "h2d")
+        return d
+
+    def sync(self):
+        self._check(self.eng.gdl_sync(self.h), "gdl_sync")
+
+    def mem_info(self):
+        v = [c_uint64() for _ in range(5)]
+        self._check(self.eng.gdl_mem_info(self.h, *[ctypes.byref(x) for x in v]), "mem_info")
+        return dict(zip(("free", "total", "in_use", "peak", "cached"), [x.value for x in v]))
+
+    def set_mem_cap(self, cap):
+        self._check(self.eng.gdl_set_mem_cap(self.h, cap), "set_mem_cap")
+
+    def release_cached(self):
+        self._check(self.eng.gdl_release_cached(self.h), "release_cached")
+
+    # primitives (numpy in, numpy out)
+    def sort(self, keys, hi_bits=32, lo_bits=32):
+        d = self.to_device(keys)
+        o = self.alloc(d.n)
+        self._check(self.eng.gdl_sort_u64(self.h, d.ptr, d.n, hi_bits, lo_bits, o.ptr), "sort")
+        r = o.numpy()
+        d.free(); o.free()
+        return r
+
+    def sort_unique(self, keys, hi_bits=32, lo_bits=32):
+        d = self.to_device(keys)
+        o = self.alloc(d.n)
+        m = c_uint64()
+        self._check(self.eng.gdl_sort_unique_u64(self.h, d.ptr, d.n, hi_bits, lo_bits, o.ptr,
+                                                 ctypes.byref(m)), "sort_unique")
+        r = o.numpy(m.value)
+        d.free(); o.free()
+        return r
+
+    def unique(self, sorted_keys):
+        d = self.to_device(sorted_keys)
+        o = self.alloc(d.n)
+        m = c_uint64()
+        self._check(self.eng.gdl_unique_u64(self.h, d.ptr, d.n, o.ptr, ctypes.byref(m)), "unique")
+        r = o.numpy(m.value)
+        d.free(); o.free()
+        return r
+
+    def key_bits(self, keys):
+        d = self.to_device(keys)
+        hi, lo = c_int(), c_int()
+        self._check(self.eng.gdl_key_bits(self.h, d.ptr, d.n, ctypes.byref(hi), ctypes.byref(lo)),
+                    "key_bits")
+        d.free()
+        return hi.value, lo.value
+
+    def merge_union(self, a, b):
+        da, db = self.to_device(a), self.to_device(b)
+        o = self.alloc(da.n + db.n)
+        m = c_uint64()
+        self._check(self.eng.gdl_merge_union_u64(self.h, da.ptr, da.n, db.ptr, db.n, o.ptr,
+                                                 ctypes.byref(m)), "merge_union")
+        r = o.numpy(m.value)
+        for x in (da, db, o):
+            x.free()
+        return r
+
+    def set_difference(self, b, a):
+        da, db = self.to_device(a), self.to_device(b)
+        o = self.alloc(db.n)
+        m = c_uint64()
+        self._check(self.eng.gdl_set_difference_u64(self.h, db.ptr, db.n, da.ptr, da.n, o.ptr,
+                                                    ctypes.byref(m)), "set_difference")
+        r = o.numpy(m.value)
+        for x in (da, db, o):
+            x.free()
+        return r
+
+    def merge_diff(self, full, newt):
+        df, dn = self.to_device(full), self.to_device(newt)
+        u, d = self.alloc(df.n + dn.n), self.alloc(dn.n)
+        nu, nd = c_uint64(), c_uint64()
+        self._check(self.eng.gdl_merge_diff_u64(self.h, df.ptr, df.n, dn.ptr, dn.n, u.ptr,
+                                                ctypes.byref(nu), d.ptr, ctypes.byref(nd)),
+                    "merge_diff")
+        ru, rd = u.numpy(nu.value), d.numpy(nd.value)
+        for x in (df, dn, u, d):
+            x.free()
+        return ru, rd
+
+    def index_build(self, dkeys, prefix_cols=1):
+        idx = Index()
+        self._check(self.eng.gdl_index_build(self.h, dkeys.ptr, dkeys.n, prefix_cols,
+                                             ctypes.byref(idx)), "index_build")
+        return idx
+
+    def index_free(self, idx):
+        self._check(self.eng.gdl_index_free(self.h, ctypes.byref(idx)), "index_free")
+
+    def index_lookup(self, idx, dkeys, prefixes):
+        dq = self.to_device(prefixes)
+        b, c = self.alloc(dq.n), self.alloc(dq.n, np.uint32)
+        self._check(self.eng.gdl_index_lookup(self.h, ctypes.byref(idx), dkeys.ptr, dkeys.n,
+                                              dq.ptr, dq.n, b.ptr, c.ptr), "index_lookup")
+        rb, rc = b.numpy(), c.numpy()
+        for x in (dq, b, c):
+            x.free()
+        return rb, rc
+
+    def join(self, inner_sorted, outer, colmap, prefix_cols=1, pred=(PRED_NONE, 0, 1)):
+        """Hash join of sorted-unique inner keys with outer keys (any order)."""
+        di, do = self.to_device(inner_sorted), self.to_device(outer)
+        idx = self.index_build(di, prefix_cols)
+        cm = ColMap(len(colmap), (c_int32 * 2)(*(list(colmap) + [0])[:2]))
+        pr = Pred(*pred)
+        out, m = c_void_p(), c_uint64()
+        self._check(self.eng.gdl_join(self.h, di.ptr, di.n, ctypes.byref(idx), do.ptr, do.n, cm,
+                                      pr, ctypes.byref(out), ctypes.byref(m)), "join")
+        res = DeviceArray(self, out.value, m.value).numpy() if m.value else \
+            np.zeros(0, dtype=np.uint64)
+        if out.value:
+            self.eng.gdl_free(self.h, out.value)
+        self.index_free(idx)
+        di.free(); do.free()
+        return res
+
+    def project(self, keys, colmap):
+        d = self.to_device(keys)
+        o = self.alloc(d.n)
+        cm = ColMap(len(colmap), (c_int32 * 2)(*(list(colmap) + [0])[:2]))
+        self._check(self.eng.gdl_project_u64(self.h, d.ptr, d.n, cm, o.ptr), "project")
+        r = o.numpy()
+        d.free(); o.free()
+        return r
+
+    def filter(self, keys, op, lhs=0, rhs=1):
+        d = self.to_device(keys)
+        o = self.alloc(d.n)
+        m = c_uint64()
+        self._check(self.eng.gdl_filter_u64(self.h, d.ptr, d.n, Pred(op, lhs, rhs), o.ptr,
+                                            ctypes.byref(m)), "filter")
+        r = o.numpy(m.value)
+        d.free(); o.free()
+        return r
+
+    # statistics
+    def stats_enable(self, on=True):
+        self._check(self.eng.gdl_stats_enable(self.h, int(on)), "stats_enable")
+
+    def stats_reset(self):
+        self._check(self.eng.gdl_stats_reset(self.h), "stats_reset")
+
+    def stats(self):
+        arr = (KernelStat * 64)()
+        n = c_int()
+        self._check(self.eng.gdl_stats_get(self.h, arr, 64, ctypes.byref(n)), "stats_get")
+        return {arr[i].name.decode(): dict(launches=arr[i].launches, total_ms=arr[i].total_ms,
+                                           alg_bytes=arr[i].alg_bytes)
+                for i in range(min(n.value, 64))}
+
+    # programs
+    def _run(self, fn, args, want_keys, want_trace, out_cap, extra_caps=(0, 0)):
+        res = ProgResult()
+        bufs = ProgBuffers()
+        keep = []
+        if want_keys:
+            o = np.zeros(max(out_cap, 1), dtype=np.uint64)
+            keep.append(o)
+            bufs.out, bufs.out_cap = _p64(o), out_cap
+            ex = []
+            for i, cap in enumerate(extra_caps):
+                e = np.zeros(max(cap, 1), dtype=np.uint64)
+                ex.append(e)
+                if cap:
+                    if i == 0:
+                        bufs.extra0, bufs.extra0_cap = _p64(e), cap
+                    else:
+                        bufs.extra1, bufs.extra1_cap = _p64(e), cap
+            keep.append(ex)
+        if want_trace:
+            tr = (TraceRec * 100000)()
+            ti = (TraceRec * 100000)()
+            keep += [tr, ti]
+            bufs.trace, bufs.trace_cap = tr, 100000
+            bufs.trace_init, bufs.trace_init_cap = ti, 100000
+        st = fn(self.h, *args, ctypes.byref(res), ctypes.byref(bufs))
+        if st != GDL_OK:
+            msg = (self.prog.gdl_prog_last_error() or b"").decode(errors="replace")
+            raise GdlError(st, msg)
+        r = dict(iterations=res.iterations, init_iterations=res.init_iterations,
+                 derived=res.derived, n_out=res.n_out, sizes=list(res.sizes),
+                 load_ms=res.load_ms, fixpoint_ms=res.fixpoint_ms)
+        if want_keys:
+            r["out"] = keep[0][:res.n_out].copy()
+            r["extra"] = [keep[1][0][:res.sizes[0]].copy() if extra_caps[0] else None,
+                          keep[1][1][:res.sizes[1]].copy() if extra_caps[1] else None]
+        if want_trace:
+            tr, ti = keep[-2], keep[-1]
+            r["trace"] = [(tr[i].iteration, tr[i].relation, tr[i].newt, tr[i].delta, tr[i].full)
+                          for i in range(res.n_trace)]
+            r["trace_init"] = [(ti[i].iteration, ti[i].relation, ti[i].newt, ti[i].delta,
+                                ti[i].full) for i in range(res.n_trace_init)]
+        return r
+
+    def _edges(self, edges, on_device):
+        if on_device:
+            if isinstance(edges, DeviceArray):
+                return edges.ptr, edges.n, None
+            d = self.to_device(edges)
+            return d.ptr, d.n, d
+        e = np.ascontiguousarray(edges, dtype=np.uint64)
+        return e.ctypes.data, e.size, e
+
+    def run_tc(self, edges, on_device=False, max_iteration=-1, want_keys=True, want_trace=True,
+               out_cap=None):
+        p, n, keep = self._edges(edges, on_device)
+        cap = out_cap if out_cap is not None else (60_000_000 if want_keys else 0)
+        try:
+            return self._run(self.prog.gdl_prog_tc, (p, n, int(on_device), max_iteration),
+                             want_keys, want_trace, cap)
+        finally:
+            if isinstance(keep, DeviceArray):
+                keep.free()
+
+    def run_sg(self, edges, on_device=False, want_keys=True, want_trace=True, out_cap=None):
+        p, n, keep = self._edges(edges, on_device)
+        cap = out_cap if out_cap is not None else (80_000_000 if want_keys else 0)
+        try:
+            return self._run(self.prog.gdl_prog_sg, (p, n, int(on_device)), want_keys,
+                             want_trace, cap)
+        finally:
+            if isinstance(keep, DeviceArray):
+                keep.free()
+
+    def run_cspa(self, assign, deref, on_device=False, want_keys=True, want_trace=True,
+                 out_cap=None, extra_cap=None):
+        pa, na, ka = self._edges(assign, on_device)
+        pd, nd, kd = self._edges(deref, on_device)
+        cap = out_cap if out_cap is not None else (20_000_000 if want_keys else 0)
+        ecap = extra_cap if extra_cap is not None else (5_000_000 if want_keys else 0)
+        try:
+            return self._run(self.prog.gdl_prog_cspa, (pa, na, pd, nd, int(on_device)),
+                             want_keys, want_trace, cap, (ecap, ecap))
+        finally:
+            for k in (ka, kd):
+                if isinstance(k, DeviceArray):
+                    k.free()

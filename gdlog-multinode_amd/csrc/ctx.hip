@@ -1,0 +1,381 @@
+// ctx.hip -- context, caching allocator, statistics, memory entry points.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "gdl_internal.hpp"
+
+namespace gdl {
+
+const char *kKernelNames[K_NUM] = {
+    "rs_upsweep",   "rs_rowscan",   "rs_downsweep", "rs_oneblock",   "key_bits",
+    "scan_reduce",  "scan_top",     "scan_down",    "compact_count", "compact_write",
+    "merge_count",  "merge_write",  "idx_count",    "idx_build",     "idx_lookup",
+    "join_count",   "join_materialize", "project",  "fill"};
+
+// ---------------------------------------------------------------- allocator
+
+size_t Allocator::size_class(size_t bytes) {
+    if (bytes <= 512) return 512;
+    int k = 63 - __builtin_clzll((unsigned long long)bytes);
+    size_t step = (size_t)1 << (k >= 3 ? k - 3 : 0);
+    return (bytes + step - 1) / step * step;
+}
+
+Allocator::~Allocator() {
+    release_cached();
+    for (auto &kv : live_) (void)hipFree(kv.first);
+}
+
+gdl_status Allocator::alloc(size_t bytes, void **out, std::string *err) {
+    std::lock_guard<std::mutex> lock(mu_);
+    size_t cls = size_class(bytes);
+    if (cap_ && in_use_ + cls > cap_) {
+        if (err)
+            *err = "allocation of " + std::to_string(bytes) + " bytes exceeds the byte cap (" +
+                   std::to_string(in_use_) + " in use, cap " + std::to_string(cap_) + ")";
+        return GDL_ERR_OUT_OF_MEMORY;
+    }
+    auto it = free_.find(cls);
+    void *p = nullptr;
+    if (it != free_.end()) {
+        p = it->second;
+        free_.erase(it);
+        cached_ -= cls;
+    } else {
+        hipError_t e = hipMalloc(&p, cls);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            // give the cache back and retry once
+            (void)hipDeviceSynchronize();
+            for (auto &kv : free_) (void)hipFree(kv.second);
+            free_.clear();
+            cached_ = 0;
+            e = hipMalloc(&p, cls);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                if (err)
+                    *err = "hipMalloc of " + std::to_string(cls) +
+                           " bytes failed: " + hipGetErrorString(e);
+                return GDL_ERR_OUT_OF_MEMORY;
+            }
+        }
+    }
+    live_[p] = cls;
+    in_use_ += cls;
+    peak_ = std::max(peak_, in_use_);
+    *out = p;
+    return GDL_OK;
+}
+
+gdl_status Allocator::free(void *p, std::string *err) {
+    if (!p) return GDL_OK;
+    std::lock_guard<std::mutex> lock(mu_);
+    auto it = live_.find(p);
+    if (it == live_.end()) {
+        if (err) *err = "gdl_free of a pointer this context did not allocate";
+        return GDL_ERR_INVALID_ARG;
+    }
+    size_t cls = it->second;
+    live_.erase(it);
+    in_use_ -= cls;
+    free_.emplace(cls, p);
+    cached_ += cls;
+    return GDL_OK;
+}
+
+void Allocator::release_cached() {
+    std::lock_guard<std::mutex> lock(mu_);
+    if (free_.empty()) return;
+    (void)hipDeviceSynchronize();
+    for (auto &kv : free_) (void)hipFree(kv.second);
+    free_.clear();
+    cached_ = 0;
+}
+
+// ------------------------------------------------------------ error helpers
+
+gdl_status fail(gdl_ctx *ctx, gdl_status st, const std::string &msg) {
+    if (ctx) ctx->last_error = msg;
+    if (ctx && ctx->cfg.verbose) fprintf(stderr, "gdl: %s\n", msg.c_str());
+    return st;
+}
+
+gdl_status hip_fail(gdl_ctx *ctx, hipError_t e, const char *what) {
+    std::string msg = std::string(what) + ": " + hipGetErrorString(e);
+    return fail(ctx, e == hipErrorOutOfMemory ? GDL_ERR_OUT_OF_MEMORY : GDL_ERR_HIP, msg);
+}
+
+gdl_status check_launch(gdl_ctx *ctx, const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(ctx, e, what);
+    return GDL_OK;
+}
+
+gdl_status read_u64(gdl_ctx *ctx, const u64 *d, u64 *h, int count) {
+    GDL_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, d, sizeof(u64) * count, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    memcpy(h, ctx->h_pinned, sizeof(u64) * count);
+    return GDL_OK;
+}
+
+gdl_status check_device_error(gdl_ctx *ctx) {
+    u32 err = 0;
+    GDL_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, ctx->d_err, sizeof(u32), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    memcpy(&err, ctx->h_pinned, sizeof(u32));
+    if (err) {
+        u32 zero = 0;
+        (void)hipMemcpy(ctx->d_err, &zero, sizeof(u32), hipMemcpyHostToDevice);
+        return fail(ctx, GDL_ERR_DEVICE, "kernel reported internal error code " +
+                                             std::to_string(err));
+    }
+    return GDL_OK;
+}
+
+// ------------------------------------------------------------------- timing
+
+static hipEvent_t get_event(gdl_ctx *ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+KTimer::KTimer(gdl_ctx *ctx, int kid, double alg_bytes) : ctx_(ctx), kid_(kid) {
+    ctx->stats[kid].launches++;
+    ctx->stats[kid].alg_bytes += alg_bytes;
+    if (ctx->stats_on) {
+        start_ = get_event(ctx);
+        if (start_) (void)hipEventRecord(start_, ctx->stream);
+    }
+}
+
+KTimer::~KTimer() {
+    if (!start_) return;
+    hipEvent_t stop = get_event(ctx_);
+    if (!stop) {
+        ctx_->event_pool.push_back(start_);
+        return;
+    }
+    (void)hipEventRecord(stop, ctx_->stream);
+    ctx_->pending.push_back({kid_, start_, stop});
+}
+
+static void resolve_pending(gdl_ctx *ctx) {
+    if (ctx->pending.empty()) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &p : ctx->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess)
+            ctx->stats[p.kid].total_ms += ms;
+        ctx->event_pool.push_back(p.start);
+        ctx->event_pool.push_back(p.stop);
+    }
+    ctx->pending.clear();
+}
+
+}  // namespace gdl
+
+using namespace gdl;
+
+// ================================================================ C ABI
+
+extern "C" {
+
+void gdl_config_default(gdl_config *cfg) {
+    if (!cfg) return;
+    memset(cfg, 0, sizeof(*cfg));
+    cfg->device = 0;
+    cfg->mem_cap_bytes = 0;
+    cfg->load_factor = 0.8f;
+    cfg->collect_stats = 0;
+    cfg->verbose = 0;
+}
+
+const char *gdl_status_string(gdl_status s) {
+    switch (s) {
+        case GDL_OK: return "ok";
+        case GDL_ERR_INVALID_ARG: return "invalid argument";
+        case GDL_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case GDL_ERR_HIP: return "HIP runtime error";
+        case GDL_ERR_UNSUPPORTED: return "unsupported";
+        case GDL_ERR_OVERFLOW: return "size limit exceeded";
+        case GDL_ERR_DEVICE: return "device-side error";
+    }
+    return "unknown status";
+}
+
+gdl_status gdl_ctx_create(const gdl_config *cfg_in, gdl_ctx **out) {
+    if (!out) return GDL_ERR_INVALID_ARG;
+    *out = nullptr;
+    gdl_config cfg;
+    if (cfg_in) cfg = *cfg_in;
+    else gdl_config_default(&cfg);
+    if (!(cfg.load_factor > 0.f && cfg.load_factor <= 0.95f)) return GDL_ERR_INVALID_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        return GDL_ERR_HIP;
+    }
+    if (cfg.device < 0 || cfg.device >= ndev) return GDL_ERR_INVALID_ARG;
+    if (hipSetDevice(cfg.device) != hipSuccess) return GDL_ERR_HIP;
+    gdl_ctx *ctx = new gdl_ctx();
+    ctx->device = cfg.device;
+    ctx->cfg = cfg;
+    ctx->stats_on = cfg.collect_stats != 0;
+    ctx->alloc = new Allocator(cfg.device);
+    ctx->alloc->set_cap(cfg.mem_cap_bytes);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, cfg.device) == hipSuccess) ctx->cu_count = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc((void **)&ctx->h_pinned, 64 * sizeof(uint64_t), hipHostMallocDefault) !=
+            hipSuccess ||
+        hipMalloc((void **)&ctx->d_err, 256) != hipSuccess ||
+        hipMemset(ctx->d_err, 0, 256) != hipSuccess) {
+        (void)hipGetLastError();
+        gdl_ctx_destroy(ctx);
+        return GDL_ERR_HIP;
+    }
+    *out = ctx;
+    return GDL_OK;
+}
+
+gdl_status gdl_ctx_destroy(gdl_ctx *ctx) {
+    if (!ctx) return GDL_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &p : ctx->pending) {
+        (void)hipEventDestroy(p.start);
+        (void)hipEventDestroy(p.stop);
+    }
+    for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+    delete ctx->alloc;
+    if (ctx->d_err) (void)hipFree(ctx->d_err);
+    if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GDL_OK;
+}
+
+const char *gdl_last_error(gdl_ctx *ctx) { return ctx ? ctx->last_error.c_str() : ""; }
+
+gdl_status gdl_sync(gdl_ctx *ctx) {
+    if (!ctx) return GDL_ERR_INVALID_ARG;
+    GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return check_device_error(ctx);
+}
+
+void *gdl_stream(gdl_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+int gdl_device(gdl_ctx *ctx) { return ctx ? ctx->device : -1; }
+
+gdl_status gdl_set_load_factor(gdl_ctx *ctx, float lf) {
+    if (!ctx || !(lf > 0.f && lf <= 0.95f)) return GDL_ERR_INVALID_ARG;
+    ctx->cfg.load_factor = lf;
+    return GDL_OK;
+}
+
+gdl_status gdl_alloc(gdl_ctx *ctx, size_t bytes, void **dptr) {
+    if (!ctx || !dptr) return GDL_ERR_INVALID_ARG;
+    std::string err;
+    gdl_status st = ctx->alloc->alloc(bytes, dptr, &err);
+    if (st != GDL_OK) return fail(ctx, st, err);
+    return GDL_OK;
+}
+
+gdl_status gdl_free(gdl_ctx *ctx, void *dptr) {
+    if (!ctx) return GDL_ERR_INVALID_ARG;
+    std::string err;
+    gdl_status st = ctx->alloc->free(dptr, &err);
+    if (st != GDL_OK) return fail(ctx, st, err);
+    return GDL_OK;
+}
+
+gdl_status gdl_memcpy_h2d(gdl_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return GDL_ERR_INVALID_ARG;
+    if (!bytes) return GDL_OK;
+    GDL_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GDL_OK;
+}
+
+gdl_status gdl_memcpy_d2h(gdl_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return GDL_ERR_INVALID_ARG;
+    if (!bytes) return GDL_OK;
+    GDL_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GDL_OK;
+}
+
+gdl_status gdl_memcpy_d2d(gdl_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return GDL_ERR_INVALID_ARG;
+    if (!bytes) return GDL_OK;
+    GDL_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return GDL_OK;
+}
+
+gdl_status gdl_mem_info(gdl_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes,
+                        uint64_t *in_use, uint64_t *peak, uint64_t *cached) {
+    if (!ctx) return GDL_ERR_INVALID_ARG;
+    size_t f = 0, t = 0;
+    GDL_HIP(ctx, hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    if (in_use) *in_use = ctx->alloc->in_use();
+    if (peak) *peak = ctx->alloc->peak();
+    if (cached) *cached = ctx->alloc->cached();
+    return GDL_OK;
+}
+
+gdl_status gdl_release_cached(gdl_ctx *ctx) {
+    if (!ctx) return GDL_ERR_INVALID_ARG;
+    GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->alloc->release_cached();
+    return GDL_OK;
+}
+
+gdl_status gdl_set_mem_cap(gdl_ctx *ctx, uint64_t cap) {
+    if (!ctx) return GDL_ERR_INVALID_ARG;
+    ctx->alloc->set_cap(cap);
+    ctx->cfg.mem_cap_bytes = cap;
+    return GDL_OK;
+}
+
+gdl_status gdl_stats_enable(gdl_ctx *ctx, int enable) {
+    if (!ctx) return GDL_ERR_INVALID_ARG;
+    ctx->stats_on = enable != 0;
+    return GDL_OK;
+}
+
+gdl_status gdl_stats_reset(gdl_ctx *ctx) {
+    if (!ctx) return GDL_ERR_INVALID_ARG;
+    resolve_pending(ctx);
+    for (auto &s : ctx->stats) s = KernelStat();
+    return GDL_OK;
+}
+
+gdl_status gdl_stats_get(gdl_ctx *ctx, gdl_kernel_stat *out, int cap, int *n_out) {
+    if (!ctx) return GDL_ERR_INVALID_ARG;
+    resolve_pending(ctx);
+    int n = 0;
+    for (int k = 0; k < K_NUM; ++k) {
+        if (out && n < cap) {
+            out[n].name = kKernelNames[k];
+            out[n].launches = ctx->stats[k].launches;
+            out[n].total_ms = ctx->stats[k].total_ms;
+            out[n].alg_bytes = ctx->stats[k].alg_bytes;
+        }
+        ++n;
+    }
+    if (n_out) *n_out = n;
+    return GDL_OK;
+}
+
+}  // extern "C"

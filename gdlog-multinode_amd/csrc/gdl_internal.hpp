@@ -1,0 +1,306 @@
+// gdl_internal.hpp -- context, allocator, statistics and wave64 device helpers
+// shared by the engine's translation units.  Written for gfx950 (CDNA4,
+// 64-lane wavefronts).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "gdl_hip.h"
+
+namespace gdl {
+
+using u64 = uint64_t;
+using u32 = uint32_t;
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------- allocator
+// Size-classed caching allocator: classes are 512 B, then 8 steps per power
+// of two (<= 12.5 % slack).  Freed blocks are kept for reuse; reuse is safe
+// because every kernel of a context runs on its one stream.
+class Allocator {
+  public:
+    explicit Allocator(int device) : device_(device) {}
+    ~Allocator();
+    gdl_status alloc(size_t bytes, void **out, std::string *err);
+    gdl_status free(void *p, std::string *err);
+    void release_cached();
+    uint64_t in_use() const { return in_use_; }
+    uint64_t peak() const { return peak_; }
+    uint64_t cached() const { return cached_; }
+    void set_cap(uint64_t cap) { cap_ = cap; }
+    uint64_t cap() const { return cap_; }
+    static size_t size_class(size_t bytes);
+
+  private:
+    int device_ [[maybe_unused]];
+    std::mutex mu_;
+    std::multimap<size_t, void *> free_;
+    std::unordered_map<void *, size_t> live_;
+    uint64_t in_use_ = 0, peak_ = 0, cached_ = 0, cap_ = 0;
+};
+
+// ------------------------------------------------------------------ stats
+enum KernelId {
+    K_RS_UPSWEEP = 0,
+    K_RS_ROWSCAN,
+    K_RS_DOWNSWEEP,
+    K_RS_ONEBLOCK,
+    K_KEY_BITS,
+    K_SCAN_REDUCE,
+    K_SCAN_TOP,
+    K_SCAN_DOWN,
+    K_COMPACT_COUNT,
+    K_COMPACT_WRITE,
+    K_MERGE_COUNT,
+    K_MERGE_WRITE,
+    K_IDX_COUNT,
+    K_IDX_BUILD,
+    K_IDX_LOOKUP,
+    K_JOIN_COUNT,
+    K_JOIN_MAT,
+    K_PROJECT,
+    K_FILL,
+    K_NUM
+};
+extern const char *kKernelNames[K_NUM];
+
+struct PendingTiming {
+    int kid;
+    hipEvent_t start, stop;
+};
+
+struct KernelStat {
+    uint64_t launches = 0;
+    double total_ms = 0.0;
+    double alg_bytes = 0.0;
+};
+
+}  // namespace gdl
+
+struct gdl_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    gdl_config cfg{};
+    gdl::Allocator *alloc = nullptr;
+    uint64_t *h_pinned = nullptr;  // pinned staging for scalars read back
+    uint32_t *d_err = nullptr;     // kernel-side error word
+    int cu_count = 0;
+    std::string last_error;
+    bool stats_on = false;
+    gdl::KernelStat stats[gdl::K_NUM];
+    std::vector<gdl::PendingTiming> pending;
+    std::vector<hipEvent_t> event_pool;
+};
+
+namespace gdl {
+
+// Records the status + message on the context and returns the status.
+gdl_status fail(gdl_ctx *ctx, gdl_status st, const std::string &msg);
+gdl_status hip_fail(gdl_ctx *ctx, hipError_t e, const char *what);
+
+#define GDL_HIP(ctx, call)                                         \
+    do {                                                           \
+        hipError_t _e = (call);                                    \
+        if (_e != hipSuccess) return ::gdl::hip_fail(ctx, _e, #call); \
+    } while (0)
+
+#define GDL_TRY(expr)                         \
+    do {                                      \
+        gdl_status _s = (expr);               \
+        if (_s != GDL_OK) return _s;          \
+    } while (0)
+
+// Device scratch owned by the context allocator, freed on scope exit.
+class Scratch {
+  public:
+    Scratch(gdl_ctx *ctx) : ctx_(ctx) {}
+    ~Scratch() {
+        for (void *p : ptrs_) ctx_->alloc->free(p, nullptr);
+    }
+    template <typename T>
+    gdl_status get(size_t count, T **out) {
+        void *p = nullptr;
+        std::string err;
+        gdl_status st = ctx_->alloc->alloc(count * sizeof(T) + 16, &p, &err);
+        if (st != GDL_OK) return fail(ctx_, st, err);
+        ptrs_.push_back(p);
+        *out = static_cast<T *>(p);
+        return GDL_OK;
+    }
+
+  private:
+    gdl_ctx *ctx_;
+    std::vector<void *> ptrs_;
+};
+
+// Kernel timing scope: with statistics on, brackets a launch with hipEvents
+// on the context stream; always counts launches and algorithmic bytes.
+class KTimer {
+  public:
+    KTimer(gdl_ctx *ctx, int kid, double alg_bytes);
+    ~KTimer();
+
+  private:
+    gdl_ctx *ctx_;
+    int kid_;
+    hipEvent_t start_ = nullptr;
+};
+
+gdl_status read_u64(gdl_ctx *ctx, const u64 *d, u64 *h, int count = 1);
+gdl_status check_launch(gdl_ctx *ctx, const char *what);
+gdl_status check_device_error(gdl_ctx *ctx);
+
+// ---- internal entry points shared between translation units
+gdl_status exclusive_scan_u32_to_u64(gdl_ctx *ctx, const u32 *d_in, u64 n, u64 *d_out,
+                                     u64 *total_host /* may be null: no sync */);
+gdl_status exclusive_scan_u64(gdl_ctx *ctx, const u64 *d_in, u64 n, u64 *d_out);
+// In-place exclusive scan of v[0..n); v[n] receives the total (v holds n + 1).
+gdl_status scan_inplace(gdl_ctx *ctx, u64 *v, u64 n);
+gdl_status sort_keys(gdl_ctx *ctx, const u64 *d_in, u64 n, int hi_bits, int lo_bits, u64 *d_out);
+gdl_status unique_keys(gdl_ctx *ctx, const u64 *d_in, u64 n, u64 *d_out, u64 *n_out);
+gdl_status sort_unique_keys(gdl_ctx *ctx, const u64 *d_in, u64 n, int hi_bits, int lo_bits,
+                            u64 *d_out, u64 *n_out);
+gdl_status filter_keys(gdl_ctx *ctx, const u64 *d_in, u64 n, gdl_pred pred, u64 *d_out,
+                       u64 *n_out);
+
+inline unsigned grid_for(u64 n, u64 per_block) {
+    u64 g = (n + per_block - 1) / per_block;
+    return (unsigned)(g ? g : 1);
+}
+
+}  // namespace gdl
+
+// ----------------------------------------------------------- device helpers
+namespace gdl {
+
+__device__ __forceinline__ u64 lane_mask_lt() {
+    return (1ull << (threadIdx.x & 63)) - 1ull;
+}
+
+// Inclusive scan of a u64 across the 64 lanes of a wave.
+__device__ __forceinline__ u64 wave_inclusive_scan(u64 v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        u64 o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan of one u64 per thread.  `s_wave` holds
+// BLOCK/64 + 1 u64.  Returns the exclusive prefix; *total gets the block sum.
+template <int BLOCK>
+__device__ __forceinline__ u64 block_exclusive_scan(u64 v, u64 *s_wave, u64 *total) {
+    constexpr int NW = BLOCK / 64;
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    u64 inc = wave_inclusive_scan(v);
+    if (lane == 63) s_wave[w] = inc;
+    __syncthreads();
+    if (w == 0) {
+        u64 x = (lane < NW) ? s_wave[lane] : 0;
+        u64 xi = wave_inclusive_scan(x);
+        if (lane < NW) s_wave[lane] = xi - x;
+        if (lane == NW - 1) s_wave[NW] = xi;
+    }
+    __syncthreads();
+    u64 res = s_wave[w] + inc - v;
+    *total = s_wave[NW];
+    __syncthreads();
+    return res;
+}
+
+__device__ __forceinline__ u32 key_col(u64 k, int c) {
+    return c ? (u32)(k & 0xffffffffull) : (u32)(k >> 32);
+}
+
+// Column map: src in {0: inner c0, 1: inner c1, 2: outer c0, 3: outer c1}.
+__device__ __forceinline__ u32 map_col(u64 ik, u64 ok, int src) {
+    u64 v = (src & 2) ? ok : ik;
+    return (src & 1) ? (u32)(v & 0xffffffffull) : (u32)(v >> 32);
+}
+
+__device__ __forceinline__ u64 apply_map(u64 ik, u64 ok, int arity, int s0, int s1) {
+    u64 hi = (u64)map_col(ik, ok, s0) << 32;
+    return arity == 2 ? (hi | (u64)map_col(ik, ok, s1)) : hi;
+}
+
+__device__ __forceinline__ bool eval_pred(u64 k, int op, int lhs, int rhs) {
+    u32 a = key_col(k, lhs), b = key_col(k, rhs);
+    switch (op) {
+        case GDL_PRED_EQ: return a == b;
+        case GDL_PRED_NE: return a != b;
+        case GDL_PRED_LT: return a < b;
+        case GDL_PRED_LE: return a <= b;
+        case GDL_PRED_GT: return a > b;
+        case GDL_PRED_GE: return a >= b;
+        case GDL_PRED_FALSE: return false;
+        default: return true;
+    }
+}
+
+// 64-bit finalizer (MurmurHash3 fmix64) for the prefix hash index.
+__host__ __device__ __forceinline__ u64 mix64(u64 k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+__device__ __forceinline__ u64 mulhi64(u64 a, u64 b) { return __umul64hi(a, b); }
+
+struct alignas(16) IdxSlot {
+    u64 key;  // prefix
+    u64 val;  // (begin << 24) | min(count, 0xFFFFFF); 0 = empty
+};
+constexpr u64 kCountBits = 24;
+constexpr u64 kCountSat = (1ull << kCountBits) - 1;
+
+// First index >= from whose key prefix differs from p (keys sorted).
+__device__ __forceinline__ u64 run_end(const u64 *keys, u64 n, u64 from, u64 p, int shift) {
+    u64 lo = from, step = 1, hi = from + 1;
+    while (hi < n && (keys[hi] >> shift) == p) {
+        lo = hi;
+        step <<= 1;
+        hi = from + step;
+    }
+    if (hi > n) hi = n;
+    // keys[lo] has prefix p; first mismatch in (lo, hi]
+    u64 a = lo + 1, b = hi;
+    while (a < b) {
+        u64 mid = a + ((b - a) >> 1);
+        if ((keys[mid] >> shift) == p) a = mid + 1;
+        else b = mid;
+    }
+    return a;
+}
+
+__device__ __forceinline__ bool idx_probe(const IdxSlot *slots, u64 n_slots, u64 p, u64 *begin,
+                                          u64 *count) {
+    u64 s = mulhi64(mix64(p), n_slots);
+    for (u64 step = 0; step < n_slots; ++step) {
+        IdxSlot e = slots[s];
+        if (e.val == 0) return false;
+        if (e.key == p) {
+            *begin = e.val >> kCountBits;
+            *count = e.val & kCountSat;
+            return true;
+        }
+        if (++s == n_slots) s = 0;
+    }
+    return false;
+}
+
+}  // namespace gdl

@@ -1,0 +1,145 @@
+// lie.cpp -- LIE::fixpoint_loop, the semi-naive loop of one SCC
+// (reference: src/lie.cu:26-274), on the C ABI.
+//
+// Per iteration: run the RA operators in insertion order (a RelationalCopy
+// from FULL only on the first iteration), drop the tmp relations' newt, then
+// for each update relation compute delta = newt \ full and full = full U newt
+// in one merge-path pass (gdl_merge_diff_u64), which is the reference's
+// thrust::set_difference followed by Relation::flush_delta's thrust::merge.
+#include <algorithm>
+#include <iostream>
+
+#include "gdlog/exception.h"
+#include "gdlog/lie.h"
+
+using namespace gdlog;
+
+void LIE::add_ra(ra_op op) { ra_ops.push_back(op); }
+
+void LIE::add_relations(Relation *rel, bool static_flag) {
+    if (static_flag) static_relations.push_back(rel);
+    else update_relations.push_back(rel);
+}
+
+void LIE::add_tmp_relation(Relation *rel) { tmp_relations.push_back(rel); }
+
+namespace {
+
+// delta := first full->tuple_counts raw rows, unsorted (copy_relation_container
+// of lie.cu:56 copies full's data_raw rows and loads them without sorting).
+void seed_delta(Relation *rel) {
+    container_free(rel->delta);
+    const tuple_size_t n = std::min(rel->full->tuple_counts, rel->raw_rows);
+    if (n == 0) return;
+    uint64_t *k = device_alloc_keys(n);
+    check(gdl_memcpy_d2d(context(), k, rel->raw_keys, n * sizeof(uint64_t)), "seed delta");
+    container_take(rel->delta, k, n, n, rel->raw_sorted);
+}
+
+}  // namespace
+
+void LIE::fixpoint_loop() {
+    trace.clear();
+    gdl_ctx *ctx = context();
+    if (verbose()) std::cout << "start lie .... " << std::endl;
+    for (Relation *rel : update_relations) {
+        rel->current_full_size = rel->full->tuple_counts;
+        seed_delta(rel);
+    }
+    int iteration_counter = 0;
+    while (true) {
+        for (auto &op : ra_ops) {
+            if (auto *j = std::get_if<RelationalJoin>(&op)) {
+                (*j)();
+            } else if (auto *a = std::get_if<RelationalACopy>(&op)) {
+                (*a)();
+            } else if (auto *c = std::get_if<RelationalCopy>(&op)) {
+                if (c->src_ver == FULL) {
+                    if (!c->copied) {
+                        (*c)();
+                        c->copied = true;
+                    }
+                } else {
+                    (*c)();
+                }
+            }
+        }
+        for (Relation *rel : tmp_relations) container_free(rel->newt);
+
+        if (verbose())
+            std::cout << "Iteration " << iteration_counter << " popluating new tuple" << std::endl;
+        bool fixpoint_flag = true;
+        int rel_idx = -1;
+        for (Relation *rel : update_relations) {
+            ++rel_idx;
+            if (verbose()) std::cout << rel->name << std::endl;
+            container_free(rel->delta);
+            GHashRelContainer *newt = rel->newt;
+            if (!newt->keys || newt->tuple_counts == 0) {
+                container_free(newt);
+                trace.push_back({iteration_counter, rel_idx, 0, 0, rel->current_full_size});
+                if (verbose())
+                    std::cout << "iteration " << iteration_counter << " relation " << rel->name
+                              << " no new tuple added" << std::endl;
+                continue;
+            }
+            container_normalize(newt);
+            const tuple_size_t nn = newt->tuple_counts;
+            const tuple_size_t nf = rel->current_full_size;
+            uint64_t *new_full = device_alloc_keys(nf + nn);
+            uint64_t *delta = device_alloc_keys(nn);
+            uint64_t nu = 0, nd = 0;
+            gdl_status st = gdl_merge_diff_u64(ctx, rel->full->keys, nf, newt->keys, nn, new_full,
+                                               &nu, delta, &nd);
+            if (st != GDL_OK) {
+                device_free(new_full);
+                device_free(delta);
+                check(st, "gdl_merge_diff_u64");
+            }
+            if (nd != 0) fixpoint_flag = false;
+            container_free(newt);
+            container_take(rel->delta, delta, nd, nn, true);
+            if (nd > 0) {
+                container_take(rel->full, new_full, nu, nf + nn, true);
+                rel->current_full_size = nu;
+            } else {
+                device_free(new_full);
+            }
+            trace.push_back({iteration_counter, rel_idx, nn, nd, rel->current_full_size});
+            if (verbose())
+                std::cout << "iteration " << iteration_counter << " relation " << rel->name
+                          << " finish dedup new tuples : " << nd
+                          << " delta tuple size: " << rel->delta->tuple_counts << " full counts "
+                          << rel->current_full_size << std::endl;
+        }
+        if (verbose())
+            std::cout << "Iteration " << iteration_counter << " finish populating" << std::endl;
+        iteration_counter++;
+        if (fixpoint_flag || iteration_counter > max_iteration) break;
+    }
+    iteration_count = iteration_counter;
+
+    for (Relation *rel : update_relations) {
+        if (reload_full_flag) {
+            // lie.cu:219-254: full reloaded from the merged tuples; its rows
+            // (data_raw) become the sorted full.
+            rel->full->tuple_counts = rel->current_full_size;
+            device_free(rel->raw_keys);
+            rel->raw_keys = nullptr;
+            rel->raw_rows = 0;
+            const tuple_size_t n = rel->current_full_size;
+            if (n) {
+                rel->raw_keys = device_alloc_keys(n);
+                check(gdl_memcpy_d2d(ctx, rel->raw_keys, rel->full->keys, n * sizeof(uint64_t)),
+                      "reload full");
+                rel->raw_rows = n;
+            }
+            rel->raw_sorted = true;
+            container_free(rel->delta);
+            container_free(rel->newt);
+        }
+        if (verbose())
+            std::cout << "Finished! " << rel->name << " has " << rel->full->tuple_counts
+                      << std::endl;
+    }
+}

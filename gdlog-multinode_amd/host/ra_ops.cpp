@@ -1,0 +1,173 @@
+// ra_ops.cpp -- RelationalJoin / RelationalCopy / RelationalACopy
+// (reference: src/join.cu, src/copy.cu, src/acopy.cu), on the C ABI.
+#include <chrono>
+#include <iostream>
+
+#include "gdlog/exception.h"
+#include "gdlog/relational_algebra.h"
+
+using namespace gdlog;
+
+namespace {
+
+// Wall-clock phase timer into detail_time[slot] (seconds); active only with
+// gdlog::set_phase_timing(true), since it synchronises the stream.
+class Phase {
+  public:
+    Phase(float *detail, int slot) : detail_(phase_timing() ? detail : nullptr), slot_(slot) {
+        if (detail_) {
+            check(gdl_sync(context()), "gdl_sync");
+            t0_ = std::chrono::steady_clock::now();
+        }
+    }
+    ~Phase() {
+        if (!detail_) return;
+        (void)gdl_sync(context());
+        detail_[slot_] +=
+            std::chrono::duration<float>(std::chrono::steady_clock::now() - t0_).count();
+    }
+
+  private:
+    float *detail_;
+    int slot_;
+    std::chrono::steady_clock::time_point t0_;
+};
+
+GHashRelContainer *version_of(Relation *r, RelationVersion v, bool inner) {
+    if (v == DELTA) return r->delta;
+    if (v == FULL || inner) return r->full;  // NEWT as inner reads FULL (join.cu:15-20)
+    return r->newt;
+}
+
+// newt := newt U sort+unique(fresh); takes ownership of `fresh`.
+void accumulate_sorted(GHashRelContainer *newt, uint64_t *fresh, tuple_size_t n, float *detail) {
+    uint64_t *su = nullptr;
+    tuple_size_t m;
+    {
+        Phase ph(detail, 3);
+        m = sort_unique(fresh, n, &su);
+    }
+    device_free(fresh);
+    if (!newt->keys || newt->tuple_counts == 0) {
+        container_take(newt, su, m, n, true);
+        return;
+    }
+    container_normalize(newt);
+    Phase ph(detail, 6);
+    const tuple_size_t na = newt->tuple_counts;
+    uint64_t *merged = device_alloc_keys(na + m);
+    uint64_t nm = 0;
+    gdl_status st = gdl_merge_union_u64(context(), newt->keys, na, su, m, merged, &nm);
+    device_free(su);
+    if (st != GDL_OK) device_free(merged);
+    check(st, "newt merge");
+    container_take(newt, merged, nm, na + m, true);
+}
+
+// tmp newt: raw concatenation, no dedup (join.cu:141-151,226-254).
+void append_raw(GHashRelContainer *newt, uint64_t *fresh, tuple_size_t n) {
+    if (!newt->keys || newt->tuple_counts == 0) {
+        container_take(newt, fresh, n, n, false);
+        return;
+    }
+    const tuple_size_t have = newt->tuple_counts;
+    if (newt->capacity >= have + n) {
+        check(gdl_memcpy_d2d(context(), newt->keys + have, fresh, n * sizeof(uint64_t)),
+              "tmp append");
+        device_free(fresh);
+        container_drop_index(newt);
+        newt->tuple_counts = have + n;
+        newt->data_raw_row_size = have + n;
+        newt->sorted = false;
+        return;
+    }
+    const tuple_size_t cap = (have + n) + (have + n) / 2;
+    uint64_t *buf = device_alloc_keys(cap);
+    check(gdl_memcpy_d2d(context(), buf, newt->keys, have * sizeof(uint64_t)), "tmp append");
+    check(gdl_memcpy_d2d(context(), buf + have, fresh, n * sizeof(uint64_t)), "tmp append");
+    device_free(fresh);
+    container_take(newt, buf, have + n, cap, false);
+}
+
+}  // namespace
+
+void RelationalJoin::operator()() {
+    const bool output_is_tmp = output_rel->tmp_flag;
+    GHashRelContainer *inner = version_of(inner_rel, inner_ver, true);
+    GHashRelContainer *outer = version_of(outer_rel, outer_ver, false);
+    if (!outer->keys || outer->tuple_counts == 0) {
+        outer->tuple_counts = 0;
+        return;
+    }
+    if (!inner->keys || inner->tuple_counts == 0) {
+        outer->tuple_counts = 0;  // join.cu:42-45: an empty inner zeroes the outer's count
+        return;
+    }
+    if (!gen_map.ok)
+        throw Error(GDL_ERR_UNSUPPORTED, "RelationalJoin into " + output_rel->name +
+                                             ": the generator hook is not a column selection");
+    if (!pred_map.ok)
+        throw Error(GDL_ERR_UNSUPPORTED, "RelationalJoin into " + output_rel->name +
+                                             ": the predicate is not a column comparison");
+    const int k = (int)outer->index_column_size;
+    if (k < 1 || k > inner->arity || k > outer->arity)
+        throw Error(GDL_ERR_INVALID_ARG, "RelationalJoin: bad join column count");
+    const gdl_index *idx;
+    {
+        Phase ph(detail_time, 5);
+        idx = container_index(inner, k);
+    }
+    uint64_t *res = nullptr;
+    uint64_t m = 0;
+    {
+        Phase ph(detail_time, 2);
+        check(gdl_join(context(), inner->keys, inner->tuple_counts, idx, outer->keys,
+                       outer->tuple_counts, gen_map.map, pred_map.pred, &res, &m),
+              "gdl_join");
+    }
+    if (verbose())
+        std::cout << output_rel->name << "   " << k << " join result size(non dedup) " << m
+                  << std::endl;
+    if (disable_load || m == 0) {
+        device_free(res);
+        return;
+    }
+    if (output_is_tmp) append_raw(output_rel->newt, res, m);
+    else accumulate_sorted(output_rel->newt, res, m, detail_time);
+}
+
+void RelationalCopy::operator()() {
+    GHashRelContainer *src = src_ver == DELTA ? src_rel->delta : src_rel->full;
+    if (verbose()) std::cout << "Copy " << src_rel->name << " to " << dest_rel->name << std::endl;
+    if (src->tuple_counts == 0) {
+        dest_rel->newt->tuple_counts = 0;  // copy.cu:23-26
+        return;
+    }
+    if (!copy_map.ok)
+        throw Error(GDL_ERR_UNSUPPORTED, "RelationalCopy into " + dest_rel->name +
+                                             ": the copy hook is not a column selection");
+    const tuple_size_t n = src->tuple_counts;
+    uint64_t *copied = device_alloc_keys(n);
+    check(gdl_project_u64(context(), src->keys, n, copy_map.map, copied), "gdl_project_u64");
+    accumulate_sorted(dest_rel->newt, copied, n, nullptr);
+}
+
+void RelationalACopy::operator()() {
+    GHashRelContainer *src = src_rel->newt;
+    GHashRelContainer *dest = dest_rel->newt;
+    if (verbose()) std::cout << "ACopy " << src_rel->name << " to " << dest_rel->name << std::endl;
+    if (src->tuple_counts == 0) {
+        container_free(dest);  // acopy.cu:19-23
+        return;
+    }
+    if (!copy_map.ok)
+        throw Error(GDL_ERR_UNSUPPORTED, "RelationalACopy into " + dest_rel->name +
+                                             ": the copy hook is not a column selection");
+    const tuple_size_t n = src->tuple_counts;
+    uint64_t *copied = device_alloc_keys(n);
+    check(gdl_project_u64(context(), src->keys, n, copy_map.map, copied), "gdl_project_u64");
+    uint64_t *su = nullptr;
+    tuple_size_t m = sort_unique(copied, n, &su);
+    device_free(copied);
+    container_take(dest, su, m, n, true);
+}

@@ -1,0 +1,327 @@
+// relation.cpp -- Relation / GHashRelContainer storage of the gdlog C++ API
+// (reference: include/relation.cuh, src/relation.cu).  Calls the engine only
+// through the C ABI of include/gdl_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+
+#include "gdlog/exception.h"
+#include "gdlog/relation.h"
+
+namespace gdlog {
+
+namespace {
+gdl_ctx *g_ctx = nullptr;
+int g_id_bits = 0;
+bool g_phase_timing = false;
+int g_verbose = -1;
+}  // namespace
+
+void check(gdl_status st, const char *what) {
+    if (st == GDL_OK) return;
+    std::string msg = std::string(what) + ": " + gdl_status_string(st);
+    const char *detail = g_ctx ? gdl_last_error(g_ctx) : "";
+    if (detail && *detail) msg += std::string(" (") + detail + ")";
+    throw Error(st, msg);
+}
+
+gdl_ctx *context() {
+    if (!g_ctx) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+        gdl_config cfg;
+        gdl_config_default(&cfg);
+        cfg.device = dev;
+        cfg.verbose = verbose() ? 1 : 0;
+        check(gdl_ctx_create(&cfg, &g_ctx), "gdl_ctx_create");
+    }
+    return g_ctx;
+}
+
+gdl_ctx *context_or_null() { return g_ctx; }
+
+void set_context(gdl_ctx *ctx) { g_ctx = ctx; }
+
+int id_bits() { return g_id_bits; }
+void note_id_bits(int bits) { g_id_bits = std::max(g_id_bits, std::min(bits, 32)); }
+void reset_id_bits() { g_id_bits = 0; }
+
+bool verbose() {
+    if (g_verbose < 0) {
+        const char *e = std::getenv("GDLOG_VERBOSE");
+        g_verbose = (e && *e && *e != '0') ? 1 : 0;
+    }
+    return g_verbose == 1;
+}
+
+void set_phase_timing(bool on) { g_phase_timing = on; }
+bool phase_timing() { return g_phase_timing; }
+
+uint64_t *device_alloc_keys(tuple_size_t n) {
+    void *p = nullptr;
+    check(gdl_alloc(context(), (n ? n : 1) * sizeof(uint64_t), &p), "gdl_alloc");
+    return static_cast<uint64_t *>(p);
+}
+
+void device_free(void *p) {
+    if (p) check(gdl_free(context(), p), "gdl_free");
+}
+
+void container_drop_index(GHashRelContainer *c) {
+    if (c->index.slots) check(gdl_index_free(context(), &c->index), "gdl_index_free");
+    c->index = gdl_index{};
+    c->index_prefix_cols = 0;
+    c->index_map_size = 0;
+}
+
+void container_free(GHashRelContainer *c) {
+    container_drop_index(c);
+    device_free(c->keys);
+    c->keys = nullptr;
+    c->capacity = 0;
+    c->tuple_counts = 0;
+    c->data_raw_row_size = 0;
+    c->sorted = true;
+}
+
+void container_take(GHashRelContainer *c, uint64_t *keys, tuple_size_t n, tuple_size_t cap,
+                    bool sorted) {
+    container_free(c);
+    c->keys = keys;
+    c->capacity = cap;
+    c->tuple_counts = n;
+    c->data_raw_row_size = n;
+    c->sorted = sorted;
+}
+
+tuple_size_t sort_unique(const uint64_t *d_in, tuple_size_t n, uint64_t **d_out) {
+    *d_out = nullptr;
+    if (n == 0) return 0;
+    uint64_t *out = device_alloc_keys(n);
+    uint64_t m = 0;
+    const int b = id_bits();
+    gdl_status st = gdl_sort_unique_u64(context(), d_in, n, b, b, out, &m);
+    if (st != GDL_OK) {
+        device_free(out);
+        check(st, "gdl_sort_unique_u64");
+    }
+    *d_out = out;
+    return m;
+}
+
+void container_normalize(GHashRelContainer *c) {
+    if (c->sorted || c->tuple_counts == 0) {
+        c->sorted = true;
+        return;
+    }
+    uint64_t *out = nullptr;
+    tuple_size_t m = sort_unique(c->keys, c->tuple_counts, &out);
+    container_take(c, out, m, c->tuple_counts, true);
+}
+
+const gdl_index *container_index(GHashRelContainer *c, int prefix_cols) {
+    if (c->index.slots && c->index_prefix_cols == prefix_cols) return &c->index;
+    container_drop_index(c);
+    container_normalize(c);
+    check(gdl_index_build(context(), c->keys, c->tuple_counts, prefix_cols, &c->index),
+          "gdl_index_build");
+    c->index_prefix_cols = prefix_cols;
+    c->index_map_size = c->index.n_slots;
+    return &c->index;
+}
+
+}  // namespace gdlog
+
+using namespace gdlog;
+
+namespace {
+
+// Packs host rows of 8-byte columns into keys, checking the u32 id range.
+std::vector<uint64_t> pack_rows(const column_type *data, tuple_size_t rows, int arity,
+                                const std::string &name) {
+    std::vector<uint64_t> keys(rows);
+    column_type mx = 0;
+    for (tuple_size_t i = 0; i < rows; ++i) {
+        const column_type c0 = data[i * arity];
+        const column_type c1 = arity > 1 ? data[i * arity + 1] : 0;
+        mx |= c0 | c1;
+        keys[i] = ((uint64_t)c0 << 32) | (uint64_t)c1;
+    }
+    if (mx > 0xffffffffull)
+        throw Error(GDL_ERR_OVERFLOW, "load_relation " + name + ": an id does not fit in 32 bits");
+    note_id_bits(mx ? 64 - __builtin_clzll((unsigned long long)mx) : 0);
+    return keys;
+}
+
+void check_shape(const std::string &name, int arity, tuple_size_t index_column_size,
+                 int dependent_column_size) {
+    if (arity < 1 || arity > 2)
+        throw Error(GDL_ERR_UNSUPPORTED,
+                    "relation " + name + ": arity " + std::to_string(arity) +
+                        " (this engine stores arity 1 and 2)");
+    if (index_column_size < 1 || index_column_size > (tuple_size_t)arity)
+        throw Error(GDL_ERR_INVALID_ARG, "relation " + name + ": bad index_column_size");
+    if (dependent_column_size != 0)
+        throw Error(GDL_ERR_UNSUPPORTED,
+                    "relation " + name + ": dependent columns are not supported");
+}
+
+void init_relation(Relation *target, const std::string &name, int arity,
+                   tuple_size_t index_column_size, int dependent_column_size, bool tmp_flag) {
+    check_shape(name, arity, index_column_size, dependent_column_size);
+    target->name = name;
+    target->arity = arity;
+    target->index_column_size = (int)index_column_size;
+    target->dependent_column_size = dependent_column_size;
+    target->tmp_flag = tmp_flag;
+    target->full = new GHashRelContainer(arity, (int)index_column_size, dependent_column_size);
+    target->delta = new GHashRelContainer(arity, (int)index_column_size, dependent_column_size);
+    target->newt = new GHashRelContainer(arity, (int)index_column_size, dependent_column_size);
+    target->raw_keys = nullptr;
+    target->raw_rows = 0;
+    target->raw_sorted = false;
+}
+
+// full := sort + unique(raw)
+void build_full_from_raw(Relation *target) {
+    if (target->raw_rows == 0) return;
+    uint64_t *out = nullptr;
+    tuple_size_t m = sort_unique(target->raw_keys, target->raw_rows, &out);
+    container_take(target->full, out, m, target->raw_rows, true);
+}
+
+}  // namespace
+
+void load_relation(Relation *target, std::string name, int arity, column_type *data,
+                   tuple_size_t data_row_size, tuple_size_t index_column_size,
+                   int dependent_column_size, int grid_size, int block_size, bool tmp_flag) {
+    (void)grid_size;
+    (void)block_size;
+    init_relation(target, name, arity, index_column_size, dependent_column_size, tmp_flag);
+    if (data_row_size == 0 || !data) return;
+    std::vector<uint64_t> keys = pack_rows(data, data_row_size, arity, name);
+    target->raw_keys = device_alloc_keys(data_row_size);
+    target->raw_rows = data_row_size;
+    check(gdl_memcpy_h2d(context(), target->raw_keys, keys.data(), keys.size() * sizeof(uint64_t)),
+          "load_relation H2D");
+    build_full_from_raw(target);
+}
+
+void load_relation_device(Relation *target, std::string name, int arity, const uint64_t *d_keys,
+                          tuple_size_t data_row_size, tuple_size_t index_column_size,
+                          int dependent_column_size, int grid_size, int block_size,
+                          bool tmp_flag) {
+    (void)grid_size;
+    (void)block_size;
+    init_relation(target, name, arity, index_column_size, dependent_column_size, tmp_flag);
+    if (data_row_size == 0 || !d_keys) return;
+    int hi = 0, lo = 0;
+    check(gdl_key_bits(context(), d_keys, data_row_size, &hi, &lo), "gdl_key_bits");
+    note_id_bits(std::max(hi, lo));
+    target->raw_keys = device_alloc_keys(data_row_size);
+    target->raw_rows = data_row_size;
+    check(gdl_memcpy_d2d(context(), target->raw_keys, d_keys, data_row_size * sizeof(uint64_t)),
+          "load_relation_device D2D");
+    build_full_from_raw(target);
+}
+
+void load_relation_container(GHashRelContainer *target, int arity, column_type *data,
+                             tuple_size_t data_row_size, tuple_size_t index_column_size,
+                             int dependent_column_size, float index_map_load_factor,
+                             int grid_size, int block_size, float *detail_time,
+                             bool gpu_data_flag, bool sorted_flag, bool build_index_flag,
+                             bool tuples_array_flag) {
+    (void)grid_size;
+    (void)block_size;
+    (void)detail_time;
+    (void)tuples_array_flag;
+    check_shape("container", arity, index_column_size, dependent_column_size);
+    container_free(target);
+    target->arity = arity;
+    target->index_column_size = index_column_size;
+    target->dependent_column_size = dependent_column_size;
+    target->index_map_load_factor = index_map_load_factor;
+    if (data_row_size == 0 || !data) return;
+    uint64_t *raw = device_alloc_keys(data_row_size);
+    if (gpu_data_flag) {
+        // device data: packed u64 keys
+        check(gdl_memcpy_d2d(context(), raw, data, data_row_size * sizeof(uint64_t)),
+              "load_relation_container D2D");
+        int hi = 0, lo = 0;
+        check(gdl_key_bits(context(), raw, data_row_size, &hi, &lo), "gdl_key_bits");
+        note_id_bits(std::max(hi, lo));
+    } else {
+        std::vector<uint64_t> keys = pack_rows(data, data_row_size, arity, "container");
+        check(gdl_memcpy_h2d(context(), raw, keys.data(), keys.size() * sizeof(uint64_t)),
+              "load_relation_container H2D");
+    }
+    if (sorted_flag) {
+        container_take(target, raw, data_row_size, data_row_size, true);
+    } else {
+        uint64_t *out = nullptr;
+        tuple_size_t m = sort_unique(raw, data_row_size, &out);
+        device_free(raw);
+        container_take(target, out, m, data_row_size, true);
+    }
+    if (build_index_flag) {
+        float old = index_map_load_factor;
+        if (old > 0.f && old <= 0.95f) check(gdl_set_load_factor(context(), old), "load factor");
+        container_index(target, (int)index_column_size);
+    }
+}
+
+void free_relation_container(GHashRelContainer *target) { container_free(target); }
+
+void copy_relation_container(GHashRelContainer *dst, GHashRelContainer *src, int grid_size,
+                             int block_size) {
+    (void)grid_size;
+    (void)block_size;
+    container_free(dst);
+    dst->arity = src->arity;
+    dst->index_column_size = src->index_column_size;
+    dst->dependent_column_size = src->dependent_column_size;
+    if (src->tuple_counts == 0) return;
+    uint64_t *k = device_alloc_keys(src->tuple_counts);
+    check(gdl_memcpy_d2d(context(), k, src->keys, src->tuple_counts * sizeof(uint64_t)),
+          "copy_relation_container");
+    container_take(dst, k, src->tuple_counts, src->tuple_counts, src->sorted);
+}
+
+void free_relation(Relation *target) {
+    for (GHashRelContainer *c : {target->full, target->delta, target->newt}) {
+        if (!c) continue;
+        container_free(c);
+        delete c;
+    }
+    target->full = target->delta = target->newt = nullptr;
+    device_free(target->raw_keys);
+    target->raw_keys = nullptr;
+    target->raw_rows = 0;
+}
+
+void Relation::flush_delta(int grid_size, int block_size, float *detail_time) {
+    (void)grid_size;
+    (void)block_size;
+    (void)detail_time;
+    if (delta->tuple_counts == 0) return;
+    container_normalize(delta);
+    const tuple_size_t nf = current_full_size, nd = delta->tuple_counts;
+    uint64_t *merged = device_alloc_keys(nf + nd);
+    uint64_t nm = 0;
+    check(gdl_merge_union_u64(context(), full->keys, nf, delta->keys, nd, merged, &nm),
+          "flush_delta merge");
+    container_take(full, merged, nm, nf + nd, true);
+    current_full_size = nm;
+}
+
+std::vector<uint64_t> relation_keys(GHashRelContainer *c) {
+    std::vector<uint64_t> out(c->tuple_counts);
+    if (c->tuple_counts)
+        check(gdl_memcpy_d2h(context(), out.data(), c->keys, c->tuple_counts * sizeof(uint64_t)),
+              "relation_keys D2H");
+    if (!c->sorted) std::sort(out.begin(), out.end());
+    return out;
+}

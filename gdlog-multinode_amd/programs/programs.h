@@ -1,0 +1,42 @@
+// programs.h -- shared input/output plumbing of the TC, SG and CSPA programs.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "gdlog/lie.h"
+
+// Edge input of a program: host rows of two 8-byte columns (the reference's
+// get_relation_from_file layout) or device-resident packed keys.
+struct EdgeInput {
+    const column_type *host = nullptr;
+    const uint64_t *dev = nullptr;
+    tuple_size_t rows = 0;
+    // load_relation of the edges (reversed: (c1, c0) rows, the reference's
+    // host-side reversal, test/tc.cu:94-101).
+    void load(Relation *r, const std::string &name, bool reversed, int grid_size,
+              int block_size, bool tmp_flag = false) const;
+};
+
+struct ProgramOutput {
+    int iterations = 0;
+    int init_iterations = 0;
+    uint64_t derived = 0;  // sum of |delta| over update relations and iterations
+    double load_ms = 0.0;
+    double fixpoint_ms = 0.0;
+    std::vector<IterationRecord> trace, trace_init;
+    bool want_keys = false;
+    uint64_t n_main = 0;
+    std::vector<uint64_t> main_keys;
+    std::vector<uint64_t> extra_keys[2];
+    uint64_t sizes[4] = {0, 0, 0, 0};
+};
+
+double now_ms();  // device-synchronised wall clock
+uint64_t sum_delta(const std::vector<IterationRecord> &t);
+
+void tc_program(const EdgeInput &graph, int max_iteration, int grid_size, int block_size,
+                ProgramOutput *out);
+void sg_program(const EdgeInput &graph, int grid_size, int block_size, ProgramOutput *out);
+void cspa_program(const EdgeInput &assign, const EdgeInput &dereference, int grid_size,
+                  int block_size, ProgramOutput *out);

@@ -1,0 +1,214 @@
+/*
+ * gdl_hip.h -- C ABI of the MI355X (gfx950) semi-naive Datalog engine.
+ *
+ * Plain pointers and sizes only.  Every call returns a gdl_status; nothing
+ * calls exit().  A context owns one HIP device, one HIP stream (default
+ * priority), a caching device allocator with an optional byte cap, and the
+ * tuning parameters.  All work of a context is ordered on its stream; calls
+ * that return a size to the host synchronise that stream.  One context per
+ * host thread; a context is not re-entrant.
+ *
+ * Tuple encoding.  A tuple of arity 1 or 2 over u32 ids is one u64 key:
+ * (c0 << 32) | c1, arity 1 as c0 << 32.  Integer order of keys is the
+ * reference's lexicographic tuple order (include/tuple.cuh:94-127) and key
+ * equality its full-tuple equality (tuple.cuh:51-65).  The join prefix of
+ * `prefix_cols` columns is key >> 32 (1 column) or the whole key (2).
+ *
+ * The reference exposes a C++ API only (it has no FFI); each entry point
+ * below names the reference function or Thrust call site it replaces.
+ */
+#ifndef GDL_HIP_H
+#define GDL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gdl_status {
+    GDL_OK = 0,
+    GDL_ERR_INVALID_ARG = 1,   /* null pointer, bad arity, bad column map ... */
+    GDL_ERR_OUT_OF_MEMORY = 2, /* device allocation failed or the byte cap was hit */
+    GDL_ERR_HIP = 3,           /* a HIP runtime call failed (see gdl_last_error) */
+    GDL_ERR_UNSUPPORTED = 4,   /* valid request outside what this engine implements */
+    GDL_ERR_OVERFLOW = 5,      /* a size exceeds a documented limit */
+    GDL_ERR_DEVICE = 6         /* a kernel reported an internal error */
+} gdl_status;
+
+typedef struct gdl_ctx gdl_ctx;
+
+typedef struct gdl_config {
+    int device;             /* HIP device ordinal */
+    uint64_t mem_cap_bytes; /* 0: no cap.  Otherwise allocations that would take the
+                               bytes in use above the cap fail (GDL_ERR_OUT_OF_MEMORY) */
+    float load_factor;      /* hash index load factor, (0, 0.95]; reference 0.8
+                               (relation.cu:603-604) */
+    int collect_stats;      /* 1: time every kernel with hipEvents on the context stream */
+    int verbose;            /* 1: print the reference-style per-op / per-iteration log */
+} gdl_config;
+
+/* Fills the defaults: device 0, no cap, load factor 0.8, no stats. */
+void gdl_config_default(gdl_config *cfg);
+
+/* Replaces the implicit device-0 / legacy-default-stream state of the
+ * reference (src/print.cu:85-124, every <<<grid,block>>> launch). */
+gdl_status gdl_ctx_create(const gdl_config *cfg, gdl_ctx **out);
+gdl_status gdl_ctx_destroy(gdl_ctx *ctx);
+const char *gdl_status_string(gdl_status s);
+/* Detail of the last failure on this context (empty string if none). */
+const char *gdl_last_error(gdl_ctx *ctx);
+/* Waits for all work of the context; reports kernel-side errors. */
+gdl_status gdl_sync(gdl_ctx *ctx);
+/* The context's hipStream_t, as void*. */
+void *gdl_stream(gdl_ctx *ctx);
+int gdl_device(gdl_ctx *ctx);
+gdl_status gdl_set_load_factor(gdl_ctx *ctx, float load_factor);
+
+/* ---- device memory (replaces the per-op cudaMalloc/cudaFree/cudaMemset of
+ *      join.cu:49-62,109-114 and relation.cu:547-561,606-616) ---------- */
+gdl_status gdl_alloc(gdl_ctx *ctx, size_t bytes, void **dptr);
+gdl_status gdl_free(gdl_ctx *ctx, void *dptr);
+gdl_status gdl_memcpy_h2d(gdl_ctx *ctx, void *dst, const void *src, size_t bytes);
+gdl_status gdl_memcpy_d2h(gdl_ctx *ctx, void *dst, const void *src, size_t bytes);
+gdl_status gdl_memcpy_d2d(gdl_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Free/total device memory (replaces get_free_memory / get_total_memory,
+ * print.cu:98-124) and the allocator's own accounting. */
+gdl_status gdl_mem_info(gdl_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes,
+                        uint64_t *in_use, uint64_t *peak, uint64_t *cached);
+/* Returns cached (freed) blocks to HIP. */
+gdl_status gdl_release_cached(gdl_ctx *ctx);
+/* Changes the byte cap (0 = none). */
+gdl_status gdl_set_mem_cap(gdl_ctx *ctx, uint64_t cap_bytes);
+
+/* ---- sort / unique ---------------------------------------------------
+ * Replaces thrust::sort(tuple_indexed_less) + thrust::unique(t_equal)
+ * (relation.cu:576-590).  LSD radix sort over 8-bit digits of the
+ * order-preserving compressed key (c0 << lo_bits) | c1, where hi_bits and
+ * lo_bits bound the bit length of c0 and c1 (0..32; pass 32/32 when unknown,
+ * or -1 to measure them on the device).  d_in and d_out must not overlap.
+ * At most 2^32 - 1 keys per call. */
+gdl_status gdl_sort_u64(gdl_ctx *ctx, const uint64_t *d_in, uint64_t n, int hi_bits,
+                        int lo_bits, uint64_t *d_out);
+/* Adjacent-duplicate removal of a sorted array (thrust::unique, t_equal). */
+gdl_status gdl_unique_u64(gdl_ctx *ctx, const uint64_t *d_sorted, uint64_t n,
+                          uint64_t *d_out, uint64_t *n_out);
+/* Bit lengths of the largest c0 and c1 over n keys (0 for an empty array). */
+gdl_status gdl_key_bits(gdl_ctx *ctx, const uint64_t *d_keys, uint64_t n, int *hi_bits,
+                        int *lo_bits);
+/* sort + unique; d_out holds n slots. */
+gdl_status gdl_sort_unique_u64(gdl_ctx *ctx, const uint64_t *d_in, uint64_t n, int hi_bits,
+                               int lo_bits, uint64_t *d_out, uint64_t *n_out);
+
+/* ---- merge / set difference over sorted duplicate-free arrays ----------
+ * gdl_merge_union_u64: thrust::merge + thrust::unique of newt accumulation
+ *   (join.cu:181-195, copy.cu:63-72); d_out holds na + nb slots.
+ * gdl_set_difference_u64: thrust::set_difference(newt, full) (lie.cu:131-138),
+ *   d_out = b \ a, holds nb slots.
+ * gdl_merge_diff_u64: set difference and Relation::flush_delta's
+ *   thrust::merge(full, delta) (relation.cu:484-487) fused in one merge path:
+ *   d_delta = newt \ full (nn slots), d_full_out = full U newt (nf + nn slots;
+ *   may be NULL to skip the union). */
+gdl_status gdl_merge_union_u64(gdl_ctx *ctx, const uint64_t *d_a, uint64_t na,
+                               const uint64_t *d_b, uint64_t nb, uint64_t *d_out,
+                               uint64_t *n_out);
+gdl_status gdl_set_difference_u64(gdl_ctx *ctx, const uint64_t *d_b, uint64_t nb,
+                                  const uint64_t *d_a, uint64_t na, uint64_t *d_out,
+                                  uint64_t *n_out);
+gdl_status gdl_merge_diff_u64(gdl_ctx *ctx, const uint64_t *d_full, uint64_t nf,
+                              const uint64_t *d_newt, uint64_t nn, uint64_t *d_full_out,
+                              uint64_t *n_full_out, uint64_t *d_delta, uint64_t *n_delta);
+
+/* ---- hash index over the distinct join prefixes of a sorted array ------
+ * Replaces init_index_map + calculate_index_hash (relation.cu:12-80,124-137):
+ * open addressing with linear probing; each distinct prefix maps to the
+ * [begin, begin + count) run of the sorted array.  Slots are 16 bytes. */
+typedef struct gdl_index {
+    void *slots;        /* device; owned by the context allocator */
+    uint64_t n_slots;
+    uint64_t n_distinct; /* distinct prefixes */
+    int prefix_cols;    /* 1 or 2 */
+} gdl_index;
+gdl_status gdl_index_build(gdl_ctx *ctx, const uint64_t *d_sorted, uint64_t n, int prefix_cols,
+                           gdl_index *out);
+gdl_status gdl_index_free(gdl_ctx *ctx, gdl_index *idx);
+/* Looks up n prefixes (test/diagnostic entry): d_begin/d_count get the run
+ * of each (count 0 when absent). */
+gdl_status gdl_index_lookup(gdl_ctx *ctx, const gdl_index *idx, const uint64_t *d_sorted,
+                            uint64_t n_sorted, const uint64_t *d_prefixes, uint64_t n,
+                            uint64_t *d_begin, uint32_t *d_count);
+
+/* ---- column maps and predicates (the build's form of the reference hooks,
+ *      tuple_generator_hook / tuple_copy_hook / tuple_predicate,
+ *      include/tuple.cuh:23-25) ------------------------------------------ */
+enum {
+    GDL_COL_INNER0 = 0, /* join: inner column 0; copy: source column 0 */
+    GDL_COL_INNER1 = 1, /* join: inner column 1; copy: source column 1 */
+    GDL_COL_OUTER0 = 2,
+    GDL_COL_OUTER1 = 3
+};
+typedef struct gdl_colmap {
+    int32_t arity;  /* output arity, 1 or 2 */
+    int32_t src[2]; /* GDL_COL_* of output columns 0 and 1 */
+} gdl_colmap;
+
+enum { GDL_PRED_NONE = 0, GDL_PRED_EQ, GDL_PRED_NE, GDL_PRED_LT, GDL_PRED_LE, GDL_PRED_GT,
+       GDL_PRED_GE, GDL_PRED_FALSE };
+/* Keeps output tuple t iff t[lhs] <op> t[rhs] (columns of the generated tuple). */
+typedef struct gdl_pred {
+    int32_t op;
+    int32_t lhs;
+    int32_t rhs;
+} gdl_pred;
+
+/* ---- join (count, scan, materialize) ----------------------------------
+ * Replaces get_join_result_size, the chunked thrust::reduce and
+ * thrust::exclusive_scan, and get_join_result (relation.cu:221-373,
+ * join.cu:68-118).  For each outer key, every inner key of the same
+ * `prefix_cols`-column prefix (via the inner index) produces colmap(inner,
+ * outer); the predicate is applied to the produced tuple.
+ *   gdl_join_count: d_counts[i] = run length, d_begins[i] = run start.
+ *   gdl_join_scan: d_offsets (n + 1 slots) = exclusive scan, *total to host.
+ *   gdl_join_materialize: writes `total` unfiltered outputs (load-balanced). */
+gdl_status gdl_join_count(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n_inner,
+                          const gdl_index *inner_idx, const uint64_t *d_outer, uint64_t n_outer,
+                          uint32_t *d_counts, uint64_t *d_begins);
+gdl_status gdl_join_scan(gdl_ctx *ctx, const uint32_t *d_counts, uint64_t n,
+                         uint64_t *d_offsets, uint64_t *total);
+gdl_status gdl_join_materialize(gdl_ctx *ctx, const uint64_t *d_inner, const uint64_t *d_outer,
+                                uint64_t n_outer, const uint64_t *d_offsets,
+                                const uint64_t *d_begins, uint64_t total, gdl_colmap map,
+                                uint64_t *d_out);
+/* Whole join: allocates *d_out from the context allocator (free with
+ * gdl_free), applies the predicate, returns the row count in *n_out. */
+gdl_status gdl_join(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n_inner,
+                    const gdl_index *inner_idx, const uint64_t *d_outer, uint64_t n_outer,
+                    gdl_colmap map, gdl_pred pred, uint64_t **d_out, uint64_t *n_out);
+
+/* ---- projection / filter ----------------------------------------------
+ * gdl_project_u64 replaces get_copy_result (relation.cu:390-403);
+ * gdl_filter_u64 keeps keys whose tuple satisfies pred. */
+gdl_status gdl_project_u64(gdl_ctx *ctx, const uint64_t *d_in, uint64_t n, gdl_colmap map,
+                           uint64_t *d_out);
+gdl_status gdl_filter_u64(gdl_ctx *ctx, const uint64_t *d_in, uint64_t n, gdl_pred pred,
+                          uint64_t *d_out, uint64_t *n_out);
+
+/* ---- phase statistics (replaces KernelTimer / detail_time[], timer.cuh:5-30,
+ *      join.cu:74-258) ---------------------------------------------------- */
+typedef struct gdl_kernel_stat {
+    const char *name;
+    uint64_t launches;
+    double total_ms;       /* sum of hipEvent durations (collect_stats = 1) */
+    double alg_bytes;      /* algorithmic HBM bytes of all launches */
+} gdl_kernel_stat;
+gdl_status gdl_stats_enable(gdl_ctx *ctx, int enable);
+gdl_status gdl_stats_reset(gdl_ctx *ctx);
+/* Resolves pending events (synchronises) and returns the number of kernel
+ * classes; fills up to `cap` entries. */
+gdl_status gdl_stats_get(gdl_ctx *ctx, gdl_kernel_stat *out, int cap, int *n_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,71 @@
+/*
+ * gdl_programs.h -- C ABI of the TC, SG and CSPA programs built on the gdlog
+ * C++ API (the reference's benchmark drivers test/tc.cu, test/sg.cu and
+ * test/cspa.cu, whose analysis_bench functions these replace), for callers
+ * that cannot link C++: tests, benchmarks, foreign-function bindings.
+ *
+ * Edges are packed keys (src << 32) | dst.  With on_device = 0 they are host
+ * arrays loaded through load_relation (the reference's host path); with
+ * on_device = 1 they are device arrays of the given context.
+ */
+#ifndef GDL_PROGRAMS_H
+#define GDL_PROGRAMS_H
+
+#include <stdint.h>
+
+#include "gdl_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gdl_trace_rec {
+    int32_t iteration;
+    int32_t relation; /* index in the LIE's update-relation order */
+    uint64_t newt;    /* newt size before the set difference */
+    uint64_t delta;   /* new tuples */
+    uint64_t full;    /* full size after the merge */
+} gdl_trace_rec;
+
+typedef struct gdl_prog_result {
+    int32_t iterations;      /* iteration counter of the (last) LIE */
+    int32_t init_iterations; /* SG / CSPA init LIE */
+    uint64_t derived;        /* sum of |delta| over update relations and iterations */
+    uint64_t n_out;          /* size of the main output (path / sg / ValueAlias) */
+    uint64_t sizes[4];       /* CSPA: ValueFlow, MemoryAlias, ValueAlias, ValueFlow mirror */
+    double load_ms;          /* relation loading (sort, unique) */
+    double fixpoint_ms;      /* fixpoint loop(s), device-synchronised */
+    uint64_t n_trace;        /* records of the (last) LIE */
+    uint64_t n_trace_init;   /* records of the init LIE */
+} gdl_prog_result;
+
+/* Optional output buffers (NULL / 0 capacity to skip).  A buffer too small
+ * for its data yields GDL_ERR_OVERFLOW after the run (counts still set). */
+typedef struct gdl_prog_buffers {
+    uint64_t *out;           /* sorted keys of the main output */
+    uint64_t out_cap;
+    uint64_t *extra0;        /* CSPA: ValueFlow */
+    uint64_t extra0_cap;
+    uint64_t *extra1;        /* CSPA: MemoryAlias */
+    uint64_t extra1_cap;
+    gdl_trace_rec *trace;
+    uint64_t trace_cap;
+    gdl_trace_rec *trace_init;
+    uint64_t trace_init_cap;
+} gdl_prog_buffers;
+
+/* max_iteration < 0: unbounded (INT_MAX, lie.cuh:30). */
+gdl_status gdl_prog_tc(gdl_ctx *ctx, const uint64_t *edges, uint64_t n, int on_device,
+                       int max_iteration, gdl_prog_result *res, const gdl_prog_buffers *bufs);
+gdl_status gdl_prog_sg(gdl_ctx *ctx, const uint64_t *edges, uint64_t n, int on_device,
+                       gdl_prog_result *res, const gdl_prog_buffers *bufs);
+gdl_status gdl_prog_cspa(gdl_ctx *ctx, const uint64_t *assign, uint64_t n_assign,
+                         const uint64_t *dereference, uint64_t n_dereference, int on_device,
+                         gdl_prog_result *res, const gdl_prog_buffers *bufs);
+/* Message of the last failed gdl_prog_* call on this thread. */
+const char *gdl_prog_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,103 @@
+// gdlog/relational_algebra.h -- RA operators of the gdlog C++ API.
+//
+// Same constructors, fields and operator() as the reference
+// (include/relational_algebra.cuh:18-117): RelationalJoin (binary hash join
+// into output_rel->newt), RelationalCopy (projection into dest newt) and
+// RelationalACopy (re-indexing copy of one relation's newt into another's).
+// The hooks are turned into column maps when the operator is constructed
+// (gdlog/hook_probe.h), so constructors must be compiled with hipcc in the
+// translation unit that defines the hooks.
+#pragma once
+
+#include <variant>
+
+#include "gdlog/hook_probe.h"
+#include "gdlog/relation.h"
+
+// Chunk size of the reference's host-side reduction (relational_algebra.cuh:8-10);
+// kept for API compatibility (the engine's scan has no 32-bit limit).
+#ifndef MAX_REDUCE_SIZE
+#define MAX_REDUCE_SIZE 80000000
+#endif
+
+struct RelationalJoin {
+    Relation *inner_rel;
+    RelationVersion inner_ver;
+    Relation *outer_rel;
+    RelationVersion outer_ver;
+    Relation *output_rel;
+    tuple_generator_hook tuple_generator;
+    tuple_predicate tuple_pred;
+    JoinDirection direction;
+    int grid_size;
+    int block_size;
+    // benchmark flag: drop the join result instead of loading it into newt
+    bool disable_load = false;
+    // accumulated phase times (seconds): [0] count [1] scan [2] materialize
+    // [3] sort [4] unique [5] index [6] merge [7] post-merge unique
+    float *detail_time;
+
+    gdlog::HookMap gen_map;
+    gdlog::PredMap pred_map;
+
+    RelationalJoin(Relation *inner_rel, RelationVersion inner_ver, Relation *outer_rel,
+                   RelationVersion outer_ver, Relation *output_rel, tuple_generator_hook tp_gen,
+                   tuple_predicate tp_pred, JoinDirection direction, int grid_size, int block_size,
+                   float *detail_time)
+        : inner_rel(inner_rel), inner_ver(inner_ver), outer_rel(outer_rel), outer_ver(outer_ver),
+          output_rel(output_rel), tuple_generator(tp_gen), tuple_pred(tp_pred),
+          direction(direction), grid_size(grid_size), block_size(block_size),
+          detail_time(detail_time) {
+        gen_map = gdlog::probe_generator(tp_gen, output_rel ? output_rel->arity : 0);
+        pred_map = gdlog::probe_predicate(tp_pred, output_rel ? output_rel->arity : 0);
+    }
+
+    void operator()();
+};
+
+struct RelationalCopy {
+    Relation *src_rel;
+    RelationVersion src_ver;
+    Relation *dest_rel;
+    tuple_copy_hook tuple_generator;
+    tuple_predicate tuple_pred;  // stored, not applied (as in the reference, copy.cu)
+    int grid_size;
+    int block_size;
+    bool copied = false;
+
+    gdlog::HookMap copy_map;
+
+    RelationalCopy(Relation *src, RelationVersion src_ver, Relation *dest,
+                   tuple_copy_hook tuple_generator, tuple_predicate tuple_pred, int grid_size,
+                   int block_size)
+        : src_rel(src), src_ver(src_ver), dest_rel(dest), tuple_generator(tuple_generator),
+          tuple_pred(tuple_pred), grid_size(grid_size), block_size(block_size) {
+        copy_map = gdlog::probe_copy(tuple_generator, dest ? dest->arity : 0);
+    }
+
+    void operator()();
+};
+
+struct RelationalACopy {
+    Relation *src_rel;
+    Relation *dest_rel;
+    tuple_copy_hook tuple_generator;
+    tuple_predicate tuple_pred;  // stored, not applied (as in the reference, acopy.cu)
+    int grid_size;
+    int block_size;
+
+    gdlog::HookMap copy_map;
+
+    RelationalACopy(Relation *src, Relation *dest, tuple_copy_hook tuple_generator,
+                    tuple_predicate tuple_pred, int grid_size, int block_size)
+        : src_rel(src), dest_rel(dest), tuple_generator(tuple_generator), tuple_pred(tuple_pred),
+          grid_size(grid_size), block_size(block_size) {
+        copy_map = gdlog::probe_copy(tuple_generator, dest ? dest->arity : 0);
+    }
+
+    void operator()();
+};
+
+using ra_op = std::variant<RelationalJoin, RelationalCopy, RelationalACopy>;
+
+enum RAtypes { JOIN, COPY, ACOPY };

@@ -1,0 +1,126 @@
+"""CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+ctypes wrapper of oracle/lib/libgdl_oracle.so, the plain-C restatement of the
+reference engine's semi-naive algorithm (see gdl_oracle.h for what each
+function restates and how it is pinned).  Only tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg may use it, and only as the checker / the CPU
+baseline -- never as the product path.
+"""
+import ctypes
+import os
+from ctypes import POINTER, Structure, c_int, c_int32, c_size_t, c_uint64
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "lib", "libgdl_oracle.so")
+
+_U64P = POINTER(c_uint64)
+
+
+class TraceRec(Structure):
+    _fields_ = [("iteration", c_int), ("rel", c_int), ("newt", c_uint64), ("delta", c_uint64),
+                ("full", c_uint64)]
+
+
+class Result(Structure):
+    _fields_ = [("iterations", c_int), ("init_iterations", c_int), ("derived", c_uint64),
+                ("n_out", c_size_t), ("out", _U64P), ("n_trace", c_size_t),
+                ("trace", POINTER(TraceRec)), ("n_trace_init", c_size_t),
+                ("trace_init", POINTER(TraceRec)), ("sizes", c_uint64 * 4),
+                ("extra", _U64P * 3), ("n_extra", c_size_t * 3)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise OSError(f"{LIB} is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB)
+        L.orc_sort_unique.restype = c_size_t
+        L.orc_sort_unique.argtypes = [_U64P, c_size_t]
+        L.orc_merge_union.restype = c_size_t
+        L.orc_merge_union.argtypes = [_U64P, c_size_t, _U64P, c_size_t, _U64P]
+        L.orc_set_difference.restype = c_size_t
+        L.orc_set_difference.argtypes = [_U64P, c_size_t, _U64P, c_size_t, _U64P]
+        L.orc_tc.restype = c_int
+        L.orc_tc.argtypes = [_U64P, c_size_t, c_int, POINTER(Result)]
+        L.orc_sg.restype = c_int
+        L.orc_sg.argtypes = [_U64P, c_size_t, POINTER(Result)]
+        L.orc_cspa.restype = c_int
+        L.orc_cspa.argtypes = [_U64P, c_size_t, _U64P, c_size_t, POINTER(Result)]
+        L.orc_result_free.restype = None
+        L.orc_result_free.argtypes = [POINTER(Result)]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_U64P)
+
+
+def _arr(a):
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+def sort_unique(keys):
+    a = _arr(keys).copy()
+    n = lib().orc_sort_unique(_p(a), a.size)
+    return a[:n]
+
+
+def merge_union(a, b):
+    a, b = _arr(a), _arr(b)
+    out = np.zeros(a.size + b.size + 1, dtype=np.uint64)
+    n = lib().orc_merge_union(_p(a), a.size, _p(b), b.size, _p(out))
+    return out[:n]
+
+
+def set_difference(b, a):
+    a, b = _arr(a), _arr(b)
+    out = np.zeros(b.size + 1, dtype=np.uint64)
+    n = lib().orc_set_difference(_p(b), b.size, _p(a), a.size, _p(out))
+    return out[:n]
+
+
+def _collect(res):
+    out = np.ctypeslib.as_array(res.out, shape=(res.n_out,)).copy() if res.n_out else \
+        np.zeros(0, dtype=np.uint64)
+    tr = [(res.trace[i].iteration, res.trace[i].rel, res.trace[i].newt, res.trace[i].delta,
+           res.trace[i].full) for i in range(res.n_trace)]
+    ti = [(res.trace_init[i].iteration, res.trace_init[i].rel, res.trace_init[i].newt,
+           res.trace_init[i].delta, res.trace_init[i].full) for i in range(res.n_trace_init)]
+    extra = []
+    for k in range(2):
+        n = res.n_extra[k]
+        extra.append(np.ctypeslib.as_array(res.extra[k], shape=(n,)).copy() if n else
+                     np.zeros(0, dtype=np.uint64))
+    r = dict(iterations=res.iterations, init_iterations=res.init_iterations,
+             derived=res.derived, n_out=res.n_out, out=out, trace=tr, trace_init=ti,
+             sizes=list(res.sizes), extra=extra)
+    lib().orc_result_free(ctypes.byref(res))
+    return r
+
+
+def tc(edges, max_iteration=-1):
+    e = _arr(edges)
+    res = Result()
+    lib().orc_tc(_p(e), e.size, max_iteration, ctypes.byref(res))
+    return _collect(res)
+
+
+def sg(edges):
+    e = _arr(edges)
+    res = Result()
+    lib().orc_sg(_p(e), e.size, ctypes.byref(res))
+    return _collect(res)
+
+
+def cspa(assign, deref):
+    a, d = _arr(assign), _arr(deref)
+    res = Result()
+    lib().orc_cspa(_p(a), a.size, _p(d), d.size, ctypes.byref(res))
+    return _collect(res)

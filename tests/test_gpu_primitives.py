@@ -1,0 +1,202 @@
+"""Kernel-level parity of the HIP engine through the C ABI (bit-exact).
+
+References are textbook numpy routines (np.sort / np.unique / set algebra /
+searchsorted), restated independently of both the engine and the oracle.
+"""
+import numpy as np
+import pytest
+
+from conftest import gdl
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(2024)
+M32 = np.uint64(0xffffffff)
+
+
+def rand_keys(n, hi_bits=32, lo_bits=32, rng=RNG):
+    hi = rng.integers(0, 1 << hi_bits, n, dtype=np.uint64) if hi_bits else np.zeros(n, np.uint64)
+    lo = rng.integers(0, 1 << lo_bits, n, dtype=np.uint64) if lo_bits else np.zeros(n, np.uint64)
+    return (hi << np.uint64(32)) | lo
+
+
+SIZES = [0, 1, 2, 63, 64, 65, 4095, 4096, 4097, 100_000, 1_000_003]
+
+
+@pytest.mark.parametrize("n", SIZES)
+@pytest.mark.parametrize("bits", [(32, 32), (18, 18), (15, 15), (1, 0), (0, 7)])
+def test_sort_matches_numpy(ctx, n, bits):
+    k = rand_keys(n, *bits)
+    got = ctx.sort(k, *bits)
+    assert np.array_equal(got, np.sort(k))
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_sort_unique_matches_numpy(ctx, n):
+    k = rand_keys(n, 12, 12)  # many duplicates
+    assert np.array_equal(ctx.sort_unique(k, 12, 12), np.unique(k))
+    assert np.array_equal(ctx.sort_unique(k, -1, -1), np.unique(k))  # measured bounds
+
+
+def test_sort_adversarial(ctx):
+    n = 300_000
+    cases = [np.full(n, 12345, np.uint64), np.arange(n, dtype=np.uint64),
+             np.arange(n, dtype=np.uint64)[::-1].copy(),
+             gdl.pack(np.full(n, 0xffffffff), np.arange(n)[::-1] % 7 + 0xfffffff0)]
+    for k in cases:
+        assert np.array_equal(ctx.sort(k, 32, 32), np.sort(k))
+        assert np.array_equal(ctx.sort_unique(k, 32, 32), np.unique(k))
+
+
+def test_key_bits(ctx):
+    k = gdl.pack([3, 200000, 7], [1, 2, 131071])
+    assert ctx.key_bits(k) == (18, 17)
+
+
+@pytest.mark.parametrize("n", [0, 1, 5000, 1_000_000])
+def test_unique_matches_numpy(ctx, n):
+    k = np.sort(rand_keys(n, 8, 8))
+    assert np.array_equal(ctx.unique(k), np.unique(k))
+
+
+@pytest.mark.parametrize("na,nb", [(0, 0), (0, 10), (10, 0), (1, 1), (3000, 2000),
+                                   (1_000_000, 300_000), (50, 2_000_000)])
+def test_merge_union_difference_match_set_algebra(ctx, na, nb):
+    a = np.unique(rand_keys(na, 11, 11))
+    b = np.unique(rand_keys(nb, 11, 11))
+    u = np.union1d(a, b)
+    d = np.setdiff1d(b, a)
+    assert np.array_equal(ctx.merge_union(a, b), u)
+    assert np.array_equal(ctx.set_difference(b, a), d)
+    gu, gd = ctx.merge_diff(a, b)
+    assert np.array_equal(gu, u) and np.array_equal(gd, d)
+
+
+def test_merge_diff_identical_and_disjoint(ctx):
+    a = np.unique(rand_keys(500_000, 20, 20))
+    gu, gd = ctx.merge_diff(a, a)
+    assert np.array_equal(gu, a) and gd.size == 0
+    b = a + np.uint64(1 << 63)
+    gu, gd = ctx.merge_diff(a, b)
+    assert np.array_equal(gu, np.concatenate([a, b])) and np.array_equal(gd, b)
+
+
+def np_runs(sorted_keys, prefixes, shift):
+    p = sorted_keys >> np.uint64(shift)
+    lo = np.searchsorted(p, prefixes, "left")
+    hi = np.searchsorted(p, prefixes, "right")
+    return lo, hi - lo
+
+
+@pytest.mark.parametrize("prefix_cols", [1, 2])
+@pytest.mark.parametrize("lf", [0.5, 0.8])
+def test_index_lookup_runs(ctx, prefix_cols, lf):
+    ctx.eng.gdl_set_load_factor(ctx.h, lf)
+    keys = np.unique(rand_keys(200_000, 14, 16))
+    shift = 32 if prefix_cols == 1 else 0
+    d = ctx.to_device(keys)
+    idx = ctx.index_build(d, prefix_cols)
+    present = np.unique(keys >> np.uint64(shift))
+    assert idx.n_distinct == present.size
+    absent = (present[:1000] + np.uint64(1 << 40)) if prefix_cols == 2 else \
+        np.arange(1 << 14, (1 << 14) + 1000, dtype=np.uint64)
+    q = np.concatenate([present, absent])
+    b, c = ctx.index_lookup(idx, d, q)
+    lo, cnt = np_runs(keys, q, shift)
+    assert np.array_equal(c.astype(np.int64), cnt)
+    hit = cnt > 0
+    assert np.array_equal(b[hit].astype(np.int64), lo[hit])
+    ctx.index_free(idx)
+    d.free()
+    ctx.eng.gdl_set_load_factor(ctx.h, 0.8)
+
+
+def test_index_saturated_run_count(ctx):
+    # one prefix with more than 2^24 - 1 tuples exercises the saturated count
+    n = (1 << 24) + 5
+    keys = np.concatenate([gdl.pack(np.full(n, 5), np.arange(n)),
+                           gdl.pack(np.full(3, 9), np.arange(3))])
+    d = ctx.to_device(keys)
+    idx = ctx.index_build(d, 1)
+    b, c = ctx.index_lookup(idx, d, np.array([5, 9, 7], dtype=np.uint64))
+    assert c.tolist() == [n, 3, 0] and b[:2].tolist() == [0, n]
+    ctx.index_free(idx)
+    d.free()
+
+
+def np_join(inner, outer, colmap, prefix_cols):
+    shift = np.uint64(32 if prefix_cols == 1 else 0)
+    lo, cnt = np_runs(inner, outer >> shift, int(shift))
+    oi = np.repeat(np.arange(outer.size), cnt)
+    start = np.repeat(lo, cnt)
+    excl = np.repeat(np.cumsum(cnt) - cnt, cnt)
+    ii = start + (np.arange(oi.size) - excl)
+    ik, ok = inner[ii], outer[oi]
+    cols = [ik >> np.uint64(32), ik & M32, ok >> np.uint64(32), ok & M32]
+    out = cols[colmap[0]] << np.uint64(32)
+    if len(colmap) == 2:
+        out = out | cols[colmap[1]]
+    return out
+
+
+@pytest.mark.parametrize("colmap", [(1, 3), (3, 1), (0, 2), (2,), (3, 3)])
+@pytest.mark.parametrize("n_outer", [0, 1, 7000, 400_000])
+def test_join_matches_numpy(ctx, colmap, n_outer):
+    inner = np.unique(rand_keys(50_000, 12, 14))
+    outer = rand_keys(n_outer, 12, 14)  # unsorted, with repeats
+    got = ctx.join(inner, outer, colmap, 1)
+    want = np_join(inner, outer, colmap, 1)
+    assert got.size == want.size
+    assert np.array_equal(np.sort(got), np.sort(want))
+
+
+def test_join_skewed_hub(ctx):
+    # one join key with 100k matches (the CSPA hub case)
+    inner = np.unique(np.concatenate([gdl.pack(np.full(100_000, 3), np.arange(100_000)),
+                                      rand_keys(20_000, 10, 10)]))
+    outer = gdl.pack(np.array([3, 3, 4, 3]), np.array([1, 2, 3, 4]))
+    got = ctx.join(inner, outer, (1, 3), 1)
+    want = np_join(inner, outer, (1, 3), 1)
+    assert np.array_equal(np.sort(got), np.sort(want))
+
+
+def test_join_two_column_prefix(ctx):
+    inner = np.unique(rand_keys(30_000, 8, 8))
+    outer = rand_keys(30_000, 8, 8)
+    got = ctx.join(inner, outer, (2, 1), 2)
+    want = np_join(inner, outer, (2, 1), 2)
+    assert np.array_equal(np.sort(got), np.sort(want))
+
+
+@pytest.mark.parametrize("op,fn", [(gdl.PRED_NE, np.not_equal), (gdl.PRED_EQ, np.equal),
+                                   (gdl.PRED_LT, np.less), (gdl.PRED_GE, np.greater_equal)])
+def test_join_with_predicate(ctx, op, fn):
+    inner = np.unique(rand_keys(20_000, 8, 6))
+    outer = rand_keys(20_000, 8, 6)
+    got = ctx.join(inner, outer, (1, 3), 1, (op, 0, 1))
+    want = np_join(inner, outer, (1, 3), 1)
+    want = want[fn(want >> np.uint64(32), want & M32)]
+    assert np.array_equal(np.sort(got), np.sort(want))
+
+
+def test_project_and_filter(ctx):
+    k = rand_keys(123_457, 20, 20)
+    assert np.array_equal(ctx.project(k, (1, 0)), ((k & M32) << np.uint64(32)) | (k >> np.uint64(32)))
+    assert np.array_equal(ctx.project(k, (0, 0)), ((k >> np.uint64(32)) << np.uint64(32)) | (k >> np.uint64(32)))
+    assert np.array_equal(ctx.project(k, (1,)), (k & M32) << np.uint64(32))
+    kk = rand_keys(200_000, 4, 4)
+    f = ctx.filter(kk, gdl.PRED_NE)
+    assert np.array_equal(f, kk[(kk >> np.uint64(32)) != (kk & M32)])
+
+
+def test_allocator_cap_reports_out_of_memory(ctx):
+    ctx.set_mem_cap(1 << 20)
+    try:
+        with pytest.raises(gdl.GdlError) as e:
+            ctx.sort(rand_keys(1_000_000))
+        assert e.value.status == 2
+    finally:
+        ctx.set_mem_cap(0)
+    # the context still works after the failure
+    k = rand_keys(10_000)
+    assert np.array_equal(ctx.sort(k), np.sort(k))
