@@ -61,7 +61,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nodes", type=int, default=200_000)
     ap.add_argument("--window", type=int, default=1000)
-    ap.add_argument("--cpu-iterations", type=int, default=60,
+    ap.add_argument("--cpu-iterations", type=int, default=100,
                     help="fixpoint iterations of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()

@@ -44,6 +44,21 @@ class Index(Structure):
                 ("prefix_cols", c_int)]
 
 
+class HashSet(Structure):
+    _fields_ = [("slots", c_void_p), ("capacity", c_uint64), ("size", c_uint64)]
+
+
+FULL_MODE_AUTO, FULL_MODE_MERGE = 0, 1
+
+
+def set_full_mode(mode):
+    """FULL maintenance of later program runs (see gdl_prog_set_full_mode)."""
+    _, prog = load()
+    st = prog.gdl_prog_set_full_mode(mode)
+    if st != GDL_OK:
+        raise GdlError(st, "gdl_prog_set_full_mode")
+
+
 class ColMap(Structure):
     _fields_ = [("arity", c_int32), ("src", c_int32 * 2)]
 
@@ -150,6 +165,11 @@ def load():
     _decl(prog, "gdl_prog_sg", c_int, [vp, vp, c_uint64, c_int, rp, bp])
     _decl(prog, "gdl_prog_cspa", c_int, [vp, vp, c_uint64, vp, c_uint64, c_int, rp, bp])
     _decl(prog, "gdl_prog_last_error", c_char_p, [])
+    _decl(prog, "gdl_prog_set_full_mode", c_int, [c_int])
+    _decl(eng, "gdl_hashset_create", c_int, [vp, c_uint64, POINTER(HashSet)])
+    _decl(eng, "gdl_hashset_free", c_int, [vp, POINTER(HashSet)])
+    _decl(eng, "gdl_hashset_insert_new", c_int,
+          [vp, POINTER(HashSet), vp, c_uint64, vp, _U64P])
     _engine, _programs = eng, prog
     return eng, prog
 
@@ -417,6 +437,26 @@ class Context:
         self.index_free(idx)
         di.free(); do.free()
         return res
+
+    def hashset_create(self, expected=0):
+        hs = HashSet()
+        self._check(self.eng.gdl_hashset_create(self.h, expected, ctypes.byref(hs)),
+                    "hashset_create")
+        return hs
+
+    def hashset_free(self, hs):
+        self._check(self.eng.gdl_hashset_free(self.h, ctypes.byref(hs)), "hashset_free")
+
+    def hashset_insert_new(self, hs, keys):
+        """Inserts keys; returns the keys that were not yet members (any order)."""
+        d = self.to_device(keys)
+        o = self.alloc(d.n)
+        m = c_uint64()
+        self._check(self.eng.gdl_hashset_insert_new(self.h, ctypes.byref(hs), d.ptr, d.n, o.ptr,
+                                                    ctypes.byref(m)), "hashset_insert_new")
+        r = o.numpy(m.value)
+        d.free(); o.free()
+        return r
 
     def project(self, keys, colmap):
         d = self.to_device(keys)

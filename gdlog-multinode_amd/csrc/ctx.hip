@@ -12,7 +12,8 @@ const char *kKernelNames[K_NUM] = {
     "rs_upsweep",   "rs_rowscan",   "rs_downsweep", "rs_oneblock",   "key_bits",
     "scan_reduce",  "scan_top",     "scan_down",    "compact_count", "compact_write",
     "merge_count",  "merge_write",  "idx_count",    "idx_build",     "idx_lookup",
-    "join_count",   "join_materialize", "project",  "fill"};
+    "join_count",   "join_materialize", "project",  "fill",          "hs_insert_new",
+    "hs_rehash"};
 
 // ---------------------------------------------------------------- allocator
 

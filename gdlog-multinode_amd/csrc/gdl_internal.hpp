@@ -69,6 +69,8 @@ enum KernelId {
     K_JOIN_MAT,
     K_PROJECT,
     K_FILL,
+    K_HS_INSERT,
+    K_HS_REHASH,
     K_NUM
 };
 extern const char *kKernelNames[K_NUM];
@@ -171,6 +173,10 @@ gdl_status sort_unique_keys(gdl_ctx *ctx, const u64 *d_in, u64 n, int hi_bits, i
                             u64 *d_out, u64 *n_out);
 gdl_status filter_keys(gdl_ctx *ctx, const u64 *d_in, u64 n, gdl_pred pred, u64 *d_out,
                        u64 *n_out);
+gdl_status hashset_insert_new(gdl_ctx *ctx, gdl_hashset *hs, const u64 *keys, u64 n, u64 *d_new,
+                              u64 *n_new);
+// Exclusive scans of two arrays in place in one launch (v[n] = total each).
+gdl_status scan_inplace2(gdl_ctx *ctx, u64 *v0, u64 *v1, u64 n, u64 *totals /* 2 */);
 
 inline unsigned grid_for(u64 n, u64 per_block) {
     u64 g = (n + per_block - 1) / per_block;
@@ -287,18 +293,31 @@ __device__ __forceinline__ u64 run_end(const u64 *keys, u64 n, u64 from, u64 p, 
     return a;
 }
 
+// The index is an array of buckets of kBucket slots (one 64-byte line);
+// a key lives in the first bucket of its probe sequence that had a free slot
+// when it was inserted, so a bucket with a free slot ends every probe.
+constexpr u64 kBucket = 4;
+
 __device__ __forceinline__ bool idx_probe(const IdxSlot *slots, u64 n_slots, u64 p, u64 *begin,
                                           u64 *count) {
-    u64 s = mulhi64(mix64(p), n_slots);
-    for (u64 step = 0; step < n_slots; ++step) {
-        IdxSlot e = slots[s];
-        if (e.val == 0) return false;
-        if (e.key == p) {
-            *begin = e.val >> kCountBits;
-            *count = e.val & kCountSat;
-            return true;
+    const u64 nbk = n_slots / kBucket;
+    u64 b = mulhi64(mix64(p), nbk);
+    for (u64 step = 0; step < nbk; ++step) {
+        const IdxSlot *bk = slots + b * kBucket;
+        bool any_empty = false;
+#pragma unroll
+        for (u64 j = 0; j < kBucket; ++j) {
+            const IdxSlot e = bk[j];
+            if (e.val == 0) {
+                any_empty = true;
+            } else if (e.key == p) {
+                *begin = e.val >> kCountBits;
+                *count = e.val & kCountSat;
+                return true;
+            }
         }
-        if (++s == n_slots) s = 0;
+        if (any_empty) return false;
+        if (++b == nbk) b = 0;
     }
     return false;
 }

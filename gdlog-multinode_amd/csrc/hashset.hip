@@ -1,0 +1,182 @@
+// hashset.hip -- device hash set of tuple keys: the membership side of an
+// append-only FULL version.
+//
+// Open addressing, linear probing, power-of-two capacity kept at load <= 1/2
+// (grown by rehashing before an insert would pass it).  EMPTY = ~0; the key
+// ~0 itself (ids 2^32 - 1 in both columns) is tracked by a flag word after
+// the slots.  insert_new inserts a batch of keys (any order, repeats allowed)
+// and emits each key that was not yet present exactly once: the thread whose
+// CAS claims the slot emits it, wave-aggregated through one atomic counter.
+#include "gdl_internal.hpp"
+
+namespace gdl {
+
+constexpr u64 kEmpty = ~0ull;
+constexpr int HB = 256;
+
+__device__ __forceinline__ bool hs_insert(u64 *slots, u64 mask, u64 k) {
+    u64 s = mix64(k) & mask;
+    for (u64 step = 0; step <= mask; ++step) {
+        u64 cur = slots[s];
+        if (cur == k) return false;
+        if (cur == kEmpty) {
+            const unsigned long long old =
+                atomicCAS((unsigned long long *)&slots[s], (unsigned long long)kEmpty,
+                          (unsigned long long)k);
+            if (old == (unsigned long long)kEmpty) return true;
+            if (old == (unsigned long long)k) return false;
+        }
+        s = (s + 1) & mask;
+    }
+    return false;  // unreachable at load <= 1/2
+}
+
+// flag word at slots[cap]: 1 when the key ~0 is a member
+__global__ __launch_bounds__(HB) void k_hs_insert_new(u64 *slots, u64 mask, const u64 *keys, u64 n,
+                                                      u64 *out, unsigned long long *counter) {
+    const u64 lt = lane_mask_lt();
+    const u64 stride = (u64)gridDim.x * HB;
+    // every lane of a wave runs the same number of rounds (ballots below)
+    for (u64 base = (u64)blockIdx.x * HB; base < n; base += stride) {
+        const u64 i = base + threadIdx.x;
+        bool fresh = false;
+        u64 k = 0;
+        if (i < n) {
+            k = keys[i];
+            if (k == kEmpty) {
+                fresh = atomicExch((unsigned int *)&slots[mask + 1], 1u) == 0u;
+            } else {
+                fresh = hs_insert(slots, mask, k);
+            }
+        }
+        const u64 bal = __ballot(fresh);
+        if (bal) {
+            unsigned long long pos = 0;
+            const int leader = __ffsll((unsigned long long)bal) - 1;
+            if ((int)(threadIdx.x & 63) == leader)
+                pos = atomicAdd(counter, (unsigned long long)__popcll(bal));
+            pos = __shfl(pos, leader, 64);
+            if (fresh) out[pos + __popcll(bal & lt)] = k;
+        }
+    }
+}
+
+// Re-inserts every member of an old table into a new one.
+__global__ __launch_bounds__(HB) void k_hs_rehash(const u64 *old_slots, u64 old_cap, u64 *slots,
+                                                  u64 mask) {
+    for (u64 i = (u64)blockIdx.x * HB + threadIdx.x; i < old_cap; i += (u64)gridDim.x * HB) {
+        const u64 k = old_slots[i];
+        if (k != kEmpty) (void)hs_insert(slots, mask, k);
+    }
+}
+
+__global__ __launch_bounds__(HB) void k_hs_fill(u64 *slots, u64 cap) {
+    for (u64 i = (u64)blockIdx.x * HB + threadIdx.x; i < cap; i += (u64)gridDim.x * HB)
+        slots[i] = i < cap - 1 ? kEmpty : 0ull;  // last word: the ~0 flag (cleared)
+}
+
+static unsigned hs_grid(gdl_ctx *ctx, u64 n) {
+    const u64 cap = (u64)(ctx->cu_count > 0 ? ctx->cu_count : 256) * 16;
+    u64 g = (n + HB - 1) / HB;
+    return (unsigned)std::max<u64>(1, std::min(g, cap));
+}
+
+static gdl_status hs_alloc(gdl_ctx *ctx, u64 cap, u64 **slots) {
+    void *p = nullptr;
+    std::string err;
+    gdl_status st = ctx->alloc->alloc((cap + 1) * sizeof(u64), &p, &err);
+    if (st != GDL_OK) return fail(ctx, st, err);
+    KTimer kt(ctx, K_FILL, 8.0 * (cap + 1));
+    hipLaunchKernelGGL(k_hs_fill, dim3(hs_grid(ctx, cap + 1)), dim3(HB), 0, ctx->stream, (u64 *)p,
+                       cap + 1);
+    gdl_status ls = check_launch(ctx, "k_hs_fill");
+    if (ls != GDL_OK) {
+        ctx->alloc->free(p, nullptr);
+        return ls;
+    }
+    *slots = (u64 *)p;
+    return GDL_OK;
+}
+
+static u64 pow2_at_least(u64 x) {
+    u64 c = 1024;
+    while (c < x) c <<= 1;
+    return c;
+}
+
+// Grows the table so that size + incoming stays at load <= 1/2.
+static gdl_status hs_reserve(gdl_ctx *ctx, gdl_hashset *hs, u64 incoming) {
+    const u64 need = pow2_at_least(2 * (hs->size + incoming));
+    if (hs->slots && hs->capacity >= need) return GDL_OK;
+    u64 *ns = nullptr;
+    GDL_TRY(hs_alloc(ctx, need, &ns));
+    if (hs->slots) {
+        KTimer kt(ctx, K_HS_REHASH, 16.0 * hs->capacity);
+        hipLaunchKernelGGL(k_hs_rehash, dim3(hs_grid(ctx, hs->capacity)), dim3(HB), 0, ctx->stream,
+                           (const u64 *)hs->slots, hs->capacity, ns, need - 1);
+        GDL_TRY(check_launch(ctx, "k_hs_rehash"));
+        // carry the ~0 flag word
+        GDL_HIP(ctx, hipMemcpyAsync(ns + need, (u64 *)hs->slots + hs->capacity, sizeof(u64),
+                                    hipMemcpyDeviceToDevice, ctx->stream));
+        ctx->alloc->free(hs->slots, nullptr);
+    }
+    hs->slots = ns;
+    hs->capacity = need;
+    return GDL_OK;
+}
+
+gdl_status hashset_insert_new(gdl_ctx *ctx, gdl_hashset *hs, const u64 *keys, u64 n, u64 *d_new,
+                              u64 *n_new) {
+    *n_new = 0;
+    if (n == 0) return GDL_OK;
+    GDL_TRY(hs_reserve(ctx, hs, n));
+    Scratch scr(ctx);
+    u64 *counter;
+    GDL_TRY(scr.get(1, &counter));
+    GDL_HIP(ctx, hipMemsetAsync(counter, 0, sizeof(u64), ctx->stream));
+    {
+        KTimer kt(ctx, K_HS_INSERT, 24.0 * n);
+        hipLaunchKernelGGL(k_hs_insert_new, dim3(hs_grid(ctx, n)), dim3(HB), 0, ctx->stream,
+                           (u64 *)hs->slots, hs->capacity - 1, keys, n, d_new,
+                           (unsigned long long *)counter);
+        GDL_TRY(check_launch(ctx, "k_hs_insert_new"));
+    }
+    GDL_TRY(read_u64(ctx, counter, n_new));
+    hs->size += *n_new;
+    return GDL_OK;
+}
+
+}  // namespace gdl
+
+using namespace gdl;
+
+extern "C" {
+
+gdl_status gdl_hashset_create(gdl_ctx *ctx, uint64_t expected, gdl_hashset *out) {
+    if (!ctx || !out) return GDL_ERR_INVALID_ARG;
+    out->slots = nullptr;
+    out->capacity = 0;
+    out->size = 0;
+    return hs_reserve(ctx, out, expected);
+}
+
+gdl_status gdl_hashset_free(gdl_ctx *ctx, gdl_hashset *hs) {
+    if (!ctx || !hs) return GDL_ERR_INVALID_ARG;
+    if (hs->slots) {
+        std::string err;
+        gdl_status st = ctx->alloc->free(hs->slots, &err);
+        if (st != GDL_OK) return fail(ctx, st, err);
+    }
+    hs->slots = nullptr;
+    hs->capacity = 0;
+    hs->size = 0;
+    return GDL_OK;
+}
+
+gdl_status gdl_hashset_insert_new(gdl_ctx *ctx, gdl_hashset *hs, const uint64_t *d_keys,
+                                  uint64_t n, uint64_t *d_new, uint64_t *n_new) {
+    if (!ctx || !hs || !n_new || (n && (!d_keys || !d_new))) return GDL_ERR_INVALID_ARG;
+    return hashset_insert_new(ctx, hs, d_keys, n, d_new, n_new);
+}
+
+}  // extern "C"

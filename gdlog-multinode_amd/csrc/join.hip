@@ -39,17 +39,22 @@ __global__ __launch_bounds__(JB) void k_idx_build(const u64 *keys, u64 n, int sh
         const u64 end = run_end(keys, n, i, p, shift);
         const u64 cnt = end - i;
         const u64 val = (i << kCountBits) | (cnt < kCountSat ? cnt : kCountSat);
-        u64 s = mulhi64(mix64(p), n_slots);
+        const u64 nbk = n_slots / kBucket;
+        u64 b = mulhi64(mix64(p), nbk);
         bool placed = false;
-        for (u64 step = 0; step < n_slots; ++step) {
-            const unsigned long long old =
-                atomicCAS((unsigned long long *)&slots[s].val, 0ull, (unsigned long long)val);
-            if (old == 0ull) {
-                slots[s].key = p;
-                placed = true;
-                break;
+        for (u64 step = 0; step < nbk && !placed; ++step) {
+            for (u64 j = 0; j < kBucket; ++j) {
+                IdxSlot *e = slots + b * kBucket + j;
+                if (e->val != 0) continue;
+                const unsigned long long old =
+                    atomicCAS((unsigned long long *)&e->val, 0ull, (unsigned long long)val);
+                if (old == 0ull) {
+                    e->key = p;
+                    placed = true;
+                    break;
+                }
             }
-            if (++s == n_slots) s = 0;
+            if (++b == nbk) b = 0;
         }
         if (!placed) atomicOr(err, 1u);
     }
@@ -174,6 +179,7 @@ gdl_status index_build(gdl_ctx *ctx, const u64 *keys, u64 n, int prefix_cols, gd
     }
     u64 n_slots = (u64)((double)distinct / (double)ctx->cfg.load_factor) + 1;
     if (n_slots <= distinct) n_slots = distinct + 1;
+    n_slots = (n_slots + kBucket - 1) / kBucket * kBucket;  // whole buckets
     void *p = nullptr;
     std::string err;
     gdl_status st = ctx->alloc->alloc(n_slots * sizeof(IdxSlot), &p, &err);

@@ -188,28 +188,28 @@ static gdl_status merge_impl(gdl_ctx *ctx, const u64 *A, u64 na, const u64 *B, u
     const u64 nt = (tot + MG_TILE - 1) / MG_TILE;
     if (nt > 0xffffffffull) return fail(ctx, GDL_ERR_OVERFLOW, "merge: too many tiles");
     Scratch scr(ctx);
-    u64 *ou, *od;
+    u64 *ou, *od, *totals;
     GDL_TRY(scr.get(nt + 1, &ou));
     GDL_TRY(scr.get(nt + 1, &od));
+    GDL_TRY(scr.get(2, &totals));
     {
         KTimer kt(ctx, K_MERGE_COUNT, 8.0 * tot);
         hipLaunchKernelGGL(k_merge_count, dim3((unsigned)nt), dim3(MG_B), 0, ctx->stream, A, na, B,
                            nb, ou, od, ctx->d_err);
         GDL_TRY(check_launch(ctx, "k_merge_count"));
     }
-    GDL_TRY(scan_inplace(ctx, ou, nt));
-    GDL_TRY(scan_inplace(ctx, od, nt));
+    GDL_TRY(scan_inplace2(ctx, ou, od, nt, totals));
     {
-        // algorithmic bytes: read A and B, write U and D (sizes known after the fact;
-        // the union is at most na + nb, the difference at most nb)
-        KTimer kt(ctx, K_MERGE_WRITE, 8.0 * tot + (U ? 8.0 * tot : 0.0) + (D ? 8.0 * nb : 0.0));
+        // algorithmic bytes: read A and B here; the U / D writes are added
+        // below once their sizes are known
+        KTimer kt(ctx, K_MERGE_WRITE, 8.0 * tot);
         hipLaunchKernelGGL(k_merge_write, dim3((unsigned)nt), dim3(MG_B), 0, ctx->stream, A, na, B,
                            nb, ou, od, U, D, ctx->d_err);
         GDL_TRY(check_launch(ctx, "k_merge_write"));
     }
     u64 h[2];
-    GDL_TRY(read_u64(ctx, ou + nt, &h[0]));
-    GDL_TRY(read_u64(ctx, od + nt, &h[1]));  // second read: no extra wait
+    GDL_TRY(read_u64(ctx, totals, h, 2));
+    ctx->stats[K_MERGE_WRITE].alg_bytes += (U ? 8.0 * h[0] : 0.0) + (D ? 8.0 * h[1] : 0.0);
     if (n_u) *n_u = h[0];
     if (n_d) *n_d = h[1];
     return GDL_OK;

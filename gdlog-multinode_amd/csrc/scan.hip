@@ -34,23 +34,36 @@ __global__ __launch_bounds__(TB) void k_scan_reduce(const T *in, u64 n, u64 *blo
     if (t == 0) block_sums[blockIdx.x] = tot;
 }
 
-// Single-block exclusive scan of v[0..n) in place; v[n] = total; *total_out too.
+// Exclusive scan in place by one block per array (block 0: v0, block 1: v1);
+// v[n] = total, and block 0 also stores it to *total_out.  Each thread scans
+// 4 consecutive elements per round of 4 * BLOCK.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_scan_top(u64 *v, u64 n, u64 *total_out) {
+__global__ __launch_bounds__(BLOCK) void k_scan_top(u64 *v0, u64 *v1, u64 n, u64 *total_out) {
     __shared__ u64 s_w[BLOCK / 64 + 1];
+    u64 *v = blockIdx.x ? v1 : v0;
     const int t = threadIdx.x;
     u64 carry = 0;
-    for (u64 b = 0; b < n; b += BLOCK) {
-        u64 i = b + t;
-        u64 x = i < n ? v[i] : 0;
+    for (u64 b = 0; b < n; b += 4 * BLOCK) {
+        const u64 i0 = b + 4 * (u64)t;
+        u64 x[4];
+        u64 sum = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            x[j] = i0 + j < n ? v[i0 + j] : 0;
+            sum += x[j];
+        }
         u64 tot;
-        u64 ex = block_exclusive_scan<BLOCK>(x, s_w, &tot);
-        if (i < n) v[i] = carry + ex;
+        u64 ex = carry + block_exclusive_scan<BLOCK>(sum, s_w, &tot);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (i0 + j < n) v[i0 + j] = ex;
+            ex += x[j];
+        }
         carry += tot;
     }
     if (t == 0) {
         v[n] = carry;
-        if (total_out) *total_out = carry;
+        if (total_out) total_out[blockIdx.x] = carry;
     }
 }
 
@@ -149,7 +162,23 @@ __global__ __launch_bounds__(TB) void k_compact_write(const u64 *in, u64 n, F fl
 
 static gdl_status scan_top(gdl_ctx *ctx, u64 *v, u64 n, u64 *total_dev) {
     KTimer kt(ctx, K_SCAN_TOP, 16.0 * n);
-    hipLaunchKernelGGL(k_scan_top<1024>, dim3(1), dim3(1024), 0, ctx->stream, v, n, total_dev);
+    hipLaunchKernelGGL(k_scan_top<1024>, dim3(1), dim3(1024), 0, ctx->stream, v, v, n,
+                       total_dev);
+    return check_launch(ctx, "k_scan_top");
+}
+
+gdl_status scan_inplace2(gdl_ctx *ctx, u64 *v0, u64 *v1, u64 n, u64 *totals) {
+    if (n > 262144) {
+        GDL_TRY(scan_inplace(ctx, v0, n));
+        GDL_TRY(scan_inplace(ctx, v1, n));
+        GDL_HIP(ctx, hipMemcpyAsync(totals, v0 + n, sizeof(u64), hipMemcpyDeviceToDevice,
+                                    ctx->stream));
+        GDL_HIP(ctx, hipMemcpyAsync(totals + 1, v1 + n, sizeof(u64), hipMemcpyDeviceToDevice,
+                                    ctx->stream));
+        return GDL_OK;
+    }
+    KTimer kt(ctx, K_SCAN_TOP, 32.0 * n);
+    hipLaunchKernelGGL(k_scan_top<1024>, dim3(2), dim3(1024), 0, ctx->stream, v0, v1, n, totals);
     return check_launch(ctx, "k_scan_top");
 }
 
@@ -195,7 +224,7 @@ gdl_status exclusive_scan_u64(gdl_ctx *ctx, const u64 *d_in, u64 n, u64 *d_out) 
 }
 
 gdl_status scan_inplace(gdl_ctx *ctx, u64 *v, u64 n) {
-    if (n <= 65536) return scan_top(ctx, v, n, nullptr);
+    if (n <= 262144) return scan_top(ctx, v, n, nullptr);
     Scratch scr(ctx);
     u64 *tmp;
     GDL_TRY(scr.get(n + 1, &tmp));

@@ -36,6 +36,46 @@ void seed_delta(Relation *rel) {
     container_take(rel->delta, k, n, n, rel->raw_sorted);
 }
 
+// True when some join of the LIE reads rel's FULL as its inner side (NEWT as
+// inner reads FULL too, join.cu:15-20).
+bool full_is_inner(const std::vector<ra_op> &ops, const Relation *rel) {
+    for (const auto &op : ops)
+        if (const auto *j = std::get_if<RelationalJoin>(&op))
+            if (j->inner_rel == rel && j->inner_ver != DELTA) return true;
+    return false;
+}
+
+void enter_append_mode(Relation *rel) {
+    gdl_ctx *ctx = context();
+    const tuple_size_t n = rel->current_full_size;
+    check(gdl_hashset_create(ctx, n + 1024, &rel->full_set), "gdl_hashset_create");
+    if (n) {
+        uint64_t *scratch = device_alloc_keys(n);
+        uint64_t added = 0;
+        gdl_status st = gdl_hashset_insert_new(ctx, &rel->full_set, rel->full->keys, n, scratch,
+                                               &added);
+        device_free(scratch);
+        check(st, "gdl_hashset_insert_new");
+    }
+    rel->append_mode = true;
+}
+
+// FULL back to one sorted array; drops the hash set.
+void leave_append_mode(Relation *rel) {
+    gdl_ctx *ctx = context();
+    const tuple_size_t n = rel->current_full_size;
+    if (n && !rel->full->sorted) {
+        uint64_t *sorted = device_alloc_keys(n);
+        const int b = id_bits();
+        gdl_status st = gdl_sort_u64(ctx, rel->full->keys, n, b, b, sorted);
+        if (st != GDL_OK) device_free(sorted);
+        check(st, "gdl_sort_u64");
+        container_take(rel->full, sorted, n, n, true);
+    }
+    check(gdl_hashset_free(ctx, &rel->full_set), "gdl_hashset_free");
+    rel->append_mode = false;
+}
+
 }  // namespace
 
 void LIE::fixpoint_loop() {
@@ -45,6 +85,7 @@ void LIE::fixpoint_loop() {
     for (Relation *rel : update_relations) {
         rel->current_full_size = rel->full->tuple_counts;
         seed_delta(rel);
+        if (full_mode() == FullMode::Auto && !full_is_inner(ra_ops, rel)) enter_append_mode(rel);
     }
     int iteration_counter = 0;
     while (true) {
@@ -81,6 +122,33 @@ void LIE::fixpoint_loop() {
                 if (verbose())
                     std::cout << "iteration " << iteration_counter << " relation " << rel->name
                               << " no new tuple added" << std::endl;
+                continue;
+            }
+            if (rel->append_mode) {
+                // delta = newt \ full through the hash set; full grows by delta
+                const tuple_size_t nn = newt->tuple_counts;
+                uint64_t *delta = device_alloc_keys(nn);
+                uint64_t nd = 0;
+                gdl_status st =
+                    gdl_hashset_insert_new(ctx, &rel->full_set, newt->keys, nn, delta, &nd);
+                if (st != GDL_OK) device_free(delta);
+                check(st, "gdl_hashset_insert_new");
+                if (nd != 0) fixpoint_flag = false;
+                container_free(newt);
+                if (nd > 0) {
+                    // flush_delta restores full's count before growing it
+                    // (relation.cu:514-515); a join may have zeroed it
+                    rel->full->tuple_counts = rel->current_full_size;
+                    container_append(rel->full, delta, nd);
+                    rel->current_full_size += nd;
+                }
+                container_take(rel->delta, delta, nd, nn, false);
+                trace.push_back({iteration_counter, rel_idx, nn, nd, rel->current_full_size});
+                if (verbose())
+                    std::cout << "iteration " << iteration_counter << " relation " << rel->name
+                              << " finish dedup new tuples : " << nd
+                              << " delta tuple size: " << nd << " full counts "
+                              << rel->current_full_size << std::endl;
                 continue;
             }
             container_normalize(newt);
@@ -120,6 +188,7 @@ void LIE::fixpoint_loop() {
     iteration_count = iteration_counter;
 
     for (Relation *rel : update_relations) {
+        if (rel->append_mode) leave_append_mode(rel);
         if (reload_full_flag) {
             // lie.cu:219-254: full reloaded from the merged tuples; its rows
             // (data_raw) become the sorted full.

@@ -65,28 +65,15 @@ void accumulate_sorted(GHashRelContainer *newt, uint64_t *fresh, tuple_size_t n,
 }
 
 // tmp newt: raw concatenation, no dedup (join.cu:141-151,226-254).
+// Also the newt of an append-mode relation, deduplicated only when it meets
+// FULL's hash set at the end of the iteration.
 void append_raw(GHashRelContainer *newt, uint64_t *fresh, tuple_size_t n) {
     if (!newt->keys || newt->tuple_counts == 0) {
         container_take(newt, fresh, n, n, false);
         return;
     }
-    const tuple_size_t have = newt->tuple_counts;
-    if (newt->capacity >= have + n) {
-        check(gdl_memcpy_d2d(context(), newt->keys + have, fresh, n * sizeof(uint64_t)),
-              "tmp append");
-        device_free(fresh);
-        container_drop_index(newt);
-        newt->tuple_counts = have + n;
-        newt->data_raw_row_size = have + n;
-        newt->sorted = false;
-        return;
-    }
-    const tuple_size_t cap = (have + n) + (have + n) / 2;
-    uint64_t *buf = device_alloc_keys(cap);
-    check(gdl_memcpy_d2d(context(), buf, newt->keys, have * sizeof(uint64_t)), "tmp append");
-    check(gdl_memcpy_d2d(context(), buf + have, fresh, n * sizeof(uint64_t)), "tmp append");
+    container_append(newt, fresh, n);
     device_free(fresh);
-    container_take(newt, buf, have + n, cap, false);
 }
 
 }  // namespace
@@ -132,7 +119,7 @@ void RelationalJoin::operator()() {
         device_free(res);
         return;
     }
-    if (output_is_tmp) append_raw(output_rel->newt, res, m);
+    if (output_is_tmp || output_rel->append_mode) append_raw(output_rel->newt, res, m);
     else accumulate_sorted(output_rel->newt, res, m, detail_time);
 }
 
@@ -149,7 +136,8 @@ void RelationalCopy::operator()() {
     const tuple_size_t n = src->tuple_counts;
     uint64_t *copied = device_alloc_keys(n);
     check(gdl_project_u64(context(), src->keys, n, copy_map.map, copied), "gdl_project_u64");
-    accumulate_sorted(dest_rel->newt, copied, n, nullptr);
+    if (dest_rel->append_mode) append_raw(dest_rel->newt, copied, n);
+    else accumulate_sorted(dest_rel->newt, copied, n, nullptr);
 }
 
 void RelationalACopy::operator()() {
@@ -166,6 +154,10 @@ void RelationalACopy::operator()() {
     const tuple_size_t n = src->tuple_counts;
     uint64_t *copied = device_alloc_keys(n);
     check(gdl_project_u64(context(), src->keys, n, copy_map.map, copied), "gdl_project_u64");
+    if (dest_rel->append_mode) {
+        container_take(dest, copied, n, n, false);
+        return;
+    }
     uint64_t *su = nullptr;
     tuple_size_t m = sort_unique(copied, n, &su);
     device_free(copied);

@@ -60,6 +60,16 @@ bool verbose() {
 void set_phase_timing(bool on) { g_phase_timing = on; }
 bool phase_timing() { return g_phase_timing; }
 
+namespace {
+FullMode g_full_mode = [] {
+    const char *e = std::getenv("GDLOG_FULL_MODE");
+    return (e && std::string(e) == "merge") ? FullMode::Merge : FullMode::Auto;
+}();
+}  // namespace
+
+void set_full_mode(FullMode m) { g_full_mode = m; }
+FullMode full_mode() { return g_full_mode; }
+
 uint64_t *device_alloc_keys(tuple_size_t n) {
     void *p = nullptr;
     check(gdl_alloc(context(), (n ? n : 1) * sizeof(uint64_t), &p), "gdl_alloc");
@@ -95,6 +105,25 @@ void container_take(GHashRelContainer *c, uint64_t *keys, tuple_size_t n, tuple_
     c->tuple_counts = n;
     c->data_raw_row_size = n;
     c->sorted = sorted;
+}
+
+void container_append(GHashRelContainer *c, const uint64_t *keys, tuple_size_t n) {
+    if (n == 0) return;
+    const tuple_size_t have = c->keys ? c->tuple_counts : 0;
+    container_drop_index(c);
+    if (!c->keys || c->capacity < have + n) {
+        const tuple_size_t cap = (have + n) + (have + n) / 2 + 1024;
+        uint64_t *buf = device_alloc_keys(cap);
+        if (have)
+            check(gdl_memcpy_d2d(context(), buf, c->keys, have * sizeof(uint64_t)), "append");
+        device_free(c->keys);
+        c->keys = buf;
+        c->capacity = cap;
+    }
+    check(gdl_memcpy_d2d(context(), c->keys + have, keys, n * sizeof(uint64_t)), "append");
+    c->tuple_counts = have + n;
+    c->data_raw_row_size = have + n;
+    c->sorted = false;
 }
 
 tuple_size_t sort_unique(const uint64_t *d_in, tuple_size_t n, uint64_t **d_out) {

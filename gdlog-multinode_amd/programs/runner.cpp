@@ -178,4 +178,10 @@ gdl_status gdl_prog_cspa(gdl_ctx *ctx, const uint64_t *assign, uint64_t n_assign
 
 const char *gdl_prog_last_error(void) { return g_last_error.c_str(); }
 
+gdl_status gdl_prog_set_full_mode(int mode) {
+    if (mode != 0 && mode != 1) return GDL_ERR_INVALID_ARG;
+    set_full_mode(mode ? FullMode::Merge : FullMode::Auto);
+    return GDL_OK;
+}
+
 }  // extern "C"

@@ -139,6 +139,23 @@ gdl_status gdl_index_lookup(gdl_ctx *ctx, const gdl_index *idx, const uint64_t *
                             uint64_t n_sorted, const uint64_t *d_prefixes, uint64_t n,
                             uint64_t *d_begin, uint32_t *d_count);
 
+/* ---- device hash set of keys (membership of an append-only FULL) --------
+ * Replaces the set-difference role of thrust::set_difference(newt, full)
+ * (lie.cu:131-138) for relations whose FULL is never a join inner, where
+ * keeping FULL sorted (Relation::flush_delta, relation.cu:484-487) is not
+ * needed until the fixpoint ends.  Load factor <= 1/2, grown by rehashing. */
+typedef struct gdl_hashset {
+    void *slots;       /* device; owned by the context allocator */
+    uint64_t capacity; /* slots, a power of two */
+    uint64_t size;     /* members */
+} gdl_hashset;
+gdl_status gdl_hashset_create(gdl_ctx *ctx, uint64_t expected, gdl_hashset *out);
+gdl_status gdl_hashset_free(gdl_ctx *ctx, gdl_hashset *hs);
+/* Inserts n keys (any order, repeats allowed); every key not yet a member is
+ * written once to d_new (n slots), in no particular order; *n_new = count. */
+gdl_status gdl_hashset_insert_new(gdl_ctx *ctx, gdl_hashset *hs, const uint64_t *d_keys,
+                                  uint64_t n, uint64_t *d_new, uint64_t *n_new);
+
 /* ---- column maps and predicates (the build's form of the reference hooks,
  *      tuple_generator_hook / tuple_copy_hook / tuple_predicate,
  *      include/tuple.cuh:23-25) ------------------------------------------ */

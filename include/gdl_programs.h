@@ -65,6 +65,12 @@ gdl_status gdl_prog_cspa(gdl_ctx *ctx, const uint64_t *assign, uint64_t n_assign
 /* Message of the last failed gdl_prog_* call on this thread. */
 const char *gdl_prog_last_error(void);
 
+/* FULL maintenance of the fixpoint for later runs: 0 = auto (append log +
+ * hash set for relations whose FULL is never a join inner), 1 = merge (sorted
+ * merge every iteration, the reference's structure).  Results are identical;
+ * the trace's newt field counts raw rows in append mode. */
+gdl_status gdl_prog_set_full_mode(int mode);
+
 #ifdef __cplusplus
 }
 #endif

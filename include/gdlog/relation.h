@@ -67,6 +67,12 @@ struct Relation {
 
     MonotonicOrder monotonic_order = MonotonicOrder::DESC;
 
+    // Set by LIE::fixpoint_loop for an update relation whose FULL is never a
+    // join inner in that LIE: FULL is then an append log (sorted at the end of
+    // the fixpoint) plus a device hash set, and newt stays unsorted.
+    bool append_mode = false;
+    gdl_hashset full_set{};
+
     // Merges DELTA into FULL (relation.cu:405-521).
     void flush_delta(int grid_size, int block_size, float *detail_time);
 };
@@ -142,5 +148,16 @@ bool verbose();
 // synchronises the stream around every phase).
 void set_phase_timing(bool on);
 bool phase_timing();
+
+// How the fixpoint keeps FULL of update relations.  Auto: append log + hash
+// set for relations whose FULL is not a join inner in the LIE, sorted merge
+// for the others.  Merge: sorted merge for every relation (the reference's
+// structure: set_difference then flush_delta's merge every iteration).
+enum class FullMode { Auto = 0, Merge = 1 };
+void set_full_mode(FullMode m);
+FullMode full_mode();
+
+// Appends n device keys to a version's array (growing it), unsorted.
+void container_append(GHashRelContainer *c, const uint64_t *keys, tuple_size_t n);
 
 }  // namespace gdlog

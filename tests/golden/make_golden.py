@@ -40,7 +40,7 @@ def workloads():
         # R-MAT scale 15, 40,000 distinct edges, 0.57/0.19/0.19/0.05, seed 1
         "sg_rmat15_40k": ("sg", lambda: gdl.gen_rmat(1, 15, 40000)),
         # CSPA at a reduced scale the full fixpoint fits (see the CSPA notes)
-        "cspa_zipf_2k": ("cspa", lambda: gdl.gen_cspa(5, 2000, 1.1, 3000, 1500)),
+        "cspa_zipf_500": ("cspa", lambda: gdl.gen_cspa(5, 500, 1.1, 750, 375)),
     }
 
 

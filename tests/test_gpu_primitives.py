@@ -189,6 +189,24 @@ def test_project_and_filter(ctx):
     assert np.array_equal(f, kk[(kk >> np.uint64(32)) != (kk & M32)])
 
 
+def test_hashset_insert_new_is_set_difference(ctx):
+    hs = ctx.hashset_create(0)
+    seen = set()
+    try:
+        for batch in range(6):  # grows and rehashes across batches
+            k = rand_keys(150_000 * (batch + 1), 10, 10)  # heavy repeats
+            if batch == 2:
+                k = np.concatenate([k, np.array([0xffffffffffffffff] * 3, dtype=np.uint64)])
+            new = ctx.hashset_insert_new(hs, k)
+            want = set(k.tolist()) - seen
+            assert new.size == len(want)
+            assert set(new.tolist()) == want
+            seen |= want
+        assert hs.size == len(seen)
+    finally:
+        ctx.hashset_free(hs)
+
+
 def test_allocator_cap_reports_out_of_memory(ctx):
     ctx.set_mem_cap(1 << 20)
     try:
