@@ -41,7 +41,7 @@ class Config(Structure):
 
 class Index(Structure):
     _fields_ = [("slots", c_void_p), ("n_slots", c_uint64), ("n_distinct", c_uint64),
-                ("prefix_cols", c_int)]
+                ("prefix_cols", c_int), ("max_run", c_uint64)]
 
 
 class HashSet(Structure):
@@ -170,6 +170,9 @@ def load():
     _decl(eng, "gdl_hashset_free", c_int, [vp, POINTER(HashSet)])
     _decl(eng, "gdl_hashset_insert_new", c_int,
           [vp, POINTER(HashSet), vp, c_uint64, vp, _U64P])
+    _decl(eng, "gdl_join_insert", c_int,
+          [vp, vp, c_uint64, POINTER(Index), vp, c_uint64, ColMap, Pred, POINTER(HashSet),
+           POINTER(vp), _U64P, _U64P, _U64P])
     _engine, _programs = eng, prog
     return eng, prog
 
@@ -275,7 +278,7 @@ class DeviceArray:
 
 
 class Context:
-    def __init__(self, device=0, mem_cap_bytes=0, load_factor=0.8, collect_stats=False,
+    def __init__(self, device=0, mem_cap_bytes=0, load_factor=0.5, collect_stats=False,
                  verbose=False):
         self.eng, self.prog = load()
         cfg = Config()
@@ -457,6 +460,24 @@ class Context:
         r = o.numpy(m.value)
         d.free(); o.free()
         return r
+
+    def join_insert(self, hs, inner_sorted, outer, colmap, prefix_cols=1, pred=(PRED_NONE, 0, 1)):
+        """Fused join + insert-if-absent; returns (new keys in any order, produced count)."""
+        di, do = self.to_device(inner_sorted), self.to_device(outer)
+        idx = self.index_build(di, prefix_cols)
+        cm = ColMap(len(colmap), (c_int32 * 2)(*(list(colmap) + [0])[:2]))
+        buf, cap, nn, raw = c_void_p(), c_uint64(0), c_uint64(0), c_uint64(0)
+        self._check(self.eng.gdl_join_insert(self.h, di.ptr, di.n, ctypes.byref(idx), do.ptr,
+                                             do.n, cm, Pred(*pred), ctypes.byref(hs),
+                                             ctypes.byref(buf), ctypes.byref(cap),
+                                             ctypes.byref(nn), ctypes.byref(raw)), "join_insert")
+        new = DeviceArray(self, buf.value, nn.value).numpy() if nn.value else \
+            np.zeros(0, dtype=np.uint64)
+        if buf.value:
+            self.eng.gdl_free(self.h, buf.value)
+        self.index_free(idx)
+        di.free(); do.free()
+        return new, raw.value
 
     def project(self, keys, colmap):
         d = self.to_device(keys)

@@ -13,7 +13,7 @@ const char *kKernelNames[K_NUM] = {
     "scan_reduce",  "scan_top",     "scan_down",    "compact_count", "compact_write",
     "merge_count",  "merge_write",  "idx_count",    "idx_build",     "idx_lookup",
     "join_count",   "join_materialize", "project",  "fill",          "hs_insert_new",
-    "hs_rehash"};
+    "hs_rehash",    "join_insert"};
 
 // ---------------------------------------------------------------- allocator
 
@@ -196,7 +196,7 @@ void gdl_config_default(gdl_config *cfg) {
     memset(cfg, 0, sizeof(*cfg));
     cfg->device = 0;
     cfg->mem_cap_bytes = 0;
-    cfg->load_factor = 0.8f;
+    cfg->load_factor = 0.5f;
     cfg->collect_stats = 0;
     cfg->verbose = 0;
 }

@@ -71,6 +71,7 @@ enum KernelId {
     K_FILL,
     K_HS_INSERT,
     K_HS_REHASH,
+    K_JOIN_INSERT,
     K_NUM
 };
 extern const char *kKernelNames[K_NUM];
@@ -175,6 +176,8 @@ gdl_status filter_keys(gdl_ctx *ctx, const u64 *d_in, u64 n, gdl_pred pred, u64 
                        u64 *n_out);
 gdl_status hashset_insert_new(gdl_ctx *ctx, gdl_hashset *hs, const u64 *keys, u64 n, u64 *d_new,
                               u64 *n_new);
+// Grows hs so that `incoming` more keys keep its load at or below 1/2.
+gdl_status hashset_reserve(gdl_ctx *ctx, gdl_hashset *hs, u64 incoming);
 // Exclusive scans of two arrays in place in one launch (v[n] = total each).
 gdl_status scan_inplace2(gdl_ctx *ctx, u64 *v0, u64 *v1, u64 n, u64 *totals /* 2 */);
 
@@ -291,6 +294,26 @@ __device__ __forceinline__ u64 run_end(const u64 *keys, u64 n, u64 from, u64 p, 
         else b = mid;
     }
     return a;
+}
+
+// Hash set of keys: open addressing over a power-of-two table, EMPTY = ~0,
+// the key ~0 tracked by the flag word slots[mask + 1].  Returns true when k
+// was not a member (this call inserted it).
+constexpr u64 kHsEmpty = ~0ull;
+
+__device__ __forceinline__ bool hs_insert_key(u64 *slots, u64 mask, u64 k) {
+    if (k == kHsEmpty) return atomicExch((unsigned int *)&slots[mask + 1], 1u) == 0u;
+    u64 s = mix64(k) & mask;
+    // CAS first: at load <= 1/2 the home slot is usually empty or holds k, so
+    // one round trip decides most inserts
+    for (u64 step = 0; step <= mask; ++step) {
+        const unsigned long long old = atomicCAS(
+            (unsigned long long *)&slots[s], (unsigned long long)kHsEmpty, (unsigned long long)k);
+        if (old == (unsigned long long)kHsEmpty) return true;
+        if (old == (unsigned long long)k) return false;
+        s = (s + 1) & mask;
+    }
+    return false;  // unreachable at load <= 1/2
 }
 
 // The index is an array of buckets of kBucket slots (one 64-byte line);

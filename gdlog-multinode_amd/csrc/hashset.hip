@@ -14,23 +14,6 @@ namespace gdl {
 constexpr u64 kEmpty = ~0ull;
 constexpr int HB = 256;
 
-__device__ __forceinline__ bool hs_insert(u64 *slots, u64 mask, u64 k) {
-    u64 s = mix64(k) & mask;
-    for (u64 step = 0; step <= mask; ++step) {
-        u64 cur = slots[s];
-        if (cur == k) return false;
-        if (cur == kEmpty) {
-            const unsigned long long old =
-                atomicCAS((unsigned long long *)&slots[s], (unsigned long long)kEmpty,
-                          (unsigned long long)k);
-            if (old == (unsigned long long)kEmpty) return true;
-            if (old == (unsigned long long)k) return false;
-        }
-        s = (s + 1) & mask;
-    }
-    return false;  // unreachable at load <= 1/2
-}
-
 // flag word at slots[cap]: 1 when the key ~0 is a member
 __global__ __launch_bounds__(HB) void k_hs_insert_new(u64 *slots, u64 mask, const u64 *keys, u64 n,
                                                       u64 *out, unsigned long long *counter) {
@@ -43,11 +26,7 @@ __global__ __launch_bounds__(HB) void k_hs_insert_new(u64 *slots, u64 mask, cons
         u64 k = 0;
         if (i < n) {
             k = keys[i];
-            if (k == kEmpty) {
-                fresh = atomicExch((unsigned int *)&slots[mask + 1], 1u) == 0u;
-            } else {
-                fresh = hs_insert(slots, mask, k);
-            }
+            fresh = hs_insert_key(slots, mask, k);
         }
         const u64 bal = __ballot(fresh);
         if (bal) {
@@ -66,7 +45,7 @@ __global__ __launch_bounds__(HB) void k_hs_rehash(const u64 *old_slots, u64 old_
                                                   u64 mask) {
     for (u64 i = (u64)blockIdx.x * HB + threadIdx.x; i < old_cap; i += (u64)gridDim.x * HB) {
         const u64 k = old_slots[i];
-        if (k != kEmpty) (void)hs_insert(slots, mask, k);
+        if (k != kEmpty) (void)hs_insert_key(slots, mask, k);
     }
 }
 
@@ -105,7 +84,7 @@ static u64 pow2_at_least(u64 x) {
 }
 
 // Grows the table so that size + incoming stays at load <= 1/2.
-static gdl_status hs_reserve(gdl_ctx *ctx, gdl_hashset *hs, u64 incoming) {
+gdl_status hashset_reserve(gdl_ctx *ctx, gdl_hashset *hs, u64 incoming) {
     const u64 need = pow2_at_least(2 * (hs->size + incoming));
     if (hs->slots && hs->capacity >= need) return GDL_OK;
     u64 *ns = nullptr;
@@ -129,7 +108,7 @@ gdl_status hashset_insert_new(gdl_ctx *ctx, gdl_hashset *hs, const u64 *keys, u6
                               u64 *n_new) {
     *n_new = 0;
     if (n == 0) return GDL_OK;
-    GDL_TRY(hs_reserve(ctx, hs, n));
+    GDL_TRY(hashset_reserve(ctx, hs, n));
     Scratch scr(ctx);
     u64 *counter;
     GDL_TRY(scr.get(1, &counter));
@@ -157,7 +136,7 @@ gdl_status gdl_hashset_create(gdl_ctx *ctx, uint64_t expected, gdl_hashset *out)
     out->slots = nullptr;
     out->capacity = 0;
     out->size = 0;
-    return hs_reserve(ctx, out, expected);
+    return hashset_reserve(ctx, out, expected);
 }
 
 gdl_status gdl_hashset_free(gdl_ctx *ctx, gdl_hashset *hs) {

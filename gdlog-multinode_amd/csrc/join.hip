@@ -21,14 +21,37 @@ constexpr int JM_TILE = JB * JM_I;
 
 __device__ __forceinline__ int prefix_shift(int prefix_cols) { return prefix_cols >= 2 ? 0 : 32; }
 
+// out[0] += distinct prefixes, out[1] = max(longest run); one atomic pair per block.
 __global__ __launch_bounds__(JB) void k_idx_count(const u64 *keys, u64 n, int shift, u64 *out) {
-    u64 acc = 0;
+    __shared__ u64 s_sum[JB / 64], s_max[JB / 64];
+    u64 acc = 0, mx = 0;
     for (u64 i = (u64)blockIdx.x * JB + threadIdx.x; i < n; i += (u64)gridDim.x * JB) {
         const u64 p = keys[i] >> shift;
-        if (i == 0 || (keys[i - 1] >> shift) != p) ++acc;
+        if (i == 0 || (keys[i - 1] >> shift) != p) {
+            ++acc;
+            const u64 len = run_end(keys, n, i, p, shift) - i;
+            mx = len > mx ? len : mx;
+        }
     }
-    for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd((unsigned long long *)out, (unsigned long long)acc);
+    for (int d = 32; d >= 1; d >>= 1) {
+        acc += __shfl_xor(acc, d, 64);
+        const u64 o = __shfl_xor(mx, d, 64);
+        mx = o > mx ? o : mx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_sum[threadIdx.x >> 6] = acc;
+        s_max[threadIdx.x >> 6] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u64 a = 0, m = 0;
+        for (int w = 0; w < JB / 64; ++w) {
+            a += s_sum[w];
+            m = s_max[w] > m ? s_max[w] : m;
+        }
+        if (a) atomicAdd((unsigned long long *)&out[0], (unsigned long long)a);
+        if (m) atomicMax((unsigned long long *)&out[1], (unsigned long long)m);
+    }
 }
 
 __global__ __launch_bounds__(JB) void k_idx_build(const u64 *keys, u64 n, int shift,
@@ -135,6 +158,98 @@ __global__ __launch_bounds__(JB) void k_join_materialize(const u64 *inner, const
     for (u32 x = threadIdx.x; x < n_tile; x += JB) out[tile0 + x] = s_out[x];
 }
 
+// Join fused with insert-if-absent into a hash set.  Each wave takes 64
+// outer tuples, probes their runs, and flattens the (outer, run position)
+// pairs with a wave prefix sum, so every lane handles one produced tuple per
+// round whatever the run lengths (a hub run is spread over the wave).  New
+// keys are staged per wave in LDS and flushed with one atomic per kStage keys
+// (counters[0] is the output cursor); counters[1] counts produced tuples.
+constexpr u32 kStage = 256;
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ void flush_stage(const u64 *stage, u32 n, u64 *out,
+                                            unsigned long long *cursor, int lane) {
+    unsigned long long pos = 0;
+    if (lane == 0) pos = atomicAdd(cursor, (unsigned long long)n);
+    pos = __shfl(pos, 0, 64);
+    for (u32 x = lane; x < n; x += 64) out[pos + x] = stage[x];
+}
+
+__global__ __launch_bounds__(JB) void k_join_insert(
+    const IdxSlot *slots, u64 n_slots, const u64 *inner, u64 n_inner, int shift, const u64 *outer,
+    u64 n_outer, int arity, int s0, int s1, int pop, int plhs, int prhs, u64 *hs_slots, u64 hs_mask,
+    u64 *out, unsigned long long *counters) {
+    __shared__ u64 s_excl[JB], s_b[JB], s_ok[JB];
+    __shared__ u64 s_stage[JB / 64][kStage];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const u64 lt = lane_mask_lt();
+    const u64 *wx = s_excl + w * 64;
+    u64 *stage = s_stage[w];
+    u32 staged = 0;  // wave-uniform
+    u64 produced = 0;
+    for (u64 base = (u64)blockIdx.x * JB; base < n_outer; base += (u64)gridDim.x * JB) {
+        const u64 i = base + threadIdx.x;
+        u64 b = 0, c = 0, ok = 0;
+        if (i < n_outer) {
+            ok = outer[i];
+            if (!lookup_run(slots, n_slots, inner, n_inner, shift, ok >> shift, &b, &c)) c = 0;
+        }
+        const u64 incl = wave_inclusive_scan(c);
+        const u64 total = __shfl(incl, 63, 64);
+        s_excl[threadIdx.x] = incl - c;
+        s_b[threadIdx.x] = b;
+        s_ok[threadIdx.x] = ok;
+        wave_sync_lds();
+        for (u64 q = 0; q < total; q += 64) {
+            const u64 t = q + lane;
+            bool fresh = false;
+            u64 key = 0;
+            if (t < total) {
+                // the last lane whose run starts at or before t owns position t
+                int lo = 0, hi = 63;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (wx[mid] <= t) lo = mid;
+                    else hi = mid - 1;
+                }
+                const u64 j = t - wx[lo];
+                key = apply_map(inner[s_b[w * 64 + lo] + j], s_ok[w * 64 + lo], arity, s0, s1);
+                if (eval_pred(key, pop, plhs, prhs)) {
+                    ++produced;
+                    fresh = hs_insert_key(hs_slots, hs_mask, key);
+                }
+            }
+            const u64 bal = __ballot(fresh);
+            const u32 cnt = (u32)__popcll(bal);
+            if (staged + cnt > kStage) {
+                wave_sync_lds();
+                flush_stage(stage, staged, out, &counters[0], lane);
+                staged = 0;
+                wave_sync_lds();
+            }
+            if (fresh) stage[staged + __popcll(bal & lt)] = key;
+            staged += cnt;
+        }
+        wave_sync_lds();  // s_excl / s_b / s_ok are rewritten next round
+    }
+    if (staged) {
+        wave_sync_lds();
+        flush_stage(stage, staged, out, &counters[0], lane);
+    }
+    for (int d = 32; d >= 1; d >>= 1) produced += __shfl_xor(produced, d, 64);
+    if (lane == 0 && produced) atomicAdd(&counters[1], (unsigned long long)produced);
+}
+
+__global__ void k_set2(u64 *p, u64 a, u64 b) {
+    p[0] = a;
+    p[1] = b;
+}
+
 __global__ __launch_bounds__(JB) void k_project(const u64 *in, u64 n, int arity, int s0, int s1,
                                                 u64 *out) {
     for (u64 i = (u64)blockIdx.x * JB + threadIdx.x; i < n; i += (u64)gridDim.x * JB) {
@@ -163,19 +278,23 @@ gdl_status index_build(gdl_ctx *ctx, const u64 *keys, u64 n, int prefix_cols, gd
     out->n_slots = 0;
     out->n_distinct = 0;
     out->prefix_cols = prefix_cols;
+    out->max_run = 0;
     if (n == 0) return GDL_OK;
     const int shift = prefix_cols >= 2 ? 0 : 32;
-    u64 distinct = 0;
+    u64 distinct = 0, max_run = 0;
     {
         Scratch scr(ctx);
         u64 *cnt;
-        GDL_TRY(scr.get(1, &cnt));
-        GDL_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(u64), ctx->stream));
+        GDL_TRY(scr.get(2, &cnt));
+        GDL_HIP(ctx, hipMemsetAsync(cnt, 0, 2 * sizeof(u64), ctx->stream));
         KTimer kt(ctx, K_IDX_COUNT, 8.0 * n);
-        hipLaunchKernelGGL(k_idx_count, dim3(stride_grid(ctx, n)), dim3(JB), 0, ctx->stream, keys,
-                           n, shift, cnt);
+        const unsigned g = std::min<unsigned>(stride_grid(ctx, n), 512u);
+        hipLaunchKernelGGL(k_idx_count, dim3(g), dim3(JB), 0, ctx->stream, keys, n, shift, cnt);
         GDL_TRY(check_launch(ctx, "k_idx_count"));
-        GDL_TRY(read_u64(ctx, cnt, &distinct));
+        u64 h[2];
+        GDL_TRY(read_u64(ctx, cnt, h, 2));
+        distinct = h[0];
+        max_run = h[1];
     }
     u64 n_slots = (u64)((double)distinct / (double)ctx->cfg.load_factor) + 1;
     if (n_slots <= distinct) n_slots = distinct + 1;
@@ -194,6 +313,7 @@ gdl_status index_build(gdl_ctx *ctx, const u64 *keys, u64 n, int prefix_cols, gd
     out->slots = p;
     out->n_slots = n_slots;
     out->n_distinct = distinct;
+    out->max_run = max_run;
     return GDL_OK;
 }
 
@@ -343,6 +463,78 @@ gdl_status gdl_join(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n_inner,
     }
     *d_out = res;
     *n_out = total;
+    return GDL_OK;
+}
+
+gdl_status gdl_join_insert(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n_inner,
+                           const gdl_index *inner_idx, const uint64_t *d_outer, uint64_t n_outer,
+                           gdl_colmap map, gdl_pred pred, gdl_hashset *hs, uint64_t **d_new,
+                           uint64_t *new_cap, uint64_t *n_new, uint64_t *n_raw) {
+    if (!ctx || !inner_idx || !hs || !d_new || !new_cap || !n_new || !n_raw ||
+        (n_outer && !d_outer) || (n_inner && !d_inner))
+        return GDL_ERR_INVALID_ARG;
+    if (!valid_map(map, true)) return fail(ctx, GDL_ERR_INVALID_ARG, "bad join column map");
+    if (pred.op < GDL_PRED_NONE || pred.op > GDL_PRED_FALSE)
+        return fail(ctx, GDL_ERR_INVALID_ARG, "bad join predicate");
+    *n_raw = 0;
+    if (n_outer == 0 || n_inner == 0 || inner_idx->n_slots == 0) return GDL_OK;
+    const int shift = inner_idx->prefix_cols >= 2 ? 0 : 32;
+    // upper bound of produced tuples: |outer| x longest inner run, or an
+    // exact count when that bound is loose
+    const u64 mr = inner_idx->max_run ? inner_idx->max_run : n_inner;
+    u64 bound = (n_outer > ~0ull / mr) ? ~0ull : n_outer * mr;
+    const u64 loose_limit = std::max<u64>(1ull << 24, 4 * n_outer);
+    if (bound > loose_limit) {
+        Scratch scr(ctx);
+        u32 *counts;
+        u64 *begins, *offsets;
+        GDL_TRY(scr.get(n_outer, &counts));
+        GDL_TRY(scr.get(n_outer, &begins));
+        GDL_TRY(scr.get(n_outer + 1, &offsets));
+        GDL_TRY(join_count(ctx, d_inner, n_inner, inner_idx, d_outer, n_outer, counts, begins));
+        GDL_TRY(exclusive_scan_u32_to_u64(ctx, counts, n_outer, offsets, &bound));
+    }
+    if (bound == 0) return GDL_OK;
+    GDL_TRY(hashset_reserve(ctx, hs, bound));
+    if (*n_new + bound > *new_cap || !*d_new) {
+        const u64 cap = *n_new + bound + (*n_new + bound) / 4 + 1024;
+        void *p = nullptr;
+        std::string err;
+        gdl_status st = ctx->alloc->alloc(cap * sizeof(u64), &p, &err);
+        if (st != GDL_OK) return fail(ctx, st, err);
+        if (*n_new)
+            GDL_HIP(ctx, hipMemcpyAsync(p, *d_new, *n_new * sizeof(u64), hipMemcpyDeviceToDevice,
+                                        ctx->stream));
+        if (*d_new) ctx->alloc->free(*d_new, nullptr);
+        *d_new = (u64 *)p;
+        *new_cap = cap;
+    }
+    Scratch scr(ctx);
+    u64 *counters;
+    GDL_TRY(scr.get(2, &counters));
+    hipLaunchKernelGGL(k_set2, dim3(1), dim3(1), 0, ctx->stream, counters, (u64)*n_new, (u64)0);
+    GDL_TRY(check_launch(ctx, "k_set2"));
+    {
+        KTimer kt(ctx, K_JOIN_INSERT, 8.0 * n_outer + 64.0 * n_outer);
+        const u64 cap_blocks = (u64)(ctx->cu_count > 0 ? ctx->cu_count : 256) * 8;
+        const unsigned g = (unsigned)std::max<u64>(1, std::min<u64>((n_outer + JB - 1) / JB,
+                                                                    cap_blocks));
+        hipLaunchKernelGGL(k_join_insert, dim3(g), dim3(JB), 0, ctx->stream,
+                           (const IdxSlot *)inner_idx->slots, inner_idx->n_slots, d_inner, n_inner,
+                           shift, d_outer, n_outer, map.arity, map.src[0],
+                           map.arity == 2 ? map.src[1] : 0, pred.op, pred.lhs, pred.rhs,
+                           (u64 *)hs->slots, hs->capacity - 1, *d_new,
+                           (unsigned long long *)counters);
+        GDL_TRY(check_launch(ctx, "k_join_insert"));
+    }
+    u64 h[2];
+    GDL_TRY(read_u64(ctx, counters, h, 2));
+    const u64 added = h[0] - *n_new;
+    // inner run reads + hash probes per produced tuple, new keys written
+    ctx->stats[K_JOIN_INSERT].alg_bytes += 16.0 * h[1] + 8.0 * added;
+    hs->size += added;
+    *n_new = h[0];
+    *n_raw = h[1];
     return GDL_OK;
 }
 

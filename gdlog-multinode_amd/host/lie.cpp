@@ -45,6 +45,24 @@ bool full_is_inner(const std::vector<ra_op> &ops, const Relation *rel) {
     return false;
 }
 
+// True when only joins write rel and no operator reads rel's FULL or NEWT
+// (reading DELTA is fine: the pending delta is a separate buffer).
+bool output_fusable(const std::vector<ra_op> &ops, const Relation *rel) {
+    for (const auto &op : ops) {
+        if (const auto *j = std::get_if<RelationalJoin>(&op)) {
+            if (j->inner_rel == rel && j->inner_ver != DELTA) return false;
+            if (j->outer_rel == rel && j->outer_ver != DELTA) return false;
+            if (j->output_rel == rel && (rel->tmp_flag || j->disable_load)) return false;
+        } else if (const auto *c = std::get_if<RelationalCopy>(&op)) {
+            if (c->dest_rel == rel) return false;
+            if (c->src_rel == rel && c->src_ver != DELTA) return false;
+        } else if (const auto *a = std::get_if<RelationalACopy>(&op)) {
+            if (a->dest_rel == rel || a->src_rel == rel) return false;
+        }
+    }
+    return true;
+}
+
 void enter_append_mode(Relation *rel) {
     gdl_ctx *ctx = context();
     const tuple_size_t n = rel->current_full_size;
@@ -73,6 +91,10 @@ void leave_append_mode(Relation *rel) {
         container_take(rel->full, sorted, n, n, true);
     }
     check(gdl_hashset_free(ctx, &rel->full_set), "gdl_hashset_free");
+    device_free(rel->pend_keys);
+    rel->pend_keys = nullptr;
+    rel->pend_cap = rel->pend_n = rel->pend_raw = 0;
+    rel->fused_output = false;
     rel->append_mode = false;
 }
 
@@ -85,7 +107,10 @@ void LIE::fixpoint_loop() {
     for (Relation *rel : update_relations) {
         rel->current_full_size = rel->full->tuple_counts;
         seed_delta(rel);
-        if (full_mode() == FullMode::Auto && !full_is_inner(ra_ops, rel)) enter_append_mode(rel);
+        if (full_mode() == FullMode::Auto && !full_is_inner(ra_ops, rel)) {
+            enter_append_mode(rel);
+            rel->fused_output = output_fusable(ra_ops, rel);
+        }
     }
     int iteration_counter = 0;
     while (true) {
@@ -115,6 +140,27 @@ void LIE::fixpoint_loop() {
             ++rel_idx;
             if (verbose()) std::cout << rel->name << std::endl;
             container_free(rel->delta);
+            if (rel->fused_output) {
+                // the set difference already happened inside gdl_join_insert
+                const tuple_size_t nd = rel->pend_n, raw = rel->pend_raw;
+                if (nd != 0) fixpoint_flag = false;
+                if (nd > 0) {
+                    rel->full->tuple_counts = rel->current_full_size;
+                    container_append(rel->full, rel->pend_keys, nd);
+                    rel->current_full_size += nd;
+                    container_take(rel->delta, rel->pend_keys, nd, rel->pend_cap, false);
+                    rel->pend_keys = nullptr;
+                    rel->pend_cap = 0;
+                }
+                rel->pend_n = rel->pend_raw = 0;
+                trace.push_back({iteration_counter, rel_idx, raw, nd, rel->current_full_size});
+                if (verbose())
+                    std::cout << "iteration " << iteration_counter << " relation " << rel->name
+                              << " finish dedup new tuples : " << nd
+                              << " delta tuple size: " << nd << " full counts "
+                              << rel->current_full_size << std::endl;
+                continue;
+            }
             GHashRelContainer *newt = rel->newt;
             if (!newt->keys || newt->tuple_counts == 0) {
                 container_free(newt);

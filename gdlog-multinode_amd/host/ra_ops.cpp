@@ -104,6 +104,22 @@ void RelationalJoin::operator()() {
         Phase ph(detail_time, 5);
         idx = container_index(inner, k);
     }
+    if (output_rel->fused_output && !output_is_tmp && !disable_load) {
+        uint64_t m = 0;
+        {
+            Phase ph(detail_time, 2);
+            check(gdl_join_insert(context(), inner->keys, inner->tuple_counts, idx, outer->keys,
+                                  outer->tuple_counts, gen_map.map, pred_map.pred,
+                                  &output_rel->full_set, &output_rel->pend_keys,
+                                  &output_rel->pend_cap, &output_rel->pend_n, &m),
+                  "gdl_join_insert");
+        }
+        output_rel->pend_raw += m;
+        if (verbose())
+            std::cout << output_rel->name << "   " << k << " join result size(non dedup) " << m
+                      << std::endl;
+        return;
+    }
     uint64_t *res = nullptr;
     uint64_t m = 0;
     {

@@ -43,13 +43,14 @@ typedef struct gdl_config {
     int device;             /* HIP device ordinal */
     uint64_t mem_cap_bytes; /* 0: no cap.  Otherwise allocations that would take the
                                bytes in use above the cap fail (GDL_ERR_OUT_OF_MEMORY) */
-    float load_factor;      /* hash index load factor, (0, 0.95]; reference 0.8
-                               (relation.cu:603-604) */
+    float load_factor;      /* hash index load factor, (0, 0.95]; default 0.5 (the
+                               reference uses 0.8, relation.cu:603-604; at 0.8
+                               probe-chain tails make misses 3-4x slower) */
     int collect_stats;      /* 1: time every kernel with hipEvents on the context stream */
     int verbose;            /* 1: print the reference-style per-op / per-iteration log */
 } gdl_config;
 
-/* Fills the defaults: device 0, no cap, load factor 0.8, no stats. */
+/* Fills the defaults: device 0, no cap, load factor 0.5, no stats. */
 void gdl_config_default(gdl_config *cfg);
 
 /* Replaces the implicit device-0 / legacy-default-stream state of the
@@ -129,6 +130,7 @@ typedef struct gdl_index {
     uint64_t n_slots;
     uint64_t n_distinct; /* distinct prefixes */
     int prefix_cols;    /* 1 or 2 */
+    uint64_t max_run;   /* longest run (tuples sharing one prefix) */
 } gdl_index;
 gdl_status gdl_index_build(gdl_ctx *ctx, const uint64_t *d_sorted, uint64_t n, int prefix_cols,
                            gdl_index *out);
@@ -202,6 +204,20 @@ gdl_status gdl_join_materialize(gdl_ctx *ctx, const uint64_t *d_inner, const uin
 gdl_status gdl_join(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n_inner,
                     const gdl_index *inner_idx, const uint64_t *d_outer, uint64_t n_outer,
                     gdl_colmap map, gdl_pred pred, uint64_t **d_out, uint64_t *n_out);
+
+/* Join fused with the set difference against an append-only FULL (the hash
+ * set): each produced tuple colmap(inner, outer) that passes pred is inserted
+ * into hs, and those that were not yet members are appended to the growable
+ * device buffer *d_new (capacity *new_cap keys, reallocated from the context
+ * allocator when needed, existing *n_new keys kept).  *n_raw gets the number
+ * of produced tuples (the reference's "join result size (non dedup)").
+ * Equivalent to RelationalJoin's newt accumulation followed by the
+ * iteration's set difference when no other operator reads the output
+ * relation's FULL or NEWT in the same iteration. */
+gdl_status gdl_join_insert(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n_inner,
+                           const gdl_index *inner_idx, const uint64_t *d_outer, uint64_t n_outer,
+                           gdl_colmap map, gdl_pred pred, gdl_hashset *hs, uint64_t **d_new,
+                           uint64_t *new_cap, uint64_t *n_new, uint64_t *n_raw);
 
 /* ---- projection / filter ----------------------------------------------
  * gdl_project_u64 replaces get_copy_result (relation.cu:390-403);

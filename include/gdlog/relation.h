@@ -72,6 +72,13 @@ struct Relation {
     // the fixpoint) plus a device hash set, and newt stays unsorted.
     bool append_mode = false;
     gdl_hashset full_set{};
+    // Append-mode relations that no operator of the LIE reads as FULL or NEWT
+    // and that only joins write: join results go straight into full_set
+    // (gdl_join_insert) and the new tuples collect in pend_keys, which becomes
+    // the next delta.
+    bool fused_output = false;
+    uint64_t *pend_keys = nullptr;
+    uint64_t pend_cap = 0, pend_n = 0, pend_raw = 0;
 
     // Merges DELTA into FULL (relation.cu:405-521).
     void flush_delta(int grid_size, int block_size, float *detail_time);

@@ -108,7 +108,7 @@ def test_index_lookup_runs(ctx, prefix_cols, lf):
     assert np.array_equal(b[hit].astype(np.int64), lo[hit])
     ctx.index_free(idx)
     d.free()
-    ctx.eng.gdl_set_load_factor(ctx.h, 0.8)
+    ctx.eng.gdl_set_load_factor(ctx.h, 0.5)
 
 
 def test_index_saturated_run_count(ctx):
@@ -218,3 +218,31 @@ def test_allocator_cap_reports_out_of_memory(ctx):
     # the context still works after the failure
     k = rand_keys(10_000)
     assert np.array_equal(ctx.sort(k), np.sort(k))
+
+
+@pytest.mark.parametrize("colmap,pred", [((1, 3), (gdl.PRED_NONE, 0, 1)),
+                                         ((3, 1), (gdl.PRED_NE, 0, 1))])
+@pytest.mark.parametrize("inner_kind", ["uniform", "hub"])
+def test_join_insert_is_join_then_set_difference(ctx, colmap, pred, inner_kind):
+    if inner_kind == "hub":  # runs above the wave-cooperative threshold
+        inner = np.unique(np.concatenate([gdl.pack(np.full(5000, 7), np.arange(5000)),
+                                          rand_keys(20_000, 9, 9)]))
+    else:
+        inner = np.unique(rand_keys(50_000, 10, 12))
+    hs = ctx.hashset_create(0)
+    try:
+        seen = set()
+        for step in range(3):
+            outer = rand_keys(30_000 * (step + 1), 10, 12)
+            if inner_kind == "hub":
+                outer = np.concatenate([outer, gdl.pack([7, 7], [1, 2])])
+            want_all = np_join(inner, outer, colmap, 1)
+            if pred[0] == gdl.PRED_NE:
+                want_all = want_all[(want_all >> np.uint64(32)) != (want_all & M32)]
+            new, raw = ctx.join_insert(hs, inner, outer, colmap, 1, pred)
+            assert raw == want_all.size
+            want_new = set(want_all.tolist()) - seen
+            assert new.size == len(want_new) and set(new.tolist()) == want_new
+            seen |= want_new
+    finally:
+        ctx.hashset_free(hs)
