@@ -41,17 +41,214 @@ def _pkg():
     return mod
 
 
-def cpu_baseline(edges, max_iteration):
-    """Oracle (plain C, one thread) on the first iterations of the workload."""
+def cpu_baseline(kind, inp, max_iteration):
+    """Oracle (plain C, one thread) on a bounded sample of the workload."""
     if REPO not in sys.path:
         sys.path.insert(0, REPO)
     import oracle
     t = time.perf_counter()
-    r = oracle.tc(edges, max_iteration=max_iteration)
+    if kind == "tc":
+        r = oracle.tc(inp, max_iteration=max_iteration)
+        what = f"first {r['iterations']} fixpoint iterations of the same TC workload"
+    elif kind == "sg":
+        r = oracle.sg(inp)
+        what = "the whole SG workload"
+    else:
+        r = oracle.cspa(*inp)
+        what = "the whole CSPA workload"
     dt = time.perf_counter() - t
     return dict(value=r["derived"] / dt, unit="derived tuples/s", cores=1, kind="port",
-                sample=f"oracle TC on the same tree, first {r['iterations']} fixpoint "
-                       f"iterations ({r['derived']} derived tuples, {dt:.1f} s)")
+                sample=f"oracle on {what} ({r['derived']} derived tuples, {dt:.1f} s)")
+
+
+def pct(xs, q):
+    xs = sorted(xs)
+    return xs[min(len(xs) - 1, int(round(q / 100.0 * (len(xs) - 1))))]
+
+
+def batched(gdl, ctx, n_inst, min_edges=1000, max_edges=50000, window=1000):
+    """Batched TC: n_inst instances (seed q: an Erdos-Renyi digraph or a
+    child->parent tree, log-uniform edge counts), all arriving at t = 0.
+    Serial = one TC program run per instance in order; batched = one TcBatch
+    admitting instances as slots and memory allow.  Latency of a query = time
+    until its result (|path|, iterations; checksum for the batch) is on the
+    host."""
+    def make(seeds):
+        out = []
+        for q in seeds:
+            _, _, n = gdl.gen_batched_shape(q, min_edges, max_edges)
+            out.append((q, gdl.gen_batched_instance(q, min_edges, max_edges, window), n))
+        return out
+
+    inst = make(range(n_inst))
+    n_edges = sum(int(e.size) for _, e, _ in inst)
+    # autotune on a calibration sample of the same distribution (other seeds)
+    calib = make(range(100_000, 100_016))
+
+    def calib_run():
+        with gdl.TcBatch(ctx, keep_keys=False) as b:
+            b.run(calib)
+    tune = importlib.import_module("gdlog_amd.tune") if "gdlog_amd.tune" in sys.modules else None
+    if tune is None:
+        spec = importlib.util.spec_from_file_location(
+            "gdlog_amd.tune", os.path.join(REPO, "gdlog-multinode_amd", "tune.py"))
+        tune = importlib.util.module_from_spec(spec)
+        sys.modules["gdlog_amd.tune"] = tune
+        spec.loader.exec_module(tune)
+    tuned = tune.autotune(ctx, calib_run, key=f"tc_batched_{min_edges}_{max_edges}",
+                          use_cache=False)
+    ctx.release_cached()
+    ctx.sync()
+    t0 = time.perf_counter()
+    lat_s, alone = [], {}
+    for q, e, _ in inst:
+        r = ctx.run_tc(e, want_keys=False, want_trace=False)
+        lat_s.append(time.perf_counter() - t0)
+        alone[q] = (r["n_out"], r["iterations"], r["derived"])
+    serial_s = time.perf_counter() - t0
+    ctx.release_cached()
+    t0 = time.perf_counter()
+    lat_b, done = [], {}
+    with gdl.TcBatch(ctx, keep_keys=False) as b:
+        pending = list(inst)
+        while len(done) < len(inst):
+            while pending and b.admit(*pending[0]):
+                pending.pop(0)
+            b.step()
+            for r in b.poll():
+                done[r["qid"]] = r
+                lat_b.append(time.perf_counter() - t0)
+        info = b.info()
+    batch_s = time.perf_counter() - t0
+    ctx.release_cached()
+    mism = [q for q, _, _ in inst
+            if (done[q]["n_out"], done[q]["iterations"], done[q]["derived"]) != alone[q]]
+    derived = sum(v[2] for v in alone.values())
+    return {
+        "workload": f"tc_batched_{n_inst}: {n_inst} TC instances, edges log-uniform "
+                    f"{min_edges}-{max_edges}, ER (nodes = edges/2.5) or child->parent tree "
+                    f"(window {window}) 50/50, seeds 0..{n_inst - 1}",
+        "instances": n_inst, "edges": n_edges, "derived_tuples": derived,
+        "queries_per_s": round(n_inst / batch_s, 3),
+        "p50_ms": round(1e3 * pct(lat_b, 50), 2), "p99_ms": round(1e3 * pct(lat_b, 99), 2),
+        "derived_tuples_per_s": round(derived / batch_s, 1),
+        "serial": {"queries_per_s": round(n_inst / serial_s, 3),
+                   "p50_ms": round(1e3 * pct(lat_s, 50), 2),
+                   "p99_ms": round(1e3 * pct(lat_s, 99), 2),
+                   "derived_tuples_per_s": round(derived / serial_s, 1)},
+        "speedup_vs_serial": round(serial_s / batch_s, 3),
+        "bodies": info["bodies"], "mismatches_vs_serial": len(mism),
+        "tuning": dict(tuned["tuning"], calibration_s=tuned["seconds"],
+                       default_s=tuned["default_seconds"], trials=tuned["trials"],
+                       sample="16 instances of the same generator, seeds 100000-100015"),
+    }
+
+
+SERVED_MIX = (("tc_er_2000_5000", 0.6), ("tc_tree_200k", 0.1), ("sg_rmat15_40k", 0.1),
+              ("cspa_zipf_500", 0.2))
+
+
+def served(gdl, ctx, n_req, rate_mult=2.0, seed=7, workers=3):
+    """Served mix: n_req queries drawn from SERVED_MIX (TC / SG at the sizes of
+    the single-query workloads, CSPA at 500 ids), Poisson arrivals at
+    rate_mult x the one-at-a-time engine's throughput.  Serial latencies
+    replay the measured one-at-a-time service times against the same
+    arrivals (a FIFO of one); the server runs them for real."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    names = [m[0] for m in SERVED_MIX]
+    picks = rng.choice(len(SERVED_MIX), size=n_req, p=[m[1] for m in SERVED_MIX])
+    reqs = []
+    for i, k in enumerate(picks):
+        s = 1000 + i
+        if k == 0:
+            reqs.append((i, gdl.QUERY_TC, gdl.gen_er(s, 2000, 5000), None))
+        elif k == 1:
+            reqs.append((i, gdl.QUERY_TC, gdl.gen_tree(s, 200_000, 1000), None))
+        elif k == 2:
+            reqs.append((i, gdl.QUERY_SG, gdl.gen_rmat(s, 15, 40000), None))
+        else:
+            a, d = gdl.gen_cspa(s, 500, 1.1, 750, 375)
+            reqs.append((i, gdl.QUERY_CSPA, a, d))
+    run = {gdl.QUERY_TC: lambda a, b: ctx.run_tc(a, want_keys=False, want_trace=False),
+           gdl.QUERY_SG: lambda a, b: ctx.run_sg(a, want_keys=False, want_trace=False),
+           gdl.QUERY_CSPA: lambda a, b: ctx.run_cspa(a, b, want_keys=False, want_trace=False)}
+    for kind in (gdl.QUERY_TC, gdl.QUERY_SG, gdl.QUERY_CSPA):  # warm code objects
+        q = next(r for r in reqs if r[1] == kind) if any(r[1] == kind for r in reqs) else None
+        if q:
+            run[kind](q[2], q[3])
+    service, alone = [], {}
+    for i, kind, a, b in reqs:
+        ctx.sync()
+        t0 = time.perf_counter()
+        r = run[kind](a, b)
+        service.append(time.perf_counter() - t0)
+        alone[i] = (r["n_out"], r["iterations"], r["checksum"])
+    ctx.release_cached()
+    serial_qps = n_req / sum(service)
+    rate = rate_mult * serial_qps
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    t, lat_s = 0.0, []
+    for arr, sv in zip(arrivals, service):
+        t = max(arr, t) + sv
+        lat_s.append(t - arr)
+    serial_span = t
+    got = {}
+    with gdl.Server(workers=workers, keep_keys=False) as srv:
+        t0 = time.perf_counter()
+        nxt = 0
+        while len(got) < n_req:
+            now = time.perf_counter() - t0
+            while nxt < n_req and arrivals[nxt] <= now:
+                i, kind, a, b = reqs[nxt]
+                srv.submit(i, kind, a, b)
+                nxt += 1
+            wait_ms = 1000.0 * (arrivals[nxt] - now) if nxt < n_req else 1000.0
+            r = srv.wait(max(0.0, wait_ms))
+            if r is not None:
+                got[r["qid"]] = (time.perf_counter() - t0, r)
+        span = time.perf_counter() - t0
+        st = srv.stats()
+    lat_v = [got[i][0] - arrivals[i] for i in range(n_req)]
+    mism = [i for i in range(n_req)
+            if got[i][1]["status"] != 0 or
+            (got[i][1]["n_out"], got[i][1]["iterations"], got[i][1]["checksum"]) != alone[i]]
+    by_kind = {}
+    for i, k in enumerate(picks):
+        by_kind.setdefault(names[k], []).append(lat_v[i])
+    return {
+        "workload": f"served_mix_{n_req}: " + ", ".join(f"{int(100 * p)}% {n}"
+                                                       for n, p in SERVED_MIX) +
+                    f"; Poisson arrivals at {rate_mult}x the one-at-a-time throughput "
+                    f"({rate:.1f} queries/s), seed {seed}",
+        "requests": n_req, "offered_qps": round(rate, 2),
+        "queries_per_s": round(n_req / span, 3),
+        "p50_ms": round(1e3 * pct(lat_v, 50), 2), "p99_ms": round(1e3 * pct(lat_v, 99), 2),
+        "p99_ms_by_kind": {k: round(1e3 * pct(v, 99), 2) for k, v in sorted(by_kind.items())},
+        "serial": {"queries_per_s": round(n_req / serial_span, 3),
+                   "capacity_qps": round(serial_qps, 3),
+                   "p50_ms": round(1e3 * pct(lat_s, 50), 2),
+                   "p99_ms": round(1e3 * pct(lat_s, 99), 2)},
+        "speedup_vs_serial": round(serial_span / span, 3),
+        "workers": workers, "batched_tc": st["batched"], "reruns_alone": st["reruns"],
+        "peak_reserved_gb": round(st["peak_reserved"] / 1e9, 2),
+        "mismatches_vs_alone": len(mism),
+    }
+
+
+def workloads(gdl, nodes, window):
+    """name -> (kind, make_input, description)."""
+    return {
+        "tree": ("tc", lambda: gdl.gen_tree(0, nodes, window),
+                 f"tc_tree_{nodes // 1000}k: TC on a {nodes}-node child->parent tree, "
+                 f"parent window {window}"),
+        "er": ("tc", lambda: gdl.gen_er(0, 2000, 5000),
+               "tc_er_2000_5000: TC on a directed Erdos-Renyi graph, 2,000 nodes, 5,000 edges"),
+        "sg": ("sg", lambda: gdl.gen_rmat(1, 15, 40000),
+               "sg_rmat15_40k: same generation on an R-MAT graph, scale 15, 40,000 edges"),
+        "cspa": ("cspa", lambda: gdl.gen_cspa(5, 500, 1.1, 750, 375),
+                 "cspa_zipf_500: CSPA, 500 ids, 750 assign / 375 dereference (Zipf 1.1)"),
+    }
 
 
 def main():
@@ -59,11 +256,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="tree", choices=["tree", "er", "sg", "cspa"])
+    ap.add_argument("--mode", default="auto", choices=["auto", "merge"],
+                    help="FULL maintenance (merge = the reference's per-iteration merge)")
     ap.add_argument("--nodes", type=int, default=200_000)
     ap.add_argument("--window", type=int, default=1000)
     ap.add_argument("--cpu-iterations", type=int, default=100,
                     help="fixpoint iterations of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batched-instances", type=int, default=256,
+                    help="instances of the batched-TC measurement (0: skip it)")
+    ap.add_argument("--served-requests", type=int, default=300,
+                    help="requests of the served-mix measurement (0: skip it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus must be 1 "
@@ -71,19 +275,34 @@ def main():
         sys.exit(2)
 
     gdl = _pkg()
-    edges = gdl.gen_tree(0, args.nodes, args.window)
+    gdl.set_full_mode(gdl.FULL_MODE_MERGE if args.mode == "merge" else gdl.FULL_MODE_AUTO)
+    kind, make, desc = workloads(gdl, args.nodes, args.window)[args.workload]
+    inp = make()
     ctx = gdl.Context(device=0)
-    d_edges = ctx.to_device(edges)
+    if kind == "cspa":
+        dev = (ctx.to_device(inp[0]), ctx.to_device(inp[1]))
+
+        def step():
+            return ctx.run_cspa(dev[0], dev[1], on_device=True, want_keys=False,
+                                want_trace=False)
+        n_in = int(inp[0].size + inp[1].size)
+    else:
+        dev = (ctx.to_device(inp),)
+        run = ctx.run_tc if kind == "tc" else ctx.run_sg
+
+        def step():
+            return run(dev[0], on_device=True, want_keys=False, want_trace=False)
+        n_in = int(inp.size)
 
     res = None
     for _ in range(args.warmup):
-        res = ctx.run_tc(d_edges, on_device=True, want_keys=False, want_trace=False)
+        res = step()
     ctx.stats_reset()
     ctx.stats_enable(True)
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        res = ctx.run_tc(d_edges, on_device=True, want_keys=False, want_trace=False)
+        res = step()
     ctx.sync()
     elapsed = time.perf_counter() - t0
     ctx.stats_enable(False)
@@ -112,9 +331,19 @@ def main():
               for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["total_ms"])
               if v["launches"]}
 
+    for d in dev:
+        d.free()
+    gdl.set_full_mode(gdl.FULL_MODE_AUTO)
+    serve = None
+    if args.served_requests > 0:
+        serve = served(gdl, ctx, args.served_requests)
+    batch = None
+    if args.batched_instances > 0:
+        batch = batched(gdl, ctx, args.batched_instances)
+
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(edges, args.cpu_iterations)
+        cpu = cpu_baseline(kind, inp, args.cpu_iterations)
 
     out = {
         "metric": METRIC,
@@ -129,17 +358,18 @@ def main():
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic (SplitMix64 seed 0)",
-        "config": {"workload": f"tc_tree_{args.nodes // 1000}k: TC on a {args.nodes}-node "
-                               f"child->parent tree, parent window {args.window}",
-                   "edges": int(edges.size), "path_tuples": res["n_out"],
-                   "iterations": res["iterations"], "derived_tuples": res["derived"]},
+        "config": {"workload": desc, "full_mode": args.mode, "input_tuples": n_in,
+                   "output_tuples": res["n_out"], "iterations": res["iterations"],
+                   "init_iterations": res["init_iterations"],
+                   "derived_tuples": res["derived"]},
         "roofline": roofline,
         "step_hbm_frac": round(tot_bytes / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9), 5),
         "kernel_busy_frac": round(kernel_ms / ms_per_step, 4),
         "phases": phases,
         "cpu_baseline": cpu,
+        "batched": batch,
+        "served": serve,
     }
-    d_edges.free()
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -22,7 +22,8 @@ INCLUDE_DIR = os.path.join(REPO, "include")
 
 GDL_OK = 0
 STATUS_NAMES = {0: "ok", 1: "invalid argument", 2: "out of device memory", 3: "HIP error",
-                4: "unsupported", 5: "size limit exceeded", 6: "device-side error"}
+                4: "unsupported", 5: "size limit exceeded", 6: "device-side error",
+                7: "busy"}
 
 COL_INNER0, COL_INNER1, COL_OUTER0, COL_OUTER1 = 0, 1, 2, 3
 PRED_NONE, PRED_EQ, PRED_NE, PRED_LT, PRED_LE, PRED_GT, PRED_GE, PRED_FALSE = range(8)
@@ -80,7 +81,8 @@ class TraceRec(Structure):
 class ProgResult(Structure):
     _fields_ = [("iterations", c_int32), ("init_iterations", c_int32), ("derived", c_uint64),
                 ("n_out", c_uint64), ("sizes", c_uint64 * 4), ("load_ms", c_double),
-                ("fixpoint_ms", c_double), ("n_trace", c_uint64), ("n_trace_init", c_uint64)]
+                ("fixpoint_ms", c_double), ("n_trace", c_uint64), ("n_trace_init", c_uint64),
+                ("checksum", c_uint64)]
 
 
 class ProgBuffers(Structure):
@@ -89,6 +91,30 @@ class ProgBuffers(Structure):
                 ("extra1", POINTER(c_uint64)), ("extra1_cap", c_uint64),
                 ("trace", POINTER(TraceRec)), ("trace_cap", c_uint64),
                 ("trace_init", POINTER(TraceRec)), ("trace_init_cap", c_uint64)]
+
+
+class HsSeg(Structure):
+    _fields_ = [("slots", c_void_p), ("mask", c_uint64)]
+
+
+class Tuning(Structure):
+    _fields_ = [("load_factor", c_float), ("blocks_per_cu", c_uint32), ("radix_bits", c_uint32),
+                ("join_block", c_uint32)]
+
+
+class TcbConfig(Structure):
+    _fields_ = [("max_slots", c_uint32), ("node_bits", c_uint32), ("mem_budget", c_uint64),
+                ("keep_keys", c_int)]
+
+
+class TcbResult(Structure):
+    _fields_ = [("qid", c_uint64), ("n_out", c_uint64), ("checksum", c_uint64),
+                ("derived", c_uint64), ("iterations", c_int32), ("slot", c_int32),
+                ("n_trace", c_uint64), ("admit_ms", c_double), ("done_ms", c_double),
+                ("has_keys", c_int)]
+
+
+GDL_ERR_BUSY = 7
 
 
 _U64P = POINTER(c_uint64)
@@ -173,6 +199,24 @@ def load():
     _decl(eng, "gdl_join_insert", c_int,
           [vp, vp, c_uint64, POINTER(Index), vp, c_uint64, ColMap, Pred, POINTER(HashSet),
            POINTER(vp), _U64P, _U64P, _U64P])
+    _decl(eng, "gdl_get_tuning", c_int, [vp, POINTER(Tuning)])
+    _decl(eng, "gdl_set_tuning", c_int, [vp, POINTER(Tuning)])
+    _decl(eng, "gdl_hashset_reserve", c_int, [vp, POINTER(HashSet), c_uint64])
+    _decl(eng, "gdl_hashset_extract", c_int, [vp, POINTER(HashSet), vp, c_uint64, _U64P])
+    _decl(eng, "gdl_join_insert_seg", c_int,
+          [vp, vp, c_uint64, POINTER(Index), vp, c_uint64, ColMap, Pred, vp, c_uint32, c_int,
+           vp, c_uint64, _U64P, _U64P])
+    _decl(eng, "gdl_key_histogram_u64", c_int, [vp, vp, c_uint64, c_int, c_uint32, vp])
+    _decl(eng, "gdl_mask_columns_u64", c_int, [vp, vp, c_uint64, c_uint32, vp])
+    _decl(eng, "gdl_tcb_config_default", None, [POINTER(TcbConfig)])
+    _decl(eng, "gdl_tcb_create", c_int, [vp, POINTER(TcbConfig), POINTER(vp)])
+    _decl(eng, "gdl_tcb_destroy", c_int, [vp])
+    _decl(eng, "gdl_tc_estimate", c_uint64, [vp, c_uint64, c_uint32, c_uint32, c_uint64])
+    _decl(eng, "gdl_tcb_admit", c_int, [vp, c_uint64, vp, c_uint64, c_uint32, c_uint64])
+    _decl(eng, "gdl_tcb_step", c_int, [vp, POINTER(c_uint32)])
+    _decl(eng, "gdl_tcb_poll", c_int, [vp, POINTER(TcbResult)])
+    _decl(eng, "gdl_tcb_fetch", c_int, [vp, c_uint64, vp, vp])
+    _decl(eng, "gdl_tcb_info", c_int, [vp] + [_U64P] * 5)
     _engine, _programs = eng, prog
     return eng, prog
 
@@ -332,6 +376,21 @@ class Context:
     def set_mem_cap(self, cap):
         self._check(self.eng.gdl_set_mem_cap(self.h, cap), "set_mem_cap")
 
+    def tuning(self):
+        t = Tuning()
+        self._check(self.eng.gdl_get_tuning(self.h, ctypes.byref(t)), "get_tuning")
+        return dict(load_factor=round(float(t.load_factor), 4), blocks_per_cu=t.blocks_per_cu,
+                    radix_bits=t.radix_bits, join_block=t.join_block)
+
+    def set_tuning(self, load_factor=None, blocks_per_cu=None, radix_bits=None,
+                   join_block=None):
+        cur = self.tuning()
+        t = Tuning(cur["load_factor"] if load_factor is None else load_factor,
+                   cur["blocks_per_cu"] if blocks_per_cu is None else blocks_per_cu,
+                   cur["radix_bits"] if radix_bits is None else radix_bits,
+                   cur["join_block"] if join_block is None else join_block)
+        self._check(self.eng.gdl_set_tuning(self.h, ctypes.byref(t)), "set_tuning")
+
     def release_cached(self):
         self._check(self.eng.gdl_release_cached(self.h), "release_cached")
 
@@ -479,6 +538,60 @@ class Context:
         di.free(); do.free()
         return new, raw.value
 
+    def hashset_extract(self, hs):
+        """Members of a set (any order)."""
+        o = self.alloc(max(hs.size, 1))
+        m = c_uint64()
+        self._check(self.eng.gdl_hashset_extract(self.h, ctypes.byref(hs), o.ptr, max(hs.size, 1),
+                                                 ctypes.byref(m)), "hashset_extract")
+        r = o.numpy(m.value)
+        o.free()
+        return r
+
+    def join_insert_seg(self, sets, seg_shift, inner_sorted, outer, colmap, prefix_cols=1,
+                        pred=(PRED_NONE, 0, 1), out_cap=None):
+        """Segmented fused join into per-segment sets (HashSet list, index = key >> seg_shift;
+        None = no set).  The sets must already hold room; returns (new keys, produced)."""
+        di, do = self.to_device(inner_sorted), self.to_device(outer)
+        idx = self.index_build(di, prefix_cols)
+        cm = ColMap(len(colmap), (c_int32 * 2)(*(list(colmap) + [0])[:2]))
+        tab = (HsSeg * len(sets))()
+        for i, hs in enumerate(sets):
+            tab[i] = HsSeg(hs.slots, hs.capacity - 1) if hs is not None else HsSeg(None, 0)
+        dt = self.alloc(2 * len(sets))
+        self._check(self.eng.gdl_memcpy_h2d(self.h, dt.ptr, ctypes.addressof(tab),
+                                            ctypes.sizeof(tab)), "h2d")
+        cap = out_cap if out_cap is not None else max(1, do.n * max(1, idx.max_run))
+        o = self.alloc(cap)
+        nn, raw = c_uint64(), c_uint64()
+        try:
+            self._check(self.eng.gdl_join_insert_seg(
+                self.h, di.ptr, di.n, ctypes.byref(idx), do.ptr, do.n, cm, Pred(*pred), dt.ptr,
+                len(sets), seg_shift, o.ptr, cap, ctypes.byref(nn), ctypes.byref(raw)),
+                "join_insert_seg")
+            return o.numpy(nn.value), raw.value
+        finally:
+            self.index_free(idx)
+            for a in (di, do, dt, o):
+                a.free()
+
+    def key_histogram(self, keys, shift, n_bins):
+        d = self.to_device(keys)
+        c = self.to_device(np.zeros(n_bins, dtype=np.uint64))
+        self._check(self.eng.gdl_key_histogram_u64(self.h, d.ptr, d.n, shift, n_bins, c.ptr),
+                    "key_histogram")
+        r = c.numpy()
+        d.free(); c.free()
+        return r
+
+    def mask_columns(self, keys, col_mask):
+        d = self.to_device(keys)
+        self._check(self.eng.gdl_mask_columns_u64(self.h, d.ptr, d.n, col_mask, d.ptr),
+                    "mask_columns")
+        r = d.numpy()
+        d.free()
+        return r
+
     def project(self, keys, colmap):
         d = self.to_device(keys)
         o = self.alloc(d.n)
@@ -544,7 +657,7 @@ class Context:
             raise GdlError(st, msg)
         r = dict(iterations=res.iterations, init_iterations=res.init_iterations,
                  derived=res.derived, n_out=res.n_out, sizes=list(res.sizes),
-                 load_ms=res.load_ms, fixpoint_ms=res.fixpoint_ms)
+                 load_ms=res.load_ms, fixpoint_ms=res.fixpoint_ms, checksum=res.checksum)
         if want_keys:
             r["out"] = keep[0][:res.n_out].copy()
             r["extra"] = [keep[1][0][:res.sizes[0]].copy() if extra_caps[0] else None,
@@ -600,3 +713,225 @@ class Context:
             for k in (ka, kd):
                 if isinstance(k, DeviceArray):
                     k.free()
+
+
+# ------------------------------------------------------------ batched TC
+
+def fmix64(keys):
+    """MurmurHash3 fmix64 of u64 keys (numpy, wrapping)."""
+    k = np.asarray(keys, dtype=np.uint64).copy()
+    with np.errstate(over="ignore"):
+        k ^= k >> np.uint64(33)
+        k *= np.uint64(0xff51afd7ed558ccd)
+        k ^= k >> np.uint64(33)
+        k *= np.uint64(0xc4ceb9fe1a85ec53)
+        k ^= k >> np.uint64(33)
+    return k
+
+
+def tuple_checksum(keys):
+    """Order-independent checksum of a tuple set: sum of fmix64, mod 2^64."""
+    with np.errstate(over="ignore"):
+        return int(fmix64(keys).sum(dtype=np.uint64))
+
+
+def tc_estimate(edges, n_nodes, samples=32, seed=0):
+    """Estimated |TC| (breadth-first search from sampled sources; exact when
+    samples >= n_nodes).  Host code of libgdl_hip.so; no device needed."""
+    eng, _ = load()
+    e = np.ascontiguousarray(edges, dtype=np.uint64)
+    return int(eng.gdl_tc_estimate(e.ctypes.data, e.size, n_nodes, samples, seed))
+
+
+class TcBatch:
+    """Many TC fixpoints in one disjoint union (include/gdl_batch.h)."""
+
+    def __init__(self, ctx, max_slots=256, node_bits=16, mem_budget=0, keep_keys=True):
+        self.ctx = ctx
+        cfg = TcbConfig()
+        ctx.eng.gdl_tcb_config_default(ctypes.byref(cfg))
+        cfg.max_slots, cfg.node_bits, cfg.mem_budget = max_slots, node_bits, mem_budget
+        cfg.keep_keys = int(keep_keys)
+        h = c_void_p()
+        ctx._check(ctx.eng.gdl_tcb_create(ctx.h, ctypes.byref(cfg), ctypes.byref(h)),
+                   "tcb_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.ctx.eng.gdl_tcb_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def admit(self, qid, edges, n_nodes, est=0):
+        """True if admitted, False if the batch is busy (no slot or budget)."""
+        e = np.ascontiguousarray(edges, dtype=np.uint64)
+        st = self.ctx.eng.gdl_tcb_admit(self.h, qid, e.ctypes.data, e.size, n_nodes, est)
+        if st == GDL_ERR_BUSY:
+            return False
+        self.ctx._check(st, "tcb_admit")
+        return True
+
+    def step(self):
+        n = c_uint32()
+        self.ctx._check(self.ctx.eng.gdl_tcb_step(self.h, ctypes.byref(n)), "tcb_step")
+        return n.value
+
+    def poll(self):
+        """Finished results (dicts) in completion order, fetched and dropped."""
+        out = []
+        r = TcbResult()
+        while self.ctx.eng.gdl_tcb_poll(self.h, ctypes.byref(r)):
+            keys = np.zeros(r.n_out if r.has_keys else 0, dtype=np.uint64)
+            tr = (TraceRec * max(int(r.n_trace), 1))()
+            self.ctx._check(self.ctx.eng.gdl_tcb_fetch(
+                self.h, r.qid, keys.ctypes.data if keys.size else None, tr), "tcb_fetch")
+            out.append(dict(qid=r.qid, n_out=r.n_out, checksum=r.checksum, derived=r.derived,
+                            iterations=r.iterations, slot=r.slot, admit_ms=r.admit_ms,
+                            done_ms=r.done_ms, keys=keys if r.has_keys else None,
+                            trace=[(tr[i].iteration, tr[i].relation, tr[i].newt, tr[i].delta,
+                                    tr[i].full) for i in range(r.n_trace)]))
+        return out
+
+    def info(self):
+        v = [c_uint64() for _ in range(5)]
+        self.ctx._check(self.ctx.eng.gdl_tcb_info(self.h, *[ctypes.byref(x) for x in v]),
+                        "tcb_info")
+        return dict(zip(("bodies", "admitted", "finished", "reserved", "budget"),
+                        [x.value for x in v]))
+
+    def run(self, instances, est=None):
+        """Runs (qid, edges, n_nodes) instances to completion, admitting in
+        order as slots and budget allow; returns results by qid."""
+        pending = list(instances)
+        done = {}
+        while pending or len(done) < len(instances):
+            while pending:
+                qid, edges, n_nodes = pending[0]
+                if not self.admit(qid, edges, n_nodes, 0 if est is None else est.get(qid, 0)):
+                    break
+                pending.pop(0)
+            self.step()
+            for r in self.poll():
+                done[r["qid"]] = r
+        return done
+
+
+# ------------------------------------------------------------------ serving
+
+QUERY_TC, QUERY_SG, QUERY_CSPA = 0, 1, 2
+
+
+class ServerConfig(Structure):
+    _fields_ = [("device", c_int), ("workers", c_int), ("batch_tc", c_int),
+                ("batch_slots", c_uint32), ("node_bits", c_uint32), ("mem_budget", c_uint64),
+                ("keep_keys", c_int)]
+
+
+class QueryResult(Structure):
+    _fields_ = [("qid", c_uint64), ("kind", c_int32), ("status", c_int32), ("n_out", c_uint64),
+                ("checksum", c_uint64), ("derived", c_uint64), ("iterations", c_int32),
+                ("init_iterations", c_int32), ("est_bytes", c_uint64), ("submit_ms", c_double),
+                ("admit_ms", c_double), ("done_ms", c_double), ("batched", c_int32),
+                ("rerun_alone", c_int32), ("n_trace", c_uint64), ("n_keys", c_uint64)]
+
+
+def _serve_lib():
+    _, prog = load()
+    if not hasattr(prog, "_serve_declared"):
+        vp = c_void_p
+        _decl(prog, "gdl_server_config_default", None, [POINTER(ServerConfig)])
+        _decl(prog, "gdl_server_create", c_int, [POINTER(ServerConfig), POINTER(vp)])
+        _decl(prog, "gdl_server_destroy", c_int, [vp])
+        _decl(prog, "gdl_query_estimate", c_uint64, [c_int, vp, c_uint64, vp, c_uint64])
+        _decl(prog, "gdl_server_submit", c_int, [vp, c_uint64, c_int, vp, c_uint64, vp, c_uint64])
+        _decl(prog, "gdl_server_poll", c_int, [vp, POINTER(QueryResult)])
+        _decl(prog, "gdl_server_wait", c_int, [vp, POINTER(QueryResult), c_double])
+        _decl(prog, "gdl_server_fetch", c_int, [vp, c_uint64, vp, vp])
+        _decl(prog, "gdl_server_stats", c_int, [vp] + [_U64P] * 7)
+        prog._serve_declared = True
+    return prog
+
+
+def query_estimate(kind, a, b=None):
+    """Admission estimate (device bytes) of a query; host code, no device needed."""
+    prog = _serve_lib()
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    b = np.ascontiguousarray(b if b is not None else np.zeros(0), dtype=np.uint64)
+    return int(prog.gdl_query_estimate(kind, a.ctypes.data if a.size else None, a.size,
+                                       b.ctypes.data if b.size else None, b.size))
+
+
+class Server:
+    """Query server of include/gdl_serve.h: TC / SG / CSPA queries admitted by
+    footprint estimate, TC queries continuously batched, the others on
+    concurrent worker contexts."""
+
+    def __init__(self, device=0, workers=3, batch_tc=True, batch_slots=64, node_bits=18,
+                 mem_budget=0, keep_keys=False):
+        self.prog = _serve_lib()
+        cfg = ServerConfig()
+        self.prog.gdl_server_config_default(ctypes.byref(cfg))
+        cfg.device, cfg.workers, cfg.batch_tc = device, workers, int(batch_tc)
+        cfg.batch_slots, cfg.node_bits, cfg.mem_budget = batch_slots, node_bits, mem_budget
+        cfg.keep_keys = int(keep_keys)
+        h = c_void_p()
+        st = self.prog.gdl_server_create(ctypes.byref(cfg), ctypes.byref(h))
+        if st != GDL_OK:
+            raise GdlError(st, "gdl_server_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.prog.gdl_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, qid, kind, a, b=None):
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        b = np.ascontiguousarray(b if b is not None else np.zeros(0), dtype=np.uint64)
+        st = self.prog.gdl_server_submit(self.h, qid, kind, a.ctypes.data if a.size else None,
+                                         a.size, b.ctypes.data if b.size else None, b.size)
+        if st != GDL_OK:
+            raise GdlError(st, "gdl_server_submit")
+
+    def wait(self, timeout_ms=-1.0):
+        """Next finished query as a dict (keys / trace fetched), or None on timeout."""
+        r = QueryResult()
+        if not self.prog.gdl_server_wait(self.h, ctypes.byref(r), float(timeout_ms)):
+            return None
+        keys = np.zeros(r.n_keys, dtype=np.uint64)
+        tr = (TraceRec * max(int(r.n_trace), 1))()
+        st = self.prog.gdl_server_fetch(self.h, r.qid, keys.ctypes.data if keys.size else None,
+                                        tr)
+        if st != GDL_OK:
+            raise GdlError(st, "gdl_server_fetch")
+        return dict(qid=r.qid, kind=r.kind, status=r.status, n_out=r.n_out,
+                    checksum=r.checksum, derived=r.derived, iterations=r.iterations,
+                    init_iterations=r.init_iterations, est_bytes=r.est_bytes,
+                    submit_ms=r.submit_ms, admit_ms=r.admit_ms, done_ms=r.done_ms,
+                    batched=bool(r.batched), rerun_alone=bool(r.rerun_alone),
+                    keys=keys if r.n_keys else None,
+                    trace=[(tr[i].iteration, tr[i].relation, tr[i].newt, tr[i].delta,
+                            tr[i].full) for i in range(r.n_trace)])
+
+    def poll(self):
+        return self.wait(0.0)
+
+    def stats(self):
+        v = [c_uint64() for _ in range(7)]
+        st = self.prog.gdl_server_stats(self.h, *[ctypes.byref(x) for x in v])
+        if st != GDL_OK:
+            raise GdlError(st, "gdl_server_stats")
+        return dict(zip(("submitted", "finished", "batched", "reruns", "reserved",
+                         "peak_reserved", "budget"), [x.value for x in v]))

@@ -13,7 +13,7 @@ const char *kKernelNames[K_NUM] = {
     "scan_reduce",  "scan_top",     "scan_down",    "compact_count", "compact_write",
     "merge_count",  "merge_write",  "idx_count",    "idx_build",     "idx_lookup",
     "join_count",   "join_materialize", "project",  "fill",          "hs_insert_new",
-    "hs_rehash",    "join_insert"};
+    "hs_rehash",    "join_insert",  "hs_extract",   "seg_hist",      "mask_cols"};
 
 // ---------------------------------------------------------------- allocator
 
@@ -210,6 +210,7 @@ const char *gdl_status_string(gdl_status s) {
         case GDL_ERR_UNSUPPORTED: return "unsupported";
         case GDL_ERR_OVERFLOW: return "size limit exceeded";
         case GDL_ERR_DEVICE: return "device-side error";
+        case GDL_ERR_BUSY: return "busy: no free slot or memory budget, retry later";
     }
     return "unknown status";
 }
@@ -231,6 +232,7 @@ gdl_status gdl_ctx_create(const gdl_config *cfg_in, gdl_ctx **out) {
     gdl_ctx *ctx = new gdl_ctx();
     ctx->device = cfg.device;
     ctx->cfg = cfg;
+    ctx->tune.load_factor = cfg.load_factor;
     ctx->stats_on = cfg.collect_stats != 0;
     ctx->alloc = new Allocator(cfg.device);
     ctx->alloc->set_cap(cfg.mem_cap_bytes);
@@ -280,6 +282,28 @@ int gdl_device(gdl_ctx *ctx) { return ctx ? ctx->device : -1; }
 gdl_status gdl_set_load_factor(gdl_ctx *ctx, float lf) {
     if (!ctx || !(lf > 0.f && lf <= 0.95f)) return GDL_ERR_INVALID_ARG;
     ctx->cfg.load_factor = lf;
+    ctx->tune.load_factor = lf;
+    return GDL_OK;
+}
+
+gdl_status gdl_get_tuning(gdl_ctx *ctx, gdl_tuning *out) {
+    if (!ctx || !out) return GDL_ERR_INVALID_ARG;
+    *out = ctx->tune;
+    out->load_factor = ctx->cfg.load_factor;
+    return GDL_OK;
+}
+
+gdl_status gdl_set_tuning(gdl_ctx *ctx, const gdl_tuning *t) {
+    if (!ctx || !t) return GDL_ERR_INVALID_ARG;
+    const bool block_ok = t->join_block == 64 || t->join_block == 128 || t->join_block == 256 ||
+                          t->join_block == 512;
+    if (!(t->load_factor > 0.f && t->load_factor <= 0.95f) || t->blocks_per_cu < 1 ||
+        t->blocks_per_cu > 32 || t->radix_bits < 4 || t->radix_bits > 8 || !block_ok)
+        return fail(ctx, GDL_ERR_INVALID_ARG,
+                    "tuning: load factor (0, 0.95], blocks per CU 1..32, radix bits 4..8, "
+                    "join block 64/128/256/512");
+    ctx->tune = *t;
+    ctx->cfg.load_factor = t->load_factor;
     return GDL_OK;
 }
 

@@ -72,6 +72,9 @@ enum KernelId {
     K_HS_INSERT,
     K_HS_REHASH,
     K_JOIN_INSERT,
+    K_HS_EXTRACT,
+    K_SEG_HIST,
+    K_MASK_COLS,
     K_NUM
 };
 extern const char *kKernelNames[K_NUM];
@@ -97,6 +100,7 @@ struct gdl_ctx {
     uint64_t *h_pinned = nullptr;  // pinned staging for scalars read back
     uint32_t *d_err = nullptr;     // kernel-side error word
     int cu_count = 0;
+    gdl_tuning tune{0.5f, 8, 8, 256};  // load_factor mirrors cfg.load_factor
     std::string last_error;
     bool stats_on = false;
     gdl::KernelStat stats[gdl::K_NUM];
@@ -105,6 +109,11 @@ struct gdl_ctx {
 };
 
 namespace gdl {
+
+// Grid cap of a grid-stride kernel: the tuned blocks per CU (x mult).
+inline uint64_t grid_cap(const gdl_ctx *ctx, uint64_t mult = 1) {
+    return (uint64_t)(ctx->cu_count > 0 ? ctx->cu_count : 256) * ctx->tune.blocks_per_cu * mult;
+}
 
 // Records the status + message on the context and returns the status.
 gdl_status fail(gdl_ctx *ctx, gdl_status st, const std::string &msg);
@@ -178,6 +187,13 @@ gdl_status hashset_insert_new(gdl_ctx *ctx, gdl_hashset *hs, const u64 *keys, u6
                               u64 *n_new);
 // Grows hs so that `incoming` more keys keep its load at or below 1/2.
 gdl_status hashset_reserve(gdl_ctx *ctx, gdl_hashset *hs, u64 incoming);
+constexpr double kHsHardLoad = 0.9;
+gdl_status hashset_reserve2(gdl_ctx *ctx, gdl_hashset *hs, u64 expected, u64 worst);
+gdl_status index_build(gdl_ctx *ctx, const u64 *keys, u64 n, int prefix_cols, gdl_index *out);
+gdl_status join_count(gdl_ctx *ctx, const u64 *inner, u64 n_inner, const gdl_index *idx,
+                      const u64 *outer, u64 n_outer, u32 *counts, u64 *begins);
+// Per-instance hash sets of a segmented fused join (one per batch slot).
+constexpr u32 kMaxSegs = 256;
 // Exclusive scans of two arrays in place in one launch (v[n] = total each).
 gdl_status scan_inplace2(gdl_ctx *ctx, u64 *v0, u64 *v1, u64 n, u64 *totals /* 2 */);
 

@@ -1,12 +1,15 @@
 // hashset.hip -- device hash set of tuple keys: the membership side of an
 // append-only FULL version.
 //
-// Open addressing, linear probing, power-of-two capacity kept at load <= 1/2
-// (grown by rehashing before an insert would pass it).  EMPTY = ~0; the key
+// Open addressing, linear probing, power-of-two capacity kept at load <= the
+// context's load factor, at most 0.8 (grown by rehashing before an insert
+// would pass it).  EMPTY = ~0; the key
 // ~0 itself (ids 2^32 - 1 in both columns) is tracked by a flag word after
 // the slots.  insert_new inserts a batch of keys (any order, repeats allowed)
 // and emits each key that was not yet present exactly once: the thread whose
 // CAS claims the slot emits it, wave-aggregated through one atomic counter.
+#include <cmath>
+
 #include "gdl_internal.hpp"
 
 namespace gdl {
@@ -49,13 +52,42 @@ __global__ __launch_bounds__(HB) void k_hs_rehash(const u64 *old_slots, u64 old_
     }
 }
 
+// Compacts the members of a table (the slots of cap words plus the ~0 flag
+// word after them) into out, one atomic per wave.
+__global__ __launch_bounds__(HB) void k_hs_extract(const u64 *slots, u64 cap, u64 *out,
+                                                   u64 out_cap, unsigned long long *counter) {
+    const u64 lt = lane_mask_lt();
+    for (u64 base = (u64)blockIdx.x * HB; base <= cap; base += (u64)gridDim.x * HB) {
+        const u64 i = base + threadIdx.x;
+        bool keep = false;
+        u64 k = 0;
+        if (i < cap) {
+            k = slots[i];
+            keep = k != kEmpty;
+        } else if (i == cap) {
+            k = kEmpty;
+            keep = slots[cap] != 0ull;
+        }
+        const u64 bal = __ballot(keep);
+        if (bal) {
+            unsigned long long pos = 0;
+            const int leader = __ffsll((unsigned long long)bal) - 1;
+            if ((int)(threadIdx.x & 63) == leader)
+                pos = atomicAdd(counter, (unsigned long long)__popcll(bal));
+            pos = __shfl(pos, leader, 64);
+            const u64 at = pos + __popcll(bal & lt);
+            if (keep && at < out_cap) out[at] = k;
+        }
+    }
+}
+
 __global__ __launch_bounds__(HB) void k_hs_fill(u64 *slots, u64 cap) {
     for (u64 i = (u64)blockIdx.x * HB + threadIdx.x; i < cap; i += (u64)gridDim.x * HB)
         slots[i] = i < cap - 1 ? kEmpty : 0ull;  // last word: the ~0 flag (cleared)
 }
 
 static unsigned hs_grid(gdl_ctx *ctx, u64 n) {
-    const u64 cap = (u64)(ctx->cu_count > 0 ? ctx->cu_count : 256) * 16;
+    const u64 cap = grid_cap(ctx, 2);
     u64 g = (n + HB - 1) / HB;
     return (unsigned)std::max<u64>(1, std::min(g, cap));
 }
@@ -83,9 +115,22 @@ static u64 pow2_at_least(u64 x) {
     return c;
 }
 
-// Grows the table so that size + incoming stays at load <= 1/2.
+// Grows the table so that size + incoming stays at load <= the context's
+// load factor (default 1/2).
 gdl_status hashset_reserve(gdl_ctx *ctx, gdl_hashset *hs, u64 incoming) {
-    const u64 need = pow2_at_least(2 * (hs->size + incoming));
+    return hashset_reserve2(ctx, hs, incoming, incoming);
+}
+
+// Grows the table so that size + expected new members stay at load <= the
+// load factor and size + worst (an upper bound of the new members) at load
+// <= kHsHardLoad: every insert still finds a free slot when the worst case
+// happens, and the table is not sized by a loose bound.
+gdl_status hashset_reserve2(gdl_ctx *ctx, gdl_hashset *hs, u64 expected, u64 worst) {
+    const double lf = std::min(0.8, ctx->cfg.load_factor > 0.f ? (double)ctx->cfg.load_factor : 0.5);
+    expected = std::min(expected, worst);
+    const u64 need = pow2_at_least(
+        std::max((u64)std::ceil((double)(hs->size + expected) / lf),
+                 (u64)std::ceil((double)(hs->size + worst) / kHsHardLoad) + 1));
     if (hs->slots && hs->capacity >= need) return GDL_OK;
     u64 *ns = nullptr;
     GDL_TRY(hs_alloc(ctx, need, &ns));
@@ -156,6 +201,35 @@ gdl_status gdl_hashset_insert_new(gdl_ctx *ctx, gdl_hashset *hs, const uint64_t 
                                   uint64_t n, uint64_t *d_new, uint64_t *n_new) {
     if (!ctx || !hs || !n_new || (n && (!d_keys || !d_new))) return GDL_ERR_INVALID_ARG;
     return hashset_insert_new(ctx, hs, d_keys, n, d_new, n_new);
+}
+
+gdl_status gdl_hashset_reserve(gdl_ctx *ctx, gdl_hashset *hs, uint64_t incoming) {
+    if (!ctx || !hs) return GDL_ERR_INVALID_ARG;
+    return hashset_reserve(ctx, hs, incoming);
+}
+
+gdl_status gdl_hashset_extract(gdl_ctx *ctx, const gdl_hashset *hs, uint64_t *d_out,
+                               uint64_t out_cap, uint64_t *n_out) {
+    if (!ctx || !hs || !n_out || (out_cap && !d_out)) return GDL_ERR_INVALID_ARG;
+    *n_out = 0;
+    if (!hs->slots || hs->capacity == 0) return GDL_OK;
+    Scratch scr(ctx);
+    u64 *counter;
+    GDL_TRY(scr.get(1, &counter));
+    GDL_HIP(ctx, hipMemsetAsync(counter, 0, sizeof(u64), ctx->stream));
+    {
+        KTimer kt(ctx, K_HS_EXTRACT, 8.0 * hs->capacity + 8.0 * hs->size);
+        hipLaunchKernelGGL(k_hs_extract, dim3(hs_grid(ctx, hs->capacity + 1)), dim3(HB), 0,
+                           ctx->stream, (const u64 *)hs->slots, hs->capacity, d_out, out_cap,
+                           (unsigned long long *)counter);
+        GDL_TRY(check_launch(ctx, "k_hs_extract"));
+    }
+    GDL_TRY(read_u64(ctx, counter, n_out));
+    if (*n_out > out_cap)
+        return fail(ctx, GDL_ERR_OVERFLOW, "hash set extract: " + std::to_string(*n_out) +
+                                               " members, output holds " +
+                                               std::to_string(out_cap));
+    return GDL_OK;
 }
 
 }  // extern "C"

@@ -165,26 +165,67 @@ __global__ __launch_bounds__(JB) void k_join_materialize(const u64 *inner, const
 // keys are staged per wave in LDS and flushed with one atomic per kStage keys
 // (counters[0] is the output cursor); counters[1] counts produced tuples.
 constexpr u32 kStage = 256;
+constexpr u64 kHubRun = 256;  // longer inner runs use the load-balanced variant
 
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
 }
 
-__device__ __forceinline__ void flush_stage(const u64 *stage, u32 n, u64 *out,
+// Writes stay below out_cap; a cursor past it tells the host the caller's
+// output bound was wrong.
+__device__ __forceinline__ void flush_stage(const u64 *stage, u32 n, u64 *out, u64 out_cap,
                                             unsigned long long *cursor, int lane) {
     unsigned long long pos = 0;
     if (lane == 0) pos = atomicAdd(cursor, (unsigned long long)n);
     pos = __shfl(pos, 0, 64);
-    for (u32 x = lane; x < n; x += 64) out[pos + x] = stage[x];
+    for (u32 x = lane; x < n; x += 64)
+        if (pos + x < out_cap) out[pos + x] = stage[x];
 }
 
-__global__ __launch_bounds__(JB) void k_join_insert(
+// Target of a fused join's inserts: one hash set, or one set per fixpoint
+// instance of a batch, chosen by the key's top bits (key >> seg_shift).
+struct HsSel {
+    u64 *slots;
+    u64 mask;
+    const gdl_hs_seg *segs;
+    u32 n_segs;
+    int seg_shift;
+};
+
+// The set a key belongs to (slots == nullptr: a segment with no set).
+template <bool SEG>
+__device__ __forceinline__ gdl_hs_seg sel_set(const HsSel &h, const gdl_hs_seg *s_segs, u64 key) {
+    if constexpr (SEG) {
+        const u64 s = key >> h.seg_shift;
+        if (s >= h.n_segs) return gdl_hs_seg{nullptr, 0};
+        return s_segs[s];
+    } else {
+        (void)s_segs;
+        return gdl_hs_seg{h.slots, h.mask};
+    }
+}
+
+template <bool SEG>
+__device__ __forceinline__ void load_segs(const HsSel &h, gdl_hs_seg *s_segs) {
+    if constexpr (SEG) {
+        for (u32 i = threadIdx.x; i < h.n_segs; i += blockDim.x) s_segs[i] = h.segs[i];
+        __syncthreads();
+    } else {
+        (void)h;
+        (void)s_segs;
+    }
+}
+
+template <int B, bool SEG>
+__global__ __launch_bounds__(B) void k_join_insert(
     const IdxSlot *slots, u64 n_slots, const u64 *inner, u64 n_inner, int shift, const u64 *outer,
-    u64 n_outer, int arity, int s0, int s1, int pop, int plhs, int prhs, u64 *hs_slots, u64 hs_mask,
-    u64 *out, unsigned long long *counters) {
-    __shared__ u64 s_excl[JB], s_b[JB], s_ok[JB];
-    __shared__ u64 s_stage[JB / 64][kStage];
+    u64 n_outer, int arity, int s0, int s1, int pop, int plhs, int prhs, HsSel sel, u64 *out,
+    u64 out_cap, unsigned long long *counters) {
+    __shared__ u64 s_excl[B], s_b[B], s_ok[B];
+    __shared__ u64 s_stage[B / 64][kStage];
+    __shared__ gdl_hs_seg s_segs[SEG ? kMaxSegs : 1];
+    load_segs<SEG>(sel, s_segs);
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const u64 lt = lane_mask_lt();
@@ -192,7 +233,7 @@ __global__ __launch_bounds__(JB) void k_join_insert(
     u64 *stage = s_stage[w];
     u32 staged = 0;  // wave-uniform
     u64 produced = 0;
-    for (u64 base = (u64)blockIdx.x * JB; base < n_outer; base += (u64)gridDim.x * JB) {
+    for (u64 base = (u64)blockIdx.x * B; base < n_outer; base += (u64)gridDim.x * B) {
         const u64 i = base + threadIdx.x;
         u64 b = 0, c = 0, ok = 0;
         if (i < n_outer) {
@@ -221,14 +262,15 @@ __global__ __launch_bounds__(JB) void k_join_insert(
                 key = apply_map(inner[s_b[w * 64 + lo] + j], s_ok[w * 64 + lo], arity, s0, s1);
                 if (eval_pred(key, pop, plhs, prhs)) {
                     ++produced;
-                    fresh = hs_insert_key(hs_slots, hs_mask, key);
+                    const gdl_hs_seg hs = sel_set<SEG>(sel, s_segs, key);
+                    if (hs.slots) fresh = hs_insert_key((u64 *)hs.slots, hs.mask, key);
                 }
             }
             const u64 bal = __ballot(fresh);
             const u32 cnt = (u32)__popcll(bal);
             if (staged + cnt > kStage) {
                 wave_sync_lds();
-                flush_stage(stage, staged, out, &counters[0], lane);
+                flush_stage(stage, staged, out, out_cap, &counters[0], lane);
                 staged = 0;
                 wave_sync_lds();
             }
@@ -239,7 +281,102 @@ __global__ __launch_bounds__(JB) void k_join_insert(
     }
     if (staged) {
         wave_sync_lds();
-        flush_stage(stage, staged, out, &counters[0], lane);
+        flush_stage(stage, staged, out, out_cap, &counters[0], lane);
+    }
+    for (int d = 32; d >= 1; d >>= 1) produced += __shfl_xor(produced, d, 64);
+    if (lane == 0 && produced) atomicAdd(&counters[1], (unsigned long long)produced);
+}
+
+// Load-balanced variant for inner relations with long runs: outputs are
+// numbered by the exclusive scan of run lengths (offsets), each thread takes
+// JM_I consecutive output positions of its block's tile, computes their keys
+// by walking the offsets, then issues the home-slot CASes of all of them
+// back to back before resolving collisions.
+template <bool SEG>
+__global__ __launch_bounds__(JB) void k_join_insert_lb(
+    const u64 *inner, const u64 *outer, u64 n_outer, const u64 *offsets, const u64 *begins,
+    u64 total, int arity, int s0, int s1, int pop, int plhs, int prhs, HsSel sel, u64 *out,
+    u64 out_cap, unsigned long long *counters) {
+    __shared__ u64 s_stage[JB / 64][kStage];
+    __shared__ gdl_hs_seg s_segs[SEG ? kMaxSegs : 1];
+    load_segs<SEG>(sel, s_segs);
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const u64 lt = lane_mask_lt();
+    u64 *stage = s_stage[w];
+    u32 staged = 0;
+    u64 produced = 0;
+    for (u64 tile0 = (u64)blockIdx.x * JM_TILE; tile0 < total; tile0 += (u64)gridDim.x * JM_TILE) {
+        const u64 p0 = tile0 + (u64)threadIdx.x * JM_I;
+        u64 keys[JM_I];
+        u32 live = 0;  // bit r: output r exists and passes the predicate
+        if (p0 < total) {
+            u64 lo = 0, hi = n_outer - 1;
+            while (lo < hi) {
+                const u64 mid = (lo + hi + 1) >> 1;
+                if (offsets[mid] <= p0) lo = mid;
+                else hi = mid - 1;
+            }
+            u64 o = lo, ofs = offsets[o], next = offsets[o + 1], ok = outer[o], b = begins[o];
+#pragma unroll
+            for (int r = 0; r < JM_I; ++r) {
+                const u64 p = p0 + r;
+                keys[r] = 0;
+                if (p >= total) continue;
+                while (next <= p && o + 1 < n_outer) {
+                    ++o;
+                    ofs = next;
+                    next = offsets[o + 1];
+                    ok = outer[o];
+                    b = begins[o];
+                }
+                keys[r] = apply_map(inner[b + (p - ofs)], ok, arity, s0, s1);
+                if (eval_pred(keys[r], pop, plhs, prhs)) live |= 1u << r;
+            }
+        }
+        produced += __popc(live);
+        // home-slot CAS of every key first (independent round trips in flight)
+        unsigned long long old[JM_I];
+#pragma unroll
+        for (int r = 0; r < JM_I; ++r) {
+            old[r] = 0;
+            if ((live >> r) & 1u) {
+                const gdl_hs_seg hs = sel_set<SEG>(sel, s_segs, keys[r]);
+                if (!hs.slots) live &= ~(1u << r);
+                else if (keys[r] != kHsEmpty)
+                    old[r] = atomicCAS(
+                        (unsigned long long *)((u64 *)hs.slots + (mix64(keys[r]) & hs.mask)),
+                        (unsigned long long)kHsEmpty, (unsigned long long)keys[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < JM_I; ++r) {
+            bool fresh = false;
+            if ((live >> r) & 1u) {
+                if (keys[r] == kHsEmpty || (old[r] != (unsigned long long)kHsEmpty &&
+                                            old[r] != (unsigned long long)keys[r])) {
+                    // collision (or the key ~0): the full probe
+                    const gdl_hs_seg hs = sel_set<SEG>(sel, s_segs, keys[r]);
+                    fresh = hs_insert_key((u64 *)hs.slots, hs.mask, keys[r]);
+                } else {
+                    fresh = old[r] == (unsigned long long)kHsEmpty;
+                }
+            }
+            const u64 bal = __ballot(fresh);
+            const u32 cnt = (u32)__popcll(bal);
+            if (staged + cnt > kStage) {
+                wave_sync_lds();
+                flush_stage(stage, staged, out, out_cap, &counters[0], lane);
+                staged = 0;
+                wave_sync_lds();
+            }
+            if (fresh) stage[staged + __popcll(bal & lt)] = keys[r];
+            staged += cnt;
+        }
+    }
+    if (staged) {
+        wave_sync_lds();
+        flush_stage(stage, staged, out, out_cap, &counters[0], lane);
     }
     for (int d = 32; d >= 1; d >>= 1) produced += __shfl_xor(produced, d, 64);
     if (lane == 0 && produced) atomicAdd(&counters[1], (unsigned long long)produced);
@@ -259,7 +396,7 @@ __global__ __launch_bounds__(JB) void k_project(const u64 *in, u64 n, int arity,
 }
 
 static unsigned stride_grid(gdl_ctx *ctx, u64 n) {
-    const u64 cap = (u64)(ctx->cu_count > 0 ? ctx->cu_count : 256) * 8;
+    const u64 cap = grid_cap(ctx);
     u64 g = (n + JB - 1) / JB;
     if (g > cap) g = cap;
     return (unsigned)(g ? g : 1);
@@ -466,6 +603,124 @@ gdl_status gdl_join(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n_inner,
     return GDL_OK;
 }
 
+}  // extern "C"
+
+// Fused join + insert.  SEG = false: into one set `hs`, growing it and the
+// output (*d_new, *new_cap) to the bound.  SEG = true: into the per-instance
+// sets of `sel`, which the caller has sized, and into the caller's output of
+// *new_cap keys (a fixed buffer; an overrun is reported, never written).
+template <bool SEG>
+static gdl_status join_insert_impl(gdl_ctx *ctx, const u64 *d_inner, u64 n_inner,
+                                   const gdl_index *inner_idx, const u64 *d_outer, u64 n_outer,
+                                   gdl_colmap map, gdl_pred pred, gdl_hashset *hs, HsSel sel,
+                                   u64 **d_new, u64 *new_cap, u64 *n_new, u64 *n_raw) {
+    *n_raw = 0;
+    if (n_outer == 0 || n_inner == 0 || inner_idx->n_slots == 0) return GDL_OK;
+    const int shift = inner_idx->prefix_cols >= 2 ? 0 : 32;
+    // upper bound of produced tuples: |outer| x longest inner run, or an
+    // exact count when that bound is loose
+    const u64 mr = inner_idx->max_run ? inner_idx->max_run : n_inner;
+    u64 bound = (n_outer > ~0ull / mr) ? ~0ull : n_outer * mr;
+    const u64 loose_limit = std::max<u64>(1ull << 24, 4 * n_outer);
+    const u64 cap_blocks = grid_cap(ctx);
+    const int fb = (int)ctx->tune.join_block;  // block of the flattened kernel
+    const bool counted = bound > loose_limit;
+    Scratch lb(ctx);
+    u32 *counts = nullptr;
+    u64 *begins = nullptr, *offsets = nullptr;
+    if (counted) {
+        GDL_TRY(lb.get(n_outer, &counts));
+        GDL_TRY(lb.get(n_outer, &begins));
+        GDL_TRY(lb.get(n_outer + 1, &offsets));
+        GDL_TRY(join_count(ctx, d_inner, n_inner, inner_idx, d_outer, n_outer, counts, begins));
+        GDL_TRY(exclusive_scan_u32_to_u64(ctx, counts, n_outer, offsets, &bound));
+    }
+    if (bound == 0) return GDL_OK;
+    // The wave-flattened kernel is balanced inside a wave but not across
+    // waves: a wave whose 64 outer tuples hit a hub walks the whole hub run.
+    // Number the outputs and spread them over the grid instead when one run
+    // is long against the expected work per wave.
+    const u64 waves = std::min<u64>((n_outer + fb - 1) / fb, cap_blocks) * (fb / 64);
+    const u64 avg_run = std::max<u64>(1, n_inner / std::max<u64>(1, inner_idx->n_distinct));
+    const u64 est_total = counted ? bound : n_outer * avg_run;
+    const bool balanced = mr > kHubRun && mr * 4 > est_total / waves;
+    if (balanced && !counted) {
+        GDL_TRY(lb.get(n_outer, &counts));
+        GDL_TRY(lb.get(n_outer, &begins));
+        GDL_TRY(lb.get(n_outer + 1, &offsets));
+        GDL_TRY(join_count(ctx, d_inner, n_inner, inner_idx, d_outer, n_outer, counts, begins));
+        GDL_TRY(exclusive_scan_u32_to_u64(ctx, counts, n_outer, offsets, &bound));
+        if (bound == 0) return GDL_OK;
+    }
+    if constexpr (!SEG) {
+        // size for about |outer| new members; the bound only has to fit
+        GDL_TRY(hashset_reserve2(ctx, hs, std::max<u64>(n_outer, 1024), bound));
+        sel.slots = (u64 *)hs->slots;
+        sel.mask = hs->capacity - 1;
+        if (*n_new + bound > *new_cap || !*d_new) {
+            const u64 cap = *n_new + bound + (*n_new + bound) / 4 + 1024;
+            void *p = nullptr;
+            std::string err;
+            gdl_status st = ctx->alloc->alloc(cap * sizeof(u64), &p, &err);
+            if (st != GDL_OK) return fail(ctx, st, err);
+            if (*n_new)
+                GDL_HIP(ctx, hipMemcpyAsync(p, *d_new, *n_new * sizeof(u64),
+                                            hipMemcpyDeviceToDevice, ctx->stream));
+            if (*d_new) ctx->alloc->free(*d_new, nullptr);
+            *d_new = (u64 *)p;
+            *new_cap = cap;
+        }
+    }
+    Scratch scr(ctx);
+    u64 *counters;
+    GDL_TRY(scr.get(2, &counters));
+    hipLaunchKernelGGL(k_set2, dim3(1), dim3(1), 0, ctx->stream, counters, (u64)*n_new, (u64)0);
+    GDL_TRY(check_launch(ctx, "k_set2"));
+    if (balanced) {
+        KTimer kt(ctx, K_JOIN_INSERT, 24.0 * n_outer);
+        const unsigned g = (unsigned)std::max<u64>(
+            1, std::min<u64>((bound + JM_TILE - 1) / JM_TILE, cap_blocks));
+        hipLaunchKernelGGL(k_join_insert_lb<SEG>, dim3(g), dim3(JB), 0, ctx->stream, d_inner,
+                           d_outer, n_outer, offsets, begins, bound, map.arity, map.src[0],
+                           map.arity == 2 ? map.src[1] : 0, pred.op, pred.lhs, pred.rhs, sel,
+                           *d_new, *new_cap, (unsigned long long *)counters);
+        GDL_TRY(check_launch(ctx, "k_join_insert_lb"));
+    } else {
+        KTimer kt(ctx, K_JOIN_INSERT, 8.0 * n_outer + 64.0 * n_outer);
+        const unsigned g = (unsigned)std::max<u64>(1, std::min<u64>((n_outer + fb - 1) / fb,
+                                                                    cap_blocks));
+        auto launch = [&](auto kern, int block) {
+            hipLaunchKernelGGL(kern, dim3(g), dim3(block), 0, ctx->stream,
+                               (const IdxSlot *)inner_idx->slots, inner_idx->n_slots, d_inner,
+                               n_inner, shift, d_outer, n_outer, map.arity, map.src[0],
+                               map.arity == 2 ? map.src[1] : 0, pred.op, pred.lhs, pred.rhs, sel,
+                               *d_new, *new_cap, (unsigned long long *)counters);
+        };
+        switch (fb) {
+            case 64: launch(k_join_insert<64, SEG>, 64); break;
+            case 128: launch(k_join_insert<128, SEG>, 128); break;
+            case 512: launch(k_join_insert<512, SEG>, 512); break;
+            default: launch(k_join_insert<256, SEG>, 256); break;
+        }
+        GDL_TRY(check_launch(ctx, "k_join_insert"));
+    }
+    u64 h[2];
+    GDL_TRY(read_u64(ctx, counters, h, 2));
+    if (h[0] > *new_cap)
+        return fail(ctx, GDL_ERR_OVERFLOW, "fused join: new tuples exceed the output capacity (" +
+                                               std::to_string(h[0]) + " > " +
+                                               std::to_string(*new_cap) + ")");
+    const u64 added = h[0] - *n_new;
+    // inner run reads + hash probes per produced tuple, new keys written
+    ctx->stats[K_JOIN_INSERT].alg_bytes += 16.0 * h[1] + 8.0 * added;
+    if constexpr (!SEG) hs->size += added;
+    *n_new = h[0];
+    *n_raw = h[1];
+    return GDL_OK;
+}
+
+extern "C" {
+
 gdl_status gdl_join_insert(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n_inner,
                            const gdl_index *inner_idx, const uint64_t *d_outer, uint64_t n_outer,
                            gdl_colmap map, gdl_pred pred, gdl_hashset *hs, uint64_t **d_new,
@@ -476,66 +731,31 @@ gdl_status gdl_join_insert(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n_inn
     if (!valid_map(map, true)) return fail(ctx, GDL_ERR_INVALID_ARG, "bad join column map");
     if (pred.op < GDL_PRED_NONE || pred.op > GDL_PRED_FALSE)
         return fail(ctx, GDL_ERR_INVALID_ARG, "bad join predicate");
-    *n_raw = 0;
-    if (n_outer == 0 || n_inner == 0 || inner_idx->n_slots == 0) return GDL_OK;
-    const int shift = inner_idx->prefix_cols >= 2 ? 0 : 32;
-    // upper bound of produced tuples: |outer| x longest inner run, or an
-    // exact count when that bound is loose
-    const u64 mr = inner_idx->max_run ? inner_idx->max_run : n_inner;
-    u64 bound = (n_outer > ~0ull / mr) ? ~0ull : n_outer * mr;
-    const u64 loose_limit = std::max<u64>(1ull << 24, 4 * n_outer);
-    if (bound > loose_limit) {
-        Scratch scr(ctx);
-        u32 *counts;
-        u64 *begins, *offsets;
-        GDL_TRY(scr.get(n_outer, &counts));
-        GDL_TRY(scr.get(n_outer, &begins));
-        GDL_TRY(scr.get(n_outer + 1, &offsets));
-        GDL_TRY(join_count(ctx, d_inner, n_inner, inner_idx, d_outer, n_outer, counts, begins));
-        GDL_TRY(exclusive_scan_u32_to_u64(ctx, counts, n_outer, offsets, &bound));
-    }
-    if (bound == 0) return GDL_OK;
-    GDL_TRY(hashset_reserve(ctx, hs, bound));
-    if (*n_new + bound > *new_cap || !*d_new) {
-        const u64 cap = *n_new + bound + (*n_new + bound) / 4 + 1024;
-        void *p = nullptr;
-        std::string err;
-        gdl_status st = ctx->alloc->alloc(cap * sizeof(u64), &p, &err);
-        if (st != GDL_OK) return fail(ctx, st, err);
-        if (*n_new)
-            GDL_HIP(ctx, hipMemcpyAsync(p, *d_new, *n_new * sizeof(u64), hipMemcpyDeviceToDevice,
-                                        ctx->stream));
-        if (*d_new) ctx->alloc->free(*d_new, nullptr);
-        *d_new = (u64 *)p;
-        *new_cap = cap;
-    }
-    Scratch scr(ctx);
-    u64 *counters;
-    GDL_TRY(scr.get(2, &counters));
-    hipLaunchKernelGGL(k_set2, dim3(1), dim3(1), 0, ctx->stream, counters, (u64)*n_new, (u64)0);
-    GDL_TRY(check_launch(ctx, "k_set2"));
-    {
-        KTimer kt(ctx, K_JOIN_INSERT, 8.0 * n_outer + 64.0 * n_outer);
-        const u64 cap_blocks = (u64)(ctx->cu_count > 0 ? ctx->cu_count : 256) * 8;
-        const unsigned g = (unsigned)std::max<u64>(1, std::min<u64>((n_outer + JB - 1) / JB,
-                                                                    cap_blocks));
-        hipLaunchKernelGGL(k_join_insert, dim3(g), dim3(JB), 0, ctx->stream,
-                           (const IdxSlot *)inner_idx->slots, inner_idx->n_slots, d_inner, n_inner,
-                           shift, d_outer, n_outer, map.arity, map.src[0],
-                           map.arity == 2 ? map.src[1] : 0, pred.op, pred.lhs, pred.rhs,
-                           (u64 *)hs->slots, hs->capacity - 1, *d_new,
-                           (unsigned long long *)counters);
-        GDL_TRY(check_launch(ctx, "k_join_insert"));
-    }
-    u64 h[2];
-    GDL_TRY(read_u64(ctx, counters, h, 2));
-    const u64 added = h[0] - *n_new;
-    // inner run reads + hash probes per produced tuple, new keys written
-    ctx->stats[K_JOIN_INSERT].alg_bytes += 16.0 * h[1] + 8.0 * added;
-    hs->size += added;
-    *n_new = h[0];
-    *n_raw = h[1];
-    return GDL_OK;
+    return join_insert_impl<false>(ctx, d_inner, n_inner, inner_idx, d_outer, n_outer, map, pred,
+                                   hs, HsSel{nullptr, 0, nullptr, 0, 0}, d_new, new_cap, n_new,
+                                   n_raw);
+}
+
+gdl_status gdl_join_insert_seg(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n_inner,
+                               const gdl_index *inner_idx, const uint64_t *d_outer,
+                               uint64_t n_outer, gdl_colmap map, gdl_pred pred,
+                               const gdl_hs_seg *d_segs, uint32_t n_segs, int seg_shift,
+                               uint64_t *d_out, uint64_t out_cap, uint64_t *n_out,
+                               uint64_t *n_raw) {
+    if (!ctx || !inner_idx || !d_segs || !n_out || !n_raw || (n_outer && !d_outer) ||
+        (n_inner && !d_inner) || (out_cap && !d_out))
+        return GDL_ERR_INVALID_ARG;
+    if (n_segs == 0 || n_segs > kMaxSegs || seg_shift < 1 || seg_shift > 63)
+        return fail(ctx, GDL_ERR_INVALID_ARG, "segmented join insert: 1..256 sets, shift 1..63");
+    if (!valid_map(map, true)) return fail(ctx, GDL_ERR_INVALID_ARG, "bad join column map");
+    if (pred.op < GDL_PRED_NONE || pred.op > GDL_PRED_FALSE)
+        return fail(ctx, GDL_ERR_INVALID_ARG, "bad join predicate");
+    *n_out = 0;
+    u64 *out = d_out;
+    u64 cap = out_cap;
+    return join_insert_impl<true>(ctx, d_inner, n_inner, inner_idx, d_outer, n_outer, map, pred,
+                                  nullptr, HsSel{nullptr, 0, d_segs, n_segs, seg_shift}, &out,
+                                  &cap, n_out, n_raw);
 }
 
 gdl_status gdl_project_u64(gdl_ctx *ctx, const uint64_t *d_in, uint64_t n, gdl_colmap map,

@@ -2,7 +2,9 @@
 //
 // The key (c0 << 32) | c1 is first compressed order-preservingly to
 // (c0 << lo_bits) | c1 (lo_bits = bit length bound of c1), so the number of
-// 8-bit digit passes is ceil((hi_bits + lo_bits) / 8) instead of 8.
+// digit passes is ceil((hi_bits + lo_bits) / r) instead of 64 / r, for the
+// context's radix width r (4..8 bits, default 8; digits narrower than 8 bits
+// leave the upper histogram rows empty).
 //
 // Per pass over n > 4096 keys, three launches:
 //   upsweep   -- per-tile digit histogram (LDS, one sub-histogram per wave),
@@ -31,7 +33,14 @@ __device__ __forceinline__ u64 compress_key(u64 k, int lo_bits) {
 __device__ __forceinline__ u64 expand_key(u64 k, int lo_bits) {
     return ((k >> lo_bits) << 32) | (k & ((1ull << lo_bits) - 1ull));
 }
-__device__ __forceinline__ u32 digit_of(u64 k, int shift) { return (u32)(k >> shift) & 255u; }
+// A pass's digit: `pass` packs the shift (bits 0..7) and the digit width
+// (bits 8..11, 4..8 bits; the tuned radix width).
+__device__ __forceinline__ u32 digit_of(u64 k, int pass) {
+    return (u32)(k >> (pass & 0xff)) & ((1u << (pass >> 8)) - 1u);
+}
+__host__ __device__ __forceinline__ int pass_code(int shift, int bits) {
+    return shift | (bits << 8);
+}
 
 __device__ __forceinline__ u64 tile_pos(int w, int r, int lane) {
     return (u64)w * RS_WSPAN + (u64)r * 64 + lane;
@@ -192,8 +201,10 @@ __global__ __launch_bounds__(RS_B) void k_rs_oneblock(const u64 *in, u64 *out, u
         if (XF) k = compress_key(k, lo_bits);
         keys[r] = k;
     }
-    for (int pass = 0; pass < passes; ++pass) {
-        const int shift = pass * 8;
+    // passes packs the pass count (bits 0..7) and the digit width (8..11)
+    const int n_pass = passes & 0xff, dbits = passes >> 8;
+    for (int pass = 0; pass < n_pass; ++pass) {
+        const int shift = pass_code(pass * dbits, dbits);
         rank_tile(keys, valid, rank, shift, s_hist, s_start, s_scan);
 #pragma unroll
         for (int r = 0; r < RS_I; ++r)
@@ -216,7 +227,7 @@ __global__ __launch_bounds__(RS_B) void k_rs_oneblock(const u64 *in, u64 *out, u
     }
     // keys[r] sits at tile position tile_pos(w, r, lane); s_keys holds the
     // sorted tile when passes > 0, otherwise reload it.
-    if (passes == 0) {
+    if (n_pass == 0) {
 #pragma unroll
         for (int r = 0; r < RS_I; ++r)
             if (valid[r]) s_keys[tile_pos(w, r, lane)] = keys[r];
@@ -296,7 +307,9 @@ static gdl_status sort_driver(gdl_ctx *ctx, const u64 *d_in, u64 n, int hi_bits,
         return fail(ctx, GDL_ERR_INVALID_ARG, "sort: key bit bounds above 32");
     const bool xf = lo_bits < 32;
     const int total_bits = hi_bits + (xf ? lo_bits : 32);
-    const int passes = (total_bits + 7) / 8;
+    const int dbits = ctx->tune.radix_bits;  // 4..8
+    const int passes = (total_bits + dbits - 1) / dbits;
+    const int pcode = passes | (dbits << 8);
 
     if (n <= (u64)RS_TILE) {
         KTimer kt(ctx, K_RS_ONEBLOCK, 16.0 * n);
@@ -306,17 +319,17 @@ static gdl_status sort_driver(gdl_ctx *ctx, const u64 *d_in, u64 n, int hi_bits,
         if (xf) {
             if (unique)
                 hipLaunchKernelGGL((k_rs_oneblock<true, true>), dim3(1), dim3(RS_B), 0, ctx->stream,
-                                   d_in, d_out, (u32)n, passes, lo_bits, cnt);
+                                   d_in, d_out, (u32)n, pcode, lo_bits, cnt);
             else
                 hipLaunchKernelGGL((k_rs_oneblock<true, false>), dim3(1), dim3(RS_B), 0,
-                                   ctx->stream, d_in, d_out, (u32)n, passes, lo_bits, cnt);
+                                   ctx->stream, d_in, d_out, (u32)n, pcode, lo_bits, cnt);
         } else {
             if (unique)
                 hipLaunchKernelGGL((k_rs_oneblock<false, true>), dim3(1), dim3(RS_B), 0,
-                                   ctx->stream, d_in, d_out, (u32)n, passes, lo_bits, cnt);
+                                   ctx->stream, d_in, d_out, (u32)n, pcode, lo_bits, cnt);
             else
                 hipLaunchKernelGGL((k_rs_oneblock<false, false>), dim3(1), dim3(RS_B), 0,
-                                   ctx->stream, d_in, d_out, (u32)n, passes, lo_bits, cnt);
+                                   ctx->stream, d_in, d_out, (u32)n, pcode, lo_bits, cnt);
         }
         GDL_TRY(check_launch(ctx, "k_rs_oneblock"));
         if (unique) {
@@ -342,7 +355,7 @@ static gdl_status sort_driver(gdl_ctx *ctx, const u64 *d_in, u64 n, int hi_bits,
     const u64 *src = d_in;
     for (int p = 0; p < passes; ++p) {
         u64 *dst = ((passes - 1 - p) % 2 == 0) ? d_out : tmp;
-        const int shift = p * 8;
+        const int shift = pass_code(p * dbits, dbits);
         const bool in_x = xf && p == 0;
         const bool out_x = xf && p == passes - 1;
         {

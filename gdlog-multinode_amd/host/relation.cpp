@@ -14,8 +14,10 @@
 namespace gdlog {
 
 namespace {
-gdl_ctx *g_ctx = nullptr;
-int g_id_bits = 0;
+// One engine context per host thread (the engine's contexts are not
+// re-entrant); programs on different threads run on their own streams.
+thread_local gdl_ctx *g_ctx = nullptr;
+thread_local int g_id_bits = 0;
 bool g_phase_timing = false;
 int g_verbose = -1;
 }  // namespace
@@ -344,6 +346,12 @@ void Relation::flush_delta(int grid_size, int block_size, float *detail_time) {
           "flush_delta merge");
     container_take(full, merged, nm, nf + nd, true);
     current_full_size = nm;
+}
+
+uint64_t relation_checksum(GHashRelContainer *c) {
+    uint64_t s = 0;
+    check(gdl_checksum_u64(context(), c->keys, c->tuple_counts, &s), "relation checksum");
+    return s;
 }
 
 std::vector<uint64_t> relation_keys(GHashRelContainer *c) {

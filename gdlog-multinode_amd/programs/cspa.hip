@@ -191,6 +191,7 @@ void cspa_program(const EdgeInput &assign, const EdgeInput &dereference, int gri
     out->sizes[1] = memory_alias_2__1_2->full->tuple_counts;
     out->sizes[2] = value_alias_2__1_2->full->tuple_counts;
     out->sizes[3] = value_flow_2__2_1->full->tuple_counts;
+    out->checksum = relation_checksum(value_alias_2__1_2->full);
     if (out->want_keys) {
         out->main_keys = relation_keys(value_alias_2__1_2->full);
         out->extra_keys[0] = relation_keys(value_flow_2__1_2->full);

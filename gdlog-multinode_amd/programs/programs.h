@@ -27,6 +27,7 @@ struct ProgramOutput {
     std::vector<IterationRecord> trace, trace_init;
     bool want_keys = false;
     uint64_t n_main = 0;
+    uint64_t checksum = 0;  // relation_checksum of the main output
     std::vector<uint64_t> main_keys;
     std::vector<uint64_t> extra_keys[2];
     uint64_t sizes[4] = {0, 0, 0, 0};
@@ -37,6 +38,14 @@ uint64_t sum_delta(const std::vector<IterationRecord> &t);
 
 void tc_program(const EdgeInput &graph, int max_iteration, int grid_size, int block_size,
                 ProgramOutput *out);
+
+// One program run (kind 0 TC, 1 SG, 2 CSPA) from host packed keys on ctx,
+// which is this thread's engine context for the run; output keys are copied
+// to the host when out->want_keys is set, the checksum always.  Errors come
+// back as the status and *err.
+gdl_status run_program_host(gdl_ctx *ctx, int kind, const uint64_t *a, uint64_t na,
+                            const uint64_t *b, uint64_t nb, ProgramOutput *out,
+                            std::string *err);
 void sg_program(const EdgeInput &graph, int grid_size, int block_size, ProgramOutput *out);
 void cspa_program(const EdgeInput &assign, const EdgeInput &dereference, int grid_size,
                   int block_size, ProgramOutput *out);

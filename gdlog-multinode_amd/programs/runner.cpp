@@ -125,6 +125,7 @@ gdl_status run_program(gdl_ctx *ctx, gdl_prog_result *res, const gdl_prog_buffer
     res->fixpoint_ms = out.fixpoint_ms;
     res->n_trace = out.trace.size();
     res->n_trace_init = out.trace_init.size();
+    res->checksum = out.checksum;
     if (bufs) {
         gdl_status s[5] = {copy_out(out.main_keys, bufs->out, bufs->out_cap),
                            copy_out(out.extra_keys[0], bufs->extra0, bufs->extra0_cap),
@@ -141,6 +142,38 @@ gdl_status run_program(gdl_ctx *ctx, gdl_prog_result *res, const gdl_prog_buffer
 }
 
 }  // namespace
+
+gdl_status run_program_host(gdl_ctx *ctx, int kind, const uint64_t *a, uint64_t na,
+                            const uint64_t *b, uint64_t nb, ProgramOutput *out,
+                            std::string *err) {
+    if (!ctx || !out || (na && !a) || (nb && !b)) return GDL_ERR_INVALID_ARG;
+    gdl_ctx *prev = context_or_null();
+    set_context(ctx);
+    reset_id_bits();
+    gdl_status st = GDL_OK;
+    try {
+        HostRows ha, hb;
+        EdgeInput ia = ha.input(a, na, 0);
+        if (kind == 0) {
+            tc_program(ia, -1, grid_for_ctx(ctx), kBlock, out);
+        } else if (kind == 1) {
+            sg_program(ia, grid_for_ctx(ctx), kBlock, out);
+        } else if (kind == 2) {
+            EdgeInput ib = hb.input(b, nb, 0);
+            cspa_program(ia, ib, grid_for_ctx(ctx), kBlock, out);
+        } else {
+            st = GDL_ERR_INVALID_ARG;
+        }
+    } catch (const Error &e) {
+        if (err) *err = e.what();
+        st = e.status;
+    } catch (const std::exception &e) {
+        if (err) *err = e.what();
+        st = GDL_ERR_INVALID_ARG;
+    }
+    set_context(prev);
+    return st;
+}
 
 extern "C" {
 

@@ -71,6 +71,7 @@ void sg_program(const EdgeInput &graph, int grid_size, int block_size, ProgramOu
     out->trace = sg_lie.trace;
     out->derived = sum_delta(init_scc.trace) + sum_delta(sg_lie.trace);
     out->n_main = sg_2__1_2->full->tuple_counts;
+    out->checksum = relation_checksum(sg_2__1_2->full);
     if (out->want_keys) out->main_keys = relation_keys(sg_2__1_2->full);
     for (Relation *r : {edge_2__1_2, sg_2__1_2, tmp}) {
         free_relation(r);

@@ -43,6 +43,7 @@ void tc_program(const EdgeInput &graph, int max_iteration, int grid_size, int bl
     out->trace = tc_scc.trace;
     out->derived = sum_delta(tc_scc.trace);
     out->n_main = path_2__1_2->full->tuple_counts;
+    out->checksum = relation_checksum(path_2__1_2->full);
     if (out->want_keys) out->main_keys = relation_keys(path_2__1_2->full);
     free_relation(edge_2__2_1);
     free_relation(path_2__1_2);

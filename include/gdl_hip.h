@@ -34,7 +34,9 @@ typedef enum gdl_status {
     GDL_ERR_HIP = 3,           /* a HIP runtime call failed (see gdl_last_error) */
     GDL_ERR_UNSUPPORTED = 4,   /* valid request outside what this engine implements */
     GDL_ERR_OVERFLOW = 5,      /* a size exceeds a documented limit */
-    GDL_ERR_DEVICE = 6         /* a kernel reported an internal error */
+    GDL_ERR_DEVICE = 6,        /* a kernel reported an internal error */
+    GDL_ERR_BUSY = 7           /* admission refused: no free batch slot or the memory
+                                  budget is taken; retry after running work retires */
 } gdl_status;
 
 typedef struct gdl_ctx gdl_ctx;
@@ -66,6 +68,21 @@ gdl_status gdl_sync(gdl_ctx *ctx);
 void *gdl_stream(gdl_ctx *ctx);
 int gdl_device(gdl_ctx *ctx);
 gdl_status gdl_set_load_factor(gdl_ctx *ctx, float load_factor);
+
+/* Launch and layout parameters of a context, as an autotuner picks them
+ * (gdlog-multinode_amd/tune.py).  The reference hard-codes them: load factor
+ * 0.8 (relation.cu:603-604), 512 threads x 32 blocks per SM (test/tc.cu:224-225)
+ * and Thrust's sort. */
+typedef struct gdl_tuning {
+    float load_factor;      /* hash index / hash set load factor, (0, 0.95]
+                               (hash sets are capped at 0.8); default 0.5 */
+    uint32_t blocks_per_cu; /* grid cap of the grid-stride kernels, 1..32; default 8 */
+    uint32_t radix_bits;    /* LSD radix sort digit width, 4..8; default 8 */
+    uint32_t join_block;    /* threads per block of the fused join: 64, 128, 256 or 512;
+                               default 256 */
+} gdl_tuning;
+gdl_status gdl_get_tuning(gdl_ctx *ctx, gdl_tuning *out);
+gdl_status gdl_set_tuning(gdl_ctx *ctx, const gdl_tuning *t);
 
 /* ---- device memory (replaces the per-op cudaMalloc/cudaFree/cudaMemset of
  *      join.cu:49-62,109-114 and relation.cu:547-561,606-616) ---------- */
@@ -145,7 +162,8 @@ gdl_status gdl_index_lookup(gdl_ctx *ctx, const gdl_index *idx, const uint64_t *
  * Replaces the set-difference role of thrust::set_difference(newt, full)
  * (lie.cu:131-138) for relations whose FULL is never a join inner, where
  * keeping FULL sorted (Relation::flush_delta, relation.cu:484-487) is not
- * needed until the fixpoint ends.  Load factor <= 1/2, grown by rehashing. */
+ * needed until the fixpoint ends.  Load kept at or below the context's
+ * load factor (at most 0.8), grown by rehashing. */
 typedef struct gdl_hashset {
     void *slots;       /* device; owned by the context allocator */
     uint64_t capacity; /* slots, a power of two */
@@ -157,6 +175,13 @@ gdl_status gdl_hashset_free(gdl_ctx *ctx, gdl_hashset *hs);
  * written once to d_new (n slots), in no particular order; *n_new = count. */
 gdl_status gdl_hashset_insert_new(gdl_ctx *ctx, gdl_hashset *hs, const uint64_t *d_keys,
                                   uint64_t n, uint64_t *d_new, uint64_t *n_new);
+/* Grows the table (rehash) so that size + incoming members fit. */
+gdl_status gdl_hashset_reserve(gdl_ctx *ctx, gdl_hashset *hs, uint64_t incoming);
+/* Writes the members, in no particular order, to d_out (out_cap keys);
+ * GDL_ERR_OVERFLOW (nothing written past out_cap) when they do not fit.
+ * Replaces flatten_tuples_raw_data of a FULL version (relation.cu:375-388). */
+gdl_status gdl_hashset_extract(gdl_ctx *ctx, const gdl_hashset *hs, uint64_t *d_out,
+                               uint64_t out_cap, uint64_t *n_out);
 
 /* ---- column maps and predicates (the build's form of the reference hooks,
  *      tuple_generator_hook / tuple_copy_hook / tuple_predicate,
@@ -218,6 +243,39 @@ gdl_status gdl_join_insert(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n_inn
                            const gdl_index *inner_idx, const uint64_t *d_outer, uint64_t n_outer,
                            gdl_colmap map, gdl_pred pred, gdl_hashset *hs, uint64_t **d_new,
                            uint64_t *new_cap, uint64_t *n_new, uint64_t *n_raw);
+
+/* Batched form of gdl_join_insert: many fixpoint instances share the inner
+ * and outer arrays, told apart by the top bits of their keys, and each has
+ * its own hash set: a produced key k goes into d_segs[k >> seg_shift]
+ * (a device array of n_segs <= 256 entries; slots == NULL: no set, the key
+ * is dropped).  The caller sizes every set for its worst case and gives a
+ * fixed output of out_cap keys; GDL_ERR_OVERFLOW reports an output bound
+ * that was too small (nothing is written past it).  Set sizes are not
+ * updated: count the new keys per instance with gdl_key_histogram_u64.
+ * No counterpart in the reference, which runs one fixpoint per process. */
+typedef struct gdl_hs_seg {
+    void *slots;   /* a gdl_hashset's slots, or NULL */
+    uint64_t mask; /* its capacity - 1 */
+} gdl_hs_seg;
+gdl_status gdl_join_insert_seg(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n_inner,
+                               const gdl_index *inner_idx, const uint64_t *d_outer,
+                               uint64_t n_outer, gdl_colmap map, gdl_pred pred,
+                               const gdl_hs_seg *d_segs, uint32_t n_segs, int seg_shift,
+                               uint64_t *d_out, uint64_t out_cap, uint64_t *n_out,
+                               uint64_t *n_raw);
+
+/* d_counts[k >> shift] += 1 for every key k with k >> shift < n_bins,
+ * n_bins <= 2048 (device u64 counters, accumulated: clear them first). */
+gdl_status gdl_key_histogram_u64(gdl_ctx *ctx, const uint64_t *d_keys, uint64_t n, int shift,
+                                 uint32_t n_bins, uint64_t *d_counts);
+/* Order-independent checksum of a tuple set: the sum, mod 2^64, of the
+ * MurmurHash3 fmix64 finalizer of every key.  Result-comparison aid (a
+ * relation's identity without copying it to the host). */
+gdl_status gdl_checksum_u64(gdl_ctx *ctx, const uint64_t *d_keys, uint64_t n, uint64_t *out);
+/* out = ((c0 & col_mask) << 32) | (c1 & col_mask): strips an instance
+ * prefix from both columns (in == out allowed). */
+gdl_status gdl_mask_columns_u64(gdl_ctx *ctx, const uint64_t *d_in, uint64_t n,
+                                uint32_t col_mask, uint64_t *d_out);
 
 /* ---- projection / filter ----------------------------------------------
  * gdl_project_u64 replaces get_copy_result (relation.cu:390-403);

@@ -37,6 +37,7 @@ typedef struct gdl_prog_result {
     double fixpoint_ms;      /* fixpoint loop(s), device-synchronised */
     uint64_t n_trace;        /* records of the (last) LIE */
     uint64_t n_trace_init;   /* records of the init LIE */
+    uint64_t checksum;       /* gdl_checksum_u64 of the main output */
 } gdl_prog_result;
 
 /* Optional output buffers (NULL / 0 capacity to skip).  A buffer too small

@@ -19,6 +19,9 @@
 
 #include <cstdint>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <utility>
 
 #include "gdl_hip.h"
 #include "gdlog/tuple.h"
@@ -86,7 +89,7 @@ static bool decode(const column_type *out, int arity, column_type tag_i, column_
 
 // Column map of a tuple_generator_hook producing `out_arity` columns from
 // (inner, outer) tuples of arity <= 2.
-static inline HookMap probe_generator(tuple_generator_hook h, int out_arity) {
+static inline HookMap probe_generator_now(tuple_generator_hook h, int out_arity) {
     using namespace probe_detail;
     HookMap r;
     if (!h || out_arity < 1 || out_arity > 2) return r;
@@ -114,7 +117,7 @@ static inline HookMap probe_generator(tuple_generator_hook h, int out_arity) {
 }
 
 // Column map of a tuple_copy_hook.
-static inline HookMap probe_copy(tuple_copy_hook h, int out_arity) {
+static inline HookMap probe_copy_now(tuple_copy_hook h, int out_arity) {
     using namespace probe_detail;
     HookMap r;
     if (!h || out_arity < 1 || out_arity > 2) return r;
@@ -142,7 +145,7 @@ static inline HookMap probe_copy(tuple_copy_hook h, int out_arity) {
 // Comparison form of a tuple_predicate over a generated tuple of `arity`
 // columns: classified on (a,a), (a,b), (b,a) with a < b, then checked on
 // pseudo-random tuples including 0 and 2^32 - 1.
-static inline PredMap probe_predicate(tuple_predicate p, int arity) {
+static inline PredMap probe_predicate_now(tuple_predicate p, int arity) {
     using namespace probe_detail;
     PredMap r;
     if (!p) return r;  // no predicate: keep everything
@@ -215,6 +218,42 @@ static inline PredMap probe_predicate(tuple_predicate p, int arity) {
     r.ok = true;
     r.pred = pr;
     return r;
+}
+
+namespace probe_detail {
+// A hook's meaning is fixed for the process, so each (hook, arity) is probed
+// once: later operators with the same hook (every run of a program) skip the
+// probe kernels, their allocations and device-wide synchronisation.
+template <typename R, typename F>
+static R cached(const void *hook, int arity, F compute) {
+    static std::mutex mu;
+    static std::map<std::pair<uintptr_t, int>, R> memo;
+    const auto key = std::make_pair((uintptr_t)hook, arity);
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = memo.find(key);
+        if (it != memo.end()) return it->second;
+    }
+    R r = compute();
+    std::lock_guard<std::mutex> g(mu);
+    memo.emplace(key, r);
+    return r;
+}
+}  // namespace probe_detail
+
+static inline HookMap probe_generator(tuple_generator_hook h, int out_arity) {
+    return probe_detail::cached<HookMap>((const void *)h, out_arity,
+                                         [&] { return probe_generator_now(h, out_arity); });
+}
+
+static inline HookMap probe_copy(tuple_copy_hook h, int out_arity) {
+    return probe_detail::cached<HookMap>((const void *)h, out_arity,
+                                         [&] { return probe_copy_now(h, out_arity); });
+}
+
+static inline PredMap probe_predicate(tuple_predicate p, int arity) {
+    return probe_detail::cached<PredMap>((const void *)p, arity,
+                                         [&] { return probe_predicate_now(p, arity); });
 }
 
 }  // namespace gdlog

@@ -121,6 +121,8 @@ void free_relation(Relation *target);
 
 // Copies a version's tuples to the host as sorted keys ((c0 << 32) | c1).
 std::vector<uint64_t> relation_keys(GHashRelContainer *c);
+// Order-independent checksum of a version's tuples (gdl_checksum_u64).
+uint64_t relation_checksum(GHashRelContainer *c);
 
 namespace gdlog {
 
