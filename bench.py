@@ -41,11 +41,43 @@ def _pkg():
     return mod
 
 
+def cpu_threads():
+    """Host threads for the CPU baseline: this process's CPU share, at most 16
+    (the share of one GPU on a shared box)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def cpu_baseline(kind, inp, max_iteration):
-    """Oracle (plain C, one thread) on a bounded sample of the workload."""
+    """CPU baseline.  TC: the same semi-naive loop on all the host threads
+    (oracle/cpu_tc_omp.c, partitioned by the path's second column), on the
+    whole workload, best of 2, with the one-thread oracle on a bounded sample
+    beside it.  SG / CSPA: the one-thread oracle on the whole workload."""
     if REPO not in sys.path:
         sys.path.insert(0, REPO)
     import oracle
+    if kind == "tc":
+        threads = cpu_threads()
+        best = None
+        for _ in range(2):
+            t = time.perf_counter()
+            r = oracle.tc_omp(inp, threads)
+            dt = time.perf_counter() - t
+            best = dt if best is None else min(best, dt)
+        t = time.perf_counter()
+        s = oracle.tc(inp, max_iteration=max_iteration)
+        dt1 = time.perf_counter() - t
+        return dict(value=round(r["derived"] / best, 1), unit="derived tuples/s", cores=threads,
+                    kind="port",
+                    sample=f"oracle/cpu_tc_omp.c on the whole workload ({r['derived']} derived "
+                           f"tuples, {r['iterations']} iterations, best of 2: {best:.2f} s, "
+                           f"{threads} threads)",
+                    single_thread=dict(value=round(s["derived"] / dt1, 1), cores=1,
+                                       sample=f"oracle, first {s['iterations']} iterations "
+                                              f"({s['derived']} derived tuples, {dt1:.1f} s)"))
     t = time.perf_counter()
     if kind == "tc":
         r = oracle.tc(inp, max_iteration=max_iteration)
@@ -268,6 +300,8 @@ def main():
                     help="instances of the batched-TC measurement (0: skip it)")
     ap.add_argument("--served-requests", type=int, default=300,
                     help="requests of the served-mix measurement (0: skip it)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 256 MiB device copy (traffic-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus must be 1 "
@@ -275,6 +309,17 @@ def main():
         sys.exit(2)
 
     gdl = _pkg()
+    if args.calibrate_copy:
+        # a plain 256 MiB device-to-device copy: what FETCH_SIZE / WRITE_SIZE
+        # read for a known number of bytes (the scale of the traffic counters)
+        ctx = gdl.Context(device=0)
+        n = 32 << 20
+        a, b = ctx.alloc(n), ctx.alloc(n)
+        ctx._check(ctx.eng.gdl_memcpy_d2d(ctx.h, b.ptr, a.ptr, n * 8), "copy")
+        ctx.sync()
+        print(json.dumps({"copied_bytes": n * 8}), flush=True)
+        a.free(); b.free(); ctx.close()
+        return
     gdl.set_full_mode(gdl.FULL_MODE_MERGE if args.mode == "merge" else gdl.FULL_MODE_AUTO)
     kind, make, desc = workloads(gdl, args.nodes, args.window)[args.workload]
     inp = make()

@@ -13,7 +13,8 @@ const char *kKernelNames[K_NUM] = {
     "scan_reduce",  "scan_top",     "scan_down",    "compact_count", "compact_write",
     "merge_count",  "merge_write",  "idx_count",    "idx_build",     "idx_lookup",
     "join_count",   "join_materialize", "project",  "fill",          "hs_insert_new",
-    "hs_rehash",    "join_insert",  "hs_extract",   "seg_hist",      "mask_cols"};
+    "hs_rehash",    "join_insert",  "hs_extract",   "seg_hist",      "mask_cols",
+    "linear_body"};
 
 // ---------------------------------------------------------------- allocator
 

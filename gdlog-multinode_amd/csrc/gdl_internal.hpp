@@ -75,6 +75,7 @@ enum KernelId {
     K_HS_EXTRACT,
     K_SEG_HIST,
     K_MASK_COLS,
+    K_LINEAR,
     K_NUM
 };
 extern const char *kKernelNames[K_NUM];

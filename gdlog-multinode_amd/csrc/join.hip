@@ -183,6 +183,41 @@ __device__ __forceinline__ void flush_stage(const u64 *stage, u32 n, u64 *out, u
         if (pos + x < out_cap) out[pos + x] = stage[x];
 }
 
+// End of a fused-join block: what the waves still stage goes out with one
+// cursor atomic for the whole block, and the produced count with one more
+// (instead of one of each per wave, all on the same two addresses).  Every
+// thread of the block calls it.
+template <int B>
+__device__ __forceinline__ void block_final_flush(u64 (*s_stage)[kStage], u32 staged,
+                                                  u64 produced, u64 *out, u64 out_cap,
+                                                  unsigned long long *counters) {
+    __shared__ u32 s_cnt[B / 64];
+    __shared__ u64 s_prod[B / 64];
+    __shared__ unsigned long long s_base;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int d = 32; d >= 1; d >>= 1) produced += __shfl_xor(produced, d, 64);
+    if (lane == 0) {
+        s_cnt[w] = staged;
+        s_prod[w] = produced;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u64 tot = 0, prod = 0;
+        for (int v = 0; v < B / 64; ++v) {
+            tot += s_cnt[v];
+            prod += s_prod[v];
+        }
+        s_base = tot ? atomicAdd(&counters[0], (unsigned long long)tot) : 0ull;
+        if (prod) atomicAdd(&counters[1], (unsigned long long)prod);
+    }
+    __syncthreads();
+    u64 off = s_base;
+    for (int v = 0; v < w; ++v) off += s_cnt[v];
+    const u64 *stage = s_stage[w];
+    for (u32 x = lane; x < staged; x += 64)
+        if (off + x < out_cap) out[off + x] = stage[x];
+}
+
 // Target of a fused join's inserts: one hash set, or one set per fixpoint
 // instance of a batch, chosen by the key's top bits (key >> seg_shift).
 struct HsSel {
@@ -217,8 +252,11 @@ __device__ __forceinline__ void load_segs(const HsSel &h, gdl_hs_seg *s_segs) {
     }
 }
 
+// One block's share of a fused join + insert (the blocks of the grid stride
+// over the outer tuples); counters[0] is the output cursor, counters[1] the
+// produced-tuple count.
 template <int B, bool SEG>
-__global__ __launch_bounds__(B) void k_join_insert(
+__device__ __forceinline__ void join_insert_blocks(
     const IdxSlot *slots, u64 n_slots, const u64 *inner, u64 n_inner, int shift, const u64 *outer,
     u64 n_outer, int arity, int s0, int s1, int pop, int plhs, int prhs, HsSel sel, u64 *out,
     u64 out_cap, unsigned long long *counters) {
@@ -279,12 +317,79 @@ __global__ __launch_bounds__(B) void k_join_insert(
         }
         wave_sync_lds();  // s_excl / s_b / s_ok are rewritten next round
     }
-    if (staged) {
-        wave_sync_lds();
-        flush_stage(stage, staged, out, out_cap, &counters[0], lane);
-    }
-    for (int d = 32; d >= 1; d >>= 1) produced += __shfl_xor(produced, d, 64);
-    if (lane == 0 && produced) atomicAdd(&counters[1], (unsigned long long)produced);
+    block_final_flush<B>(s_stage, staged, produced, out, out_cap, counters);
+}
+
+template <int B, bool SEG>
+__global__ __launch_bounds__(B) void k_join_insert(
+    const IdxSlot *slots, u64 n_slots, const u64 *inner, u64 n_inner, int shift, const u64 *outer,
+    u64 n_outer, int arity, int s0, int s1, int pop, int plhs, int prhs, HsSel sel, u64 *out,
+    u64 out_cap, unsigned long long *counters) {
+    join_insert_blocks<B, SEG>(slots, n_slots, inner, n_inner, shift, outer, n_outer, arity, s0,
+                               s1, pop, plhs, prhs, sel, out, out_cap, counters);
+}
+
+// State of a stream-ordered linear fixpoint (gdl_fixpoint_linear) in device
+// memory.  cursor and produced come first: they are a body's counters.
+struct LinState {
+    unsigned long long cursor;    // log end while a body runs
+    unsigned long long produced;  // tuples the body produced
+    u64 delta_begin, delta_end;   // the body's delta: log[delta_begin, delta_end)
+    u64 size;                     // hash set members
+    u64 hs_limit;                 // members the set may reach (a fraction of its capacity)
+    u64 log_cap;                  // log capacity, keys
+    u64 max_run;                  // longest inner run: new tuples <= |delta| x max_run
+    u32 stop;                     // 0 run, 1 fixpoint, 2 grow first, 3 body limit, 4 overflow
+    u32 body;                     // bodies finished in this call
+    u32 blocks_done;
+    u32 max_bodies;
+};
+enum : u32 { kLinRun = 0, kLinFixpoint = 1, kLinGrow = 2, kLinLimit = 3, kLinOverflow = 4 };
+
+// One body of out := out U map(inner x delta) with FULL = hash set + append
+// log: the delta is the log's last segment, new tuples are appended after it.
+// The last block to finish records the body's counts and moves the delta
+// window; a stop code makes the bodies enqueued after it return at once.
+template <int B>
+__global__ __launch_bounds__(B) void k_linear_body(const IdxSlot *slots, u64 n_slots,
+                                                   const u64 *inner, u64 n_inner, int shift,
+                                                   int arity, int s0, int s1, int pop, int plhs,
+                                                   int prhs, u64 *hs_slots, u64 hs_mask, u64 *log,
+                                                   LinState *st, u64 *counts) {
+    __shared__ bool s_last;
+    if (st->stop != kLinRun) return;  // set by an earlier body: the same for every block
+    const u64 d0 = st->delta_begin, d1 = st->delta_end;
+    join_insert_blocks<B, false>(slots, n_slots, inner, n_inner, shift, log + d0, d1 - d0, arity,
+                                 s0, s1, pop, plhs, prhs, HsSel{hs_slots, hs_mask, nullptr, 0, 0},
+                                 log, st->log_cap, &st->cursor);
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0)
+        s_last = atomicAdd(&st->blocks_done, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!s_last || threadIdx.x != 0) return;
+    __threadfence();
+    const u64 end = atomicAdd(&st->cursor, 0ull);
+    const u64 produced = atomicAdd(&st->produced, 0ull);
+    const u32 b = st->body;
+    const u64 nd = end > d1 ? end - d1 : 0;
+    counts[2 * b] = nd;
+    counts[2 * b + 1] = produced;
+    const u64 size = st->size + nd;
+    st->size = size;
+    st->delta_begin = d1;
+    st->delta_end = end;
+    st->produced = 0;
+    st->blocks_done = 0;
+    st->body = b + 1;
+    u32 stop = kLinRun;
+    if (end > st->log_cap) stop = kLinOverflow;
+    else if (nd == 0) stop = kLinFixpoint;
+    else if (b + 1 >= st->max_bodies) stop = kLinLimit;
+    else if (size + nd * st->max_run > st->hs_limit || end + nd * st->max_run > st->log_cap)
+        stop = kLinGrow;
+    st->stop = stop;
+    __threadfence();
 }
 
 // Load-balanced variant for inner relations with long runs: outputs are
@@ -374,12 +479,7 @@ __global__ __launch_bounds__(JB) void k_join_insert_lb(
             staged += cnt;
         }
     }
-    if (staged) {
-        wave_sync_lds();
-        flush_stage(stage, staged, out, out_cap, &counters[0], lane);
-    }
-    for (int d = 32; d >= 1; d >>= 1) produced += __shfl_xor(produced, d, 64);
-    if (lane == 0 && produced) atomicAdd(&counters[1], (unsigned long long)produced);
+    block_final_flush<JB>(s_stage, staged, produced, out, out_cap, counters);
 }
 
 __global__ void k_set2(u64 *p, u64 a, u64 b) {
@@ -756,6 +856,108 @@ gdl_status gdl_join_insert_seg(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
     return join_insert_impl<true>(ctx, d_inner, n_inner, inner_idx, d_outer, n_outer, map, pred,
                                   nullptr, HsSel{nullptr, 0, d_segs, n_segs, seg_shift}, &out,
                                   &cap, n_out, n_raw);
+}
+
+gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n_inner,
+                               const gdl_index *inner_idx, gdl_colmap map, gdl_pred pred,
+                               gdl_hashset *hs, uint64_t **d_log, uint64_t *log_cap,
+                               uint64_t *delta_begin, uint64_t *delta_end, int max_bodies,
+                               uint64_t *h_new, uint64_t *h_raw, int *bodies) {
+    if (!ctx || !inner_idx || !hs || !d_log || !log_cap || !delta_begin || !delta_end ||
+        !h_new || !h_raw || !bodies || (n_inner && !d_inner) || *delta_begin > *delta_end ||
+        (*delta_end && !*d_log) || *delta_end > *log_cap)
+        return GDL_ERR_INVALID_ARG;
+    if (!valid_map(map, true)) return fail(ctx, GDL_ERR_INVALID_ARG, "bad join column map");
+    if (pred.op < GDL_PRED_NONE || pred.op > GDL_PRED_FALSE)
+        return fail(ctx, GDL_ERR_INVALID_ARG, "bad join predicate");
+    *bodies = 0;
+    if (max_bodies <= 0) return GDL_OK;
+    if (n_inner == 0 || inner_idx->n_slots == 0 || *delta_end == *delta_begin) {
+        h_new[0] = h_raw[0] = 0;  // one body, nothing produced: the fixpoint
+        *delta_begin = *delta_end;
+        *bodies = 1;
+        return GDL_OK;
+    }
+    constexpr int kChunk = 32;
+    constexpr int LB = 256;
+    const int shift = inner_idx->prefix_cols >= 2 ? 0 : 32;
+    const u64 mr = std::max<u64>(inner_idx->max_run, 1);
+    Scratch scr(ctx);
+    u64 *st_words, *counts;
+    GDL_TRY(scr.get((sizeof(LinState) + 7) / 8, &st_words));
+    GDL_TRY(scr.get(2 * kChunk, &counts));
+    LinState *st = (LinState *)st_words;
+    u64 d0 = *delta_begin, d1 = *delta_end;
+    int done = 0;
+    while (done < max_bodies) {
+        const u64 nd = d1 - d0;
+        const u64 worst = nd > (~0ull >> 20) / mr ? (~0ull >> 20) : nd * mr;
+        // the set: expected growth at the load factor, the worst body at 0.9;
+        // the log: the worst body after the delta, grown geometrically
+        GDL_TRY(hashset_reserve2(ctx, hs, std::max<u64>({nd, hs->size / 2, 1024}), worst));
+        const u64 need_log = d1 + std::max<u64>({worst, d1 / 2, 4096});
+        if (*log_cap < need_log) {
+            void *p = nullptr;
+            std::string err;
+            gdl_status s = ctx->alloc->alloc(need_log * sizeof(u64), &p, &err);
+            if (s != GDL_OK) return fail(ctx, s, err);
+            if (d1)
+                GDL_HIP(ctx, hipMemcpyAsync(p, *d_log, d1 * sizeof(u64), hipMemcpyDeviceToDevice,
+                                            ctx->stream));
+            if (*d_log) ctx->alloc->free(*d_log, nullptr);
+            *d_log = (u64 *)p;
+            *log_cap = need_log;
+        }
+        const int k = std::min(kChunk, max_bodies - done);
+        LinState h{};
+        h.cursor = d1;
+        h.delta_begin = d0;
+        h.delta_end = d1;
+        h.size = hs->size;
+        h.hs_limit = (u64)(kHsHardLoad * (double)hs->capacity);
+        h.log_cap = *log_cap;
+        h.max_run = mr;
+        h.max_bodies = (u32)k;
+        GDL_HIP(ctx, hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
+        // the delta can grow within a chunk: at least 4 blocks per CU
+        const u64 cus = (u64)(ctx->cu_count > 0 ? ctx->cu_count : 256);
+        const unsigned g = (unsigned)std::min<u64>(grid_cap(ctx),
+                                                   std::max<u64>(4 * cus, (4 * nd + LB - 1) / LB));
+        for (int i = 0; i < k; ++i) {
+            KTimer kt(ctx, K_LINEAR, 0.0);
+            hipLaunchKernelGGL(k_linear_body<LB>, dim3(g), dim3(LB), 0, ctx->stream,
+                               (const IdxSlot *)inner_idx->slots, inner_idx->n_slots, d_inner,
+                               n_inner, shift, map.arity, map.src[0],
+                               map.arity == 2 ? map.src[1] : 0, pred.op, pred.lhs, pred.rhs,
+                               (u64 *)hs->slots, hs->capacity - 1, *d_log, st, counts);
+            GDL_TRY(check_launch(ctx, "k_linear_body"));
+        }
+        LinState r{};
+        u64 c[2 * kChunk];
+        GDL_HIP(ctx, hipMemcpyAsync(&r, st, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
+        GDL_HIP(ctx, hipMemcpyAsync(c, counts, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+        GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        GDL_TRY(check_device_error(ctx));
+        if (r.stop == kLinOverflow)
+            return fail(ctx, GDL_ERR_OVERFLOW, "linear fixpoint: a body outgrew its log bound");
+        u64 outer_n = nd;
+        for (u32 b = 0; b < r.body && done < max_bodies; ++b, ++done) {
+            h_new[done] = c[2 * b];
+            h_raw[done] = c[2 * b + 1];
+            // probe + outer read per outer tuple, inner read + hash probe per
+            // produced tuple, the new keys written
+            ctx->stats[K_LINEAR].alg_bytes += 72.0 * outer_n + 16.0 * c[2 * b + 1] + 8.0 * c[2 * b];
+            outer_n = c[2 * b];
+        }
+        hs->size = r.size;
+        d0 = r.delta_begin;
+        d1 = r.delta_end;
+        if (r.stop == kLinFixpoint || r.body == 0) break;
+    }
+    *bodies = done;
+    *delta_begin = d0;
+    *delta_end = d1;
+    return GDL_OK;
 }
 
 gdl_status gdl_project_u64(gdl_ctx *ctx, const uint64_t *d_in, uint64_t n, gdl_colmap map,

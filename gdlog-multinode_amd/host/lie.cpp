@@ -112,8 +112,46 @@ void LIE::fixpoint_loop() {
             rel->fused_output = output_fusable(ra_ops, rel);
         }
     }
+    // A one-join LIE of TC's shape runs its bodies after the first as one
+    // stream-ordered loop (RelationalJoin::run_linear): the same bodies, no
+    // host synchronisation between them.
+    RelationalJoin *linear = nullptr;
+    if (ra_ops.size() == 1 && update_relations.size() == 1 && tmp_relations.empty()) {
+        auto *j = std::get_if<RelationalJoin>(&ra_ops[0]);
+        if (j && update_relations[0]->fused_output &&
+            std::find(static_relations.begin(), static_relations.end(), j->inner_rel) !=
+                static_relations.end())
+            linear = j;
+    }
     int iteration_counter = 0;
     while (true) {
+        if (linear && iteration_counter > 0) {
+            Relation *rel = update_relations[0];
+            const long long left = (long long)max_iteration + 1 - iteration_counter;
+            const int bodies = (int)std::min<long long>(left, 1 << 30);
+            std::vector<uint64_t> nn, rr;
+            tuple_size_t full = rel->current_full_size;
+            if (bodies > 0 && linear->run_linear(bodies, &nn, &rr)) {
+                bool fixpoint = false;
+                for (size_t b = 0; b < nn.size(); ++b) {
+                    full += nn[b];
+                    trace.push_back({iteration_counter, 0, rr[b], nn[b], full});
+                    if (verbose())
+                        std::cout << "iteration " << iteration_counter << " relation "
+                                  << rel->name << " finish dedup new tuples : " << nn[b]
+                                  << " delta tuple size: " << nn[b] << " full counts " << full
+                                  << std::endl;
+                    iteration_counter++;
+                    if (nn[b] == 0) {
+                        fixpoint = true;
+                        break;
+                    }
+                }
+                if (fixpoint || iteration_counter > max_iteration) break;
+                continue;
+            }
+            linear = nullptr;  // not that shape any more: one body at a time
+        }
         for (auto &op : ra_ops) {
             if (auto *j = std::get_if<RelationalJoin>(&op)) {
                 (*j)();

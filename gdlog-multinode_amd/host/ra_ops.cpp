@@ -139,6 +139,53 @@ void RelationalJoin::operator()() {
     else accumulate_sorted(output_rel->newt, res, m, detail_time);
 }
 
+bool RelationalJoin::run_linear(int max_bodies, std::vector<uint64_t> *new_tuples,
+                                std::vector<uint64_t> *raw_tuples) {
+    Relation *out = output_rel;
+    if (max_bodies <= 0 || !out || !out->fused_output || out->tmp_flag || disable_load ||
+        outer_rel != out || outer_ver != DELTA || inner_rel == out || inner_ver != FULL ||
+        !gen_map.ok || !pred_map.ok)
+        return false;
+    GHashRelContainer *inner = inner_rel->full;
+    GHashRelContainer *delta = out->delta;
+    GHashRelContainer *full = out->full;
+    if (!inner->keys || inner->tuple_counts == 0) return false;  // the quirk path
+    const int k = (int)delta->index_column_size;
+    if (k < 1 || k > inner->arity || k > delta->arity) return false;
+    // the delta is the log's tail (the fused output appends it to FULL)
+    const tuple_size_t nd = delta->tuple_counts, nf = out->current_full_size;
+    if (nd == 0 || nd > nf || !full->keys) return false;
+    const gdl_index *idx = container_index(inner, k);
+    if (idx->max_run > 256) return false;  // hub runs: the per-body path balances them
+    uint64_t *log = full->keys;
+    uint64_t cap = full->capacity, d0 = nf - nd, d1 = nf;
+    new_tuples->assign(max_bodies, 0);
+    raw_tuples->assign(max_bodies, 0);
+    int bodies = 0;
+    container_drop_index(full);
+    check(gdl_fixpoint_linear(context(), inner->keys, inner->tuple_counts, idx, gen_map.map,
+                              pred_map.pred, &out->full_set, &log, &cap, &d0, &d1, max_bodies,
+                              new_tuples->data(), raw_tuples->data(), &bodies),
+          "gdl_fixpoint_linear");
+    full->keys = log;
+    full->capacity = cap;
+    full->tuple_counts = d1;
+    full->data_raw_row_size = d1;
+    full->sorted = false;
+    out->current_full_size = d1;
+    new_tuples->resize(bodies);
+    raw_tuples->resize(bodies);
+    // the delta after the last body: its new tuples (none at the fixpoint)
+    container_free(delta);
+    if (d1 > d0) {
+        uint64_t *dk = device_alloc_keys(d1 - d0);
+        check(gdl_memcpy_d2d(context(), dk, log + d0, (d1 - d0) * sizeof(uint64_t)),
+              "linear delta");
+        container_take(delta, dk, d1 - d0, d1 - d0, false);
+    }
+    return true;
+}
+
 void RelationalCopy::operator()() {
     GHashRelContainer *src = src_ver == DELTA ? src_rel->delta : src_rel->full;
     if (verbose()) std::cout << "Copy " << src_rel->name << " to " << dest_rel->name << std::endl;

@@ -244,6 +244,24 @@ gdl_status gdl_join_insert(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n_inn
                            gdl_colmap map, gdl_pred pred, gdl_hashset *hs, uint64_t **d_new,
                            uint64_t *new_cap, uint64_t *n_new, uint64_t *n_raw);
 
+/* The loop of a linear recursive rule, out := out U map(inner x delta(out)),
+ * with out's FULL an append log *d_log (capacity *log_cap keys) plus the hash
+ * set hs, and the delta the log's last segment [*delta_begin, *delta_end).
+ * Runs up to max_bodies bodies -- each one gdl_join_insert whose new tuples
+ * are appended to the log and become the next delta -- enqueued back to back
+ * on the stream: the host waits once per 32 bodies (or when the set or the
+ * log must grow), not once per body.  Stops after the first body with no new
+ * tuple (the fixpoint; that body is counted).  h_new / h_raw (max_bodies
+ * entries) get each body's new and produced tuple counts, *bodies the bodies
+ * run; the log and the set are grown as needed.  Equivalent to max_bodies
+ * iterations of LIE::fixpoint_loop (lie.cu:26-274) over a one-join LIE whose
+ * output is the outer relation. */
+gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n_inner,
+                               const gdl_index *inner_idx, gdl_colmap map, gdl_pred pred,
+                               gdl_hashset *hs, uint64_t **d_log, uint64_t *log_cap,
+                               uint64_t *delta_begin, uint64_t *delta_end, int max_bodies,
+                               uint64_t *h_new, uint64_t *h_raw, int *bodies);
+
 /* Batched form of gdl_join_insert: many fixpoint instances share the inner
  * and outer arrays, told apart by the top bits of their keys, and each has
  * its own hash set: a produced key k goes into d_segs[k >> seg_shift]

@@ -53,6 +53,16 @@ struct RelationalJoin {
     }
 
     void operator()();
+
+    // Runs up to max_bodies iterations of a LIE whose only operator is this
+    // join and whose only update relation is its outer (DELTA) and fused
+    // output, with the inner a static FULL (TC's shape, test/tc.cu:80-149), as
+    // one stream-ordered loop (gdl_fixpoint_linear).  The output's delta must
+    // be the last segment of its FULL log.  new_tuples / raw_tuples get each
+    // body's counts.  Returns false, doing nothing, when the join does not
+    // have that shape.
+    bool run_linear(int max_bodies, std::vector<uint64_t> *new_tuples,
+                    std::vector<uint64_t> *raw_tuples);
 };
 
 struct RelationalCopy {

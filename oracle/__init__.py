@@ -54,8 +54,25 @@ def lib():
         L.orc_cspa.argtypes = [_U64P, c_size_t, _U64P, c_size_t, POINTER(Result)]
         L.orc_result_free.restype = None
         L.orc_result_free.argtypes = [POINTER(Result)]
+        L.cpu_tc_omp.restype = c_int
+        L.cpu_tc_omp.argtypes = [_U64P, c_uint64, c_int, c_int, _U64P, _U64P,
+                                 POINTER(c_int), _U64P, c_int]
         _lib = L
     return _lib
+
+
+def tc_omp(edges, threads, max_iteration=-1, trace_cap=1 << 20):
+    """Multi-core CPU baseline of the TC loop (cpu_tc_omp.c): |path|,
+    derived, iterations and per-body |delta| (no tuples)."""
+    e = _arr(edges)
+    tr = np.zeros(trace_cap, dtype=np.uint64)
+    n_out, der = c_uint64(), c_uint64()
+    it = c_int()
+    if lib().cpu_tc_omp(_p(e), e.size, threads, max_iteration, ctypes.byref(n_out),
+                        ctypes.byref(der), ctypes.byref(it), _p(tr), trace_cap) != 0:
+        raise MemoryError("cpu_tc_omp")
+    return dict(n_out=n_out.value, derived=der.value, iterations=it.value,
+                deltas=[int(x) for x in tr[:min(it.value, trace_cap)]])
 
 
 def _p(a):

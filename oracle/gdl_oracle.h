@@ -119,6 +119,15 @@ int orc_sg(const orc_key *edges, size_t n, orc_result *res);
 int orc_cspa(const orc_key *assign, size_t na, const orc_key *deref, size_t nd, orc_result *res);
 void orc_result_free(orc_result *res);
 
+/* Multi-core CPU baseline of the same TC loop (cpu_tc_omp.c): the fixpoint
+ * split by the path's second column over `threads` OpenMP threads.  Returns
+ * |path|, the derived count, the iteration count and the per-body |delta|
+ * (trace_cap entries).  0 on success, -1 when memory ran out.  Checked
+ * against orc_tc in tests/test_oracle.py. */
+int cpu_tc_omp(const uint64_t *edges, uint64_t n, int threads, int max_iteration,
+               uint64_t *n_out, uint64_t *derived, int *iterations, uint64_t *delta_trace,
+               int trace_cap);
+
 #ifdef __cplusplus
 }
 #endif

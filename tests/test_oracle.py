@@ -256,6 +256,25 @@ def test_cspa_matches_naive_datalog(seed):
     assert pairs_of(r["out"]) == VA
 
 
+@pytest.mark.parametrize("name", ["hpc_talk", "data_7035_dups", "er", "tree", "chain"])
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_multicore_cpu_baseline_is_the_same_loop(name, threads):
+    # the CPU baseline (oracle/cpu_tc_omp.c) runs the oracle's TC loop split
+    # by the path's second column: same size, iterations and per-body delta
+    edges = {"hpc_talk": lambda: read_edges("hpc_talk.txt"),
+             "data_7035_dups": lambda: read_edges("data_7035.txt"),
+             "er": lambda: gdl.gen_er(0, 2000, 5000),
+             "tree": lambda: gdl.gen_tree(0, 20000, 1000),
+             "chain": lambda: gdl.pack(np.arange(299), np.arange(1, 300))}[name]()
+    want = oracle.tc(edges)
+    got = oracle.tc_omp(edges, threads)
+    assert got["n_out"] == want["n_out"] and got["iterations"] == want["iterations"]
+    assert got["deltas"] == [t[3] for t in want["trace"]]
+    assert got["derived"] == want["derived"]
+    capped = oracle.tc_omp(edges, threads, max_iteration=2)
+    assert capped["deltas"] == [t[3] for t in oracle.tc(edges, max_iteration=2)["trace"]]
+
+
 def test_cspa_delta_conservation():
     a, d = gdl.gen_cspa(5, 150, 1.1, 200, 100)
     r = oracle.cspa(a, d)
