@@ -93,6 +93,17 @@ def cpu_baseline(kind, inp, max_iteration):
                 sample=f"oracle on {what} ({r['derived']} derived tuples, {dt:.1f} s)")
 
 
+_last_note = [0.0]
+
+
+def note(msg, every=20.0):
+    """Progress line on stderr, at most one per `every` seconds."""
+    now = time.perf_counter()
+    if now - _last_note[0] >= every:
+        _last_note[0] = now
+        print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def pct(xs, q):
     xs = sorted(xs)
     return xs[min(len(xs) - 1, int(round(q / 100.0 * (len(xs) - 1))))]
@@ -120,6 +131,7 @@ def batched(gdl, ctx, n_inst, min_edges=1000, max_edges=50000, window=1000):
     def calib_run():
         with gdl.TcBatch(ctx, keep_keys=False) as b:
             b.run(calib)
+        note("batched TC: autotuning")
     tune = importlib.import_module("gdlog_amd.tune") if "gdlog_amd.tune" in sys.modules else None
     if tune is None:
         spec = importlib.util.spec_from_file_location(
@@ -137,6 +149,7 @@ def batched(gdl, ctx, n_inst, min_edges=1000, max_edges=50000, window=1000):
         r = ctx.run_tc(e, want_keys=False, want_trace=False)
         lat_s.append(time.perf_counter() - t0)
         alone[q] = (r["n_out"], r["iterations"], r["derived"])
+        note(f"batched TC: serial {q + 1}/{len(inst)}")
     serial_s = time.perf_counter() - t0
     ctx.release_cached()
     t0 = time.perf_counter()
@@ -150,6 +163,7 @@ def batched(gdl, ctx, n_inst, min_edges=1000, max_edges=50000, window=1000):
             for r in b.poll():
                 done[r["qid"]] = r
                 lat_b.append(time.perf_counter() - t0)
+            note(f"batched TC: batch {len(done)}/{len(inst)} done")
         info = b.info()
     batch_s = time.perf_counter() - t0
     ctx.release_cached()
@@ -216,6 +230,7 @@ def served(gdl, ctx, n_req, rate_mult=2.0, seed=7, workers=3):
         r = run[kind](a, b)
         service.append(time.perf_counter() - t0)
         alone[i] = (r["n_out"], r["iterations"], r["checksum"])
+        note(f"served mix: one-at-a-time {i + 1}/{n_req}")
     ctx.release_cached()
     serial_qps = n_req / sum(service)
     rate = rate_mult * serial_qps
@@ -236,6 +251,7 @@ def served(gdl, ctx, n_req, rate_mult=2.0, seed=7, workers=3):
                 srv.submit(i, kind, a, b)
                 nxt += 1
             wait_ms = 1000.0 * (arrivals[nxt] - now) if nxt < n_req else 1000.0
+            note(f"served mix: server {len(got)}/{n_req} done, {nxt} submitted")
             r = srv.wait(max(0.0, wait_ms))
             if r is not None:
                 got[r["qid"]] = (time.perf_counter() - t0, r)

@@ -339,6 +339,7 @@ struct LinState {
     u64 hs_limit;                 // members the set may reach (a fraction of its capacity)
     u64 log_cap;                  // log capacity, keys
     u64 max_run;                  // longest inner run: new tuples <= |delta| x max_run
+    u64 delta_limit;              // a larger delta ends the chunk (the grid was sized smaller)
     u32 stop;                     // 0 run, 1 fixpoint, 2 grow first, 3 body limit, 4 overflow
     u32 body;                     // bodies finished in this call
     u32 blocks_done;
@@ -385,7 +386,7 @@ __global__ __launch_bounds__(B) void k_linear_body(const IdxSlot *slots, u64 n_s
     u32 stop = kLinRun;
     if (end > st->log_cap) stop = kLinOverflow;
     else if (nd == 0) stop = kLinFixpoint;
-    else if (b + 1 >= st->max_bodies) stop = kLinLimit;
+    else if (b + 1 >= st->max_bodies || nd > st->delta_limit) stop = kLinLimit;
     else if (size + nd * st->max_run > st->hs_limit || end + nd * st->max_run > st->log_cap)
         stop = kLinGrow;
     st->stop = stop;
@@ -753,8 +754,9 @@ static gdl_status join_insert_impl(gdl_ctx *ctx, const u64 *d_inner, u64 n_inner
         if (bound == 0) return GDL_OK;
     }
     if constexpr (!SEG) {
-        // size for about |outer| new members; the bound only has to fit
-        GDL_TRY(hashset_reserve2(ctx, hs, std::max<u64>(n_outer, 1024), bound));
+        // size for up to 2 |outer| new members at the load factor; the bound
+        // only has to fit
+        GDL_TRY(hashset_reserve2(ctx, hs, std::max<u64>(2 * n_outer, 1024), bound));
         sel.slots = (u64 *)hs->slots;
         sel.mask = hs->capacity - 1;
         if (*n_new + bound > *new_cap || !*d_new) {
@@ -895,8 +897,9 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
         // the set: expected growth at the load factor, the worst body at 0.9;
         // the log: the worst body after the delta, grown geometrically
         GDL_TRY(hashset_reserve2(ctx, hs, std::max<u64>({nd, hs->size / 2, 1024}), worst));
-        const u64 need_log = d1 + std::max<u64>({worst, d1 / 2, 4096});
+        u64 need_log = d1 + std::max<u64>(worst, 4096);
         if (*log_cap < need_log) {
+            need_log = std::max<u64>(need_log, *log_cap + *log_cap / 2);  // geometric
             void *p = nullptr;
             std::string err;
             gdl_status s = ctx->alloc->alloc(need_log * sizeof(u64), &p, &err);
@@ -918,11 +921,13 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
         h.log_cap = *log_cap;
         h.max_run = mr;
         h.max_bodies = (u32)k;
+        // a grid for this delta (about 2 outer tuples per thread); a body
+        // whose delta outgrows it 4x ends the chunk, so the next chunk is
+        // launched wider
+        const u64 want = (2 * nd + LB - 1) / LB;
+        const unsigned g = (unsigned)std::min<u64>(grid_cap(ctx), std::max<u64>(want, 16));
+        h.delta_limit = g >= grid_cap(ctx) ? ~0ull : (u64)g * LB * 8;
         GDL_HIP(ctx, hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
-        // the delta can grow within a chunk: at least 4 blocks per CU
-        const u64 cus = (u64)(ctx->cu_count > 0 ? ctx->cu_count : 256);
-        const unsigned g = (unsigned)std::min<u64>(grid_cap(ctx),
-                                                   std::max<u64>(4 * cus, (4 * nd + LB - 1) / LB));
         for (int i = 0; i < k; ++i) {
             KTimer kt(ctx, K_LINEAR, 0.0);
             hipLaunchKernelGGL(k_linear_body<LB>, dim3(g), dim3(LB), 0, ctx->stream,

@@ -159,22 +159,33 @@ bool RelationalJoin::run_linear(int max_bodies, std::vector<uint64_t> *new_tuple
     if (idx->max_run > 256) return false;  // hub runs: the per-body path balances them
     uint64_t *log = full->keys;
     uint64_t cap = full->capacity, d0 = nf - nd, d1 = nf;
-    new_tuples->assign(max_bodies, 0);
-    raw_tuples->assign(max_bodies, 0);
-    int bodies = 0;
+    new_tuples->clear();
+    raw_tuples->clear();
     container_drop_index(full);
-    check(gdl_fixpoint_linear(context(), inner->keys, inner->tuple_counts, idx, gen_map.map,
-                              pred_map.pred, &out->full_set, &log, &cap, &d0, &d1, max_bodies,
-                              new_tuples->data(), raw_tuples->data(), &bodies),
-          "gdl_fixpoint_linear");
+    // in slices of at most kSlice bodies (the counts' buffers), to the
+    // fixpoint or max_bodies
+    constexpr int kSlice = 4096;
+    std::vector<uint64_t> nb(kSlice), rb(kSlice);
+    gdl_status st = GDL_OK;
+    int left = max_bodies;
+    while (left > 0) {
+        int bodies = 0;
+        st = gdl_fixpoint_linear(context(), inner->keys, inner->tuple_counts, idx, gen_map.map,
+                                 pred_map.pred, &out->full_set, &log, &cap, &d0, &d1,
+                                 std::min(left, kSlice), nb.data(), rb.data(), &bodies);
+        if (st != GDL_OK) break;
+        new_tuples->insert(new_tuples->end(), nb.begin(), nb.begin() + bodies);
+        raw_tuples->insert(raw_tuples->end(), rb.begin(), rb.begin() + bodies);
+        left -= bodies;
+        if (bodies == 0 || nb[bodies - 1] == 0) break;  // the fixpoint
+    }
     full->keys = log;
     full->capacity = cap;
     full->tuple_counts = d1;
     full->data_raw_row_size = d1;
     full->sorted = false;
     out->current_full_size = d1;
-    new_tuples->resize(bodies);
-    raw_tuples->resize(bodies);
+    check(st, "gdl_fixpoint_linear");
     // the delta after the last body: its new tuples (none at the fixpoint)
     container_free(delta);
     if (d1 > d0) {
