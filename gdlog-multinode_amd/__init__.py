@@ -830,7 +830,7 @@ QUERY_TC, QUERY_SG, QUERY_CSPA = 0, 1, 2
 class ServerConfig(Structure):
     _fields_ = [("device", c_int), ("workers", c_int), ("batch_tc", c_int),
                 ("batch_slots", c_uint32), ("node_bits", c_uint32), ("mem_budget", c_uint64),
-                ("keep_keys", c_int)]
+                ("keep_keys", c_int), ("partition_cus", c_int)]
 
 
 class QueryResult(Structure):
@@ -872,14 +872,14 @@ class Server:
     footprint estimate, TC queries continuously batched, the others on
     concurrent worker contexts."""
 
-    def __init__(self, device=0, workers=3, batch_tc=True, batch_slots=64, node_bits=18,
-                 mem_budget=0, keep_keys=False):
+    def __init__(self, device=0, workers=1, batch_tc=True, batch_slots=64, node_bits=18,
+                 mem_budget=0, keep_keys=False, partition_cus=False):
         self.prog = _serve_lib()
         cfg = ServerConfig()
         self.prog.gdl_server_config_default(ctypes.byref(cfg))
         cfg.device, cfg.workers, cfg.batch_tc = device, workers, int(batch_tc)
         cfg.batch_slots, cfg.node_bits, cfg.mem_budget = batch_slots, node_bits, mem_budget
-        cfg.keep_keys = int(keep_keys)
+        cfg.keep_keys, cfg.partition_cus = int(keep_keys), int(partition_cus)
         h = c_void_p()
         st = self.prog.gdl_server_create(ctypes.byref(cfg), ctypes.byref(h))
         if st != GDL_OK:

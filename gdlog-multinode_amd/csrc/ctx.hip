@@ -39,29 +39,33 @@ gdl_status Allocator::alloc(size_t bytes, void **out, std::string *err) {
                    std::to_string(in_use_) + " in use, cap " + std::to_string(cap_) + ")";
         return GDL_ERR_OUT_OF_MEMORY;
     }
-    auto it = free_.find(cls);
+    // best fit among the cached blocks up to twice the request: a cache that
+    // only reused exact size classes grew without bound across workloads
+    auto it = free_.lower_bound(cls);
     void *p = nullptr;
-    if (it != free_.end()) {
+    if (it != free_.end() && it->first <= 2 * cls) {
+        cls = it->first;
         p = it->second;
         free_.erase(it);
         cached_ -= cls;
     } else {
         hipError_t e = hipMalloc(&p, cls);
+        // out of device memory: give cached blocks back, largest first,
+        // until the allocation fits (hipFree waits for the device)
+        while (e != hipSuccess && !free_.empty()) {
+            (void)hipGetLastError();
+            auto big = std::prev(free_.end());
+            (void)hipFree(big->second);
+            cached_ -= big->first;
+            free_.erase(big);
+            e = hipMalloc(&p, cls);
+        }
         if (e != hipSuccess) {
             (void)hipGetLastError();
-            // give the cache back and retry once
-            (void)hipDeviceSynchronize();
-            for (auto &kv : free_) (void)hipFree(kv.second);
-            free_.clear();
-            cached_ = 0;
-            e = hipMalloc(&p, cls);
-            if (e != hipSuccess) {
-                (void)hipGetLastError();
-                if (err)
-                    *err = "hipMalloc of " + std::to_string(cls) +
-                           " bytes failed: " + hipGetErrorString(e);
-                return GDL_ERR_OUT_OF_MEMORY;
-            }
+            if (err)
+                *err = "hipMalloc of " + std::to_string(cls) + " bytes failed: " +
+                       hipGetErrorString(e);
+            return GDL_ERR_OUT_OF_MEMORY;
         }
     }
     live_[p] = cls;
@@ -84,6 +88,13 @@ gdl_status Allocator::free(void *p, std::string *err) {
     in_use_ -= cls;
     free_.emplace(cls, p);
     cached_ += cls;
+    // over the cache limit: drop the largest cached blocks
+    while (cache_limit_ && cached_ > cache_limit_ && !free_.empty()) {
+        auto big = std::prev(free_.end());
+        (void)hipFree(big->second);
+        cached_ -= big->first;
+        free_.erase(big);
+    }
     return GDL_OK;
 }
 
@@ -371,6 +382,31 @@ gdl_status gdl_set_mem_cap(gdl_ctx *ctx, uint64_t cap) {
     if (!ctx) return GDL_ERR_INVALID_ARG;
     ctx->alloc->set_cap(cap);
     ctx->cfg.mem_cap_bytes = cap;
+    return GDL_OK;
+}
+
+gdl_status gdl_set_cu_mask(gdl_ctx *ctx, const uint32_t *mask, uint32_t words) {
+    if (!ctx || (words && !mask)) return GDL_ERR_INVALID_ARG;
+    uint32_t n = 0;
+    for (uint32_t i = 0; i < words; ++i) n += (uint32_t)__builtin_popcount(mask[i]);
+    if (words && n == 0) return fail(ctx, GDL_ERR_INVALID_ARG, "CU mask selects no CU");
+    GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    hipStream_t s = nullptr;
+    if (words) GDL_HIP(ctx, hipExtStreamCreateWithCUMask(&s, words, mask));
+    else GDL_HIP(ctx, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    (void)hipStreamDestroy(ctx->stream);
+    ctx->stream = s;
+    // grids are sized per CU: count the CUs this stream can use
+    hipDeviceProp_t prop;
+    int all = 0;
+    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) all = prop.multiProcessorCount;
+    ctx->cu_count = words ? (int)std::min<uint32_t>(n, (uint32_t)std::max(all, 1)) : all;
+    return GDL_OK;
+}
+
+gdl_status gdl_set_cache_limit(gdl_ctx *ctx, uint64_t bytes) {
+    if (!ctx) return GDL_ERR_INVALID_ARG;
+    ctx->alloc->set_cache_limit(bytes);
     return GDL_OK;
 }
 

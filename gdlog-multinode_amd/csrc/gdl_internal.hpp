@@ -38,6 +38,7 @@ class Allocator {
     uint64_t cached() const { return cached_; }
     void set_cap(uint64_t cap) { cap_ = cap; }
     uint64_t cap() const { return cap_; }
+    void set_cache_limit(uint64_t bytes) { cache_limit_ = bytes; }
     static size_t size_class(size_t bytes);
 
   private:
@@ -45,7 +46,7 @@ class Allocator {
     std::mutex mu_;
     std::multimap<size_t, void *> free_;
     std::unordered_map<void *, size_t> live_;
-    uint64_t in_use_ = 0, peak_ = 0, cached_ = 0, cap_ = 0;
+    uint64_t in_use_ = 0, peak_ = 0, cached_ = 0, cap_ = 0, cache_limit_ = 0;
 };
 
 // ------------------------------------------------------------------ stats

@@ -342,36 +342,34 @@ struct LinState {
     u64 delta_limit;              // a larger delta ends the chunk (the grid was sized smaller)
     u32 stop;                     // 0 run, 1 fixpoint, 2 grow first, 3 body limit, 4 overflow
     u32 body;                     // bodies finished in this call
-    u32 blocks_done;
     u32 max_bodies;
 };
 enum : u32 { kLinRun = 0, kLinFixpoint = 1, kLinGrow = 2, kLinLimit = 3, kLinOverflow = 4 };
 
 // One body of out := out U map(inner x delta) with FULL = hash set + append
 // log: the delta is the log's last segment, new tuples are appended after it.
-// The last block to finish records the body's counts and moves the delta
-// window; a stop code makes the bodies enqueued after it return at once.
+// A stop code set by an earlier body's update makes it return at once.
 template <int B>
 __global__ __launch_bounds__(B) void k_linear_body(const IdxSlot *slots, u64 n_slots,
                                                    const u64 *inner, u64 n_inner, int shift,
                                                    int arity, int s0, int s1, int pop, int plhs,
                                                    int prhs, u64 *hs_slots, u64 hs_mask, u64 *log,
-                                                   LinState *st, u64 *counts) {
-    __shared__ bool s_last;
-    if (st->stop != kLinRun) return;  // set by an earlier body: the same for every block
+                                                   LinState *st) {
+    if (st->stop != kLinRun) return;  // the same for every block
     const u64 d0 = st->delta_begin, d1 = st->delta_end;
     join_insert_blocks<B, false>(slots, n_slots, inner, n_inner, shift, log + d0, d1 - d0, arity,
                                  s0, s1, pop, plhs, prhs, HsSel{hs_slots, hs_mask, nullptr, 0, 0},
                                  log, st->log_cap, &st->cursor);
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0)
-        s_last = atomicAdd(&st->blocks_done, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!s_last || threadIdx.x != 0) return;
-    __threadfence();
-    const u64 end = atomicAdd(&st->cursor, 0ull);
-    const u64 produced = atomicAdd(&st->produced, 0ull);
+}
+
+// After a body (stream order makes its atomics and stores complete): record
+// its counts, move the delta window to the tuples it appended, and decide
+// whether the next enqueued body runs.
+__global__ void k_linear_update(LinState *st, u64 *counts) {
+    if (threadIdx.x != 0 || st->stop != kLinRun) return;
+    const u64 d1 = st->delta_end;
+    const u64 end = st->cursor;
+    const u64 produced = st->produced;
     const u32 b = st->body;
     const u64 nd = end > d1 ? end - d1 : 0;
     counts[2 * b] = nd;
@@ -381,7 +379,6 @@ __global__ __launch_bounds__(B) void k_linear_body(const IdxSlot *slots, u64 n_s
     st->delta_begin = d1;
     st->delta_end = end;
     st->produced = 0;
-    st->blocks_done = 0;
     st->body = b + 1;
     u32 stop = kLinRun;
     if (end > st->log_cap) stop = kLinOverflow;
@@ -390,7 +387,6 @@ __global__ __launch_bounds__(B) void k_linear_body(const IdxSlot *slots, u64 n_s
     else if (size + nd * st->max_run > st->hs_limit || end + nd * st->max_run > st->log_cap)
         stop = kLinGrow;
     st->stop = stop;
-    __threadfence();
 }
 
 // Load-balanced variant for inner relations with long runs: outputs are
@@ -890,7 +886,7 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
     GDL_TRY(scr.get(2 * kChunk, &counts));
     LinState *st = (LinState *)st_words;
     u64 d0 = *delta_begin, d1 = *delta_end;
-    int done = 0;
+    int done = 0, ramp = 4;
     while (done < max_bodies) {
         const u64 nd = d1 - d0;
         const u64 worst = nd > (~0ull >> 20) / mr ? (~0ull >> 20) : nd * mr;
@@ -911,7 +907,7 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
             *d_log = (u64 *)p;
             *log_cap = need_log;
         }
-        const int k = std::min(kChunk, max_bodies - done);
+        const int k = std::min({ramp, kChunk, max_bodies - done});
         LinState h{};
         h.cursor = d1;
         h.delta_begin = d0;
@@ -929,13 +925,17 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
         h.delta_limit = g >= grid_cap(ctx) ? ~0ull : (u64)g * LB * 8;
         GDL_HIP(ctx, hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
         for (int i = 0; i < k; ++i) {
-            KTimer kt(ctx, K_LINEAR, 0.0);
-            hipLaunchKernelGGL(k_linear_body<LB>, dim3(g), dim3(LB), 0, ctx->stream,
-                               (const IdxSlot *)inner_idx->slots, inner_idx->n_slots, d_inner,
-                               n_inner, shift, map.arity, map.src[0],
-                               map.arity == 2 ? map.src[1] : 0, pred.op, pred.lhs, pred.rhs,
-                               (u64 *)hs->slots, hs->capacity - 1, *d_log, st, counts);
-            GDL_TRY(check_launch(ctx, "k_linear_body"));
+            {
+                KTimer kt(ctx, K_LINEAR, 0.0);
+                hipLaunchKernelGGL(k_linear_body<LB>, dim3(g), dim3(LB), 0, ctx->stream,
+                                   (const IdxSlot *)inner_idx->slots, inner_idx->n_slots,
+                                   d_inner, n_inner, shift, map.arity, map.src[0],
+                                   map.arity == 2 ? map.src[1] : 0, pred.op, pred.lhs, pred.rhs,
+                                   (u64 *)hs->slots, hs->capacity - 1, *d_log, st);
+                GDL_TRY(check_launch(ctx, "k_linear_body"));
+            }
+            hipLaunchKernelGGL(k_linear_update, dim3(1), dim3(64), 0, ctx->stream, st, counts);
+            GDL_TRY(check_launch(ctx, "k_linear_update"));
         }
         LinState r{};
         u64 c[2 * kChunk];
@@ -958,6 +958,9 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
         d0 = r.delta_begin;
         d1 = r.delta_end;
         if (r.stop == kLinFixpoint || r.body == 0) break;
+        // chunks double while every enqueued body runs (a stopped chunk's
+        // remaining launches return at once, but still cost a launch)
+        ramp = (int)r.body == k ? std::min(2 * ramp, kChunk) : std::max(4, (int)r.body);
     }
     *bodies = done;
     *delta_begin = d0;

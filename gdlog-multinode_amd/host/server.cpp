@@ -5,9 +5,9 @@
 // submission order when their estimate fits the budget beside the running
 // ones and a place is free (a batch slot for TC, else an idle worker); a
 // query that does not fit may be passed by later ones at most kMaxPasses
-// times, then it blocks the queue until it fits.  A query that outgrew its
-// reservation is rerun alone: it waits until nothing runs and nothing is
-// admitted beside it.
+// times, then it blocks the queue until it fits.  A query that ran out of
+// memory (its worker's cap, or the device) is rerun alone: it waits until
+// nothing runs and nothing is admitted beside it.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -29,6 +29,7 @@ namespace {
 
 constexpr int kMaxPasses = 8;
 constexpr uint64_t kMinReservation = 64ull << 20;
+constexpr uint64_t kBatchMaxTuples = 8ull << 20;  // larger TC queries run on a worker
 
 double clock_ms() {
     using namespace std::chrono;
@@ -180,19 +181,44 @@ struct gdl_server {
         cv_done.notify_all();
     }
 
-    void worker_main() {
+    // Executor e of n gets CU i for every i with i mod n == e.
+    gdl_status partition(gdl_ctx *ctx, int e) {
+        const int n = cfg.workers + (cfg.batch_tc ? 1 : 0);
+        if (!cfg.partition_cus || n < 2) return GDL_OK;
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, cfg.device) !=
+                hipSuccess ||
+            cus < n)
+            return GDL_OK;  // too few CUs to split: share them
+        std::vector<uint32_t> mask((cus + 31) / 32, 0u);
+        for (int i = e; i < cus; i += n) mask[i / 32] |= 1u << (i % 32);
+        return gdl_set_cu_mask(ctx, mask.data(), (uint32_t)mask.size());
+    }
+
+    void worker_main(int e) {
         gdl_config c;
         gdl_config_default(&c);
         c.device = cfg.device;
         gdl_ctx *ctx = nullptr;
         gdl_status st = gdl_ctx_create(&c, &ctx);
+        if (st == GDL_OK) st = partition(ctx, e);
+        if (st == GDL_OK) {
+            // keep freed blocks for the next query (returning them to HIP
+            // synchronises the whole device, stalling every other worker),
+            // up to half this worker's share of the budget
+            const uint64_t share = budget / (uint64_t)(cfg.workers + (cfg.batch_tc ? 1 : 0));
+            gdl_set_cache_limit(ctx, share / 2);
+        }
         {
             std::lock_guard<std::mutex> g(mu);
             if (st != GDL_OK) init_status = st;
             ++ready_threads;
             cv_ready.notify_all();
         }
-        if (st != GDL_OK) return;
+        if (st != GDL_OK) {
+            if (ctx) gdl_ctx_destroy(ctx);
+            return;
+        }
         for (;;) {
             QueryPtr q;
             {
@@ -203,14 +229,16 @@ struct gdl_server {
                 work.pop_front();
             }
             const bool alone = q->alone;
-            gdl_set_mem_cap(ctx, alone ? 0 : std::max(q->est, kMinReservation));
+            // admission counts the estimate; the cap only stops a runaway
+            // query (4x its estimate, at least an even share of the budget)
+            const uint64_t share = budget / (uint64_t)(cfg.workers + (cfg.batch_tc ? 1 : 0));
+            gdl_set_mem_cap(ctx, alone ? 0 : std::max({4 * q->est, share, kMinReservation}));
             ProgramOutput out;
             out.want_keys = cfg.keep_keys != 0;
             std::string err;
             gdl_status rs = run_program_host(ctx, q->kind, q->a.data(), q->a.size(),
                                              q->b.data(), q->b.size(), &out, &err);
             gdl_set_mem_cap(ctx, 0);
-            gdl_release_cached(ctx);
             if (rs == GDL_ERR_OUT_OF_MEMORY && !alone) {
                 // outgrew its reservation: rerun alone, first in line
                 std::lock_guard<std::mutex> g(mu);
@@ -249,6 +277,7 @@ struct gdl_server {
         gdl_ctx *ctx = nullptr;
         gdl_tcb *b = nullptr;
         gdl_status st = gdl_ctx_create(&c, &ctx);
+        if (st == GDL_OK) st = partition(ctx, cfg.workers);
         if (st == GDL_OK) {
             gdl_tcb_config bc;
             gdl_tcb_config_default(&bc);
@@ -343,12 +372,13 @@ extern "C" {
 void gdl_server_config_default(gdl_server_config *cfg) {
     if (!cfg) return;
     cfg->device = 0;
-    cfg->workers = 3;
+    cfg->workers = 1;
     cfg->batch_tc = 1;
     cfg->batch_slots = 64;
     cfg->node_bits = 18;
     cfg->mem_budget = 0;
     cfg->keep_keys = 0;
+    cfg->partition_cus = 0;
 }
 
 uint64_t gdl_query_estimate(int kind, const uint64_t *a, uint64_t na, const uint64_t *b,
@@ -380,7 +410,8 @@ gdl_status gdl_server_create(const gdl_server_config *cfg_in, gdl_server **out) 
     }
     const int n_threads = cfg.workers + (cfg.batch_tc ? 1 : 0);
     s->idle_workers = cfg.workers;
-    for (int i = 0; i < cfg.workers; ++i) s->threads.emplace_back([p = s.get()] { p->worker_main(); });
+    for (int i = 0; i < cfg.workers; ++i)
+        s->threads.emplace_back([p = s.get(), i] { p->worker_main(i); });
     if (cfg.batch_tc) s->threads.emplace_back([p = s.get()] { p->batch_main(); });
     gdl_status st;
     {
@@ -421,8 +452,12 @@ gdl_status gdl_server_submit(gdl_server *s, uint64_t qid, int kind, const uint64
     q->a.assign(a, a + na);
     if (kind == GDL_QUERY_CSPA) q->b.assign(b, b + nb);
     q->est_tuples = estimate_tuples(kind, a, na, b, nb, &q->n_nodes);
+    // the batch is for many small instances; a large one (a deep tree) runs
+    // the stream-ordered linear loop on a worker instead of holding the
+    // batch's host-driven bodies for hundreds of iterations
     q->to_batch = kind == GDL_QUERY_TC && s->cfg.batch_tc &&
-                  (uint64_t)q->n_nodes <= (1ull << s->cfg.node_bits);
+                  (uint64_t)q->n_nodes <= (1ull << s->cfg.node_bits) &&
+                  q->est_tuples <= kBatchMaxTuples;
     if (q->to_batch) {
         // the batch's own accounting: hash set at load 1/2, delta share, edges
         q->est = 8 * (pow2_ceil(2 * q->est_tuples) + 1) + 8 * q->est_tuples + 32 * na +

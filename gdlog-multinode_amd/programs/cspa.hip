@@ -36,7 +36,8 @@ __device__ tuple_generator_hook join_01_11_device = join_01_11;
 template <typename T>
 static T host_hook(const T &device_symbol) {
     T h;
-    (void)hipMemcpyFromSymbol(&h, HIP_SYMBOL(device_symbol), sizeof(T));
+    (void)hipMemcpyFromSymbol(&h, (const void *)&device_symbol, sizeof(T), 0,
+                              hipMemcpyDeviceToHost);
     return h;
 }
 

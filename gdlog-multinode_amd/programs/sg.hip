@@ -33,11 +33,11 @@ void sg_program(const EdgeInput &graph, int grid_size, int block_size, ProgramOu
     init_scc.add_relations(edge_2__1_2, true);
     init_scc.add_relations(sg_2__1_2, false);
     tuple_generator_hook reorder_path_host;
-    (void)hipMemcpyFromSymbol(&reorder_path_host, HIP_SYMBOL(reorder_path_device),
-                              sizeof(tuple_generator_hook));
+    (void)hipMemcpyFromSymbol(&reorder_path_host, (const void *)&reorder_path_device,
+                              sizeof(tuple_generator_hook), 0, hipMemcpyDeviceToHost);
     tuple_predicate tuple_pred_eq_11_host;
-    (void)hipMemcpyFromSymbol(&tuple_pred_eq_11_host, HIP_SYMBOL(tuple_pred_eq_11_device),
-                              sizeof(tuple_predicate));
+    (void)hipMemcpyFromSymbol(&tuple_pred_eq_11_host, (const void *)&tuple_pred_eq_11_device,
+                              sizeof(tuple_predicate), 0, hipMemcpyDeviceToHost);
     init_scc.add_ra(RelationalJoin(edge_2__1_2, FULL, edge_2__1_2, FULL, sg_2__1_2,
                                    reorder_path_host, tuple_pred_eq_11_host, LEFT, grid_size,
                                    block_size, join_detail));
@@ -52,8 +52,8 @@ void sg_program(const EdgeInput &graph, int grid_size, int block_size, ProgramOu
     sg_lie.add_relations(sg_2__1_2, false);
     sg_lie.add_tmp_relation(tmp);
     tuple_generator_hook reorder_path1_host;
-    (void)hipMemcpyFromSymbol(&reorder_path1_host, HIP_SYMBOL(reorder_path1_device),
-                              sizeof(tuple_generator_hook));
+    (void)hipMemcpyFromSymbol(&reorder_path1_host, (const void *)&reorder_path1_device,
+                              sizeof(tuple_generator_hook), 0, hipMemcpyDeviceToHost);
     sg_lie.add_ra(RelationalJoin(edge_2__1_2, FULL, sg_2__1_2, DELTA, tmp, reorder_path1_host,
                                  nullptr, LEFT, grid_size, block_size, join_detail));
     sg_lie.add_ra(RelationalJoin(edge_2__1_2, FULL, tmp, NEWT, sg_2__1_2, reorder_path1_host,

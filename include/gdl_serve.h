@@ -10,10 +10,12 @@
  *     program (include/gdl_programs.h) on one of `workers` worker threads,
  *     each with its own engine context and stream, so their kernels overlap
  *     on the device.
- * Relations stay device-resident for the whole query.  A worker's context
- * is capped at the query's reservation; a query that outgrows it is rerun
- * alone (no cap, nothing else admitted beside it), so an estimate that was
- * too small costs time, not correctness.  Each result is the program's
+ * Relations stay device-resident for the whole query.  Admission counts the
+ * estimates; a worker's context is capped at 4x its query's estimate (at
+ * least an even share of the budget) so a runaway query cannot starve the
+ * others, and a query that runs out of memory is rerun alone (no cap,
+ * nothing else admitted beside it): an estimate that was too small costs
+ * time, not correctness.  Each result is the program's
  * result alone: the same tuples, per-iteration |delta| and |full|, and
  * iteration counts.
  *
@@ -36,13 +38,20 @@ enum { GDL_QUERY_TC = 0, GDL_QUERY_SG = 1, GDL_QUERY_CSPA = 2 };
 
 typedef struct gdl_server_config {
     int device;
-    int workers;          /* program worker threads (default 3) */
+    int workers;          /* program worker threads (default 1: two or more heavy
+                             queries sharing the device slowed each other down
+                             several-fold on MI355X, see the design notes) */
     int batch_tc;         /* 1: TC queries join one continuous TC batch (default 1) */
     uint32_t batch_slots; /* instances of that batch (default 64) */
     uint32_t node_bits;   /* batch: ids per instance < 2^node_bits (default 18) */
     uint64_t mem_budget;  /* device bytes the admitted estimates may take
                              (0: 80% of the device memory free at create) */
     int keep_keys;        /* 1: results keep the sorted output tuples */
+    int partition_cus;    /* 1: each worker and the batch get their own CUs
+                             (gdl_set_cu_mask, CU i to executor i mod executors),
+                             so one query's kernels never queue behind another's;
+                             0: all share every CU (default 0: the partitioned
+                             configuration has not been validated on the device) */
 } gdl_server_config;
 
 typedef struct gdl_query_result {
