@@ -209,6 +209,10 @@ __device__ __forceinline__ void block_final_flush(u64 (*s_stage)[kStage], u32 st
         }
         s_base = tot ? atomicAdd(&counters[0], (unsigned long long)tot) : 0ull;
         if (prod) atomicAdd(&counters[1], (unsigned long long)prod);
+        // both counter updates complete (acknowledged at the device's
+        // coherence point) before this thread goes on: a caller may count
+        // finished blocks after this function and read the counters then
+        __builtin_amdgcn_s_waitcnt(0);
     }
     __syncthreads();
     u64 off = s_base;
@@ -342,34 +346,19 @@ struct LinState {
     u64 delta_limit;              // a larger delta ends the chunk (the grid was sized smaller)
     u32 stop;                     // 0 run, 1 fixpoint, 2 grow first, 3 body limit, 4 overflow
     u32 body;                     // bodies finished in this call
+    u32 blocks_done;              // blocks of the running body that finished
     u32 max_bodies;
 };
 enum : u32 { kLinRun = 0, kLinFixpoint = 1, kLinGrow = 2, kLinLimit = 3, kLinOverflow = 4 };
 
-// One body of out := out U map(inner x delta) with FULL = hash set + append
-// log: the delta is the log's last segment, new tuples are appended after it.
-// A stop code set by an earlier body's update makes it return at once.
-template <int B>
-__global__ __launch_bounds__(B) void k_linear_body(const IdxSlot *slots, u64 n_slots,
-                                                   const u64 *inner, u64 n_inner, int shift,
-                                                   int arity, int s0, int s1, int pop, int plhs,
-                                                   int prhs, u64 *hs_slots, u64 hs_mask, u64 *log,
-                                                   LinState *st) {
-    if (st->stop != kLinRun) return;  // the same for every block
-    const u64 d0 = st->delta_begin, d1 = st->delta_end;
-    join_insert_blocks<B, false>(slots, n_slots, inner, n_inner, shift, log + d0, d1 - d0, arity,
-                                 s0, s1, pop, plhs, prhs, HsSel{hs_slots, hs_mask, nullptr, 0, 0},
-                                 log, st->log_cap, &st->cursor);
-}
-
-// After a body (stream order makes its atomics and stores complete): record
-// its counts, move the delta window to the tuples it appended, and decide
-// whether the next enqueued body runs.
-__global__ void k_linear_update(LinState *st, u64 *counts) {
-    if (threadIdx.x != 0 || st->stop != kLinRun) return;
-    const u64 d1 = st->delta_end;
-    const u64 end = st->cursor;
-    const u64 produced = st->produced;
+// End of a body (thread 0 of the last block to finish): record its counts,
+// move the delta window to the tuples it appended, and decide whether the
+// next enqueued body runs.  Every block's cursor / produced updates are
+// complete before it counts itself finished, so the atomic reads below see
+// the final values; the plain stores reach the next body at its launch.
+__device__ void linear_epilogue(LinState *st, u64 *counts, u64 d1) {
+    const u64 end = atomicAdd(&st->cursor, 0ull);
+    const u64 produced = atomicAdd(&st->produced, 0ull);
     const u32 b = st->body;
     const u64 nd = end > d1 ? end - d1 : 0;
     counts[2 * b] = nd;
@@ -379,6 +368,7 @@ __global__ void k_linear_update(LinState *st, u64 *counts) {
     st->delta_begin = d1;
     st->delta_end = end;
     st->produced = 0;
+    st->blocks_done = 0;
     st->body = b + 1;
     u32 stop = kLinRun;
     if (end > st->log_cap) stop = kLinOverflow;
@@ -387,6 +377,28 @@ __global__ void k_linear_update(LinState *st, u64 *counts) {
     else if (size + nd * st->max_run > st->hs_limit || end + nd * st->max_run > st->log_cap)
         stop = kLinGrow;
     st->stop = stop;
+}
+
+// One body of out := out U map(inner x delta) with FULL = hash set + append
+// log: the delta is the log's last segment, new tuples are appended after it.
+// A stop code set by an earlier body makes it return at once.
+template <int B>
+__global__ __launch_bounds__(B) void k_linear_body(const IdxSlot *slots, u64 n_slots,
+                                                   const u64 *inner, u64 n_inner, int shift,
+                                                   int arity, int s0, int s1, int pop, int plhs,
+                                                   int prhs, u64 *hs_slots, u64 hs_mask, u64 *log,
+                                                   LinState *st, u64 *counts) {
+    __shared__ bool s_last;
+    if (st->stop != kLinRun) return;  // the same for every block
+    const u64 d0 = st->delta_begin, d1 = st->delta_end;
+    join_insert_blocks<B, false>(slots, n_slots, inner, n_inner, shift, log + d0, d1 - d0, arity,
+                                 s0, s1, pop, plhs, prhs, HsSel{hs_slots, hs_mask, nullptr, 0, 0},
+                                 log, st->log_cap, &st->cursor);
+    // join_insert_blocks ends with this block's counter updates acknowledged
+    // (thread 0, before a barrier)
+    if (threadIdx.x == 0) s_last = atomicAdd(&st->blocks_done, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (s_last && threadIdx.x == 0) linear_epilogue(st, counts, d1);
 }
 
 // Load-balanced variant for inner relations with long runs: outputs are
@@ -890,9 +902,11 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
     while (done < max_bodies) {
         const u64 nd = d1 - d0;
         const u64 worst = nd > (~0ull >> 20) / mr ? (~0ull >> 20) : nd * mr;
-        // the set: expected growth at the load factor, the worst body at 0.9;
-        // the log: the worst body after the delta, grown geometrically
-        GDL_TRY(hashset_reserve2(ctx, hs, std::max<u64>({nd, hs->size / 2, 1024}), worst));
+        // the set: room for 2x its size more at the load factor (a grow stop
+        // costs a host wait and a rehash, so it grows at least 3x at a time),
+        // the worst body at 0.9; the log: the worst body after the delta,
+        // grown geometrically
+        GDL_TRY(hashset_reserve2(ctx, hs, std::max<u64>({nd, 2 * hs->size, 1024}), worst));
         u64 need_log = d1 + std::max<u64>(worst, 4096);
         if (*log_cap < need_log) {
             need_log = std::max<u64>(need_log, *log_cap + *log_cap / 2);  // geometric
@@ -925,17 +939,13 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
         h.delta_limit = g >= grid_cap(ctx) ? ~0ull : (u64)g * LB * 8;
         GDL_HIP(ctx, hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
         for (int i = 0; i < k; ++i) {
-            {
-                KTimer kt(ctx, K_LINEAR, 0.0);
-                hipLaunchKernelGGL(k_linear_body<LB>, dim3(g), dim3(LB), 0, ctx->stream,
-                                   (const IdxSlot *)inner_idx->slots, inner_idx->n_slots,
-                                   d_inner, n_inner, shift, map.arity, map.src[0],
-                                   map.arity == 2 ? map.src[1] : 0, pred.op, pred.lhs, pred.rhs,
-                                   (u64 *)hs->slots, hs->capacity - 1, *d_log, st);
-                GDL_TRY(check_launch(ctx, "k_linear_body"));
-            }
-            hipLaunchKernelGGL(k_linear_update, dim3(1), dim3(64), 0, ctx->stream, st, counts);
-            GDL_TRY(check_launch(ctx, "k_linear_update"));
+            KTimer kt(ctx, K_LINEAR, 0.0);
+            hipLaunchKernelGGL(k_linear_body<LB>, dim3(g), dim3(LB), 0, ctx->stream,
+                               (const IdxSlot *)inner_idx->slots, inner_idx->n_slots, d_inner,
+                               n_inner, shift, map.arity, map.src[0],
+                               map.arity == 2 ? map.src[1] : 0, pred.op, pred.lhs, pred.rhs,
+                               (u64 *)hs->slots, hs->capacity - 1, *d_log, st, counts);
+            GDL_TRY(check_launch(ctx, "k_linear_body"));
         }
         LinState r{};
         u64 c[2 * kChunk];
