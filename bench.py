@@ -48,6 +48,22 @@ def _pkg():
     return mod
 
 
+def profiled_traffic(kernel, workload_tag):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 --pmc summary profiles/<kernel>_<workload>_traffic.md
+    (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by what the counter reads
+    for a plain device copy of known size), or None when there is none."""
+    path = os.path.join(REPO, "profiles", f"{kernel}_{workload_tag}_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def cpu_threads():
     """Host threads for the CPU baseline: this process's CPU share, at most 16
     (the share of one GPU on a shared box)."""
@@ -396,10 +412,14 @@ def main():
         avg_ms = s["total_ms"] / s["launches"]
         avg_bytes = s["alg_bytes"] / s["launches"]
         achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+        traffic = profiled_traffic(name, desc.split(":")[0])
         roofline = dict(bound="hbm", kernel=name, achieved=round(achieved, 2),
                         peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 5),
-                        traffic=None, avg_launch_us=round(avg_ms * 1e3, 3),
+                        traffic=traffic, avg_launch_us=round(avg_ms * 1e3, 3),
                         avg_alg_bytes=round(avg_bytes))
+        if traffic is not None:
+            roofline["traffic_unit"] = "HBM bytes per launch (rocprofv3 --pmc, profiles/)"
+            roofline["traffic_gbs"] = round(traffic / (avg_ms * 1e-3) / 1e9, 2)
     # whole-step HBM fraction: all algorithmic bytes over the step time
     tot_bytes = sum(v["alg_bytes"] for v in stats.values()) / args.steps
     kernel_ms = sum(v["total_ms"] for v in stats.values()) / args.steps

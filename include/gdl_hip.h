@@ -103,7 +103,8 @@ gdl_status gdl_set_mem_cap(gdl_ctx *ctx, uint64_t cap_bytes);
  * bit i = CU i; words = 0: all CUs again), so concurrent contexts on one
  * device can run on disjoint CU sets instead of queueing behind each other's
  * kernels.  Waits for the context's pending work first; grids are then sized
- * for the selected CU count. */
+ * for the selected CU count.  Experimental: no test exercises it on a device
+ * yet, and the server does not use it unless asked (partition_cus). */
 gdl_status gdl_set_cu_mask(gdl_ctx *ctx, const uint32_t *mask, uint32_t words);
 /* Bytes of freed blocks the allocator keeps for reuse (0 = no limit); past
  * it the largest cached blocks go back to HIP.  Freed blocks are reused best
