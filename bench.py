@@ -163,7 +163,10 @@ def batched(gdl, ctx, n_inst, min_edges=1000, max_edges=50000, window=1000):
         sys.modules["gdlog_amd.tune"] = tune
         spec.loader.exec_module(tune)
     tuned = tune.autotune(ctx, calib_run, key=f"tc_batched_{min_edges}_{max_edges}",
-                          use_cache=False)
+                          use_cache=False, space=tune.BATCH_SPACE, start=tune.BATCH_DEFAULT)
+    # one at a time: the single-query defaults (each query has the device to
+    # itself, so the sparser hash table of DEFAULT pays)
+    ctx.set_tuning(**tune.DEFAULT)
     ctx.release_cached()
     ctx.sync()
     t0 = time.perf_counter()
@@ -174,6 +177,7 @@ def batched(gdl, ctx, n_inst, min_edges=1000, max_edges=50000, window=1000):
         alone[q] = (r["n_out"], r["iterations"], r["derived"])
         note(f"batched TC: serial {q + 1}/{len(inst)}")
     serial_s = time.perf_counter() - t0
+    ctx.set_tuning(**tuned["tuning"])
     ctx.release_cached()
     t0 = time.perf_counter()
     lat_b, done = [], {}

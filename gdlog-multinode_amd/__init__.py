@@ -322,7 +322,7 @@ class DeviceArray:
 
 
 class Context:
-    def __init__(self, device=0, mem_cap_bytes=0, load_factor=0.5, collect_stats=False,
+    def __init__(self, device=0, mem_cap_bytes=0, load_factor=0.35, collect_stats=False,
                  verbose=False):
         self.eng, self.prog = load()
         cfg = Config()

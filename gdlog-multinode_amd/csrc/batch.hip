@@ -227,8 +227,15 @@ static u64 pow2_ceil(u64 x) {
 // Device bytes an instance is expected to hold: its hash set at the load
 // factor (power-of-two capacity), its share of the two delta buffers, its
 // edges and the transient arrays of admission.
+// Load factor of the instances' hash sets: the context's, but at least 1/2.
+// A batch holds many instances' sets at once, and they are sized for each
+// body's worst case, so memory (not probe length) is what limits it.
+static double batch_load(const gdl_ctx *ctx) {
+    return std::min(0.8, std::max(0.5, (double)ctx->cfg.load_factor));
+}
+
 static u64 footprint_bytes(const gdl_tcb *b, u64 est, u64 n_edges) {
-    const double lf = std::min(0.8, (double)b->ctx->cfg.load_factor);
+    const double lf = batch_load(b->ctx);
     const u64 hs = 8 * (pow2_ceil((u64)std::ceil((double)est / lf)) + 1);
     return hs + 16 * est / 2 + 32 * n_edges + (1ull << 16);
 }
@@ -349,7 +356,8 @@ static gdl_status tcb_admit(gdl_tcb *b, u64 qid, const u64 *h_edges, u64 n, u32 
         GDL_HIP(ctx, hipMemcpyAsync(d_raw, raw.data(), n * sizeof(u64), hipMemcpyHostToDevice,
                                     ctx->stream));
         // FULL = the distinct rows; the first |FULL| raw rows are the delta
-        GDL_TRY(gdl_hashset_create(ctx, est, &in->hs));
+        in->hs = gdl_hashset{};
+        GDL_TRY(hashset_reserve2(ctx, &in->hs, est, est, batch_load(ctx)));
         u64 n_full = 0;
         GDL_TRY(hashset_insert_new(ctx, &in->hs, d_raw, n, d_tmp, &n_full));
         GDL_TRY(ensure_cap(ctx, &b->delta, &b->cap_delta, b->n_delta + n_full, b->n_delta));
@@ -415,7 +423,8 @@ static gdl_status tcb_step(gdl_tcb *b, u32 *n_active_out) {
         if (exact) bound = std::min(bound, b->h_hist[in->slot]);
         const u64 cap0 = in->hs.capacity;
         const void *s0 = in->hs.slots;
-        GDL_TRY(hashset_reserve2(ctx, &in->hs, std::max<u64>(in->delta, 1024), bound));
+        GDL_TRY(hashset_reserve2(ctx, &in->hs, std::max<u64>(in->delta, 1024), bound,
+                                 batch_load(ctx)));
         if (in->hs.capacity != cap0 || in->hs.slots != s0) {
             b->h_segs[in->slot] = gdl_hs_seg{in->hs.slots, in->hs.capacity - 1};
             b->segs_dirty = true;

@@ -208,7 +208,7 @@ void gdl_config_default(gdl_config *cfg) {
     memset(cfg, 0, sizeof(*cfg));
     cfg->device = 0;
     cfg->mem_cap_bytes = 0;
-    cfg->load_factor = 0.5f;
+    cfg->load_factor = GDL_DEFAULT_LOAD_FACTOR;
     cfg->collect_stats = 0;
     cfg->verbose = 0;
 }

@@ -102,7 +102,7 @@ struct gdl_ctx {
     uint64_t *h_pinned = nullptr;  // pinned staging for scalars read back
     uint32_t *d_err = nullptr;     // kernel-side error word
     int cu_count = 0;
-    gdl_tuning tune{0.5f, 8, 8, 256};  // load_factor mirrors cfg.load_factor
+    gdl_tuning tune{GDL_DEFAULT_LOAD_FACTOR, 8, 8, 256};  // load_factor mirrors cfg.load_factor
     std::string last_error;
     bool stats_on = false;
     gdl::KernelStat stats[gdl::K_NUM];
@@ -190,7 +190,8 @@ gdl_status hashset_insert_new(gdl_ctx *ctx, gdl_hashset *hs, const u64 *keys, u6
 // Grows hs so that `incoming` more keys keep its load at or below 1/2.
 gdl_status hashset_reserve(gdl_ctx *ctx, gdl_hashset *hs, u64 incoming);
 constexpr double kHsHardLoad = 0.9;
-gdl_status hashset_reserve2(gdl_ctx *ctx, gdl_hashset *hs, u64 expected, u64 worst);
+gdl_status hashset_reserve2(gdl_ctx *ctx, gdl_hashset *hs, u64 expected, u64 worst,
+                            double load = 0.0);
 gdl_status index_build(gdl_ctx *ctx, const u64 *keys, u64 n, int prefix_cols, gdl_index *out);
 gdl_status join_count(gdl_ctx *ctx, const u64 *inner, u64 n_inner, const gdl_index *idx,
                       const u64 *outer, u64 n_outer, u32 *counts, u64 *begins);

@@ -116,17 +116,19 @@ static u64 pow2_at_least(u64 x) {
 }
 
 // Grows the table so that size + incoming stays at load <= the context's
-// load factor (default 1/2).
+// load factor.
 gdl_status hashset_reserve(gdl_ctx *ctx, gdl_hashset *hs, u64 incoming) {
     return hashset_reserve2(ctx, hs, incoming, incoming);
 }
 
 // Grows the table so that size + expected new members stay at load <= the
-// load factor and size + worst (an upper bound of the new members) at load
-// <= kHsHardLoad: every insert still finds a free slot when the worst case
-// happens, and the table is not sized by a loose bound.
-gdl_status hashset_reserve2(gdl_ctx *ctx, gdl_hashset *hs, u64 expected, u64 worst) {
-    const double lf = std::min(0.8, ctx->cfg.load_factor > 0.f ? (double)ctx->cfg.load_factor : 0.5);
+// load factor (`load`, or the context's when 0; at most 0.8) and size + worst
+// (an upper bound of the new members) at load <= kHsHardLoad: every insert
+// still finds a free slot when the worst case happens, and the table is not
+// sized by a loose bound.
+gdl_status hashset_reserve2(gdl_ctx *ctx, gdl_hashset *hs, u64 expected, u64 worst, double load) {
+    if (!(load > 0.0)) load = ctx->cfg.load_factor > 0.f ? (double)ctx->cfg.load_factor : 0.5;
+    const double lf = std::min(0.8, load);
     expected = std::min(expected, worst);
     const u64 need = pow2_at_least(
         std::max((u64)std::ceil((double)(hs->size + expected) / lf),

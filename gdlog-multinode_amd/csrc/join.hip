@@ -889,7 +889,12 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
         return GDL_OK;
     }
     constexpr int kChunk = 32;
-    constexpr int LB = 256;
+    // a body is bound by the latency of each tuple's chain of dependent
+    // accesses, so every outer tuple gets its own lane (more chains in
+    // flight) in blocks of 512 (fewer blocks: fewer barriers and end-of-block
+    // counter atomics on the shared state); on the 200k-node tree both beat
+    // 256-thread blocks with two tuples per thread
+    constexpr int LB = 512;
     const int shift = inner_idx->prefix_cols >= 2 ? 0 : 32;
     const u64 mr = std::max<u64>(inner_idx->max_run, 1);
     Scratch scr(ctx);
@@ -931,10 +936,10 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
         h.log_cap = *log_cap;
         h.max_run = mr;
         h.max_bodies = (u32)k;
-        // a grid for this delta (about 2 outer tuples per thread); a body
-        // whose delta outgrows it 4x ends the chunk, so the next chunk is
-        // launched wider
-        const u64 want = (2 * nd + LB - 1) / LB;
+        // a grid for this delta (one outer tuple per thread); a body whose
+        // delta outgrows it 8x ends the chunk, so the next chunk is launched
+        // wider
+        const u64 want = (nd + LB - 1) / LB;
         const unsigned g = (unsigned)std::min<u64>(grid_cap(ctx), std::max<u64>(want, 16));
         h.delta_limit = g >= grid_cap(ctx) ? ~0ull : (u64)g * LB * 8;
         GDL_HIP(ctx, hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));

@@ -49,6 +49,12 @@ uint64_t pow2_ceil(uint64_t x) {
     return c;
 }
 
+// Bytes of a hash set holding `tuples` keys at the default load factor
+// (power-of-two slots plus the flag word).
+uint64_t hash_set_bytes(uint64_t tuples) {
+    return 8 * (pow2_ceil((uint64_t)std::ceil((double)tuples / GDL_DEFAULT_LOAD_FACTOR)) + 1);
+}
+
 // Tuple count of a query's output relations, estimated from its inputs.
 uint64_t estimate_tuples(int kind, const uint64_t *a, uint64_t na, const uint64_t *b,
                          uint64_t nb, uint32_t *n_nodes) {
@@ -76,7 +82,7 @@ uint64_t estimate_tuples(int kind, const uint64_t *a, uint64_t na, const uint64_
 // hash set of the output relations, the delta / newt buffers and the join
 // outputs of an iteration.
 uint64_t estimate_bytes(int kind, uint64_t tuples, uint64_t na, uint64_t nb) {
-    const uint64_t hs = 8 * pow2_ceil(2 * tuples);
+    const uint64_t hs = hash_set_bytes(tuples);
     const uint64_t per = kind == GDL_QUERY_TC ? 24 : 64;  // SG / CSPA: materialized joins
     return hs + per * tuples + 64 * (na + nb) + kMinReservation;
 }
@@ -459,9 +465,8 @@ gdl_status gdl_server_submit(gdl_server *s, uint64_t qid, int kind, const uint64
                   (uint64_t)q->n_nodes <= (1ull << s->cfg.node_bits) &&
                   q->est_tuples <= kBatchMaxTuples;
     if (q->to_batch) {
-        // the batch's own accounting: hash set at load 1/2, delta share, edges
-        q->est = 8 * (pow2_ceil(2 * q->est_tuples) + 1) + 8 * q->est_tuples + 32 * na +
-                 (1ull << 16);
+        // the batch's own accounting: hash set, delta share, edges
+        q->est = hash_set_bytes(q->est_tuples) + 8 * q->est_tuples + 32 * na + (1ull << 16);
     } else {
         q->est = estimate_bytes(kind, q->est_tuples, na, nb);
     }

@@ -7,7 +7,7 @@ Thrust's sort.  Here they are a `gdl_tuning` of the context
 width and grid cap.  `autotune` times a calibration workload under candidate
 settings by coordinate descent over
 
-    load_factor   0.5, 0.7, 0.8
+    load_factor   0.35, 0.5, 0.7, 0.8
     join_block    128, 256, 512
     radix_bits    6, 7, 8
     blocks_per_cu 4, 8, 16
@@ -15,18 +15,24 @@ settings by coordinate descent over
 (each candidate: best of `reps` runs), keeps the fastest, and caches the
 choice per (device, workload key) in a JSON file, so a later run of the same
 workload on the same device reuses it without timing anything.
+
+A TC batch holds many instances' hash sets at once and keeps them at load 1/2
+or more (memory, not probe length, limits it), so BATCH_SPACE / BATCH_DEFAULT
+search load factors 0.5, 0.7 and 0.8 only.
 """
 import json
 import os
 import time
 
 SPACE = {
-    "load_factor": [0.5, 0.7, 0.8],
+    "load_factor": [0.35, 0.5, 0.7, 0.8],
     "join_block": [128, 256, 512],
     "radix_bits": [6, 7, 8],
     "blocks_per_cu": [4, 8, 16],
 }
-DEFAULT = {"load_factor": 0.5, "join_block": 256, "radix_bits": 8, "blocks_per_cu": 8}
+DEFAULT = {"load_factor": 0.35, "join_block": 256, "radix_bits": 8, "blocks_per_cu": 8}
+BATCH_SPACE = dict(SPACE, load_factor=[0.5, 0.7, 0.8])
+BATCH_DEFAULT = dict(DEFAULT, load_factor=0.5)
 
 
 def cache_path():
@@ -54,8 +60,9 @@ def _store_cache(path, cache):
 
 
 def autotune(ctx, workload, key, device_key="device0", reps=2, sweeps=1, use_cache=True,
-             path=None):
-    """Picks ctx's tuning for `workload` (a callable running it on ctx).
+             path=None, space=None, start=None):
+    """Picks ctx's tuning for `workload` (a callable running it on ctx),
+    searching `space` (default SPACE) from `start` (default DEFAULT).
 
     Returns {"tuning": {...}, "seconds": best time, "default_seconds": time
     with the defaults, "trials": n, "cached": bool}; ctx is left set to the
@@ -86,12 +93,13 @@ def autotune(ctx, workload, key, device_key="device0", reps=2, sweeps=1, use_cac
         return timed[sig]
 
     workload()  # warm-up: caches, code objects, allocator
-    cur = dict(DEFAULT)
+    space = SPACE if space is None else space
+    cur = dict(DEFAULT if start is None else start)
     cur_s = run(cur)
     default_s = cur_s
     for _ in range(sweeps):
         changed = False
-        for name, values in SPACE.items():
+        for name, values in space.items():
             for v in values:
                 if v == cur[name]:
                     continue

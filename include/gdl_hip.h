@@ -27,6 +27,13 @@
 extern "C" {
 #endif
 
+/* Default load factor of the hash index and hash sets.  Lower than the
+ * reference's 0.8 (relation.cu:603-604): an insert into an open-addressing
+ * set pays one dependent atomic round trip per occupied slot it passes, and
+ * a fixpoint body is bound by that chain's latency, so memory is traded for
+ * shorter chains. */
+#define GDL_DEFAULT_LOAD_FACTOR 0.35f
+
 typedef enum gdl_status {
     GDL_OK = 0,
     GDL_ERR_INVALID_ARG = 1,   /* null pointer, bad arity, bad column map ... */
@@ -45,14 +52,13 @@ typedef struct gdl_config {
     int device;             /* HIP device ordinal */
     uint64_t mem_cap_bytes; /* 0: no cap.  Otherwise allocations that would take the
                                bytes in use above the cap fail (GDL_ERR_OUT_OF_MEMORY) */
-    float load_factor;      /* hash index load factor, (0, 0.95]; default 0.5 (the
-                               reference uses 0.8, relation.cu:603-604; at 0.8
-                               probe-chain tails make misses 3-4x slower) */
+    float load_factor;      /* hash index / hash set load factor, (0, 0.95];
+                               default GDL_DEFAULT_LOAD_FACTOR */
     int collect_stats;      /* 1: time every kernel with hipEvents on the context stream */
     int verbose;            /* 1: print the reference-style per-op / per-iteration log */
 } gdl_config;
 
-/* Fills the defaults: device 0, no cap, load factor 0.5, no stats. */
+/* Fills the defaults: device 0, no cap, GDL_DEFAULT_LOAD_FACTOR, no stats. */
 void gdl_config_default(gdl_config *cfg);
 
 /* Replaces the implicit device-0 / legacy-default-stream state of the
@@ -75,7 +81,8 @@ gdl_status gdl_set_load_factor(gdl_ctx *ctx, float load_factor);
  * and Thrust's sort. */
 typedef struct gdl_tuning {
     float load_factor;      /* hash index / hash set load factor, (0, 0.95]
-                               (hash sets are capped at 0.8); default 0.5 */
+                               (hash sets are capped at 0.8);
+                               default GDL_DEFAULT_LOAD_FACTOR */
     uint32_t blocks_per_cu; /* grid cap of the grid-stride kernels, 1..32; default 8 */
     uint32_t radix_bits;    /* LSD radix sort digit width, 4..8; default 8 */
     uint32_t join_block;    /* threads per block of the fused join: 64, 128, 256 or 512;

@@ -37,7 +37,7 @@ def test_status_strings_and_defaults():
     assert eng.gdl_status_string(2) == b"out of device memory"
     cfg = gdl.Config()
     eng.gdl_config_default(ctypes.byref(cfg))
-    assert cfg.device == 0 and abs(cfg.load_factor - 0.5) < 1e-6 and cfg.mem_cap_bytes == 0
+    assert cfg.device == 0 and abs(cfg.load_factor - 0.35) < 1e-6 and cfg.mem_cap_bytes == 0
 
 
 def test_null_arguments_are_rejected_without_a_device():
