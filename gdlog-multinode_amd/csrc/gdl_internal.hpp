@@ -187,7 +187,8 @@ gdl_status filter_keys(gdl_ctx *ctx, const u64 *d_in, u64 n, gdl_pred pred, u64 
                        u64 *n_out);
 gdl_status hashset_insert_new(gdl_ctx *ctx, gdl_hashset *hs, const u64 *keys, u64 n, u64 *d_new,
                               u64 *n_new);
-// Grows hs so that `incoming` more keys keep its load at or below 1/2.
+// Grows hs so that `incoming` more keys keep its load at or below the
+// context's load factor.
 gdl_status hashset_reserve(gdl_ctx *ctx, gdl_hashset *hs, u64 incoming);
 constexpr double kHsHardLoad = 0.9;
 gdl_status hashset_reserve2(gdl_ctx *ctx, gdl_hashset *hs, u64 expected, u64 worst,
@@ -320,19 +321,56 @@ __device__ __forceinline__ u64 run_end(const u64 *keys, u64 n, u64 from, u64 p, 
 // was not a member (this call inserted it).
 constexpr u64 kHsEmpty = ~0ull;
 
-__device__ __forceinline__ bool hs_insert_key(u64 *slots, u64 mask, u64 k) {
-    if (k == kHsEmpty) return atomicExch((unsigned int *)&slots[mask + 1], 1u) == 0u;
-    u64 s = mix64(k) & mask;
-    // CAS first: at load <= 1/2 the home slot is usually empty or holds k, so
-    // one round trip decides most inserts
-    for (u64 step = 0; step <= mask; ++step) {
+constexpr int kHsScan = 8;  // slots read together while probing (one 64-byte line)
+
+// Probe from slot s: read the next kHsScan slots with independent plain loads
+// (one round trip), then CAS only the first that read EMPTY, so a run of
+// occupied slots costs about two round trips instead of one atomic per slot.
+// Slots are written once, so a read can only be stale by showing EMPTY for a
+// slot filled since; the CAS catches that and the probe goes on after it.
+__device__ __forceinline__ bool hs_probe_from(u64 *slots, u64 mask, u64 k, u64 s) {
+    for (u64 seen = 0; seen <= mask;) {
+        u64 v[kHsScan];
+#pragma unroll
+        for (int j = 0; j < kHsScan; ++j) v[j] = slots[(s + j) & mask];
+        int pos = kHsScan;
+        bool is_k = false;
+#pragma unroll
+        for (int j = 0; j < kHsScan; ++j) {
+            if (pos == kHsScan && (v[j] == k || v[j] == kHsEmpty)) {
+                pos = j;
+                is_k = v[j] == k;
+            }
+        }
+        if (is_k) return false;
+        if (pos == kHsScan) {
+            s = (s + kHsScan) & mask;
+            seen += kHsScan;
+            continue;
+        }
+        const u64 t = (s + pos) & mask;
         const unsigned long long old = atomicCAS(
-            (unsigned long long *)&slots[s], (unsigned long long)kHsEmpty, (unsigned long long)k);
+            (unsigned long long *)&slots[t], (unsigned long long)kHsEmpty, (unsigned long long)k);
         if (old == (unsigned long long)kHsEmpty) return true;
         if (old == (unsigned long long)k) return false;
-        s = (s + 1) & mask;
+        s = (t + 1) & mask;
+        seen += (u64)pos + 1;
     }
-    return false;  // unreachable at load <= 1/2
+    return false;  // unreachable: the table is never full
+}
+
+// CAS the home slot first: one round trip decides most inserts (reading
+// first instead costs a second round trip for every key that is new, and
+// was slower even where most keys are already members); past the home slot
+// the probe reads kHsScan slots at a time.
+__device__ __forceinline__ bool hs_insert_key(u64 *slots, u64 mask, u64 k) {
+    if (k == kHsEmpty) return atomicExch((unsigned int *)&slots[mask + 1], 1u) == 0u;
+    const u64 s = mix64(k) & mask;
+    const unsigned long long old = atomicCAS(
+        (unsigned long long *)&slots[s], (unsigned long long)kHsEmpty, (unsigned long long)k);
+    if (old == (unsigned long long)kHsEmpty) return true;
+    if (old == (unsigned long long)k) return false;
+    return hs_probe_from(slots, mask, k, (s + 1) & mask);
 }
 
 // The index is an array of buckets of kBucket slots (one 64-byte line);
