@@ -192,7 +192,7 @@ gdl_status hashset_insert_new(gdl_ctx *ctx, gdl_hashset *hs, const u64 *keys, u6
 gdl_status hashset_reserve(gdl_ctx *ctx, gdl_hashset *hs, u64 incoming);
 constexpr double kHsHardLoad = 0.9;
 gdl_status hashset_reserve2(gdl_ctx *ctx, gdl_hashset *hs, u64 expected, u64 worst,
-                            double load = 0.0);
+                            double load = 0.0, u64 grow_to = 0);
 gdl_status index_build(gdl_ctx *ctx, const u64 *keys, u64 n, int prefix_cols, gdl_index *out);
 gdl_status join_count(gdl_ctx *ctx, const u64 *inner, u64 n_inner, const gdl_index *idx,
                       const u64 *outer, u64 n_outer, u32 *counts, u64 *begins);

@@ -125,15 +125,19 @@ gdl_status hashset_reserve(gdl_ctx *ctx, gdl_hashset *hs, u64 incoming) {
 // load factor (`load`, or the context's when 0; at most 0.8) and size + worst
 // (an upper bound of the new members) at load <= kHsHardLoad: every insert
 // still finds a free slot when the worst case happens, and the table is not
-// sized by a loose bound.
-gdl_status hashset_reserve2(gdl_ctx *ctx, gdl_hashset *hs, u64 expected, u64 worst, double load) {
+// sized by a loose bound.  A table that has to grow is made large enough for
+// size + grow_to members at the load factor as well, so a set that keeps
+// growing is rehashed every few multiples of its size, not at every doubling.
+gdl_status hashset_reserve2(gdl_ctx *ctx, gdl_hashset *hs, u64 expected, u64 worst, double load,
+                            u64 grow_to) {
     if (!(load > 0.0)) load = ctx->cfg.load_factor > 0.f ? (double)ctx->cfg.load_factor : 0.5;
     const double lf = std::min(0.8, load);
     expected = std::min(expected, worst);
-    const u64 need = pow2_at_least(
+    u64 need = pow2_at_least(
         std::max((u64)std::ceil((double)(hs->size + expected) / lf),
                  (u64)std::ceil((double)(hs->size + worst) / kHsHardLoad) + 1));
     if (hs->slots && hs->capacity >= need) return GDL_OK;
+    if (grow_to) need = std::max(need, pow2_at_least((u64)std::ceil((double)(hs->size + grow_to) / lf)));
     u64 *ns = nullptr;
     GDL_TRY(hs_alloc(ctx, need, &ns));
     if (hs->slots) {

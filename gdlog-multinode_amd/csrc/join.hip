@@ -907,14 +907,18 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
     while (done < max_bodies) {
         const u64 nd = d1 - d0;
         const u64 worst = nd > (~0ull >> 20) / mr ? (~0ull >> 20) : nd * mr;
-        // the set: room for 2x its size more at the load factor (a grow stop
-        // costs a host wait and a rehash, so it grows at least 3x at a time),
-        // the worst body at 0.9; the log: the worst body after the delta,
-        // grown geometrically
-        GDL_TRY(hashset_reserve2(ctx, hs, std::max<u64>({nd, 2 * hs->size, 1024}), worst));
+        // the set: the next chunk's bodies (about |delta| each) at the load
+        // factor and the worst body at 0.9; when it has to grow, it grows to
+        // room for 4x its size more (each growth is a rehash of every member,
+        // so a set that keeps growing is rehashed every ~5x, not every 2x);
+        // the log: the worst body after the delta, grown 2x at a time (each
+        // growth copies it)
+        const u64 next = std::max<u64>(nd, 1024) * (u64)std::min(ramp, kChunk);
+        GDL_TRY(hashset_reserve2(ctx, hs, next, std::max(worst, next), 0.0,
+                                 std::max<u64>(next, 4 * hs->size)));
         u64 need_log = d1 + std::max<u64>(worst, 4096);
         if (*log_cap < need_log) {
-            need_log = std::max<u64>(need_log, *log_cap + *log_cap / 2);  // geometric
+            need_log = std::max<u64>(need_log, 2 * *log_cap);  // geometric
             void *p = nullptr;
             std::string err;
             gdl_status s = ctx->alloc->alloc(need_log * sizeof(u64), &p, &err);
