@@ -42,7 +42,7 @@ class Config(Structure):
 
 class Index(Structure):
     _fields_ = [("slots", c_void_p), ("n_slots", c_uint64), ("n_distinct", c_uint64),
-                ("prefix_cols", c_int), ("max_run", c_uint64)]
+                ("prefix_cols", c_int), ("max_run", c_uint64), ("dense", c_int)]
 
 
 class HashSet(Structure):

@@ -378,8 +378,23 @@ __device__ __forceinline__ bool hs_insert_key(u64 *slots, u64 mask, u64 k) {
 // when it was inserted, so a bucket with a free slot ends every probe.
 constexpr u64 kBucket = 4;
 
+// Kernels get an index's n_slots with this bit set when it is direct-mapped:
+// slots is then an array of run words ((begin << 24) | count, 0 = no run)
+// indexed by the prefix, one load per lookup whether the prefix is present
+// or not (a hash probe for an absent prefix walks to a bucket with room).
+constexpr u64 kIdxDense = 1ull << 63;
+inline u64 idx_arg(const gdl_index *idx) { return idx->n_slots | (idx->dense ? kIdxDense : 0); }
+
 __device__ __forceinline__ bool idx_probe(const IdxSlot *slots, u64 n_slots, u64 p, u64 *begin,
                                           u64 *count) {
+    if (n_slots & kIdxDense) {
+        if (p >= (n_slots & ~kIdxDense)) return false;
+        const u64 v = ((const u64 *)slots)[p];
+        if (v == 0) return false;
+        *begin = v >> kCountBits;
+        *count = v & kCountSat;
+        return true;
+    }
     const u64 nbk = n_slots / kBucket;
     u64 b = mulhi64(mix64(p), nbk);
     for (u64 step = 0; step < nbk; ++step) {

@@ -21,9 +21,11 @@ constexpr int JM_TILE = JB * JM_I;
 
 __device__ __forceinline__ int prefix_shift(int prefix_cols) { return prefix_cols >= 2 ? 0 : 32; }
 
-// out[0] += distinct prefixes, out[1] = max(longest run); one atomic pair per block.
+// out[0] += distinct prefixes, out[1] = max(longest run); one atomic pair per
+// block; out[2] = the largest prefix (the last key's: keys are sorted).
 __global__ __launch_bounds__(JB) void k_idx_count(const u64 *keys, u64 n, int shift, u64 *out) {
     __shared__ u64 s_sum[JB / 64], s_max[JB / 64];
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[2] = keys[n - 1] >> shift;
     u64 acc = 0, mx = 0;
     for (u64 i = (u64)blockIdx.x * JB + threadIdx.x; i < n; i += (u64)gridDim.x * JB) {
         const u64 p = keys[i] >> shift;
@@ -80,6 +82,17 @@ __global__ __launch_bounds__(JB) void k_idx_build(const u64 *keys, u64 n, int sh
             if (++b == nbk) b = 0;
         }
         if (!placed) atomicOr(err, 1u);
+    }
+}
+
+// Direct-mapped index: the run word of each prefix (run starts only).
+__global__ __launch_bounds__(JB) void k_idx_build_dense(const u64 *keys, u64 n, int shift,
+                                                        u64 *words, u64 range) {
+    for (u64 i = (u64)blockIdx.x * JB + threadIdx.x; i < n; i += (u64)gridDim.x * JB) {
+        const u64 p = keys[i] >> shift;
+        if (i != 0 && (keys[i - 1] >> shift) == p) continue;
+        const u64 cnt = run_end(keys, n, i, p, shift) - i;
+        if (p < range) words[p] = (i << kCountBits) | (cnt < kCountSat ? cnt : kCountSat);
     }
 }
 
@@ -525,22 +538,45 @@ gdl_status index_build(gdl_ctx *ctx, const u64 *keys, u64 n, int prefix_cols, gd
     out->n_distinct = 0;
     out->prefix_cols = prefix_cols;
     out->max_run = 0;
+    out->dense = 0;
     if (n == 0) return GDL_OK;
     const int shift = prefix_cols >= 2 ? 0 : 32;
-    u64 distinct = 0, max_run = 0;
+    u64 distinct = 0, max_run = 0, max_prefix = 0;
     {
         Scratch scr(ctx);
         u64 *cnt;
-        GDL_TRY(scr.get(2, &cnt));
-        GDL_HIP(ctx, hipMemsetAsync(cnt, 0, 2 * sizeof(u64), ctx->stream));
+        GDL_TRY(scr.get(3, &cnt));
+        GDL_HIP(ctx, hipMemsetAsync(cnt, 0, 3 * sizeof(u64), ctx->stream));
         KTimer kt(ctx, K_IDX_COUNT, 8.0 * n);
         const unsigned g = std::min<unsigned>(stride_grid(ctx, n), 512u);
         hipLaunchKernelGGL(k_idx_count, dim3(g), dim3(JB), 0, ctx->stream, keys, n, shift, cnt);
         GDL_TRY(check_launch(ctx, "k_idx_count"));
-        u64 h[2];
-        GDL_TRY(read_u64(ctx, cnt, h, 2));
+        u64 h[3];
+        GDL_TRY(read_u64(ctx, cnt, h, 3));
         distinct = h[0];
         max_run = h[1];
+        max_prefix = h[2];
+    }
+    out->n_distinct = distinct;
+    out->max_run = max_run;
+    // one column of ids that are dense enough: a direct-mapped array of run
+    // words no larger than the hash index (or at most 1 MB)
+    const u64 hash_slots = (u64)((double)distinct / (double)ctx->cfg.load_factor) + 1;
+    if (prefix_cols == 1 && max_prefix < std::max<u64>(2 * hash_slots, 1ull << 17)) {
+        const u64 range = max_prefix + 1;
+        void *p = nullptr;
+        std::string err;
+        gdl_status st = ctx->alloc->alloc(range * sizeof(u64), &p, &err);
+        if (st != GDL_OK) return fail(ctx, st, err);
+        GDL_HIP(ctx, hipMemsetAsync(p, 0, range * sizeof(u64), ctx->stream));
+        KTimer kt(ctx, K_IDX_BUILD, 8.0 * n + 8.0 * (double)distinct);
+        hipLaunchKernelGGL(k_idx_build_dense, dim3(stride_grid(ctx, n)), dim3(JB), 0, ctx->stream,
+                           keys, n, shift, (u64 *)p, range);
+        GDL_TRY(check_launch(ctx, "k_idx_build_dense"));
+        out->slots = p;
+        out->n_slots = range;
+        out->dense = 1;
+        return GDL_OK;
     }
     u64 n_slots = (u64)((double)distinct / (double)ctx->cfg.load_factor) + 1;
     if (n_slots <= distinct) n_slots = distinct + 1;
@@ -574,7 +610,7 @@ gdl_status join_count(gdl_ctx *ctx, const u64 *inner, u64 n_inner, const gdl_ind
     }
     KTimer kt(ctx, K_JOIN_COUNT, 8.0 * n_outer + 16.0 * n_outer + 12.0 * n_outer);
     hipLaunchKernelGGL(k_join_count, dim3(stride_grid(ctx, n_outer)), dim3(JB), 0, ctx->stream,
-                       (const IdxSlot *)idx->slots, idx->n_slots, inner, n_inner, shift, outer,
+                       (const IdxSlot *)idx->slots, idx_arg(idx), inner, n_inner, shift, outer,
                        n_outer, counts, begins);
     return check_launch(ctx, "k_join_count");
 }
@@ -615,6 +651,7 @@ gdl_status gdl_index_free(gdl_ctx *ctx, gdl_index *idx) {
     idx->slots = nullptr;
     idx->n_slots = 0;
     idx->n_distinct = 0;
+    idx->dense = 0;
     return GDL_OK;
 }
 
@@ -631,7 +668,7 @@ gdl_status gdl_index_lookup(gdl_ctx *ctx, const gdl_index *idx, const uint64_t *
     const int shift = idx->prefix_cols >= 2 ? 0 : 32;
     KTimer kt(ctx, K_IDX_LOOKUP, 28.0 * n);
     hipLaunchKernelGGL(k_idx_lookup, dim3(stride_grid(ctx, n)), dim3(JB), 0, ctx->stream,
-                       (const IdxSlot *)idx->slots, idx->n_slots, d_sorted, n_sorted, shift,
+                       (const IdxSlot *)idx->slots, idx_arg(idx), d_sorted, n_sorted, shift,
                        d_prefixes, n, d_begin, d_count);
     return check_launch(ctx, "k_idx_lookup");
 }
@@ -801,7 +838,7 @@ static gdl_status join_insert_impl(gdl_ctx *ctx, const u64 *d_inner, u64 n_inner
                                                                     cap_blocks));
         auto launch = [&](auto kern, int block) {
             hipLaunchKernelGGL(kern, dim3(g), dim3(block), 0, ctx->stream,
-                               (const IdxSlot *)inner_idx->slots, inner_idx->n_slots, d_inner,
+                               (const IdxSlot *)inner_idx->slots, idx_arg(inner_idx), d_inner,
                                n_inner, shift, d_outer, n_outer, map.arity, map.src[0],
                                map.arity == 2 ? map.src[1] : 0, pred.op, pred.lhs, pred.rhs, sel,
                                *d_new, *new_cap, (unsigned long long *)counters);
@@ -950,7 +987,7 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
         for (int i = 0; i < k; ++i) {
             KTimer kt(ctx, K_LINEAR, 0.0);
             hipLaunchKernelGGL(k_linear_body<LB>, dim3(g), dim3(LB), 0, ctx->stream,
-                               (const IdxSlot *)inner_idx->slots, inner_idx->n_slots, d_inner,
+                               (const IdxSlot *)inner_idx->slots, idx_arg(inner_idx), d_inner,
                                n_inner, shift, map.arity, map.src[0],
                                map.arity == 2 ? map.src[1] : 0, pred.op, pred.lhs, pred.rhs,
                                (u64 *)hs->slots, hs->capacity - 1, *d_log, st, counts);

@@ -166,6 +166,10 @@ typedef struct gdl_index {
     uint64_t n_distinct; /* distinct prefixes */
     int prefix_cols;    /* 1 or 2 */
     uint64_t max_run;   /* longest run (tuples sharing one prefix) */
+    int dense;          /* 1: slots is a direct-mapped array of n_slots run words
+                           indexed by the prefix itself (prefix_cols = 1 and the
+                           prefixes below n_slots, dense enough that the array is
+                           no larger than the hash index); 0: the hash index */
 } gdl_index;
 gdl_status gdl_index_build(gdl_ctx *ctx, const uint64_t *d_sorted, uint64_t n, int prefix_cols,
                            gdl_index *out);

@@ -88,18 +88,33 @@ def np_runs(sorted_keys, prefixes, shift):
     return lo, hi - lo
 
 
-@pytest.mark.parametrize("prefix_cols", [1, 2])
+@pytest.mark.parametrize("prefix_cols,layout", [(1, "dense"), (1, "holes"), (1, "spread"),
+                                               (2, "dense")])
 @pytest.mark.parametrize("lf", [0.5, 0.8])
-def test_index_lookup_runs(ctx, prefix_cols, lf):
+def test_index_lookup_runs(ctx, prefix_cols, layout, lf):
+    # one column of small ids gets the direct-mapped index; ids spread over
+    # 32 bits (or a two-column prefix) get the hash index
     ctx.eng.gdl_set_load_factor(ctx.h, lf)
-    keys = np.unique(rand_keys(200_000, 14, 16))
+    keys = rand_keys(200_000, 14, 16)
+    if layout == "holes":  # even ids only: odd ids inside the range are absent
+        keys = gdl.pack((keys >> np.uint64(32)) & ~np.uint64(1), keys & M32)
+    elif layout == "spread":
+        keys = gdl.pack((keys >> np.uint64(32)) * np.uint64(100_003), keys & M32)
+    keys = np.unique(keys)
     shift = 32 if prefix_cols == 1 else 0
     d = ctx.to_device(keys)
     idx = ctx.index_build(d, prefix_cols)
+    assert idx.dense == (1 if prefix_cols == 1 and layout != "spread" else 0)
     present = np.unique(keys >> np.uint64(shift))
     assert idx.n_distinct == present.size
-    absent = (present[:1000] + np.uint64(1 << 40)) if prefix_cols == 2 else \
-        np.arange(1 << 14, (1 << 14) + 1000, dtype=np.uint64)
+    if prefix_cols == 2:
+        absent = present[:1000] + np.uint64(1 << 40)
+    elif layout == "holes":
+        absent = np.arange(1, 2001, 2, dtype=np.uint64)
+    elif layout == "spread":
+        absent = present[:1000] + np.uint64(1)
+    else:
+        absent = np.arange(1 << 14, (1 << 14) + 1000, dtype=np.uint64)
     q = np.concatenate([present, absent])
     b, c = ctx.index_lookup(idx, d, q)
     lo, cnt = np_runs(keys, q, shift)
@@ -108,7 +123,7 @@ def test_index_lookup_runs(ctx, prefix_cols, lf):
     assert np.array_equal(b[hit].astype(np.int64), lo[hit])
     ctx.index_free(idx)
     d.free()
-    ctx.eng.gdl_set_load_factor(ctx.h, 0.5)
+    ctx.eng.gdl_set_load_factor(ctx.h, 0.35)
 
 
 def test_index_saturated_run_count(ctx):
