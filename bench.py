@@ -64,6 +64,17 @@ def profiled_traffic(kernel, workload_tag):
     return None
 
 
+def phase_child(phase, extra):
+    """Runs `bench.py --phase <phase>` as a child process (its progress
+    lines pass through on stderr) and returns the JSON object it prints."""
+    import subprocess
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--phase", phase] + extra
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"bench.py --phase {phase} exited with {r.returncode}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def cpu_threads():
     """Host threads for the CPU baseline: this process's CPU share, at most 16
     (the share of one GPU on a shared box)."""
@@ -77,15 +88,17 @@ def cpu_threads():
 def cpu_baseline(kind, inp, max_iteration):
     """CPU baseline.  TC: the same semi-naive loop on all the host threads
     (oracle/cpu_tc_omp.c, partitioned by the path's second column), on the
-    whole workload, best of 2, with the one-thread oracle on a bounded sample
-    beside it.  SG / CSPA: the one-thread oracle on the whole workload."""
+    whole workload, best of 3 after a warm-up run, with the one-thread oracle
+    on a bounded sample beside it.  SG / CSPA: the one-thread oracle on the
+    whole workload."""
     if REPO not in sys.path:
         sys.path.insert(0, REPO)
     import oracle
     if kind == "tc":
         threads = cpu_threads()
+        oracle.tc_omp(inp, threads)  # warm-up: page-in, thread pool
         best = None
-        for _ in range(2):
+        for _ in range(3):
             t = time.perf_counter()
             r = oracle.tc_omp(inp, threads)
             dt = time.perf_counter() - t
@@ -96,7 +109,7 @@ def cpu_baseline(kind, inp, max_iteration):
         return dict(value=round(r["derived"] / best, 1), unit="derived tuples/s", cores=threads,
                     kind="port",
                     sample=f"oracle/cpu_tc_omp.c on the whole workload ({r['derived']} derived "
-                           f"tuples, {r['iterations']} iterations, best of 2: {best:.2f} s, "
+                           f"tuples, {r['iterations']} iterations, best of 3: {best:.2f} s, "
                            f"{threads} threads)",
                     single_thread=dict(value=round(s["derived"] / dt1, 1), cores=1,
                                        sample=f"oracle, first {s['iterations']} iterations "
@@ -167,7 +180,6 @@ def batched(gdl, ctx, n_inst, min_edges=1000, max_edges=50000, window=1000):
     # one at a time: the single-query defaults (each query has the device to
     # itself, so the sparser hash table of DEFAULT pays)
     ctx.set_tuning(**tune.DEFAULT)
-    ctx.release_cached()
     ctx.sync()
     t0 = time.perf_counter()
     lat_s, alone = [], {}
@@ -178,7 +190,6 @@ def batched(gdl, ctx, n_inst, min_edges=1000, max_edges=50000, window=1000):
         note(f"batched TC: serial {q + 1}/{len(inst)}")
     serial_s = time.perf_counter() - t0
     ctx.set_tuning(**tuned["tuning"])
-    ctx.release_cached()
     t0 = time.perf_counter()
     lat_b, done = [], {}
     with gdl.TcBatch(ctx, keep_keys=False) as b:
@@ -193,7 +204,6 @@ def batched(gdl, ctx, n_inst, min_edges=1000, max_edges=50000, window=1000):
             note(f"batched TC: batch {len(done)}/{len(inst)} done")
         info = b.info()
     batch_s = time.perf_counter() - t0
-    ctx.release_cached()
     mism = [q for q, _, _ in inst
             if (done[q]["n_out"], done[q]["iterations"], done[q]["derived"]) != alone[q]]
     derived = sum(v[2] for v in alone.values())
@@ -258,6 +268,7 @@ def served(gdl, ctx, n_req, rate_mult=0.8, seed=7, workers=1):
         service.append(time.perf_counter() - t0)
         alone[i] = (r["n_out"], r["iterations"], r["checksum"])
         note(f"served mix: one-at-a-time {i + 1}/{n_req}")
+    # the server's contexts get the device memory this context holds cached
     ctx.release_cached()
     serial_qps = n_req / sum(service)
     rate = rate_mult * serial_qps
@@ -341,10 +352,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched-instances", type=int, default=256,
                     help="instances of the batched-TC measurement (0: skip it)")
-    ap.add_argument("--served-requests", type=int, default=300,
+    ap.add_argument("--served-requests", type=int, default=1000,
                     help="requests of the served-mix measurement (0: skip it)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device copy (traffic-counter calibration)")
+    ap.add_argument("--phase", choices=["batched", "served"],
+                    help="only run that measurement and print its JSON object")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus must be 1 "
@@ -362,6 +375,13 @@ def main():
         ctx.sync()
         print(json.dumps({"copied_bytes": n * 8}), flush=True)
         a.free(); b.free(); ctx.close()
+        return
+    if args.phase:
+        ctx = gdl.Context(device=0)
+        out = batched(gdl, ctx, args.batched_instances) if args.phase == "batched" else \
+            served(gdl, ctx, args.served_requests)
+        ctx.close()
+        print(json.dumps(out), flush=True)
         return
     gdl.set_full_mode(gdl.FULL_MODE_MERGE if args.mode == "merge" else gdl.FULL_MODE_AUTO)
     kind, make, desc = workloads(gdl, args.nodes, args.window)[args.workload]
@@ -435,12 +455,16 @@ def main():
     for d in dev:
         d.free()
     gdl.set_full_mode(gdl.FULL_MODE_AUTO)
-    serve = None
-    if args.served_requests > 0:
-        serve = served(gdl, ctx, args.served_requests)
-    batch = None
+    # the batched and served measurements each run in a process of their own
+    # (as a batch or a server would): device memory that one long run of
+    # allocations and frees gave back comes back to the next one much more
+    # slowly, so in one process the second measurement paid for the first
+    ctx.release_cached()
+    batch = serve = None
     if args.batched_instances > 0:
-        batch = batched(gdl, ctx, args.batched_instances)
+        batch = phase_child("batched", ["--batched-instances", str(args.batched_instances)])
+    if args.served_requests > 0:
+        serve = phase_child("served", ["--served-requests", str(args.served_requests)])
 
     cpu = None
     if not args.no_cpu:

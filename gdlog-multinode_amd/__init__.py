@@ -830,7 +830,7 @@ QUERY_TC, QUERY_SG, QUERY_CSPA = 0, 1, 2
 class ServerConfig(Structure):
     _fields_ = [("device", c_int), ("workers", c_int), ("batch_tc", c_int),
                 ("batch_slots", c_uint32), ("node_bits", c_uint32), ("mem_budget", c_uint64),
-                ("keep_keys", c_int), ("partition_cus", c_int)]
+                ("keep_keys", c_int), ("partition_cus", c_int), ("take_turns", c_int)]
 
 
 class QueryResult(Structure):
@@ -869,17 +869,19 @@ def query_estimate(kind, a, b=None):
 
 class Server:
     """Query server of include/gdl_serve.h: TC / SG / CSPA queries admitted by
-    footprint estimate, TC queries continuously batched, the others on
-    concurrent worker contexts."""
+    footprint estimate, TC queries continuously batched, the others on worker
+    contexts; with take_turns the batch and the workers take turns on the
+    device instead of running concurrently."""
 
     def __init__(self, device=0, workers=1, batch_tc=True, batch_slots=64, node_bits=18,
-                 mem_budget=0, keep_keys=False, partition_cus=False):
+                 mem_budget=0, keep_keys=False, partition_cus=False, take_turns=False):
         self.prog = _serve_lib()
         cfg = ServerConfig()
         self.prog.gdl_server_config_default(ctypes.byref(cfg))
         cfg.device, cfg.workers, cfg.batch_tc = device, workers, int(batch_tc)
         cfg.batch_slots, cfg.node_bits, cfg.mem_budget = batch_slots, node_bits, mem_budget
         cfg.keep_keys, cfg.partition_cus = int(keep_keys), int(partition_cus)
+        cfg.take_turns = int(take_turns)
         h = c_void_p()
         st = self.prog.gdl_server_create(ctypes.byref(cfg), ctypes.byref(h))
         if st != GDL_OK:

@@ -30,6 +30,31 @@ namespace {
 constexpr int kMaxPasses = 8;
 constexpr uint64_t kMinReservation = 64ull << 20;
 constexpr uint64_t kBatchMaxTuples = 8ull << 20;  // larger TC queries run on a worker
+constexpr double kBatchSliceMs = 20.0;             // take_turns: the batch's turn
+
+// Executors of a server take turns on the device (take_turns): one at a time,
+// in the order they asked (tickets), so neither the batch nor a worker can
+// hold it while the other waits for more than its own turn.
+class DeviceTurn {
+  public:
+    void acquire() {
+        std::unique_lock<std::mutex> lk(m_);
+        const uint64_t t = next_++;
+        cv_.wait(lk, [&] { return serving_ == t; });
+    }
+    void release() {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            ++serving_;
+        }
+        cv_.notify_all();
+    }
+
+  private:
+    std::mutex m_;
+    std::condition_variable cv_;
+    uint64_t next_ = 0, serving_ = 0;
+};
 
 double clock_ms() {
     using namespace std::chrono;
@@ -109,6 +134,7 @@ using QueryPtr = std::unique_ptr<Query>;
 struct gdl_server {
     gdl_server_config cfg{};
     std::mutex mu;
+    DeviceTurn turn;  // take_turns: one executor on the device at a time
     std::condition_variable cv_work, cv_batch, cv_done;
     std::deque<QueryPtr> pending, work, batch_in, done;
     std::map<uint64_t, QueryPtr> popped;
@@ -209,11 +235,14 @@ struct gdl_server {
         gdl_status st = gdl_ctx_create(&c, &ctx);
         if (st == GDL_OK) st = partition(ctx, e);
         if (st == GDL_OK) {
-            // keep freed blocks for the next query (returning them to HIP
-            // synchronises the whole device, stalling every other worker),
-            // up to half this worker's share of the budget
+            // keep freed blocks for the next query, up to this worker's share
+            // of the budget: a block returned to HIP synchronises the whole
+            // device, and allocating a multi-GB block again costs tens of ms,
+            // so a cache that keeps evicting the blocks the next query needs
+            // stalls every executor (on MI355X a 1,000-query mix spent a
+            // third of its time in that churn at half this limit)
             const uint64_t share = budget / (uint64_t)(cfg.workers + (cfg.batch_tc ? 1 : 0));
-            gdl_set_cache_limit(ctx, share / 2);
+            gdl_set_cache_limit(ctx, share);
         }
         {
             std::lock_guard<std::mutex> g(mu);
@@ -242,8 +271,10 @@ struct gdl_server {
             ProgramOutput out;
             out.want_keys = cfg.keep_keys != 0;
             std::string err;
+            if (cfg.take_turns) turn.acquire();
             gdl_status rs = run_program_host(ctx, q->kind, q->a.data(), q->a.size(),
                                              q->b.data(), q->b.size(), &out, &err);
+            if (cfg.take_turns) turn.release();
             gdl_set_mem_cap(ctx, 0);
             if (rs == GDL_ERR_OUT_OF_MEMORY && !alone) {
                 // outgrew its reservation: rerun alone, first in line
@@ -285,6 +316,12 @@ struct gdl_server {
         gdl_status st = gdl_ctx_create(&c, &ctx);
         if (st == GDL_OK) st = partition(ctx, cfg.workers);
         if (st == GDL_OK) {
+            // the same cache limit as a worker: instances of every size come
+            // and go, and an unbounded cache of their freed sets fills the
+            // device until every allocation miss (in any thread) starts
+            // returning blocks to HIP, each return synchronising the device
+            const uint64_t share = budget / (uint64_t)(cfg.workers + 1);
+            gdl_set_cache_limit(ctx, share);
             gdl_tcb_config bc;
             gdl_tcb_config_default(&bc);
             bc.max_slots = cfg.batch_slots;
@@ -304,6 +341,10 @@ struct gdl_server {
             return;
         }
         std::map<uint64_t, QueryPtr> inflight;
+        // take_turns: the batch holds the device for a slice of bodies, then
+        // lets the worker whose ticket is next run its query
+        bool holding = false;
+        double t_turn = 0.0;
         for (;;) {
             std::deque<QueryPtr> fresh;
             {
@@ -311,6 +352,11 @@ struct gdl_server {
                 cv_batch.wait(lk, [&] { return stop || !batch_in.empty() || !inflight.empty(); });
                 if (stop && batch_in.empty() && inflight.empty()) break;
                 fresh.swap(batch_in);
+            }
+            if (cfg.take_turns && !holding && (!fresh.empty() || !inflight.empty())) {
+                turn.acquire();
+                holding = true;
+                t_turn = clock_ms();
             }
             for (auto &q : fresh) {
                 gdl_status as = gdl_tcb_admit(b, q->qid, q->a.data(), q->a.size(), q->n_nodes,
@@ -324,7 +370,11 @@ struct gdl_server {
                 const uint64_t id = q->qid;
                 inflight[id] = std::move(q);
             }
-            if (inflight.empty()) continue;
+            if (inflight.empty()) {
+                if (holding) turn.release();
+                holding = false;
+                continue;
+            }
             uint32_t n_active = 0;
             gdl_status ss = gdl_tcb_step(b, &n_active);
             if (ss != GDL_OK) {
@@ -367,7 +417,12 @@ struct gdl_server {
                 std::lock_guard<std::mutex> g(mu);
                 finish_locked(std::move(q), false, true);
             }
+            if (holding && (inflight.empty() || clock_ms() - t_turn > kBatchSliceMs)) {
+                turn.release();
+                holding = false;
+            }
         }
+        if (holding) turn.release();
         if (b) gdl_tcb_destroy(b);
         gdl_ctx_destroy(ctx);
     }
@@ -385,6 +440,7 @@ void gdl_server_config_default(gdl_server_config *cfg) {
     cfg->mem_budget = 0;
     cfg->keep_keys = 0;
     cfg->partition_cus = 0;
+    cfg->take_turns = 0;
 }
 
 uint64_t gdl_query_estimate(int kind, const uint64_t *a, uint64_t na, const uint64_t *b,

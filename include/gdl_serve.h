@@ -52,6 +52,13 @@ typedef struct gdl_server_config {
                              so one query's kernels never queue behind another's;
                              0: all share every CU (default 0: the partitioned
                              configuration has not been validated on the device) */
+    int take_turns;       /* 1: the TC batch and the workers take turns on the device
+                             (one at a time, in the order they asked; the batch's
+                             turn is a slice of about 20 ms of bodies), so no
+                             executor's kernels share the device with another's;
+                             0: they run concurrently (default 0: on the TC / SG /
+                             CSPA mix concurrency gave the same p99 and a 4x lower
+                             p50) */
 } gdl_server_config;
 
 typedef struct gdl_query_result {

@@ -57,10 +57,10 @@ def run_mix(server, queries):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("batch_tc", [True, False])
-def test_server_mix_equals_alone(ctx, batch_tc):
+@pytest.mark.parametrize("batch_tc,take_turns", [(True, False), (False, False), (True, True)])
+def test_server_mix_equals_alone(ctx, batch_tc, take_turns):
     queries = mix_queries()
-    with gdl.Server(workers=3, batch_tc=batch_tc, keep_keys=True) as s:
+    with gdl.Server(workers=3, batch_tc=batch_tc, keep_keys=True, take_turns=take_turns) as s:
         got = run_mix(s, queries)
         st = s.stats()
     assert st["finished"] == len(queries) and st["reserved"] == 0
