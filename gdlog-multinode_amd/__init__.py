@@ -21,6 +21,7 @@ REPO = os.path.dirname(HERE)
 INCLUDE_DIR = os.path.join(REPO, "include")
 
 GDL_OK = 0
+GDL_ERR_INVALID_ARG = 1
 STATUS_NAMES = {0: "ok", 1: "invalid argument", 2: "out of device memory", 3: "HIP error",
                 4: "unsupported", 5: "size limit exceeded", 6: "device-side error",
                 7: "busy"}
@@ -58,6 +59,16 @@ def set_full_mode(mode):
     st = prog.gdl_prog_set_full_mode(mode)
     if st != GDL_OK:
         raise GdlError(st, "gdl_prog_set_full_mode")
+
+
+def set_checkpoint(dir=None, every=0, resume=False, stop_after=-1):
+    """Checkpoint / resume of later program runs on this thread (see
+    gdl_prog_set_checkpoint); dir=None turns it off."""
+    _, prog = load()
+    d = None if dir is None else os.fsencode(dir)
+    st = prog.gdl_prog_set_checkpoint(d, every, int(resume), stop_after)
+    if st != GDL_OK:
+        raise GdlError(st, "gdl_prog_set_checkpoint")
 
 
 class ColMap(Structure):
@@ -192,6 +203,7 @@ def load():
     _decl(prog, "gdl_prog_cspa", c_int, [vp, vp, c_uint64, vp, c_uint64, c_int, rp, bp])
     _decl(prog, "gdl_prog_last_error", c_char_p, [])
     _decl(prog, "gdl_prog_set_full_mode", c_int, [c_int])
+    _decl(prog, "gdl_prog_set_checkpoint", c_int, [ctypes.c_char_p, c_int, c_int, c_int])
     _decl(eng, "gdl_hashset_create", c_int, [vp, c_uint64, POINTER(HashSet)])
     _decl(eng, "gdl_hashset_free", c_int, [vp, POINTER(HashSet)])
     _decl(eng, "gdl_hashset_insert_new", c_int,

@@ -7,6 +7,8 @@
 // in one merge-path pass (gdl_merge_diff_u64), which is the reference's
 // thrust::set_difference followed by Relation::flush_delta's thrust::merge.
 #include <algorithm>
+#include <cstdio>
+#include <cstring>
 #include <iostream>
 
 #include "gdlog/exception.h"
@@ -98,20 +100,250 @@ void leave_append_mode(Relation *rel) {
     rel->append_mode = false;
 }
 
+// ---- checkpoint / resume ---------------------------------------------------
+// Snapshot file (host byte order):
+//   "GDLCKPT1", u32 update relations, u32 static relations, i32 iteration
+//   counter, i32 finished, u64 trace records, the records (i32, i32, u64 x 3);
+//   per static relation: name, u64 tuples, u64 checksum (a fingerprint: a
+//   resumed run must have loaded the same inputs);
+//   per update relation: name, i32 arity, u8 FULL sorted, u8 delta sorted,
+//   u64 |FULL|, u64 |delta|, the FULL keys, the delta keys.
+// A name is u32 length + bytes.
+constexpr char kCkptMagic[8] = {'G', 'D', 'L', 'C', 'K', 'P', 'T', '1'};
+
+struct CkptDefaults {
+    std::string dir;
+    int every = 0;
+    bool resume = false;
+    int stop_after = -1;
+    int next = 0;
+};
+thread_local CkptDefaults g_ckpt;
+
+class CkptFile {
+  public:
+    CkptFile(const std::string &path, const char *mode) : f_(std::fopen(path.c_str(), mode)) {}
+    ~CkptFile() {
+        if (f_) std::fclose(f_);
+    }
+    bool ok() const { return f_ != nullptr; }
+    void put(const void *p, size_t n) {
+        if (n && std::fwrite(p, 1, n, f_) != n) throw Error(GDL_ERR_INVALID_ARG, "checkpoint write failed");
+    }
+    void get(void *p, size_t n) {
+        if (n && std::fread(p, 1, n, f_) != n)
+            throw Error(GDL_ERR_INVALID_ARG, "checkpoint file is truncated");
+    }
+    template <typename T>
+    void put_v(T v) { put(&v, sizeof(v)); }
+    template <typename T>
+    T get_v() {
+        T v;
+        get(&v, sizeof(v));
+        return v;
+    }
+    void put_name(const std::string &s) {
+        put_v<uint32_t>((uint32_t)s.size());
+        put(s.data(), s.size());
+    }
+    std::string get_name() {
+        const uint32_t n = get_v<uint32_t>();
+        if (n > (1u << 20)) throw Error(GDL_ERR_INVALID_ARG, "checkpoint file is corrupt");
+        std::string s(n, '\0');
+        get(&s[0], n);
+        return s;
+    }
+    void close() {
+        if (f_ && std::fclose(f_) != 0) {
+            f_ = nullptr;
+            throw Error(GDL_ERR_INVALID_ARG, "checkpoint write failed");
+        }
+        f_ = nullptr;
+    }
+
+  private:
+    FILE *f_;
+};
+
+void put_keys(CkptFile &f, const uint64_t *d_keys, tuple_size_t n) {
+    if (n == 0) return;
+    std::vector<uint64_t> h(n);
+    check(gdl_memcpy_d2h(context(), h.data(), d_keys, n * sizeof(uint64_t)), "checkpoint D2H");
+    f.put(h.data(), n * sizeof(uint64_t));
+}
+
+uint64_t *get_keys(CkptFile &f, tuple_size_t n) {
+    if (n == 0) return nullptr;
+    std::vector<uint64_t> h(n);
+    f.get(h.data(), n * sizeof(uint64_t));
+    uint64_t hi = 0, lo = 0;
+    for (uint64_t k : h) {
+        hi |= k >> 32;
+        lo |= k & 0xffffffffull;
+    }
+    note_id_bits(std::max(hi ? 64 - __builtin_clzll(hi) : 0, lo ? 64 - __builtin_clzll(lo) : 0));
+    uint64_t *d = device_alloc_keys(n);
+    check(gdl_memcpy_h2d(context(), d, h.data(), n * sizeof(uint64_t)), "checkpoint H2D");
+    return d;
+}
+
+// The state at an iteration boundary: every update relation's newt has been
+// consumed, its delta is the next iteration's outer, and its FULL holds
+// current_full_size keys (a sorted array, or the append log).
+void write_snapshot(const LIE &lie, const std::string &path, int iteration, bool finished) {
+    const std::string tmp = path + ".tmp";
+    {
+        CkptFile f(tmp, "wb");
+        if (!f.ok()) throw Error(GDL_ERR_INVALID_ARG, "cannot write checkpoint " + tmp);
+        f.put(kCkptMagic, sizeof(kCkptMagic));
+        f.put_v<uint32_t>((uint32_t)lie.update_relations.size());
+        f.put_v<uint32_t>((uint32_t)lie.static_relations.size());
+        f.put_v<int32_t>(iteration);
+        f.put_v<int32_t>(finished ? 1 : 0);
+        f.put_v<uint64_t>(lie.trace.size());
+        for (const auto &t : lie.trace) {
+            f.put_v<int32_t>(t.iteration);
+            f.put_v<int32_t>(t.relation);
+            f.put_v<uint64_t>(t.newt);
+            f.put_v<uint64_t>(t.delta);
+            f.put_v<uint64_t>(t.full);
+        }
+        for (Relation *rel : lie.static_relations) {
+            f.put_name(rel->name);
+            f.put_v<uint64_t>(rel->full->tuple_counts);
+            f.put_v<uint64_t>(relation_checksum(rel->full));
+        }
+        for (Relation *rel : lie.update_relations) {
+            const tuple_size_t nf = rel->current_full_size;
+            const tuple_size_t nd = rel->delta->keys ? rel->delta->tuple_counts : 0;
+            f.put_name(rel->name);
+            f.put_v<int32_t>(rel->arity);
+            f.put_v<uint8_t>(rel->full->sorted ? 1 : 0);
+            f.put_v<uint8_t>(rel->delta->sorted ? 1 : 0);
+            f.put_v<uint64_t>(nf);
+            f.put_v<uint64_t>(nd);
+            put_keys(f, rel->full->keys, nf);
+            put_keys(f, rel->delta->keys, nd);
+        }
+        f.close();
+    }
+    if (std::rename(tmp.c_str(), path.c_str()) != 0)
+        throw Error(GDL_ERR_INVALID_ARG, "cannot move checkpoint into place: " + path);
+}
+
+// Restores a snapshot into the LIE's update relations; returns false when
+// the file does not exist.  The static relations must match the snapshot's.
+bool read_snapshot(LIE &lie, const std::string &path, int *iteration, bool *finished) {
+    CkptFile f(path, "rb");
+    if (!f.ok()) return false;
+    char magic[8];
+    f.get(magic, sizeof(magic));
+    if (std::memcmp(magic, kCkptMagic, sizeof(magic)) != 0)
+        throw Error(GDL_ERR_INVALID_ARG, "not a checkpoint file: " + path);
+    const uint32_t nu = f.get_v<uint32_t>(), ns = f.get_v<uint32_t>();
+    if (nu != lie.update_relations.size() || ns != lie.static_relations.size())
+        throw Error(GDL_ERR_INVALID_ARG, "checkpoint " + path + " is of another LIE");
+    *iteration = f.get_v<int32_t>();
+    *finished = f.get_v<int32_t>() != 0;
+    const uint64_t nt = f.get_v<uint64_t>();
+    if (nt > (1ull << 32)) throw Error(GDL_ERR_INVALID_ARG, "checkpoint file is corrupt");
+    lie.trace.clear();
+    for (uint64_t i = 0; i < nt; ++i) {
+        IterationRecord t;
+        t.iteration = f.get_v<int32_t>();
+        t.relation = f.get_v<int32_t>();
+        t.newt = f.get_v<uint64_t>();
+        t.delta = f.get_v<uint64_t>();
+        t.full = f.get_v<uint64_t>();
+        lie.trace.push_back(t);
+    }
+    for (Relation *rel : lie.static_relations) {
+        const std::string name = f.get_name();
+        const uint64_t n = f.get_v<uint64_t>(), sum = f.get_v<uint64_t>();
+        if (name != rel->name || n != rel->full->tuple_counts || sum != relation_checksum(rel->full))
+            throw Error(GDL_ERR_INVALID_ARG, "checkpoint " + path + ": static relation " +
+                                                 rel->name + " differs from the snapshot's");
+    }
+    for (Relation *rel : lie.update_relations) {
+        const std::string name = f.get_name();
+        const int arity = f.get_v<int32_t>();
+        const bool full_sorted = f.get_v<uint8_t>() != 0, delta_sorted = f.get_v<uint8_t>() != 0;
+        const uint64_t nf = f.get_v<uint64_t>(), nd = f.get_v<uint64_t>();
+        if (name != rel->name || arity != rel->arity)
+            throw Error(GDL_ERR_INVALID_ARG, "checkpoint " + path + ": relation " + rel->name +
+                                                 " is not the snapshot's " + name);
+        uint64_t *full = get_keys(f, nf);
+        container_take(rel->full, full, nf, nf, full_sorted);
+        rel->current_full_size = nf;
+        uint64_t *delta = get_keys(f, nd);
+        container_free(rel->delta);
+        if (nd) container_take(rel->delta, delta, nd, nd, delta_sorted);
+        container_free(rel->newt);
+    }
+    return true;
+}
+
 }  // namespace
+
+namespace gdlog {
+void set_checkpoint_defaults(const char *dir, int every, bool resume, int stop_after) {
+    g_ckpt.dir = dir ? dir : "";
+    g_ckpt.every = every > 0 ? every : 0;
+    g_ckpt.resume = resume;
+    g_ckpt.stop_after = stop_after;
+    g_ckpt.next = 0;
+}
+void reset_checkpoint_sequence() { g_ckpt.next = 0; }
+}  // namespace gdlog
 
 void LIE::fixpoint_loop() {
     trace.clear();
     gdl_ctx *ctx = context();
     if (verbose()) std::cout << "start lie .... " << std::endl;
+    // checkpoint files: the LIE's own, else the thread's defaults
+    std::string ck_file = checkpoint_file, rs_file = resume_file;
+    int ck_every = checkpoint_every, ck_stop = stop_after;
+    if (ck_file.empty() && rs_file.empty() && !g_ckpt.dir.empty()) {
+        ck_file = g_ckpt.dir + "/lie" + std::to_string(g_ckpt.next++) + ".ckpt";
+        ck_every = g_ckpt.every;
+        ck_stop = g_ckpt.stop_after;
+        if (g_ckpt.resume) rs_file = ck_file;
+    }
+    int iteration_counter = 0;
+    bool finished = false;
+    const bool resumed =
+        !rs_file.empty() && read_snapshot(*this, rs_file, &iteration_counter, &finished);
+    if (resumed) {
+        // a snapshot is taken after iteration 0, which ran the FULL copies
+        for (auto &op : ra_ops)
+            if (auto *c = std::get_if<RelationalCopy>(&op))
+                if (c->src_ver == FULL) c->copied = true;
+    }
     for (Relation *rel : update_relations) {
-        rel->current_full_size = rel->full->tuple_counts;
-        seed_delta(rel);
+        if (!resumed) {
+            rel->current_full_size = rel->full->tuple_counts;
+            seed_delta(rel);
+        }
         if (full_mode() == FullMode::Auto && !full_is_inner(ra_ops, rel)) {
             enter_append_mode(rel);
             rel->fused_output = output_fusable(ra_ops, rel);
+        } else if (resumed && !rel->full->sorted) {
+            container_normalize(rel->full);  // the merge needs a sorted FULL
+            rel->current_full_size = rel->full->tuple_counts;
         }
     }
+    // an iteration boundary: the snapshot when one is due; stop_after ends
+    // the run right after its snapshot
+    auto boundary = [&](bool fixpoint) {
+        if (ck_file.empty()) return;
+        const bool stop = !fixpoint && ck_stop >= 0 && iteration_counter >= ck_stop;
+        if (fixpoint || stop || iteration_counter > max_iteration ||
+            (ck_every > 0 && iteration_counter % ck_every == 0))
+            write_snapshot(*this, ck_file, iteration_counter, fixpoint);
+        if (stop)
+            throw Error(GDL_ERR_BUSY, "fixpoint stopped after its checkpoint at iteration " +
+                                          std::to_string(iteration_counter));
+    };
     // A one-join LIE of TC's shape runs its bodies after the first as one
     // stream-ordered loop (RelationalJoin::run_linear): the same bodies, no
     // host synchronisation between them.
@@ -123,11 +355,15 @@ void LIE::fixpoint_loop() {
                 static_relations.end())
             linear = j;
     }
-    int iteration_counter = 0;
-    while (true) {
+    while (!finished) {
         if (linear && iteration_counter > 0) {
             Relation *rel = update_relations[0];
-            const long long left = (long long)max_iteration + 1 - iteration_counter;
+            long long left = (long long)max_iteration + 1 - iteration_counter;
+            if (!ck_file.empty()) {  // land on the next snapshot
+                if (ck_every > 0) left = std::min<long long>(left, ck_every - iteration_counter % ck_every);
+                if (ck_stop > iteration_counter)
+                    left = std::min<long long>(left, ck_stop - iteration_counter);
+            }
             const int bodies = (int)std::min<long long>(left, 1 << 30);
             std::vector<uint64_t> nn, rr;
             tuple_size_t full = rel->current_full_size;
@@ -147,6 +383,7 @@ void LIE::fixpoint_loop() {
                         break;
                     }
                 }
+                boundary(fixpoint);
                 if (fixpoint || iteration_counter > max_iteration) break;
                 continue;
             }
@@ -267,6 +504,7 @@ void LIE::fixpoint_loop() {
         if (verbose())
             std::cout << "Iteration " << iteration_counter << " finish populating" << std::endl;
         iteration_counter++;
+        boundary(fixpoint_flag);
         if (fixpoint_flag || iteration_counter > max_iteration) break;
     }
     iteration_count = iteration_counter;

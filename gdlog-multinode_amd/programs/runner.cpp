@@ -101,6 +101,7 @@ gdl_status run_program(gdl_ctx *ctx, gdl_prog_result *res, const gdl_prog_buffer
     gdl_ctx *prev = context_or_null();
     set_context(ctx);
     reset_id_bits();
+    reset_checkpoint_sequence();
     ProgramOutput out;
     out.want_keys = bufs && (bufs->out || bufs->extra0 || bufs->extra1);
     gdl_status st = GDL_OK;
@@ -150,6 +151,7 @@ gdl_status run_program_host(gdl_ctx *ctx, int kind, const uint64_t *a, uint64_t 
     gdl_ctx *prev = context_or_null();
     set_context(ctx);
     reset_id_bits();
+    reset_checkpoint_sequence();
     gdl_status st = GDL_OK;
     try {
         HostRows ha, hb;
@@ -210,6 +212,12 @@ gdl_status gdl_prog_cspa(gdl_ctx *ctx, const uint64_t *assign, uint64_t n_assign
 }
 
 const char *gdl_prog_last_error(void) { return g_last_error.c_str(); }
+
+gdl_status gdl_prog_set_checkpoint(const char *dir, int every, int resume, int stop_after) {
+    if (every < 0 || (resume != 0 && resume != 1)) return GDL_ERR_INVALID_ARG;
+    set_checkpoint_defaults(dir, every, resume != 0, stop_after);
+    return GDL_OK;
+}
 
 gdl_status gdl_prog_set_full_mode(int mode) {
     if (mode != 0 && mode != 1) return GDL_ERR_INVALID_ARG;

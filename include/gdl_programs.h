@@ -66,6 +66,20 @@ gdl_status gdl_prog_cspa(gdl_ctx *ctx, const uint64_t *assign, uint64_t n_assign
 /* Message of the last failed gdl_prog_* call on this thread. */
 const char *gdl_prog_last_error(void);
 
+/* Checkpoint / resume of the programs' fixpoints on this thread (an
+ * extension: the reference has none).  With dir != NULL, the k-th LIE that a
+ * later gdl_prog_* call runs (k from 0 in each call: TC has one, SG and CSPA
+ * two) writes a snapshot of its update relations' FULL and delta, its
+ * iteration counter and trace to <dir>/lie<k>.ckpt every `every` iterations
+ * (0: none in between), when max_iteration stops it, and at its fixpoint.
+ * resume = 1: a LIE whose file exists starts from it, and the result and
+ * trace are those of an uninterrupted run (the file must come from the same
+ * program and inputs: GDL_ERR_INVALID_ARG otherwise).  stop_after >= 0: a LIE
+ * that reaches that iteration stops right after its snapshot, and the call
+ * returns GDL_ERR_BUSY (a preempted run; later LIEs of the call do not run).
+ * dir = NULL turns all of this off. */
+gdl_status gdl_prog_set_checkpoint(const char *dir, int every, int resume, int stop_after);
+
 /* FULL maintenance of the fixpoint for later runs: 0 = auto (append log +
  * hash set for relations whose FULL is never a join inner), 1 = merge (sorted
  * merge every iteration, the reference's structure).  Results are identical;

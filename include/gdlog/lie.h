@@ -3,6 +3,7 @@
 #pragma once
 
 #include <climits>
+#include <string>
 #include <vector>
 
 #include "gdlog/relational_algebra.h"
@@ -30,6 +31,22 @@ struct LIE {
     bool reload_full_flag = true;
     int max_iteration = INT_MAX;
 
+    // Checkpoint / resume (an extension: the reference has none).
+    // checkpoint_file: the loop writes a snapshot there (every update
+    // relation's FULL and delta, the iteration counter, the trace, and a
+    // fingerprint of the static relations) every checkpoint_every iterations
+    // (0: none in between), when max_iteration or stop_after stops it, and
+    // at the fixpoint.  resume_file: when that file exists, the loop starts
+    // from its snapshot instead of iteration 0 (a finished snapshot restores
+    // the result without running a body); the trace and result are those of
+    // an uninterrupted run.  stop_after >= 0: right after the snapshot at that
+    // iteration the loop throws gdlog::Error(GDL_ERR_BUSY) (a preempted run).
+    // With none of them set, the thread's defaults apply (set_checkpoint_defaults).
+    std::string checkpoint_file;
+    int checkpoint_every = 0;
+    std::string resume_file;
+    int stop_after = -1;
+
     // filled by fixpoint_loop
     int iteration_count = 0;
     std::vector<IterationRecord> trace;
@@ -41,3 +58,12 @@ struct LIE {
     void add_tmp_relation(Relation *rel);
     void add_ra(ra_op op);
 };
+
+namespace gdlog {
+// Thread defaults for LIEs that set no checkpoint fields themselves: the
+// k-th LIE whose fixpoint_loop runs after reset_checkpoint_sequence() uses
+// <dir>/lie<k>.ckpt as its checkpoint file (and, with resume, as its resume
+// file).  dir = nullptr or "" turns the defaults off.
+void set_checkpoint_defaults(const char *dir, int every, bool resume, int stop_after);
+void reset_checkpoint_sequence();
+}  // namespace gdlog
