@@ -271,7 +271,7 @@ gdl_status gdl_join_insert(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n_inn
  * set hs, and the delta the log's last segment [*delta_begin, *delta_end).
  * Runs up to max_bodies bodies -- each one gdl_join_insert whose new tuples
  * are appended to the log and become the next delta -- enqueued back to back
- * on the stream: the host waits once per 32 bodies (or when the set or the
+ * on the stream: the host waits once per 128 bodies (or when the set or the
  * log must grow), not once per body.  Stops after the first body with no new
  * tuple (the fixpoint; that body is counted).  h_new / h_raw (max_bodies
  * entries) get each body's new and produced tuple counts, *bodies the bodies
