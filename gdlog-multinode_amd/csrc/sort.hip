@@ -4,7 +4,9 @@
 // (c0 << lo_bits) | c1 (lo_bits = bit length bound of c1), so the number of
 // digit passes is ceil((hi_bits + lo_bits) / r) instead of 64 / r, for the
 // context's radix width r (4..8 bits, default 8; digits narrower than 8 bits
-// leave the upper histogram rows empty).
+// leave the upper histogram rows empty).  With r = 8, 9-bit digits (512
+// bins, two per thread) are used where they save a pass: 36-bit keys (two
+// 18-bit ids) sort in 4 passes instead of 5.
 //
 // Per pass over n > 4096 keys, three launches:
 //   upsweep   -- per-tile digit histogram (LDS, one sub-histogram per wave),
@@ -25,7 +27,7 @@ constexpr int RS_B = 256;           // threads per block
 constexpr int RS_I = 16;            // keys per thread
 constexpr int RS_TILE = RS_B * RS_I; // 4096
 constexpr int RS_WSPAN = 64 * RS_I;  // keys per wave
-constexpr int RADIX = 256;
+constexpr int RADIX = 256;          // 8-bit digits (the one-block sort)
 
 __device__ __forceinline__ u64 compress_key(u64 k, int lo_bits) {
     return ((k >> 32) << lo_bits) | (k & 0xffffffffull);
@@ -46,11 +48,12 @@ __device__ __forceinline__ u64 tile_pos(int w, int r, int lane) {
     return (u64)w * RS_WSPAN + (u64)r * 64 + lane;
 }
 
-// Lanes of this wave that hold the same 8-bit digit (among `valid` lanes).
+// Lanes of this wave that hold the same RB-bit digit (among `valid` lanes).
+template <int RB>
 __device__ __forceinline__ u64 match_digit(u32 d, bool valid) {
     u64 peers = __ballot(valid);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < RB; ++b) {
         const bool bit = (d >> b) & 1u;
         const u64 bal = __ballot(bit);
         peers &= bit ? bal : ~bal;
@@ -58,20 +61,23 @@ __device__ __forceinline__ u64 match_digit(u32 d, bool valid) {
     return peers;
 }
 
-// Ranks the 16 keys a thread holds (wave-contiguous layout) within the tile:
-// on return s_hist[w][d] = exclusive offset of wave w's keys of digit d inside
-// the tile's digit-d run, s_start[d] = start of digit d's run in the tile, and
+// Ranks the 16 keys a thread holds (wave-contiguous layout) within the tile,
+// for digits of up to RB bits (2^RB bins, 2^RB / 256 per thread): on return
+// s_hist[w][d] = exclusive offset of wave w's keys of digit d inside the
+// tile's digit-d run, s_start[d] = start of digit d's run in the tile, and
 // rank[r] = rank of key r among its wave's keys of the same digit.
+template <int RB>
 __device__ __forceinline__ void rank_tile(const u64 *keys, const bool *valid, u32 *rank, int shift,
-                                          u32 (*s_hist)[RADIX], u32 *s_start, u64 *s_scan) {
+                                          u32 (*s_hist)[1 << RB], u32 *s_start, u64 *s_scan) {
+    constexpr int R = 1 << RB, DPT = R / RS_B;
     const int t = threadIdx.x, w = t >> 6;
     const u64 lt = lane_mask_lt();
-    for (int x = t; x < 4 * RADIX; x += RS_B) (&s_hist[0][0])[x] = 0;
+    for (int x = t; x < 4 * R; x += RS_B) (&s_hist[0][0])[x] = 0;
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < RS_I; ++r) {
         const u32 d = digit_of(keys[r], shift);
-        const u64 peers = match_digit(d, valid[r]);
+        const u64 peers = match_digit<RB>(d, valid[r]);
         const u32 before = s_hist[w][d];
         const u32 within = (u32)__popcll(peers & lt);
         __builtin_amdgcn_wave_barrier();
@@ -80,23 +86,40 @@ __device__ __forceinline__ void rank_tile(const u64 *keys, const bool *valid, u3
         rank[r] = before + within;
     }
     __syncthreads();
-    const u32 c0 = s_hist[0][t], c1 = s_hist[1][t], c2 = s_hist[2][t], c3 = s_hist[3][t];
+    // thread t owns digits t * DPT .. t * DPT + DPT - 1 (consecutive)
+    u32 c[DPT][4];
+    u32 sum = 0;
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) {
+        const int d = t * DPT + j;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            c[j][v] = s_hist[v][d];
+            sum += c[j][v];
+        }
+    }
     u64 tot;
-    const u32 start = (u32)block_exclusive_scan<RS_B>((u64)(c0 + c1 + c2 + c3), s_scan, &tot);
-    s_start[t] = start;
-    s_hist[0][t] = start;
-    s_hist[1][t] = start + c0;
-    s_hist[2][t] = start + c0 + c1;
-    s_hist[3][t] = start + c0 + c1 + c2;
+    u32 start = (u32)block_exclusive_scan<RS_B>((u64)sum, s_scan, &tot);
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) {
+        const int d = t * DPT + j;
+        s_start[d] = start;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            s_hist[v][d] = start;
+            start += c[j][v];
+        }
+    }
     __syncthreads();
 }
 
-template <bool IN_X>
+template <bool IN_X, int RB>
 __global__ __launch_bounds__(RS_B) void k_rs_upsweep(const u64 *in, u64 n, int shift, int lo_bits,
                                                      u32 *counts, u32 nb) {
-    __shared__ u32 s_h[4][RADIX];
+    constexpr int R = 1 << RB;
+    __shared__ u32 s_h[4][R];
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-    for (int x = t; x < 4 * RADIX; x += RS_B) (&s_h[0][0])[x] = 0;
+    for (int x = t; x < 4 * R; x += RS_B) (&s_h[0][0])[x] = 0;
     __syncthreads();
     const u64 base = (u64)blockIdx.x * RS_TILE;
 #pragma unroll
@@ -109,7 +132,8 @@ __global__ __launch_bounds__(RS_B) void k_rs_upsweep(const u64 *in, u64 n, int s
         }
     }
     __syncthreads();
-    counts[(u64)t * nb + blockIdx.x] = s_h[0][t] + s_h[1][t] + s_h[2][t] + s_h[3][t];
+    for (int d = t; d < R; d += RS_B)
+        counts[(u64)d * nb + blockIdx.x] = s_h[0][d] + s_h[1][d] + s_h[2][d] + s_h[3][d];
 }
 
 // One block per digit: exclusive scan of counts[d][0..nb) in place, totals[d].
@@ -128,21 +152,36 @@ __global__ __launch_bounds__(1024) void k_rs_rowscan(u32 *counts, u32 nb, u32 *t
     if (threadIdx.x == 0) totals[blockIdx.x] = (u32)carry;
 }
 
-template <bool IN_X, bool OUT_X>
+template <bool IN_X, bool OUT_X, int RB>
 __global__ __launch_bounds__(RS_B) void k_rs_downsweep(const u64 *in, u64 *out, u64 n, int shift,
                                                        int lo_bits, const u32 *counts,
                                                        const u32 *totals, u32 nb, u32 *err) {
+    constexpr int R = 1 << RB, DPT = R / RS_B;
     __shared__ u64 s_keys[RS_TILE];
-    __shared__ u32 s_hist[4][RADIX];
-    __shared__ u32 s_start[RADIX];
-    __shared__ u32 s_gbase[RADIX];
+    __shared__ u32 s_hist[4][R];
+    __shared__ u32 s_start[R];
+    __shared__ u32 s_gbase[R];
     __shared__ u64 s_scan[RS_B / 64 + 1];
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
     const u64 base = (u64)blockIdx.x * RS_TILE;
 
+    // global start of each digit (thread t: digits t * DPT ..) plus this
+    // tile's offset inside it
+    u32 tl[DPT];
+    u64 tsum = 0;
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) {
+        tl[j] = totals[t * DPT + j];
+        tsum += tl[j];
+    }
     u64 gtot;
-    const u64 gex = block_exclusive_scan<RS_B>((u64)totals[t], s_scan, &gtot);
-    s_gbase[t] = (u32)gex + counts[(u64)t * nb + blockIdx.x];
+    u64 gex = block_exclusive_scan<RS_B>(tsum, s_scan, &gtot);
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) {
+        const int d = t * DPT + j;
+        s_gbase[d] = (u32)gex + counts[(u64)d * nb + blockIdx.x];
+        gex += tl[j];
+    }
 
     u64 keys[RS_I];
     bool valid[RS_I];
@@ -155,7 +194,7 @@ __global__ __launch_bounds__(RS_B) void k_rs_downsweep(const u64 *in, u64 *out, 
         if (IN_X) k = compress_key(k, lo_bits);
         keys[r] = k;
     }
-    rank_tile(keys, valid, rank, shift, s_hist, s_start, s_scan);
+    rank_tile<RB>(keys, valid, rank, shift, s_hist, s_start, s_scan);
 #pragma unroll
     for (int r = 0; r < RS_I; ++r)
         if (valid[r]) {
@@ -205,7 +244,7 @@ __global__ __launch_bounds__(RS_B) void k_rs_oneblock(const u64 *in, u64 *out, u
     const int n_pass = passes & 0xff, dbits = passes >> 8;
     for (int pass = 0; pass < n_pass; ++pass) {
         const int shift = pass_code(pass * dbits, dbits);
-        rank_tile(keys, valid, rank, shift, s_hist, s_start, s_scan);
+        rank_tile<8>(keys, valid, rank, shift, s_hist, s_start, s_scan);
 #pragma unroll
         for (int r = 0; r < RS_I; ++r)
             if (valid[r]) {
@@ -295,6 +334,66 @@ static gdl_status measure_bits(gdl_ctx *ctx, const u64 *d_in, u64 n, int *hi_bit
     return GDL_OK;
 }
 
+// The multi-tile LSD passes over n > 4096 keys with digits of dbits <= RB
+// bits (2^RB histogram bins); the last pass lands in d_out.
+template <int RB>
+static gdl_status tile_passes(gdl_ctx *ctx, const u64 *d_in, u64 n, int lo_bits, bool xf,
+                              int dbits, int passes, u64 *d_out) {
+    constexpr int R = 1 << RB;
+    const u32 nb = (u32)((n + RS_TILE - 1) / RS_TILE);
+    Scratch scr(ctx);
+    u32 *counts, *totals;
+    u64 *tmp = nullptr;
+    GDL_TRY(scr.get((u64)R * nb, &counts));
+    GDL_TRY(scr.get(R, &totals));
+    if (passes > 1) GDL_TRY(scr.get(n, &tmp));
+    const u64 *src = d_in;
+    for (int p = 0; p < passes; ++p) {
+        u64 *dst = ((passes - 1 - p) % 2 == 0) ? d_out : tmp;
+        const int shift = pass_code(p * dbits, dbits);
+        const bool in_x = xf && p == 0;
+        const bool out_x = xf && p == passes - 1;
+        {
+            KTimer kt(ctx, K_RS_UPSWEEP, 8.0 * n);
+            if (in_x)
+                hipLaunchKernelGGL((k_rs_upsweep<true, RB>), dim3(nb), dim3(RS_B), 0, ctx->stream,
+                                   src, n, shift, lo_bits, counts, nb);
+            else
+                hipLaunchKernelGGL((k_rs_upsweep<false, RB>), dim3(nb), dim3(RS_B), 0, ctx->stream,
+                                   src, n, shift, lo_bits, counts, nb);
+            GDL_TRY(check_launch(ctx, "k_rs_upsweep"));
+        }
+        {
+            KTimer kt(ctx, K_RS_ROWSCAN, 8.0 * R * nb);
+            hipLaunchKernelGGL(k_rs_rowscan, dim3(R), dim3(1024), 0, ctx->stream, counts, nb,
+                               totals);
+            GDL_TRY(check_launch(ctx, "k_rs_rowscan"));
+        }
+        {
+            KTimer kt(ctx, K_RS_DOWNSWEEP, 16.0 * n);
+            if (in_x && out_x)
+                hipLaunchKernelGGL((k_rs_downsweep<true, true, RB>), dim3(nb), dim3(RS_B), 0,
+                                   ctx->stream, src, dst, n, shift, lo_bits, counts, totals, nb,
+                                   ctx->d_err);
+            else if (in_x)
+                hipLaunchKernelGGL((k_rs_downsweep<true, false, RB>), dim3(nb), dim3(RS_B), 0,
+                                   ctx->stream, src, dst, n, shift, lo_bits, counts, totals, nb,
+                                   ctx->d_err);
+            else if (out_x)
+                hipLaunchKernelGGL((k_rs_downsweep<false, true, RB>), dim3(nb), dim3(RS_B), 0,
+                                   ctx->stream, src, dst, n, shift, lo_bits, counts, totals, nb,
+                                   ctx->d_err);
+            else
+                hipLaunchKernelGGL((k_rs_downsweep<false, false, RB>), dim3(nb), dim3(RS_B), 0,
+                                   ctx->stream, src, dst, n, shift, lo_bits, counts, totals, nb,
+                                   ctx->d_err);
+            GDL_TRY(check_launch(ctx, "k_rs_downsweep"));
+        }
+        src = dst;
+    }
+    return GDL_OK;
+}
+
 // Shared driver of sort and sort+unique.  For n <= 4096 with `unique`,
 // *n_unique receives the unique count (the caller then skips unique_keys).
 static gdl_status sort_driver(gdl_ctx *ctx, const u64 *d_in, u64 n, int hi_bits, int lo_bits,
@@ -344,55 +443,11 @@ static gdl_status sort_driver(gdl_ctx *ctx, const u64 *d_in, u64 n, int hi_bits,
                                     ctx->stream));
         return GDL_OK;
     }
-    const u32 nb = (u32)((n + RS_TILE - 1) / RS_TILE);
-    Scratch scr(ctx);
-    u32 *counts, *totals;
-    u64 *tmp;
-    GDL_TRY(scr.get((u64)RADIX * nb, &counts));
-    GDL_TRY(scr.get(RADIX, &totals));
-    if (passes > 1) GDL_TRY(scr.get(n, &tmp));
-    // the last pass must land in d_out
-    const u64 *src = d_in;
-    for (int p = 0; p < passes; ++p) {
-        u64 *dst = ((passes - 1 - p) % 2 == 0) ? d_out : tmp;
-        const int shift = pass_code(p * dbits, dbits);
-        const bool in_x = xf && p == 0;
-        const bool out_x = xf && p == passes - 1;
-        {
-            KTimer kt(ctx, K_RS_UPSWEEP, 8.0 * n);
-            if (in_x)
-                hipLaunchKernelGGL(k_rs_upsweep<true>, dim3(nb), dim3(RS_B), 0, ctx->stream, src,
-                                   n, shift, lo_bits, counts, nb);
-            else
-                hipLaunchKernelGGL(k_rs_upsweep<false>, dim3(nb), dim3(RS_B), 0, ctx->stream, src,
-                                   n, shift, lo_bits, counts, nb);
-            GDL_TRY(check_launch(ctx, "k_rs_upsweep"));
-        }
-        {
-            KTimer kt(ctx, K_RS_ROWSCAN, 8.0 * RADIX * nb);
-            hipLaunchKernelGGL(k_rs_rowscan, dim3(RADIX), dim3(1024), 0, ctx->stream, counts, nb,
-                               totals);
-            GDL_TRY(check_launch(ctx, "k_rs_rowscan"));
-        }
-        {
-            KTimer kt(ctx, K_RS_DOWNSWEEP, 16.0 * n);
-            if (in_x && out_x)
-                hipLaunchKernelGGL((k_rs_downsweep<true, true>), dim3(nb), dim3(RS_B), 0,
-                                   ctx->stream, src, dst, n, shift, lo_bits, counts, totals, nb, ctx->d_err);
-            else if (in_x)
-                hipLaunchKernelGGL((k_rs_downsweep<true, false>), dim3(nb), dim3(RS_B), 0,
-                                   ctx->stream, src, dst, n, shift, lo_bits, counts, totals, nb, ctx->d_err);
-            else if (out_x)
-                hipLaunchKernelGGL((k_rs_downsweep<false, true>), dim3(nb), dim3(RS_B), 0,
-                                   ctx->stream, src, dst, n, shift, lo_bits, counts, totals, nb, ctx->d_err);
-            else
-                hipLaunchKernelGGL((k_rs_downsweep<false, false>), dim3(nb), dim3(RS_B), 0,
-                                   ctx->stream, src, dst, n, shift, lo_bits, counts, totals, nb, ctx->d_err);
-            GDL_TRY(check_launch(ctx, "k_rs_downsweep"));
-        }
-        src = dst;
-    }
-    return GDL_OK;
+    // nine-bit digits where they save a pass (36-bit keys: 4 passes, not 5;
+    // 18-bit keys: 2, not 3)
+    if (dbits == 8 && (total_bits + 8) / 9 < passes)
+        return tile_passes<9>(ctx, d_in, n, lo_bits, xf, 9, (total_bits + 8) / 9, d_out);
+    return tile_passes<8>(ctx, d_in, n, lo_bits, xf, dbits, passes, d_out);
 }
 
 gdl_status sort_keys(gdl_ctx *ctx, const u64 *d_in, u64 n, int hi_bits, int lo_bits, u64 *d_out) {

@@ -24,8 +24,11 @@ SIZES = [0, 1, 2, 63, 64, 65, 4095, 4096, 4097, 100_000, 1_000_003]
 
 
 @pytest.mark.parametrize("n", SIZES)
-@pytest.mark.parametrize("bits", [(32, 32), (18, 18), (15, 15), (1, 0), (0, 7)])
+@pytest.mark.parametrize("bits", [(32, 32), (18, 18), (15, 15), (1, 0), (0, 7),
+                                  (9, 9), (13, 14), (17, 17), (27, 0)])
 def test_sort_matches_numpy(ctx, n, bits):
+    # 8-bit digits, and 9-bit digits where they save a pass (18, 27, 34 and
+    # 36 key bits: 2, 3, 4 and 4 passes)
     k = rand_keys(n, *bits)
     got = ctx.sort(k, *bits)
     assert np.array_equal(got, np.sort(k))
