@@ -770,7 +770,11 @@ static gdl_status join_insert_impl(gdl_ctx *ctx, const u64 *d_inner, u64 n_inner
     const u64 loose_limit = std::max<u64>(1ull << 24, 4 * n_outer);
     const u64 cap_blocks = grid_cap(ctx);
     const int fb = (int)ctx->tune.join_block;  // block of the flattened kernel
-    const bool counted = bound > loose_limit;
+    // the count sizes the output set and buffer; with per-instance sets (SEG)
+    // the caller has sized both already (the output capacity is checked after
+    // the kernel), so only the load-balanced kernel below counts, for its
+    // offsets
+    const bool counted = !SEG && bound > loose_limit;
     Scratch lb(ctx);
     u32 *counts = nullptr;
     u64 *begins = nullptr, *offsets = nullptr;
