@@ -235,8 +235,9 @@ def served(gdl, ctx, n_req, rate_mult=0.8, seed=7, workers=1):
     """Served mix: n_req queries drawn from SERVED_MIX (TC / SG at the sizes of
     the single-query workloads, CSPA at 500 ids), Poisson arrivals at
     rate_mult x the one-at-a-time engine's throughput.  Serial latencies
-    replay the measured one-at-a-time service times against the same
-    arrivals (a FIFO of one); the server runs them for real."""
+    replay the one-at-a-time service times (measured on a second pass over the
+    requests) against the same arrivals (a FIFO of one); the server runs them
+    for real."""
     import numpy as np
     rng = np.random.default_rng(seed)
     names = [m[0] for m in SERVED_MIX]
@@ -256,10 +257,14 @@ def served(gdl, ctx, n_req, rate_mult=0.8, seed=7, workers=1):
     run = {gdl.QUERY_TC: lambda a, b: ctx.run_tc(a, want_keys=False, want_trace=False),
            gdl.QUERY_SG: lambda a, b: ctx.run_sg(a, want_keys=False, want_trace=False),
            gdl.QUERY_CSPA: lambda a, b: ctx.run_cspa(a, b, want_keys=False, want_trace=False)}
-    for kind in (gdl.QUERY_TC, gdl.QUERY_SG, gdl.QUERY_CSPA):  # warm code objects
-        q = next(r for r in reqs if r[1] == kind) if any(r[1] == kind for r in reqs) else None
-        if q:
-            run[kind](q[2], q[3])
+    # an untimed pass over every request first: code objects, the allocator's
+    # cache, and a device that an earlier process may have left with much of
+    # its memory still being reclaimed (its first large allocations then take
+    # seconds, which would inflate the one-at-a-time baseline)
+    for i, kind, a, b in reqs:
+        run[kind](a, b)
+        note(f"served mix: warm-up {i + 1}/{n_req}")
+    ctx.sync()
     service, alone = [], {}
     for i, kind, a, b in reqs:
         ctx.sync()
