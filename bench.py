@@ -66,13 +66,18 @@ def profiled_traffic(kernel, workload_tag):
 
 def phase_child(phase, extra):
     """Runs `bench.py --phase <phase>` as a child process (its progress
-    lines pass through on stderr) and returns the JSON object it prints."""
+    lines pass through on stderr) and returns the JSON object it prints, or
+    {"error": ...} when it fails (the headline line is still printed)."""
     import subprocess
     cmd = [sys.executable, "-u", os.path.abspath(__file__), "--phase", phase] + extra
     r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"bench.py --phase {phase} exited with {r.returncode}")
-    return json.loads(r.stdout.strip().splitlines()[-1])
+    lines = r.stdout.strip().splitlines()
+    if r.returncode != 0 or not lines:
+        return {"error": f"bench.py --phase {phase} exited with {r.returncode}"}
+    try:
+        return json.loads(lines[-1])
+    except ValueError:
+        return {"error": f"bench.py --phase {phase} printed no JSON line"}
 
 
 def cpu_threads():
