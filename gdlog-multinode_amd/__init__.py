@@ -173,6 +173,7 @@ def load():
     _decl(eng, "gdl_mem_info", c_int, [vp] + [_U64P] * 5)
     _decl(eng, "gdl_release_cached", c_int, [vp])
     _decl(eng, "gdl_set_mem_cap", c_int, [vp, c_uint64])
+    _decl(eng, "gdl_set_cache_limit", c_int, [vp, c_uint64])
     _decl(eng, "gdl_sort_u64", c_int, [vp, vp, c_uint64, c_int, c_int, vp])
     _decl(eng, "gdl_unique_u64", c_int, [vp, vp, c_uint64, vp, _U64P])
     _decl(eng, "gdl_key_bits", c_int, [vp, vp, c_uint64, POINTER(c_int), POINTER(c_int)])
@@ -387,6 +388,9 @@ class Context:
 
     def set_mem_cap(self, cap):
         self._check(self.eng.gdl_set_mem_cap(self.h, cap), "set_mem_cap")
+
+    def set_cache_limit(self, nbytes):
+        self._check(self.eng.gdl_set_cache_limit(self.h, nbytes), "set_cache_limit")
 
     def tuning(self):
         t = Tuning()

@@ -238,6 +238,30 @@ def test_allocator_cap_reports_out_of_memory(ctx):
     assert np.array_equal(ctx.sort(k), np.sort(k))
 
 
+def test_allocator_cache_limit_and_best_fit_reuse():
+    with gdl.Context(device=0) as c:
+        c.release_cached()
+        # a freed block is reused for a request up to 2x smaller, not larger
+        a = c.alloc(1 << 20)
+        a.free()
+        assert c.mem_info()["cached"] >= 8 << 20
+        b = c.alloc(600_000)  # 4.8 MB <= 8 MB <= 9.6 MB: the cached block
+        assert c.mem_info()["cached"] == 0
+        b.free()
+        # past the limit, freed blocks go back (largest first): 8 + 4 MB
+        # exceed 10 MB, the 8 MB block goes, then 2 and 1 MB fit beside 4
+        c.set_cache_limit(10 << 20)
+        blocks = [c.alloc(n) for n in (1 << 20, 1 << 19, 1 << 18, 1 << 17)]
+        for x in blocks:
+            x.free()
+        info = c.mem_info()
+        assert info["in_use"] == 0 and info["cached"] == 7 << 20
+        c.set_cache_limit(0)
+        # the context still computes after its cache was trimmed
+        k = rand_keys(50_000)
+        assert np.array_equal(c.sort(k), np.sort(k))
+
+
 @pytest.mark.parametrize("colmap,pred", [((1, 3), (gdl.PRED_NONE, 0, 1)),
                                          ((3, 1), (gdl.PRED_NE, 0, 1))])
 @pytest.mark.parametrize("inner_kind", ["uniform", "hub"])
