@@ -2,6 +2,19 @@
 // executables of the same names, CMakeLists.txt:58-91): read a tab-separated
 // edge file (CSPA: a directory holding assign.facts and dereference.facts),
 // run the program and print the result size and times.
+//
+// Options (extensions; the reference prints sizes only):
+//   --out PATH        the result as sorted tab-separated rows, one per tuple
+//                     (CSPA: a directory receiving ValueFlow.csv,
+//                     MemoryAlias.csv and ValueAlias.csv), for parity diffs
+//   --out-bin PATH    the main result as sorted little-endian u64 keys
+//                     (c0 << 32) | c1
+//   --checkpoint DIR  snapshot each fixpoint to DIR/lie<k>.ckpt (at the end,
+//                     and every N iterations with --every N); with --resume,
+//                     start each fixpoint from its snapshot there
+//   --max-iteration K (TC) stop after iteration K, as the reference's LIE
+//                     max_iteration
+// and always print the result checksum (sum of fmix64 over the keys).
 #include <cstdio>
 #include <cstdlib>
 #include <iostream>
@@ -30,6 +43,23 @@ std::vector<column_type> read_pairs(const std::string &path) {
     return rows;
 }
 
+void write_tsv(const std::string &path, const std::vector<uint64_t> &keys) {
+    FILE *f = std::fopen(path.c_str(), "w");
+    if (!f) throw gdlog::Error(GDL_ERR_INVALID_ARG, "cannot write " + path);
+    for (uint64_t k : keys)
+        std::fprintf(f, "%u\t%u\n", (unsigned)(k >> 32), (unsigned)(k & 0xffffffffu));
+    if (std::fclose(f) != 0) throw gdlog::Error(GDL_ERR_INVALID_ARG, "cannot write " + path);
+}
+
+void write_bin(const std::string &path, const std::vector<uint64_t> &keys) {
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) throw gdlog::Error(GDL_ERR_INVALID_ARG, "cannot write " + path);
+    const size_t n = keys.size();
+    const bool ok = n == 0 || std::fwrite(keys.data(), sizeof(uint64_t), n, f) == n;
+    if (std::fclose(f) != 0 || !ok)
+        throw gdlog::Error(GDL_ERR_INVALID_ARG, "cannot write " + path);
+}
+
 EdgeInput host_input(const std::vector<column_type> &rows) {
     EdgeInput in;
     in.host = rows.data();
@@ -40,29 +70,66 @@ EdgeInput host_input(const std::vector<column_type> &rows) {
 }  // namespace
 
 int main(int argc, char *argv[]) {
-    if (argc < 2) {
-        std::cerr << "usage: " << argv[0] << " <dataset>" << std::endl;
+    std::string dataset, out_path, out_bin, ckpt_dir;
+    int every = 0, max_iteration = -1;
+    bool resume = false;
+    for (int i = 1; i < argc; ++i) {
+        const std::string a = argv[i];
+        auto value = [&]() -> std::string {
+            if (i + 1 >= argc) {
+                std::cerr << a << " needs a value" << std::endl;
+                std::exit(2);
+            }
+            return argv[++i];
+        };
+        if (a == "--out") out_path = value();
+        else if (a == "--out-bin") out_bin = value();
+        else if (a == "--checkpoint") ckpt_dir = value();
+        else if (a == "--every") every = std::atoi(value().c_str());
+        else if (a == "--resume") resume = true;
+        else if (a == "--max-iteration") max_iteration = std::atoi(value().c_str());
+        else if (!a.empty() && a[0] == '-') {
+            std::cerr << "unknown option " << a << std::endl;
+            return 2;
+        } else dataset = a;
+    }
+    if (dataset.empty()) {
+        std::cerr << "usage: " << argv[0]
+                  << " <dataset> [--out PATH] [--out-bin PATH] [--checkpoint DIR [--every N]"
+                     " [--resume]] [--max-iteration K]"
+                  << std::endl;
         return 2;
     }
+#if !defined(GDL_PROGRAM_TC)
+    if (max_iteration >= 0) {
+        std::cerr << "--max-iteration applies to TC only" << std::endl;
+        return 2;
+    }
+#endif
     const int block_size = 512, grid_size = 32 * 256;
     ProgramOutput out;
+    out.want_keys = !out_path.empty() || !out_bin.empty();
+    gdlog::set_checkpoint_defaults(ckpt_dir.empty() ? nullptr : ckpt_dir.c_str(), every, resume,
+                                   -1);
+    gdlog::reset_checkpoint_sequence();
     try {
 #if defined(GDL_PROGRAM_TC)
-        std::vector<column_type> g = read_pairs(argv[1]);
+        std::vector<column_type> g = read_pairs(dataset);
         std::cout << "Input graph rows: " << g.size() / 2 << std::endl;
-        tc_program(host_input(g), -1, grid_size, block_size, &out);
+        tc_program(host_input(g), max_iteration, grid_size, block_size, &out);
         std::cout << "Path counts " << out.n_main << std::endl;
         std::cout << "TC time: " << out.fixpoint_ms / 1000.0 << std::endl;
+        if (!out_path.empty()) write_tsv(out_path, out.main_keys);
 #elif defined(GDL_PROGRAM_SG)
-        std::vector<column_type> g = read_pairs(argv[1]);
+        std::vector<column_type> g = read_pairs(dataset);
         std::cout << "Input graph rows: " << g.size() / 2 << std::endl;
         sg_program(host_input(g), grid_size, block_size, &out);
         std::cout << "sg counts " << out.n_main << std::endl;
         std::cout << "sg time: " << out.fixpoint_ms / 1000.0 << std::endl;
+        if (!out_path.empty()) write_tsv(out_path, out.main_keys);
 #elif defined(GDL_PROGRAM_CSPA)
-        std::string dir = argv[1];
-        std::vector<column_type> a = read_pairs(dir + "/assign.facts");
-        std::vector<column_type> d = read_pairs(dir + "/dereference.facts");
+        std::vector<column_type> a = read_pairs(dataset + "/assign.facts");
+        std::vector<column_type> d = read_pairs(dataset + "/dereference.facts");
         std::cout << "Input assign rows: " << a.size() / 2 << std::endl;
         std::cout << "Input dereference rows: " << d.size() / 2 << std::endl;
         cspa_program(host_input(a), host_input(d), grid_size, block_size, &out);
@@ -70,9 +137,16 @@ int main(int argc, char *argv[]) {
         std::cout << "MemoryAlias " << out.sizes[1] << std::endl;
         std::cout << "ValueAlias " << out.sizes[2] << std::endl;
         std::cout << "analysis scc time: " << out.fixpoint_ms / 1000.0 << std::endl;
+        if (!out_path.empty()) {
+            write_tsv(out_path + "/ValueFlow.csv", out.extra_keys[0]);
+            write_tsv(out_path + "/MemoryAlias.csv", out.extra_keys[1]);
+            write_tsv(out_path + "/ValueAlias.csv", out.main_keys);
+        }
 #endif
+        if (!out_bin.empty()) write_bin(out_bin, out.main_keys);
         std::cout << "iterations " << out.iterations << " derived tuples " << out.derived
                   << std::endl;
+        std::printf("checksum 0x%016llx\n", (unsigned long long)out.checksum);
     } catch (const gdlog::Error &e) {
         std::cerr << "error: " << e.what() << std::endl;
         return 1;
