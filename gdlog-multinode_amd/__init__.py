@@ -205,6 +205,16 @@ def load():
     _decl(prog, "gdl_prog_last_error", c_char_p, [])
     _decl(prog, "gdl_prog_set_full_mode", c_int, [c_int])
     _decl(prog, "gdl_prog_set_checkpoint", c_int, [ctypes.c_char_p, c_int, c_int, c_int])
+    _decl(prog, "gdl_datalog_compile", c_int, [c_char_p, POINTER(vp)])
+    _decl(prog, "gdl_datalog_destroy", None, [vp])
+    _decl(prog, "gdl_datalog_plan", c_char_p, [vp])
+    _decl(prog, "gdl_datalog_run", c_int,
+          [vp, vp, c_uint32, POINTER(c_char_p), POINTER(_U64P), _U64P])
+    _decl(prog, "gdl_datalog_result", c_int, [vp, c_char_p, _U64P, c_uint64, _U64P])
+    _decl(prog, "gdl_datalog_stats", c_int,
+          [vp, _U64P, POINTER(c_int32), c_uint32, POINTER(c_uint32)])
+    _decl(prog, "gdl_datalog_kept", c_char_p, [vp, c_uint32])
+    _decl(prog, "gdl_datalog_last_error", c_char_p, [])
     _decl(eng, "gdl_hashset_create", c_int, [vp, c_uint64, POINTER(HashSet)])
     _decl(eng, "gdl_hashset_free", c_int, [vp, POINTER(HashSet)])
     _decl(eng, "gdl_hashset_insert_new", c_int,
@@ -732,6 +742,91 @@ class Context:
 
 
 # ------------------------------------------------------------ batched TC
+
+class Datalog:
+    """A Datalog program over binary relations, compiled by the engine's front
+    end (include/gdl_datalog.h) into LIEs of RA operators.  Compiling needs no
+    device; run() executes on a Context.  Relations hold packed keys
+    (c0 << 32) | c1.
+
+        prog = Datalog(".decl edge(a: number, b: number) .input edge ..."
+                       "path(x, y) :- edge(x, y). ...")
+        out = prog.run(ctx, edge=keys)   # {"path": sorted uint64 array, ...}
+    """
+
+    def __init__(self, source):
+        _, self.prog = load()
+        h = c_void_p()
+        st = self.prog.gdl_datalog_compile(source.encode(), ctypes.byref(h))
+        if st != GDL_OK:
+            msg = (self.prog.gdl_datalog_last_error() or b"").decode(errors="replace")
+            raise GdlError(st, f"datalog compile: {msg}")
+        self.h = h
+
+    @property
+    def plan(self):
+        return self.prog.gdl_datalog_plan(self.h).decode()
+
+    def run(self, ctx, inputs=None, **kw):
+        """Runs on ctx with {relation: packed keys} for every .input relation;
+        returns {kept relation: sorted duplicate-free uint64 array}."""
+        inputs = dict(inputs or {}, **kw)
+        names = list(inputs)
+        arrs = [np.ascontiguousarray(inputs[n], dtype=np.uint64) for n in names]
+        k = len(names)
+        c_names = (c_char_p * max(k, 1))(*[n.encode() for n in names])
+        c_keys = (_U64P * max(k, 1))(*[_p64(a) for a in arrs])
+        c_counts = (c_uint64 * max(k, 1))(*[a.size for a in arrs])
+        st = self.prog.gdl_datalog_run(ctx.h, self.h, k, c_names, c_keys, c_counts)
+        if st != GDL_OK:
+            msg = (self.prog.gdl_datalog_last_error() or b"").decode(errors="replace")
+            raise GdlError(st, f"datalog run: {msg}")
+        return {n: self.result(n) for n in self.kept()}
+
+    def kept(self):
+        """Relations whose tuples run() keeps: the .output / .printsize ones,
+        else every derived one."""
+        out, i = [], 0
+        while True:
+            n = self.prog.gdl_datalog_kept(self.h, i)
+            if n is None:
+                return out
+            out.append(n.decode())
+            i += 1
+
+    def result(self, relation):
+        n = c_uint64()
+        st = self.prog.gdl_datalog_result(self.h, relation.encode(), None, 0, ctypes.byref(n))
+        if st != GDL_OK:
+            msg = (self.prog.gdl_datalog_last_error() or b"").decode(errors="replace")
+            raise GdlError(st, msg)
+        out = np.zeros(max(n.value, 1), dtype=np.uint64)
+        st = self.prog.gdl_datalog_result(self.h, relation.encode(), _p64(out), n.value,
+                                          ctypes.byref(n))
+        if st != GDL_OK:
+            raise GdlError(st, f"datalog result {relation}")
+        return out[:n.value].copy()
+
+    def stats(self):
+        """(new tuples, [iterations of each LIE]) of the last run."""
+        d = c_uint64()
+        n = c_uint32()
+        self.prog.gdl_datalog_stats(self.h, ctypes.byref(d), None, 0, ctypes.byref(n))
+        its = (c_int32 * max(n.value, 1))()
+        self.prog.gdl_datalog_stats(self.h, ctypes.byref(d), its, n.value, ctypes.byref(n))
+        return d.value, list(its[:n.value])
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.prog.gdl_datalog_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
 
 def fmix64(keys):
     """MurmurHash3 fmix64 of u64 keys (numpy, wrapping)."""

@@ -87,7 +87,8 @@ void RelationalJoin::operator()() {
         return;
     }
     if (!inner->keys || inner->tuple_counts == 0) {
-        outer->tuple_counts = 0;  // join.cu:42-45: an empty inner zeroes the outer's count
+        // join.cu:42-45: an empty inner zeroes the outer's count
+        if (empty_inner_zeroes_outer) outer->tuple_counts = 0;
         return;
     }
     if (!gen_map.ok)
@@ -201,7 +202,7 @@ void RelationalCopy::operator()() {
     GHashRelContainer *src = src_ver == DELTA ? src_rel->delta : src_rel->full;
     if (verbose()) std::cout << "Copy " << src_rel->name << " to " << dest_rel->name << std::endl;
     if (src->tuple_counts == 0) {
-        dest_rel->newt->tuple_counts = 0;  // copy.cu:23-26
+        if (empty_source_zeroes_dest) dest_rel->newt->tuple_counts = 0;  // copy.cu:23-26
         return;
     }
     if (!copy_map.ok)

@@ -1,0 +1,83 @@
+// gdlog/datalog.h -- a Datalog front end: lowers a program over binary
+// relations to LIEs of RA operators, the shape of the reference's
+// hand-written programs (test/tc.cu, test/sg.cu, test/cspa.cu, which encode
+// test/souffle/{tc,sg,cspa}.dl by hand), and runs them on the current engine
+// context.  An extension: the reference builds its RA-operator lists by hand.
+//
+// Accepted (a subset of Souffle's syntax):
+//   .decl R(a: number, b: number)       binary relations of number/unsigned
+//   .input R, S(IO=file, ...)           relations given to run() (attributes
+//                                       in parentheses are ignored)
+//   .output R   .printsize R            relations whose tuples run() keeps
+//                                       (every derived relation when a
+//                                       program names none)
+//   H(x, y) :- A(x, z), B(z, y), x != y.
+//     1 to 3 body atoms over variables (_ for an unused column); every head
+//     variable bound by the body; no variable twice in one atom; each atom
+//     after the first shares exactly one variable with what is joined so
+//     far, and at most two variables are live between two joins; at most one
+//     comparison (= != < <= > >=) between two head variables, in rules of two
+//     or more atoms.  Line and block comments.
+// Facts, constants, negation, aggregates, functors, other arities and other
+// directives are rejected (GDL_ERR_UNSUPPORTED), syntax errors and unknown
+// relations with GDL_ERR_INVALID_ARG; the message names the line.
+//
+// Lowering.  Strata are the SCCs of the relation dependency graph, in
+// dependency order.  A stratum's rules whose bodies read no relation of the
+// stratum form its init LIE (all reads FULL; a one-atom rule is a
+// RelationalCopy).  Its other rules form the recursive LIE: one semi-naive
+// variant per body atom of the stratum (that atom's DELTA joined with the
+// FULL of the others), each a chain of RelationalJoins through binary tmp
+// relations, then a RelationalCopy from DELTA per one-atom rule, then one
+// RelationalACopy per mirror.  The engine joins on column 0, so each relation
+// is stored in the column order its stratum reads most (a linear rule then
+// needs no second copy), and a read in the other order goes to a reversed
+// copy: a mirror kept by ACopy inside the relation's own recursive LIE, or a
+// static copy built from FULL once the relation is final.
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "gdl_hip.h"
+
+namespace gdlog {
+
+class DatalogProgram {
+  public:
+    DatalogProgram();
+    ~DatalogProgram();
+    DatalogProgram(const DatalogProgram &) = delete;
+    DatalogProgram &operator=(const DatalogProgram &) = delete;
+
+    // Parses and plans the program; on failure returns false with *status
+    // and *err set, and the program stays empty.
+    bool compile(const std::string &source, gdl_status *status, std::string *err);
+    // Runs every stratum on the current context.  inputs: packed keys
+    // ((c0 << 32) | c1), any order, duplicates allowed, for each .input
+    // relation.  Throws gdlog::Error (GDL_ERR_INVALID_ARG: not compiled, a
+    // missing or unknown input; engine errors as raised).
+    void run(const std::map<std::string, std::vector<uint64_t>> &inputs);
+    // Sorted, duplicate-free keys of a kept relation after run(); nullptr for
+    // a relation that is not kept.
+    const std::vector<uint64_t> *result(const std::string &relation) const;
+    // Kept relations, in declaration order.
+    const std::vector<std::string> &kept() const;
+    // The lowered plan: one line per step, LIE and operator.
+    const std::string &plan() const;
+    // Iteration count of each LIE of the last run, in plan order.
+    const std::vector<int> &iterations() const;
+    // New tuples of the last run: sum of |delta| over LIEs, iterations and
+    // stored relations (a reversed mirror's tuples are not counted again).
+    uint64_t derived() const;
+
+    struct Impl;
+
+  private:
+    std::unique_ptr<Impl> impl_;
+};
+
+}  // namespace gdlog

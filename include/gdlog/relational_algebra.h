@@ -39,6 +39,11 @@ struct RelationalJoin {
 
     gdlog::HookMap gen_map;
     gdlog::PredMap pred_map;
+    // The reference's handling of an empty inner: it zeroes the outer's tuple
+    // count (join.cu:42-45), so later operators reading the same outer version
+    // in that iteration see nothing.  The column-map constructor turns it off:
+    // generated plans read one delta from several joins.
+    bool empty_inner_zeroes_outer = true;
 
     RelationalJoin(Relation *inner_rel, RelationVersion inner_ver, Relation *outer_rel,
                    RelationVersion outer_ver, Relation *output_rel, tuple_generator_hook tp_gen,
@@ -50,6 +55,21 @@ struct RelationalJoin {
           detail_time(detail_time) {
         gen_map = gdlog::probe_generator(tp_gen, output_rel ? output_rel->arity : 0);
         pred_map = gdlog::probe_predicate(tp_pred, output_rel ? output_rel->arity : 0);
+    }
+
+    // Column-map form (an extension for generated plans, gdlog/datalog.h): the
+    // output columns and the predicate are given instead of probed from hooks.
+    RelationalJoin(Relation *inner_rel, RelationVersion inner_ver, Relation *outer_rel,
+                   RelationVersion outer_ver, Relation *output_rel, gdl_colmap map,
+                   gdl_pred pred)
+        : inner_rel(inner_rel), inner_ver(inner_ver), outer_rel(outer_rel), outer_ver(outer_ver),
+          output_rel(output_rel), tuple_generator(nullptr), tuple_pred(nullptr),
+          direction(LEFT), grid_size(0), block_size(0), detail_time(nullptr),
+          empty_inner_zeroes_outer(false) {
+        gen_map.ok = true;
+        gen_map.map = map;
+        pred_map.ok = true;
+        pred_map.pred = pred;
     }
 
     void operator()();
@@ -76,6 +96,10 @@ struct RelationalCopy {
     bool copied = false;
 
     gdlog::HookMap copy_map;
+    // The reference's handling of an empty source: it zeroes the
+    // destination's newt count (copy.cu:23-26), dropping what earlier
+    // operators put there in that iteration.  Off in the column-map form.
+    bool empty_source_zeroes_dest = true;
 
     RelationalCopy(Relation *src, RelationVersion src_ver, Relation *dest,
                    tuple_copy_hook tuple_generator, tuple_predicate tuple_pred, int grid_size,
@@ -83,6 +107,14 @@ struct RelationalCopy {
         : src_rel(src), src_ver(src_ver), dest_rel(dest), tuple_generator(tuple_generator),
           tuple_pred(tuple_pred), grid_size(grid_size), block_size(block_size) {
         copy_map = gdlog::probe_copy(tuple_generator, dest ? dest->arity : 0);
+    }
+
+    // Column-map form (source columns as GDL_COL_INNER0 / GDL_COL_INNER1).
+    RelationalCopy(Relation *src, RelationVersion src_ver, Relation *dest, gdl_colmap map)
+        : src_rel(src), src_ver(src_ver), dest_rel(dest), tuple_generator(nullptr),
+          tuple_pred(nullptr), grid_size(0), block_size(0), empty_source_zeroes_dest(false) {
+        copy_map.ok = true;
+        copy_map.map = map;
     }
 
     void operator()();
@@ -103,6 +135,14 @@ struct RelationalACopy {
         : src_rel(src), dest_rel(dest), tuple_generator(tuple_generator), tuple_pred(tuple_pred),
           grid_size(grid_size), block_size(block_size) {
         copy_map = gdlog::probe_copy(tuple_generator, dest ? dest->arity : 0);
+    }
+
+    // Column-map form (source columns as GDL_COL_INNER0 / GDL_COL_INNER1).
+    RelationalACopy(Relation *src, Relation *dest, gdl_colmap map)
+        : src_rel(src), dest_rel(dest), tuple_generator(nullptr), tuple_pred(nullptr),
+          grid_size(0), block_size(0) {
+        copy_map.ok = true;
+        copy_map.map = map;
     }
 
     void operator()();

@@ -12,7 +12,8 @@ HEADERS = {
     "libgdl_hip.so": [os.path.join(REPO, "include", "gdl_hip.h"),
                       os.path.join(REPO, "include", "gdl_batch.h")],
     "libgdlog.so": [os.path.join(REPO, "include", "gdl_programs.h"),
-                    os.path.join(REPO, "include", "gdl_serve.h")],
+                    os.path.join(REPO, "include", "gdl_serve.h"),
+                    os.path.join(REPO, "include", "gdl_datalog.h")],
 }
 
 

@@ -1,0 +1,317 @@
+"""The Datalog front end (include/gdl_datalog.h): programs in the Souffle subset
+lowered to LIEs of RA operators and run on the engine.
+
+Parity: the reference ships its programs as Souffle sources
+(test/souffle/tc.dl, sg.dl, cspa.dl) next to the hand-written RA encodings
+(test/tc.cu, test/sg.cu, test/cspa.cu).  The same rules, run through the
+front end, must give exactly the oracle's TC, SG and CSPA relations (the
+restated hand encodings), bit for bit.  Programs with no hand encoding
+(several strata, reversed static copies, copy rules inside a recursive
+stratum, an input relation that rules extend, comparisons in 3-atom rules)
+are checked against tests/datalog_naive.py, itself pinned here against the
+oracle.  Inputs are duplicate-free: with duplicate rows the hand encodings
+seed their first delta from raw rows (the oracle restates that), while a
+Datalog program's result depends only on the set of input tuples.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import gdl, read_edges
+from datalog_naive import evaluate
+
+TC_LEFT = """
+.decl edge(from: number, to: number)
+.input edge(IO=file, deliminator="\\t")
+.decl path(from: number, to: number)
+.output path
+path(x, y) :- edge(x, y).
+path(x, y) :- path(x, z), edge(z, y).
+"""
+TC_RIGHT = TC_LEFT.replace("path(x, z), edge(z, y)", "edge(x, z), path(z, y)")
+# test/souffle/tc.dl's form: both body atoms recursive
+TC_SQUARE = TC_LEFT.replace("path(x, z), edge(z, y)", "path(x, z), path(z, y)")
+
+# test/souffle/sg.dl
+SG = """
+.decl edge(x: number, y: number)
+.input edge
+.decl sg(x: number, y: number)
+.printsize sg
+sg(x, y) :- edge(p, x), edge(p, y), x != y.
+sg(x, y) :- edge(a, x), sg(a, b), edge(b, y).
+"""
+
+# test/souffle/cspa.dl:25-39
+CSPA = """
+.decl assign(src: number, dest: number)
+.input assign
+.decl dereference(src: number, dest: number)
+.input dereference
+.decl ValueFlow(src: number, dest: number)
+.output ValueFlow
+.decl ValueAlias(src: number, dest: number)
+.output ValueAlias
+.decl MemoryAlias(src: number, dest: number)
+.printsize MemoryAlias
+ValueFlow(x, y) :- ValueFlow(x, z), ValueFlow(z, y).
+ValueAlias(x, y) :- ValueFlow(z, x), ValueFlow(z, y).
+ValueFlow(x, y) :- assign(x, z), MemoryAlias(z, y).
+MemoryAlias(x, w) :- dereference(y, x), ValueAlias(y, z), dereference(z, w).
+ValueAlias(x, y) :- ValueFlow(z, x), MemoryAlias(z, w), ValueFlow(w, y).
+ValueFlow(y, x) :- assign(y, x).
+ValueFlow(x, x) :- assign(x, y).
+ValueFlow(x, x) :- assign(y, x).
+MemoryAlias(x, x) :- assign(y, x).
+MemoryAlias(x, x) :- assign(x, y).
+"""
+
+# several strata; reach is stored reversed (its recursive join reads it by
+# column 1), so `pairs` reads it through a reversed static copy; sym is an
+# input that a copy rule extends inside its own recursive stratum; odd/even
+# are mutually recursive with an always-empty inner joined before a
+# non-empty one on the same delta; `wide` is a 3-atom join with a comparison
+# in an init LIE; `back` is a copy from an earlier stratum.
+MIXED = """
+/* a block comment */
+.decl e(a: number, b: number)
+.decl seed(a: number, b: number)
+.decl sym(a: number, b: number)
+.decl nothing(a: number, b: number)
+.input e, seed, sym
+.input nothing
+.decl reach(a: number, b: number)
+.decl back(a: number, b: number)
+.decl pairs(a: number, b: number)
+.decl odd(a: number, b: number)
+.decl even(a: number, b: number)
+.decl wide(a: number, b: number)
+.output reach, back, pairs, sym
+.output odd, even, wide
+reach(x, y) :- seed(x, y).
+reach(x, y) :- reach(x, z), e(z, y).
+back(y, x) :- reach(x, y).
+pairs(x, y) :- reach(z, x), reach(z, y), x != y.
+sym(x, y) :- e(x, y).
+sym(x, y) :- sym(y, x).
+odd(x, y) :- e(x, y).
+even(x, y) :- odd(x, z), nothing(z, y).
+even(x, y) :- odd(x, z), e(z, y).
+odd(x, y) :- even(x, z), e(z, y).
+wide(x, y) :- e(x, z), sym(z, w), e(w, y), x < y.
+"""
+
+
+def uniq(keys):
+    return np.unique(np.asarray(keys, dtype=np.uint64))
+
+
+def mixed_inputs(seed):
+    rng = np.random.default_rng(seed)
+    e = uniq(gdl.gen_er(seed, 60, 110))
+    s = rng.integers(0, 60, size=(8, 2))
+    sy = rng.integers(0, 60, size=(5, 2))
+    return dict(e=e, seed=uniq(gdl.pack(s[:, 0], s[:, 1])),
+                sym=uniq(gdl.pack(sy[:, 0], sy[:, 1])), nothing=np.zeros(0, dtype=np.uint64))
+
+
+# ---- CPU: the checker, plans, rejections ---------------------------------------
+
+@pytest.mark.parametrize("src", [TC_LEFT, TC_RIGHT, TC_SQUARE])
+def test_naive_evaluator_matches_the_oracle_tc(src):
+    e = uniq(gdl.gen_er(7, 80, 160))
+    assert np.array_equal(evaluate(src, {"edge": e})["path"], oracle.tc(e)["out"])
+
+
+def test_naive_evaluator_matches_the_oracle_sg():
+    e = uniq(gdl.gen_rmat(3, 6, 120))
+    assert np.array_equal(evaluate(SG, {"edge": e})["sg"], oracle.sg(e)["out"])
+
+
+def test_naive_evaluator_matches_the_oracle_cspa():
+    a, d = gdl.gen_cspa(9, 40, 1.1, 50, 25)
+    a, d = uniq(a), uniq(d)
+    got = evaluate(CSPA, {"assign": a, "dereference": d})
+    want = oracle.cspa(a, d)
+    assert np.array_equal(got["ValueAlias"], want["out"])
+    assert np.array_equal(got["ValueFlow"], want["extra"][0])
+    assert np.array_equal(got["MemoryAlias"], want["extra"][1])
+
+
+def lie_blocks(plan):
+    """[(header line, [operator lines])] of a plan."""
+    out = []
+    for line in plan.splitlines():
+        if line.startswith("lie "):
+            out.append((line, []))
+        elif line.startswith("  "):
+            out[-1][1].append(line.strip())
+    return out
+
+
+def test_linear_tc_lowers_to_one_join_in_the_stored_order():
+    left = lie_blocks(gdl.Datalog(TC_LEFT).plan)
+    assert [len(ops) for _, ops in left] == [1, 1]
+    assert "update: path_2__2_1" in left[1][0]  # path(x, z) is read by z: stored reversed
+    assert left[1][1] == ["join edge_2__1_2 FULL x path_2__2_1 DELTA -> path_2__2_1 (i1, o1)"]
+    right = gdl.Datalog(TC_RIGHT).plan
+    assert "build edge_2__2_1 from edge_2__1_2" in right
+    assert "join edge_2__2_1 FULL x path_2__1_2 DELTA -> path_2__1_2 (i1, o1)" in right
+
+
+def test_square_tc_keeps_a_mirror():
+    blocks = lie_blocks(gdl.Datalog(TC_SQUARE).plan)
+    rec = blocks[1][1]
+    assert sum(op.startswith("join") for op in rec) == 2
+    assert rec[-1] == "acopy path_2__1_2 NEWT -> path_2__2_1 (i1, i0)"
+
+
+def test_sg_lowers_to_the_hand_encoding():
+    blocks = lie_blocks(gdl.Datalog(SG).plan)
+    assert blocks[0][1] == ["join edge_2__1_2 FULL x edge_2__1_2 FULL -> sg_2__1_2 (o1, i1) "
+                            "if c0 != c1"]
+    tmp = blocks[1][1][0].split("-> ")[1].split()[0]
+    assert blocks[1][1] == [f"join edge_2__1_2 FULL x sg_2__1_2 DELTA -> {tmp} (o1, i1)",
+                            f"join edge_2__1_2 FULL x {tmp} NEWT -> sg_2__1_2 (o1, i1)"]
+
+
+def test_cspa_plan_shape():
+    p = gdl.Datalog(CSPA)
+    blocks = lie_blocks(p.plan)
+    assert len(blocks) == 2
+    init, rec = blocks[0][1], blocks[1][1]
+    assert len(init) == 5 and all(op.startswith("copy assign_2__1_2 FULL") for op in init)
+    assert sum(op.startswith("join") for op in rec) == 13
+    assert [op for op in rec if op.startswith("acopy")] == [
+        "acopy ValueFlow_2__1_2 NEWT -> ValueFlow_2__2_1 (i1, i0)",
+        "acopy MemoryAlias_2__1_2 NEWT -> MemoryAlias_2__2_1 (i1, i0)"]
+    assert p.kept() == ["ValueFlow", "ValueAlias", "MemoryAlias"]
+
+
+def test_strata_run_in_dependency_order():
+    plan = gdl.Datalog(MIXED).plan
+    order = [line.split(":")[0][4:] for line in plan.splitlines() if line.startswith("lie ")]
+    assert order.index("reach") < order.index("back") < order.index("pairs")
+    assert order.index("sym") < order.index("wide")
+    assert "odd, even" in order or "even, odd" in order
+    assert "build reach_2__1_2 from reach_2__2_1" in plan  # pairs reads reach by column 0
+
+
+def test_kept_defaults_to_every_derived_relation():
+    src = TC_LEFT.replace(".output path\n", "") + ".decl hop(a: number, b: number)\n" \
+        "hop(x, y) :- path(x, z), edge(z, y).\n"
+    assert gdl.Datalog(src).kept() == ["path", "hop"]
+
+
+BAD = [
+    ("path(x, y) :- edge(x, y).", gdl.GDL_ERR_INVALID_ARG, "relation path is not declared"),
+    (".decl r(a: number)", 4, "arity 1"),
+    (".decl r(a: symbol, b: number)", 4, "attribute type symbol"),
+    (".decl r(a: number, b: number)\n.decl r(a: number, b: number)", gdl.GDL_ERR_INVALID_ARG,
+     "declared twice"),
+    (".decl r(a: number, b: number)\nr(1, 2).", 4, "constants in atoms"),
+    (".decl r(a: number, b: number)\nr(x, y).", 4, "facts"),
+    (".decl r(a: number, b: number)\nr(x, y) :- r(x, y), !r(y, x).", 4, "negation"),
+    (".decl r(a: number, b: number)\nr(x, y) :- r(x, z), r(w, y).", 4, "cross product"),
+    (".decl r(a: number, b: number)\nr(x, y) :- r(x, y), r(y, x).", 4, "two variables"),
+    (".decl r(a: number, b: number)\nr(x, y) :- r(x, x), r(x, y).", 4, "twice in one atom"),
+    (".decl r(a: number, b: number)\nr(x, y) :- r(x, z)", gdl.GDL_ERR_INVALID_ARG, "line 2"),
+    (".decl r(a: number, b: number)\nr(x, y) :- r(x, z).", gdl.GDL_ERR_INVALID_ARG,
+     "head variable y is not bound"),
+    (".decl r(a: number, b: number)\nr(x, y) :- r(x, y), x != 3.", 4, "constants in comparisons"),
+    (".decl r(a: number, b: number)\nr(x, y) :- r(x, y), x != y.", 4, "one-atom rule"),
+    (".decl r(a: number, b: number)\nr(x, y) :- r(x, z), r(z, y), z != y.", 4,
+     "not a head variable"),
+    (".decl r(a: number, b: number)\nr(x, y) :- r(x, a), r(a, b), r(b, c), r(c, y).", 4,
+     "more than three"),
+    (".decl r(a: number, b: number)\nr(x, y) :- r(x, z), r(z, y); r(y, x).", 4, "disjunction"),
+    (".decl r(a: number, b: number)\n.type T = number", 4, "directive .type"),
+    (".decl r(a: number, b: number)\nr(x, y) :- r(x, z), r(z, y), r(z, w).", 4,
+     "three variables live"),
+    ("/* open", gdl.GDL_ERR_INVALID_ARG, "unterminated comment"),
+]
+
+
+@pytest.mark.parametrize("src,status,fragment", BAD)
+def test_unsupported_and_invalid_programs_are_rejected(src, status, fragment):
+    with pytest.raises(gdl.GdlError) as e:
+        gdl.Datalog(src)
+    assert e.value.status == status
+    assert fragment in str(e.value)
+
+
+# ---- GPU: parity --------------------------------------------------------------
+
+TC_INPUTS = {
+    "er_500": lambda: gdl.gen_er(3, 500, 1500),
+    "tree_5k": lambda: gdl.gen_tree(11, 5000, 100),
+    "data_7035": lambda: read_edges("data_7035.txt"),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(TC_INPUTS))
+@pytest.mark.parametrize("form", ["left", "right", "square"])
+def test_tc_programs_match_the_oracle(ctx, form, name):
+    e = uniq(TC_INPUTS[name]())
+    src = {"left": TC_LEFT, "right": TC_RIGHT, "square": TC_SQUARE}[form]
+    prog = gdl.Datalog(src)
+    got = prog.run(ctx, edge=e)
+    want = oracle.tc(e)
+    assert np.array_equal(got["path"], want["out"])
+    derived, its = prog.stats()
+    assert derived == want["out"].size and len(its) == 2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,make", [("rmat_10", lambda: gdl.gen_rmat(6, 10, 3000)),
+                                       ("tree_2k", lambda: gdl.gen_tree(2, 2000, 30))])
+def test_sg_program_matches_the_oracle(ctx, name, make):
+    e = uniq(make())
+    got = gdl.Datalog(SG).run(ctx, edge=e)
+    assert np.array_equal(got["sg"], oracle.sg(e)["out"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed,n_ids,na,nd", [(5, 150, 200, 100), (8, 400, 600, 300)])
+def test_cspa_program_matches_the_oracle(ctx, seed, n_ids, na, nd):
+    a, d = gdl.gen_cspa(seed, n_ids, 1.1, na, nd)
+    a, d = uniq(a), uniq(d)
+    got = gdl.Datalog(CSPA).run(ctx, assign=a, dereference=d)
+    want = oracle.cspa(a, d)
+    assert np.array_equal(got["ValueAlias"], want["out"])
+    assert np.array_equal(got["ValueFlow"], want["extra"][0])
+    assert np.array_equal(got["MemoryAlias"], want["extra"][1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_mixed_program_matches_the_naive_evaluator(ctx, seed):
+    inputs = mixed_inputs(seed)
+    want = evaluate(MIXED, inputs)
+    got = gdl.Datalog(MIXED).run(ctx, inputs)
+    assert sorted(got) == sorted(want)
+    for rel in want:
+        assert np.array_equal(got[rel], want[rel]), rel
+    assert got["even"].size > 0  # the empty inner did not starve the other join
+
+
+@pytest.mark.gpu
+def test_runs_replace_results_and_reject_bad_inputs(ctx):
+    prog = gdl.Datalog(TC_LEFT)
+    e1 = uniq(gdl.gen_er(1, 200, 400))
+    e2 = uniq(gdl.gen_er(2, 100, 150))
+    assert np.array_equal(prog.run(ctx, edge=e1)["path"], oracle.tc(e1)["out"])
+    assert np.array_equal(prog.run(ctx, edge=e2)["path"], oracle.tc(e2)["out"])
+    # duplicates and order of the input rows do not matter
+    shuffled = np.concatenate([e2[::-1], e2[:10]])
+    assert np.array_equal(prog.run(ctx, edge=shuffled)["path"], oracle.tc(e2)["out"])
+    assert prog.run(ctx, edge=np.zeros(0, dtype=np.uint64))["path"].size == 0
+    with pytest.raises(gdl.GdlError) as e:
+        prog.run(ctx)
+    assert e.value.status == gdl.GDL_ERR_INVALID_ARG and "no tuples given" in str(e.value)
+    with pytest.raises(gdl.GdlError) as e:
+        prog.run(ctx, edge=e1, path=e1)
+    assert "not an .input relation" in str(e.value)
+    with pytest.raises(gdl.GdlError):
+        prog.result("edge")
