@@ -1,6 +1,6 @@
 # Build of the MI355X (gfx950) semi-naive Datalog engine.
 #   make            -- engine (libgdl_hip.so), C++ API + programs (libgdlog.so),
-#                      program executables, generators, CPU oracle
+#                      program executables (TC, SG, CSPA, DL), generators, CPU oracle
 #   make clean
 HIPCC    ?= /opt/rocm/bin/hipcc
 CC       ?= gcc
@@ -19,7 +19,7 @@ API_OBJ    := $(patsubst $(PKG)/%,$(LIB)/obj/%.o,$(API_SRC))
 HEADERS    := $(wildcard include/*.h include/gdlog/*.h $(PKG)/csrc/*.hpp $(PKG)/programs/*.h)
 
 all: $(LIB)/libgdl_hip.so $(LIB)/libgdlog.so $(LIB)/libgdl_gen.so oracle/lib/libgdl_oracle.so \
-     $(LIB)/TC $(LIB)/SG $(LIB)/CSPA
+     $(LIB)/TC $(LIB)/SG $(LIB)/CSPA $(LIB)/DL
 
 $(LIB)/obj/engine_%.o: $(PKG)/csrc/%.hip $(HEADERS)
 	@mkdir -p $(dir $@)
@@ -47,6 +47,13 @@ oracle/lib/libgdl_oracle.so: oracle/gdl_oracle.c oracle/cpu_tc_omp.c oracle/gdl_
 $(LIB)/obj/main_%.o: $(PKG)/programs/main.cpp $(HEADERS)
 	@mkdir -p $(dir $@)
 	$(HIPCC) $(HIPFLAGS) -DGDL_PROGRAM_$* -x hip -c $< -o $@
+
+$(LIB)/obj/dl_main.o: $(PKG)/programs/dl_main.cpp $(HEADERS)
+	@mkdir -p $(dir $@)
+	$(HIPCC) $(HIPFLAGS) -x hip -c $< -o $@
+
+$(LIB)/DL: $(LIB)/obj/dl_main.o $(LIB)/libgdlog.so
+	$(HIPCC) --offload-arch=$(ARCH) $< -L$(LIB) -lgdlog -lgdl_hip -Wl,-rpath,'$$ORIGIN' -o $@
 
 $(LIB)/%: $(LIB)/obj/main_%.o $(LIB)/libgdlog.so
 	$(HIPCC) --offload-arch=$(ARCH) $< -L$(LIB) -lgdlog -lgdl_hip -Wl,-rpath,'$$ORIGIN' -o $@

@@ -21,7 +21,10 @@ Prints one JSON line.  Also reports:
                 one continuous batch vs one program run at a time, with the
                 autotuned launch / layout parameters;
   served        a Poisson stream of TC / SG / CSPA queries through the query
-                server vs one-at-a-time service, p50 / p99 latency.
+                server vs one-at-a-time service, p50 / p99 latency;
+  datalog       tc.dl and cspa.dl through the Datalog front end vs the
+                hand-written RA encodings, whole runs from host inputs to
+                host results.
 """
 import argparse
 import importlib.util
@@ -332,6 +335,45 @@ def served(gdl, ctx, n_req, rate_mult=0.8, seed=7, workers=1):
     }
 
 
+def datalog_vs_hand(gdl, ctx, reps):
+    """Whole runs, host inputs to host results: the package's tc.dl and
+    cspa.dl lowered by the Datalog front end beside the hand-written RA
+    encodings of the same rules (the TC and CSPA programs); best of `reps`
+    after one warm-up run each."""
+    import numpy as np
+    dl_dir = os.path.join(REPO, "gdlog-multinode_amd", "programs", "dl")
+    tree = np.unique(gdl.gen_tree(0, 200_000, 1000))
+    a, d = gdl.gen_cspa(5, 500, 1.1, 750, 375)
+    a, d = np.unique(a), np.unique(d)
+    cases = {
+        "tc_tree_200k": ("tc.dl", dict(edge=tree), "path",
+                         lambda: ctx.run_tc(tree, want_trace=False)["out"]),
+        "cspa_zipf_500": ("cspa.dl", dict(assign=a, dereference=d), "ValueAlias",
+                          lambda: ctx.run_cspa(a, d, want_trace=False)["out"]),
+    }
+
+    def best(f):
+        r = f()
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            r = f()
+            ts.append(time.perf_counter() - t)
+        return r, 1000.0 * min(ts)
+
+    out = {}
+    for name, (fn, inputs, rel, hand) in cases.items():
+        with open(os.path.join(dl_dir, fn)) as f:
+            prog = gdl.Datalog(f.read())
+        got, dl_ms = best(lambda: prog.run(ctx, inputs)[rel])
+        want, hand_ms = best(hand)
+        out[name] = dict(datalog_ms=round(dl_ms, 2), hand_ms=round(hand_ms, 2),
+                         equal=bool(np.array_equal(got, want)), tuples=int(got.size))
+        prog.close()
+        print(f"bench: datalog {name}: {out[name]}", file=sys.stderr, flush=True)
+    return out
+
+
 def workloads(gdl, nodes, window):
     """name -> (kind, make_input, description)."""
     return {
@@ -364,6 +406,8 @@ def main():
                     help="instances of the batched-TC measurement (0: skip it)")
     ap.add_argument("--served-requests", type=int, default=1000,
                     help="requests of the served-mix measurement (0: skip it)")
+    ap.add_argument("--datalog-reps", type=int, default=3,
+                    help="timed runs of the Datalog front-end comparison (0: skip it)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device copy (traffic-counter calibration)")
     ap.add_argument("--phase", choices=["batched", "served"],
@@ -465,6 +509,7 @@ def main():
     for d in dev:
         d.free()
     gdl.set_full_mode(gdl.FULL_MODE_AUTO)
+    dlog = datalog_vs_hand(gdl, ctx, args.datalog_reps) if args.datalog_reps > 0 else None
     # the batched and served measurements each run in a process of their own
     # (as a batch or a server would): device memory that one long run of
     # allocations and frees gave back comes back to the next one much more
@@ -505,6 +550,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batch,
         "served": serve,
+        "datalog": dlog,
     }
     ctx.close()
     print(json.dumps(out), flush=True)

@@ -155,7 +155,8 @@ struct DatalogProgram::Impl {
     std::vector<char> prim_rev, other_built;
     std::vector<Step> steps;
     std::string plan_text;
-    std::vector<std::string> kept;  // relations run() keeps, declaration order
+    std::vector<std::string> kept;          // relations run() keeps, declaration order
+    std::vector<std::string> input_names;   // .input relations, declaration order
     bool compiled = false;
     // last run
     std::vector<int> iterations;
@@ -925,9 +926,11 @@ bool DatalogProgram::compile(const std::string &source, gdl_status *status, std:
         impl_.reset(new Impl);
         return false;
     }
-    for (size_t rel = 0; rel < fresh->rel_names.size(); ++rel)
+    for (size_t rel = 0; rel < fresh->rel_names.size(); ++rel) {
         if (fresh->any_kept ? fresh->is_kept[rel] : !fresh->is_input[rel])
             fresh->kept.push_back(fresh->rel_names[rel]);
+        if (fresh->is_input[rel]) fresh->input_names.push_back(fresh->rel_names[rel]);
+    }
     fresh->compiled = true;
     impl_ = std::move(fresh);
     if (status) *status = GDL_OK;
@@ -945,6 +948,7 @@ const std::vector<uint64_t> *DatalogProgram::result(const std::string &relation)
 }
 
 const std::vector<std::string> &DatalogProgram::kept() const { return impl_->kept; }
+const std::vector<std::string> &DatalogProgram::inputs() const { return impl_->input_names; }
 
 const std::string &DatalogProgram::plan() const { return impl_->plan_text; }
 const std::vector<int> &DatalogProgram::iterations() const { return impl_->iterations; }

@@ -1,0 +1,133 @@
+// dl_main.cpp -- DL, the command-line driver of the Datalog front end
+// (gdlog/datalog.h): runs a .dl program on fact files, in the manner of
+// `souffle -F <facts> -D <out> <program.dl>` (the reference runs its Souffle
+// programs that way, test/souffle/*.dl).
+//
+//   DL <program.dl> [-F DIR] [-D DIR] [--plan]
+//                   [--checkpoint DIR [--every N] [--resume]]
+//   -F DIR            each .input relation R is read from DIR/R.facts (rows of
+//                     two ids separated by a tab, space or comma; default .)
+//   -D DIR            each kept relation R is written to DIR/R.csv, sorted
+//                     tab-separated rows
+//   --plan            print the lowered plan before running
+//   --checkpoint ...  snapshot each LIE's fixpoint to DIR/lie<k>.ckpt, as the
+//                     TC / SG / CSPA drivers do
+// Prints each kept relation's size and checksum (sum of fmix64 over its keys),
+// the iteration count of each LIE and the run time.
+#include <chrono>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+
+#include "gdlog/datalog.h"
+#include "gdlog/lie.h"
+#include "tsv_io.h"
+
+namespace {
+
+// MurmurHash3's 64-bit finalizer; the checksum the engine's gdl_checksum_u64
+// computes on the device.
+uint64_t fmix64(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+}  // namespace
+
+int main(int argc, char *argv[]) {
+    std::string program, facts = ".", out_dir, ckpt_dir;
+    bool show_plan = false, resume = false;
+    int every = 0;
+    for (int i = 1; i < argc; ++i) {
+        const std::string a = argv[i];
+        auto value = [&]() -> std::string {
+            if (i + 1 >= argc) {
+                std::cerr << a << " needs a value" << std::endl;
+                std::exit(2);
+            }
+            return argv[++i];
+        };
+        if (a == "-F") facts = value();
+        else if (a == "-D") out_dir = value();
+        else if (a == "--plan") show_plan = true;
+        else if (a == "--checkpoint") ckpt_dir = value();
+        else if (a == "--every") every = std::atoi(value().c_str());
+        else if (a == "--resume") resume = true;
+        else if (!a.empty() && a[0] == '-') {
+            std::cerr << "unknown option " << a << std::endl;
+            return 2;
+        } else program = a;
+    }
+    if (program.empty()) {
+        std::cerr << "usage: " << argv[0]
+                  << " <program.dl> [-F DIR] [-D DIR] [--plan] [--checkpoint DIR [--every N]"
+                     " [--resume]]"
+                  << std::endl;
+        return 2;
+    }
+    std::ifstream in(program);
+    if (!in) {
+        std::cerr << "cannot open " << program << std::endl;
+        return 2;
+    }
+    std::stringstream text;
+    text << in.rdbuf();
+
+    gdlog::DatalogProgram prog;
+    gdl_status st = GDL_OK;
+    std::string err;
+    if (!prog.compile(text.str(), &st, &err)) {
+        std::cerr << program << ": " << err << std::endl;
+        return 1;
+    }
+    if (show_plan) std::cout << prog.plan();
+
+    // each .input relation comes from its .facts file
+    std::map<std::string, std::vector<uint64_t>> inputs;
+    for (const std::string &rel : prog.inputs()) {
+        const std::vector<column_type> rows = gdlog_cli::read_pairs(facts + "/" + rel + ".facts");
+        std::vector<uint64_t> &keys = inputs[rel];
+        keys.reserve(rows.size() / 2);
+        column_type mx = 0;
+        for (size_t r = 0; r + 1 < rows.size(); r += 2) {
+            mx |= rows[r] | rows[r + 1];
+            keys.push_back(((uint64_t)rows[r] << 32) | (uint64_t)(rows[r + 1] & 0xffffffffu));
+        }
+        if (mx >> 32) {
+            std::cerr << rel << ".facts: an id does not fit in 32 bits" << std::endl;
+            return 1;
+        }
+        std::cout << "Input " << rel << " rows: " << keys.size() << std::endl;
+    }
+    gdlog::set_checkpoint_defaults(ckpt_dir.empty() ? nullptr : ckpt_dir.c_str(), every, resume,
+                                   -1);
+    try {
+        const auto t0 = std::chrono::steady_clock::now();
+        prog.run(inputs);
+        const double ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                .count();
+        for (const std::string &rel : prog.kept()) {
+            const std::vector<uint64_t> &keys = *prog.result(rel);
+            uint64_t sum = 0;
+            for (uint64_t k : keys) sum += fmix64(k);
+            std::printf("%s %zu checksum 0x%016llx\n", rel.c_str(), keys.size(),
+                        (unsigned long long)sum);
+            if (!out_dir.empty()) gdlog_cli::write_tsv(out_dir + "/" + rel + ".csv", keys);
+        }
+        std::cout << "iterations";
+        for (int it : prog.iterations()) std::cout << " " << it;
+        std::cout << std::endl;
+        std::cout << "derived tuples " << prog.derived() << std::endl;
+        std::cout << "run time: " << ms / 1000.0 << std::endl;
+    } catch (const gdlog::Error &e) {
+        std::cerr << "error: " << e.what() << std::endl;
+        return 1;
+    }
+    return 0;
+}

@@ -23,42 +23,13 @@
 
 #include "gdlog/exception.h"
 #include "programs.h"
+#include "tsv_io.h"
 
 namespace {
 
-// get_relation_from_file (test/tc.cu:17-57): rows of `%ld<sep>%ld`.
-std::vector<column_type> read_pairs(const std::string &path) {
-    std::vector<column_type> rows;
-    FILE *f = std::fopen(path.c_str(), "r");
-    if (!f) {
-        std::cerr << "cannot open " << path << std::endl;
-        std::exit(2);
-    }
-    long a, b;
-    while (std::fscanf(f, "%ld%*[ \t,]%ld", &a, &b) == 2) {
-        rows.push_back((column_type)a);
-        rows.push_back((column_type)b);
-    }
-    std::fclose(f);
-    return rows;
-}
-
-void write_tsv(const std::string &path, const std::vector<uint64_t> &keys) {
-    FILE *f = std::fopen(path.c_str(), "w");
-    if (!f) throw gdlog::Error(GDL_ERR_INVALID_ARG, "cannot write " + path);
-    for (uint64_t k : keys)
-        std::fprintf(f, "%u\t%u\n", (unsigned)(k >> 32), (unsigned)(k & 0xffffffffu));
-    if (std::fclose(f) != 0) throw gdlog::Error(GDL_ERR_INVALID_ARG, "cannot write " + path);
-}
-
-void write_bin(const std::string &path, const std::vector<uint64_t> &keys) {
-    FILE *f = std::fopen(path.c_str(), "wb");
-    if (!f) throw gdlog::Error(GDL_ERR_INVALID_ARG, "cannot write " + path);
-    const size_t n = keys.size();
-    const bool ok = n == 0 || std::fwrite(keys.data(), sizeof(uint64_t), n, f) == n;
-    if (std::fclose(f) != 0 || !ok)
-        throw gdlog::Error(GDL_ERR_INVALID_ARG, "cannot write " + path);
-}
+using gdlog_cli::read_pairs;
+using gdlog_cli::write_bin;
+using gdlog_cli::write_tsv;
 
 EdgeInput host_input(const std::vector<column_type> &rows) {
     EdgeInput in;

@@ -66,6 +66,8 @@ class DatalogProgram {
     const std::vector<uint64_t> *result(const std::string &relation) const;
     // Kept relations, in declaration order.
     const std::vector<std::string> &kept() const;
+    // .input relations, in declaration order.
+    const std::vector<std::string> &inputs() const;
     // The lowered plan: one line per step, LIE and operator.
     const std::string &plan() const;
     // Iteration count of each LIE of the last run, in plan order.

@@ -2,7 +2,9 @@
 tab-separated input files in, sorted tab-separated and binary results out,
 compared with the oracle on the same files; the printed checksum is the
 order-independent one of the result; --checkpoint with --max-iteration, then
---resume, finishes the same fixpoint."""
+--resume, finishes the same fixpoint.  DL, the Datalog front end's driver,
+runs the package's tc.dl / sg.dl / cspa.dl on .facts files to the same
+results."""
 import os
 import subprocess
 
@@ -79,3 +81,65 @@ def test_tc_checkpoint_then_resume(tmp_path):
     out = run("TC", fn, "--checkpoint", ck, "--resume", "--out", tmp_path / "o.tsv")
     assert np.array_equal(read_tsv(tmp_path / "o.tsv"), want["out"])
     assert f"iterations {want['iterations']} " in out
+
+
+# ---- DL: the Datalog front end's driver ----------------------------------------
+
+DL_DIR = os.path.join(os.path.dirname(gdl.__file__), "programs", "dl")
+
+
+def relation_lines(stdout):
+    """{relation: (size, checksum)} of DL's report lines."""
+    out = {}
+    for line in stdout.splitlines():
+        parts = line.split()
+        if len(parts) == 4 and parts[2] == "checksum":
+            out[parts[0]] = (int(parts[1]), int(parts[3], 16))
+    return out
+
+
+@pytest.mark.parametrize("prog", ["tc", "sg", "cspa"])
+def test_dl_programs_match_the_oracle(tmp_path, prog):
+    facts, res = tmp_path / "facts", tmp_path / "out"
+    facts.mkdir()
+    res.mkdir()
+    if prog == "cspa":
+        a, d = gdl.gen_cspa(5, 150, 1.1, 200, 100)
+        a, d = np.unique(a), np.unique(d)
+        write_pairs(facts / "assign.facts", a)
+        write_pairs(facts / "dereference.facts", d)
+        o = oracle.cspa(a, d)
+        want = {"ValueFlow": o["extra"][0], "MemoryAlias": o["extra"][1], "ValueAlias": o["out"]}
+    else:
+        e = np.unique(gdl.gen_er(3, 500, 1500) if prog == "tc" else gdl.gen_rmat(6, 10, 3000))
+        write_pairs(facts / "edge.facts", e)
+        want = {"path" if prog == "tc" else "sg": (oracle.tc if prog == "tc" else oracle.sg)(e)["out"]}
+    out = run("DL", os.path.join(DL_DIR, prog + ".dl"), "-F", facts, "-D", res, "--plan")
+    assert out.startswith("lie ")
+    lines = relation_lines(out)
+    assert sorted(lines) == sorted(want)
+    for rel, keys in want.items():
+        assert np.array_equal(read_tsv(res / f"{rel}.csv"), keys), rel
+        assert lines[rel] == (keys.size, gdl.tuple_checksum(keys)), rel
+
+
+def test_dl_checkpoint_then_resume_restores_the_result(tmp_path):
+    facts, ck = tmp_path / "facts", tmp_path / "ck"
+    facts.mkdir()
+    ck.mkdir()
+    e = gdl.gen_tree(11, 3000, 60)
+    write_pairs(facts / "edge.facts", e)
+    want = oracle.tc(e)["out"]
+    first = run("DL", os.path.join(DL_DIR, "tc.dl"), "-F", facts, "--checkpoint", ck)
+    assert sorted(os.listdir(ck)) == ["lie0.ckpt", "lie1.ckpt"]
+    again = run("DL", os.path.join(DL_DIR, "tc.dl"), "-F", facts, "--checkpoint", ck, "--resume")
+    assert relation_lines(first) == relation_lines(again) == {
+        "path": (want.size, gdl.tuple_checksum(want))}
+
+
+def test_dl_reports_compile_errors(tmp_path):
+    bad = tmp_path / "bad.dl"
+    bad.write_text(".decl r(a: number, b: number)\nr(x, y) :- r(x, y), !r(y, x).\n")
+    r = subprocess.run([os.path.join(BIN, "DL"), str(bad)], stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=60)
+    assert r.returncode == 1 and "line 2: negation is not supported" in r.stderr
