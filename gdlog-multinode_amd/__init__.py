@@ -800,12 +800,12 @@ class Datalog:
         if st != GDL_OK:
             msg = (self.prog.gdl_datalog_last_error() or b"").decode(errors="replace")
             raise GdlError(st, msg)
-        out = np.zeros(max(n.value, 1), dtype=np.uint64)
+        out = np.empty(max(n.value, 1), dtype=np.uint64)
         st = self.prog.gdl_datalog_result(self.h, relation.encode(), _p64(out), n.value,
                                           ctypes.byref(n))
         if st != GDL_OK:
             raise GdlError(st, f"datalog result {relation}")
-        return out[:n.value].copy()
+        return out[:n.value]
 
     def stats(self):
         """(new tuples, [iterations of each LIE]) of the last run."""
