@@ -214,6 +214,7 @@ def load():
     _decl(prog, "gdl_datalog_stats", c_int,
           [vp, _U64P, POINTER(c_int32), c_uint32, POINTER(c_uint32)])
     _decl(prog, "gdl_datalog_kept", c_char_p, [vp, c_uint32])
+    _decl(prog, "gdl_datalog_arity", c_int, [vp, c_char_p])
     _decl(prog, "gdl_datalog_last_error", c_char_p, [])
     _decl(eng, "gdl_hashset_create", c_int, [vp, c_uint64, POINTER(HashSet)])
     _decl(eng, "gdl_hashset_free", c_int, [vp, POINTER(HashSet)])
@@ -744,10 +745,10 @@ class Context:
 # ------------------------------------------------------------ batched TC
 
 class Datalog:
-    """A Datalog program over binary relations, compiled by the engine's front
-    end (include/gdl_datalog.h) into LIEs of RA operators.  Compiling needs no
-    device; run() executes on a Context.  Relations hold packed keys
-    (c0 << 32) | c1.
+    """A Datalog program over unary and binary relations, compiled by the
+    engine's front end (include/gdl_datalog.h) into LIEs of RA operators.
+    Compiling needs no device; run() executes on a Context.  Relations hold
+    packed keys (c0 << 32) | c1 (c0 << 32 for a unary relation).
 
         prog = Datalog(".decl edge(a: number, b: number) .input edge ..."
                        "path(x, y) :- edge(x, y). ...")
@@ -793,6 +794,11 @@ class Datalog:
                 return out
             out.append(n.decode())
             i += 1
+
+    def arity(self, relation):
+        """Declared arity (1 or 2); 0 for an unknown relation.  A unary
+        relation's tuples are keys c0 << 32."""
+        return self.prog.gdl_datalog_arity(self.h, relation.encode())
 
     def result(self, relation):
         n = c_uint64()

@@ -100,11 +100,15 @@ std::vector<Tok> lex(const std::string &src) {
 
 struct Atom {
     int rel = -1;
-    std::string v[2];  // variables; an anonymous one gets a name no other has
+    // variables; an anonymous one gets a name no other has; a unary atom
+    // repeats its variable in v[1], so it joins and projects like a binary
+    // one that is never read in reversed order
+    std::string v[2];
 };
 
 struct Rule {
     int head = -1;
+    int head_arity = 2;
     std::string hv[2];
     std::vector<Atom> body;
     int cmp_op = GDL_PRED_NONE;  // one comparison between head variables
@@ -146,6 +150,8 @@ struct DatalogProgram::Impl {
     // program
     std::vector<std::string> rel_names;
     std::map<std::string, int> rel_id;
+    std::vector<int> arity;                   // 1 or 2
+    std::vector<std::vector<uint64_t>> facts;  // ground atoms of the program, packed
     std::vector<char> is_input, is_kept;
     bool any_kept = false;
     std::vector<Rule> rules;
@@ -243,11 +249,14 @@ struct Parser {
             next();
         }
         expect(")");
-        if (arity != 2)
-            fail(GDL_ERR_UNSUPPORTED, line, "relation " + name + " has arity " +
-                                               std::to_string(arity) + " (binary relations only)");
+        if (arity != 1 && arity != 2)
+            fail(GDL_ERR_UNSUPPORTED, line,
+                 "relation " + name + " has arity " + std::to_string(arity) +
+                     " (unary and binary relations only)");
         m.rel_id[name] = (int)m.rel_names.size();
         m.rel_names.push_back(name);
+        m.arity.push_back(arity);
+        m.facts.emplace_back();
         m.is_input.push_back(0);
         m.is_kept.push_back(0);
     }
@@ -287,17 +296,28 @@ struct Parser {
         return next().s;
     }
 
-    Atom atom() {
+    // An atom; with consts, a ground atom's numbers go there instead (a fact).
+    Atom atom(std::vector<uint64_t> *consts = nullptr) {
         Atom a;
         const int line = peek().line;
         const std::string name = ident("a relation name");
         a.rel = relation(name, line);
         expect("(");
-        int k = 0;
+        int k = 0, nconst = 0;
         while (!is(")")) {
-            std::string v = variable("in atoms");
-            if (v == "_") v = kAnon + std::to_string(++anon);
-            if (k < 2) a.v[k] = v;
+            const Tok &x = peek();
+            if (x.kind == Tok::Num && consts) {
+                if (x.s.size() > 10 || x.s.find_first_not_of("0123456789") != std::string::npos ||
+                    std::stoull(x.s) > 0xffffffffull)
+                    fail(GDL_ERR_INVALID_ARG, x.line, "bad id " + x.s + " (decimal, below 2^32)");
+                consts->push_back(std::stoull(x.s));
+                next();
+                ++nconst;
+            } else {
+                std::string v = variable("in rules");
+                if (v == "_") v = kAnon + std::to_string(++anon);
+                if (k < 2) a.v[k] = v;
+            }
             ++k;
             if (peek().kind == Tok::Punct && std::strchr("+-*/%^&|", peek().s[0]))
                 fail(GDL_ERR_UNSUPPORTED, peek().line, "arithmetic is not supported");
@@ -305,18 +325,31 @@ struct Parser {
             next();
         }
         expect(")");
-        if (k != 2)
-            fail(GDL_ERR_INVALID_ARG, line, name + " takes 2 arguments, not " + std::to_string(k));
+        const int want = m.arity[a.rel];
+        if (k != want)
+            fail(GDL_ERR_INVALID_ARG, line,
+                 name + " takes " + std::to_string(want) + " argument" + (want > 1 ? "s" : "") +
+                     ", not " + std::to_string(k));
+        if (nconst && nconst != k)
+            fail(GDL_ERR_UNSUPPORTED, line, "constants in rules are not supported");
+        if (want == 1) a.v[1] = a.v[0];
         return a;
     }
 
     void rule() {
         Rule r;
         r.line = peek().line;
-        const Atom h = atom();
+        std::vector<uint64_t> ground;
+        const Atom h = atom(&ground);
+        if (!ground.empty()) {  // a fact
+            if (!is("."))
+                fail(GDL_ERR_UNSUPPORTED, r.line, "constants in rules are not supported");
+            next();
+            m.facts[h.rel].push_back((ground[0] << 32) | (ground.size() > 1 ? ground[1] : 0));
+            return;
+        }
         if (is(",")) fail(GDL_ERR_UNSUPPORTED, r.line, "rules with several heads are not supported");
-        if (is("."))
-            fail(GDL_ERR_UNSUPPORTED, r.line, "facts are not supported (give them as input tuples)");
+        if (is(".")) fail(GDL_ERR_INVALID_ARG, r.line, "a fact with variables");
         expect(":-");
         for (;;) {
             const int line = peek().line;
@@ -348,6 +381,7 @@ struct Parser {
         }
         expect(".");
         r.head = h.rel;
+        r.head_arity = m.arity[h.rel];
         r.hv[0] = h.v[0];
         r.hv[1] = h.v[1];
         validate(r);
@@ -358,7 +392,7 @@ struct Parser {
         if (r.body.empty()) fail(GDL_ERR_UNSUPPORTED, r.line, "a rule needs a body atom");
         if (r.body.size() > 3) fail(GDL_ERR_UNSUPPORTED, r.line, "more than three body atoms");
         for (const Atom &a : r.body)
-            if (a.v[0] == a.v[1])
+            if (m.arity[a.rel] == 2 && a.v[0] == a.v[1])
                 fail(GDL_ERR_UNSUPPORTED, r.line, "variable " + a.v[0] + " twice in one atom");
         auto bound = [&](const std::string &v) {
             for (const Atom &a : r.body)
@@ -461,7 +495,7 @@ void plan_chain(const Rule &r, int s, RelationVersion ver, std::vector<AOp> *ops
         };
         if (step == k - 1) {
             op.head = r.head;
-            op.map = {2, {src_of(r.hv[0]), src_of(r.hv[1])}};
+            op.map = {r.head_arity, {src_of(r.hv[0]), src_of(r.hv[1])}};
             if (r.cmp_op != GDL_PRED_NONE)
                 op.pred = {r.cmp_op, r.cl == r.hv[0] ? 0 : 1, r.cr == r.hv[0] ? 0 : 1};
         } else {
@@ -497,7 +531,7 @@ AOp plan_copy(const Rule &r, RelationVersion ver) {
     op.head = r.head;
     for (int i = 0; i < 2; ++i)
         op.map.src[i] = r.body[0].v[0] == r.hv[i] ? GDL_COL_INNER0 : GDL_COL_INNER1;
-    op.map.arity = 2;
+    op.map.arity = r.head_arity;
     return op;
 }
 
@@ -571,9 +605,10 @@ int DatalogProgram::Impl::new_phys(const std::string &name, int rel, bool rev, P
 }
 
 int DatalogProgram::Impl::phys_base(int rel) {
-    if (base[rel] < 0)
-        base[rel] = new_phys(rel_names[rel] + (prim_rev[rel] ? "_2__2_1" : "_2__1_2"), rel,
-                             prim_rev[rel], kBase);
+    if (base[rel] < 0) {
+        const char *order = arity[rel] == 1 ? "_1__1" : prim_rev[rel] ? "_2__2_1" : "_2__1_2";
+        base[rel] = new_phys(rel_names[rel] + order, rel, prim_rev[rel], kBase);
+    }
     return base[rel];
 }
 
@@ -749,8 +784,9 @@ void DatalogProgram::Impl::describe() {
                 os << phys[op.src].name << " " << ver_name(op.src_ver);
             else
                 os << phys[op.src].name << " NEWT";
-            os << " -> " << phys[op.dst].name << " (" << col_name(op.map.src[0]) << ", "
-               << col_name(op.map.src[1]) << ")";
+            os << " -> " << phys[op.dst].name << " (" << col_name(op.map.src[0]);
+            if (op.map.arity == 2) os << ", " << col_name(op.map.src[1]);
+            os << ")";
             if (op.pred.op != GDL_PRED_NONE)
                 os << " if c" << op.pred.lhs << " " << pred_name(op.pred.op) << " c" << op.pred.rhs;
             os << "\n";
@@ -805,9 +841,17 @@ void DatalogProgram::Impl::run(const std::map<std::string, std::vector<uint64_t>
         if (!is_input[it->second])
             throw Error(GDL_ERR_INVALID_ARG, "datalog: " + kv.first + " is not an .input relation");
     }
-    for (size_t rel = 0; rel < rel_names.size(); ++rel)
-        if (is_input[rel] && !inputs.count(rel_names[rel]))
+    for (size_t rel = 0; rel < rel_names.size(); ++rel) {
+        if (!is_input[rel]) continue;
+        auto it = inputs.find(rel_names[rel]);
+        if (it == inputs.end())
             throw Error(GDL_ERR_INVALID_ARG, "datalog: no tuples given for input " + rel_names[rel]);
+        if (arity[rel] == 1)
+            for (uint64_t k : it->second)
+                if (k & 0xffffffffull)
+                    throw Error(GDL_ERR_INVALID_ARG, "datalog: unary relation " + rel_names[rel] +
+                                                         ": a key with a nonzero column 1");
+    }
     results.clear();
     iterations.clear();
     derived = 0;
@@ -820,21 +864,35 @@ void DatalogProgram::Impl::run(const std::map<std::string, std::vector<uint64_t>
         const Phys &ph = phys[p];
         if (ph.kind == kMirror || ph.kind == kStaticRev) continue;  // built by their steps
         Relation *r = R.v[p] = new Relation();
-        if (ph.kind == kTmp) r->index_flag = false;
-        if (ph.kind == kBase && is_input[ph.rel]) {
-            const std::vector<uint64_t> &keys = inputs.at(rel_names[ph.rel]);
-            std::vector<column_type> rows(keys.size() * 2);
-            for (size_t i = 0; i < keys.size(); ++i) {
-                const column_type a = (column_type)(keys[i] >> 32);
-                const column_type b = (column_type)(keys[i] & 0xffffffffull);
+        if (ph.kind == kTmp) {
+            r->index_flag = false;
+            load_relation(r, ph.name, 2, nullptr, 0, 1, 0, kGrid, kBlock);
+            continue;
+        }
+        // the relation starts with its input tuples and the program's facts
+        const int ar = arity[ph.rel];
+        const std::vector<uint64_t> *given =
+            is_input[ph.rel] ? &inputs.at(rel_names[ph.rel]) : nullptr;
+        const std::vector<uint64_t> &fk = facts[ph.rel];
+        const size_t n = (given ? given->size() : 0) + fk.size();
+        if (n == 0) {
+            load_relation(r, ph.name, ar, nullptr, 0, 1, 0, kGrid, kBlock);
+            continue;
+        }
+        std::vector<column_type> rows(n * ar);
+        for (size_t i = 0; i < n; ++i) {
+            const uint64_t k = given && i < given->size() ? (*given)[i] : fk[i - (given ? given->size() : 0)];
+            const column_type a = (column_type)(k >> 32);
+            const column_type b = (column_type)(k & 0xffffffffull);
+            if (ar == 1) {
+                rows[i] = a;
+            } else {
                 rows[2 * i] = ph.rev ? b : a;
                 rows[2 * i + 1] = ph.rev ? a : b;
             }
-            load_relation(r, ph.name, 2, rows.data(), keys.size(), 1, 0, kGrid, kBlock);
-            raw_from_full(r);
-        } else {
-            load_relation(r, ph.name, 2, nullptr, 0, 1, 0, kGrid, kBlock);
         }
+        load_relation(r, ph.name, ar, rows.data(), n, 1, 0, kGrid, kBlock);
+        raw_from_full(r);
     }
     for (const Step &s : steps) {
         if (!s.lie) {
@@ -950,6 +1008,11 @@ const std::vector<uint64_t> *DatalogProgram::result(const std::string &relation)
 const std::vector<std::string> &DatalogProgram::kept() const { return impl_->kept; }
 const std::vector<std::string> &DatalogProgram::inputs() const { return impl_->input_names; }
 
+int DatalogProgram::arity(const std::string &relation) const {
+    auto it = impl_->rel_id.find(relation);
+    return it == impl_->rel_id.end() ? 0 : impl_->arity[it->second];
+}
+
 const std::string &DatalogProgram::plan() const { return impl_->plan_text; }
 const std::vector<int> &DatalogProgram::iterations() const { return impl_->iterations; }
 uint64_t DatalogProgram::derived() const { return impl_->derived; }
@@ -1046,6 +1109,10 @@ const char *gdl_datalog_kept(const gdl_datalog *prog, uint32_t i) {
     if (!prog) return nullptr;
     const std::vector<std::string> &k = prog->prog.kept();
     return i < k.size() ? k[i].c_str() : nullptr;
+}
+
+int gdl_datalog_arity(const gdl_datalog *prog, const char *relation) {
+    return prog && relation ? prog->prog.arity(relation) : 0;
 }
 
 const char *gdl_datalog_last_error(void) { return g_dl_error.c_str(); }

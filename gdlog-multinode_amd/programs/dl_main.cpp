@@ -6,7 +6,8 @@
 //   DL <program.dl> [-F DIR] [-D DIR] [--plan]
 //                   [--checkpoint DIR [--every N] [--resume]]
 //   -F DIR            each .input relation R is read from DIR/R.facts (rows of
-//                     two ids separated by a tab, space or comma; default .)
+//                     two ids separated by a tab, space or comma, or one id
+//                     for a unary relation; default .)
 //   -D DIR            each kept relation R is written to DIR/R.csv, sorted
 //                     tab-separated rows
 //   --plan            print the lowered plan before running
@@ -90,13 +91,17 @@ int main(int argc, char *argv[]) {
     // each .input relation comes from its .facts file
     std::map<std::string, std::vector<uint64_t>> inputs;
     for (const std::string &rel : prog.inputs()) {
-        const std::vector<column_type> rows = gdlog_cli::read_pairs(facts + "/" + rel + ".facts");
+        const std::string fn = facts + "/" + rel + ".facts";
+        const int ar = prog.arity(rel);
+        const std::vector<column_type> rows =
+            ar == 1 ? gdlog_cli::read_ids(fn) : gdlog_cli::read_pairs(fn);
         std::vector<uint64_t> &keys = inputs[rel];
-        keys.reserve(rows.size() / 2);
+        keys.reserve(rows.size() / ar);
         column_type mx = 0;
-        for (size_t r = 0; r + 1 < rows.size(); r += 2) {
-            mx |= rows[r] | rows[r + 1];
-            keys.push_back(((uint64_t)rows[r] << 32) | (uint64_t)(rows[r + 1] & 0xffffffffu));
+        for (size_t r = 0; r + ar <= rows.size(); r += ar) {
+            const column_type b = ar == 2 ? rows[r + 1] : 0;
+            mx |= rows[r] | b;
+            keys.push_back(((uint64_t)rows[r] << 32) | (uint64_t)(b & 0xffffffffu));
         }
         if (mx >> 32) {
             std::cerr << rel << ".facts: an id does not fit in 32 bits" << std::endl;
@@ -118,7 +123,8 @@ int main(int argc, char *argv[]) {
             for (uint64_t k : keys) sum += fmix64(k);
             std::printf("%s %zu checksum 0x%016llx\n", rel.c_str(), keys.size(),
                         (unsigned long long)sum);
-            if (!out_dir.empty()) gdlog_cli::write_tsv(out_dir + "/" + rel + ".csv", keys);
+            if (!out_dir.empty())
+                gdlog_cli::write_tsv(out_dir + "/" + rel + ".csv", keys, prog.arity(rel));
         }
         std::cout << "iterations";
         for (int it : prog.iterations()) std::cout << " " << it;

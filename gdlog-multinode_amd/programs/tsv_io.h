@@ -30,12 +30,29 @@ inline std::vector<column_type> read_pairs(const std::string &path) {
     return rows;
 }
 
-// Sorted keys ((c0 << 32) | c1) as tab-separated rows.
-inline void write_tsv(const std::string &path, const std::vector<uint64_t> &keys) {
+// One id per row (a unary relation's facts).  A missing file ends the process.
+inline std::vector<column_type> read_ids(const std::string &path) {
+    std::vector<column_type> rows;
+    FILE *f = std::fopen(path.c_str(), "r");
+    if (!f) {
+        std::cerr << "cannot open " << path << std::endl;
+        std::exit(2);
+    }
+    long a;
+    while (std::fscanf(f, "%ld", &a) == 1) rows.push_back((column_type)a);
+    std::fclose(f);
+    return rows;
+}
+
+// Sorted keys ((c0 << 32) | c1) as tab-separated rows; arity 1 writes c0 only.
+inline void write_tsv(const std::string &path, const std::vector<uint64_t> &keys,
+                      int arity = 2) {
     FILE *f = std::fopen(path.c_str(), "w");
     if (!f) throw gdlog::Error(GDL_ERR_INVALID_ARG, "cannot write " + path);
-    for (uint64_t k : keys)
-        std::fprintf(f, "%u\t%u\n", (unsigned)(k >> 32), (unsigned)(k & 0xffffffffu));
+    for (uint64_t k : keys) {
+        if (arity == 1) std::fprintf(f, "%u\n", (unsigned)(k >> 32));
+        else std::fprintf(f, "%u\t%u\n", (unsigned)(k >> 32), (unsigned)(k & 0xffffffffu));
+    }
     if (std::fclose(f) != 0) throw gdlog::Error(GDL_ERR_INVALID_ARG, "cannot write " + path);
 }
 

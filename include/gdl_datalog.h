@@ -6,7 +6,8 @@
  * test/cspa.cu encode test/souffle/{tc,sg,cspa}.dl); this is the entry point
  * a binding would use to run such a .dl program without writing C++.
  *
- * Tuples are packed keys (c0 << 32) | c1 in host memory.
+ * Tuples are packed keys (c0 << 32) | c1 in host memory; a tuple of a unary
+ * relation is c0 << 32.
  */
 #ifndef GDL_DATALOG_H
 #define GDL_DATALOG_H
@@ -23,7 +24,7 @@ typedef struct gdl_datalog gdl_datalog;
 
 /* Parses and plans `source`.  GDL_ERR_INVALID_ARG: a syntax error or an
  * undeclared relation; GDL_ERR_UNSUPPORTED: a construct outside the subset
- * (facts, constants, negation, arity other than 2, ...).  The message, with
+ * (constants in rules, negation, arity above 2, ...).  The message, with
  * its line, is in gdl_datalog_last_error(). */
 gdl_status gdl_datalog_compile(const char *source, gdl_datalog **out);
 void gdl_datalog_destroy(gdl_datalog *prog);
@@ -44,6 +45,8 @@ gdl_status gdl_datalog_result(const gdl_datalog *prog, const char *relation, uin
 /* Name of the i-th kept relation, in declaration order; NULL past the last.
  * Valid until the program is destroyed. */
 const char *gdl_datalog_kept(const gdl_datalog *prog, uint32_t i);
+/* Declared arity of a relation: 1 or 2, 0 if the program has no such relation. */
+int gdl_datalog_arity(const gdl_datalog *prog, const char *relation);
 /* Statistics of the last run: new tuples over all LIEs and iterations, and
  * the iteration count of each LIE in plan order (*n_lies of them). */
 gdl_status gdl_datalog_stats(const gdl_datalog *prog, uint64_t *derived, int32_t *iterations,

@@ -5,12 +5,15 @@
 // context.  An extension: the reference builds its RA-operator lists by hand.
 //
 // Accepted (a subset of Souffle's syntax):
-//   .decl R(a: number, b: number)       binary relations of number/unsigned
+//   .decl R(a: number, b: number)       unary and binary relations of
+//   .decl S(a: number)                  number / unsigned ids below 2^32
 //   .input R, S(IO=file, ...)           relations given to run() (attributes
 //                                       in parentheses are ignored)
 //   .output R   .printsize R            relations whose tuples run() keeps
 //                                       (every derived relation when a
 //                                       program names none)
+//   R(1, 2).  S(0).                     facts: ground atoms, added to the
+//                                       relation's starting tuples
 //   H(x, y) :- A(x, z), B(z, y), x != y.
 //     1 to 3 body atoms over variables (_ for an unused column); every head
 //     variable bound by the body; no variable twice in one atom; each atom
@@ -18,9 +21,10 @@
 //     far, and at most two variables are live between two joins; at most one
 //     comparison (= != < <= > >=) between two head variables, in rules of two
 //     or more atoms.  Line and block comments.
-// Facts, constants, negation, aggregates, functors, other arities and other
+// Constants in rules, negation, aggregates, functors, other arities and other
 // directives are rejected (GDL_ERR_UNSUPPORTED), syntax errors and unknown
-// relations with GDL_ERR_INVALID_ARG; the message names the line.
+// relations with GDL_ERR_INVALID_ARG; the message names the line.  A tuple of
+// a unary relation is the key c0 << 32.
 //
 // Lowering.  Strata are the SCCs of the relation dependency graph, in
 // dependency order.  A stratum's rules whose bodies read no relation of the
@@ -57,8 +61,8 @@ class DatalogProgram {
     // and *err set, and the program stays empty.
     bool compile(const std::string &source, gdl_status *status, std::string *err);
     // Runs every stratum on the current context.  inputs: packed keys
-    // ((c0 << 32) | c1), any order, duplicates allowed, for each .input
-    // relation.  Throws gdlog::Error (GDL_ERR_INVALID_ARG: not compiled, a
+    // ((c0 << 32) | c1; c0 << 32 for a unary relation), any order, duplicates
+    // allowed, for each .input relation.  Throws gdlog::Error (GDL_ERR_INVALID_ARG: not compiled, a
     // missing or unknown input; engine errors as raised).
     void run(const std::map<std::string, std::vector<uint64_t>> &inputs);
     // Sorted, duplicate-free keys of a kept relation after run(); nullptr for
@@ -68,6 +72,8 @@ class DatalogProgram {
     const std::vector<std::string> &kept() const;
     // .input relations, in declaration order.
     const std::vector<std::string> &inputs() const;
+    // Declared arity of a relation (1 or 2; 0 if not declared).
+    int arity(const std::string &relation) const;
     // The lowered plan: one line per step, LIE and operator.
     const std::string &plan() const;
     // Iteration count of each LIE of the last run, in plan order.

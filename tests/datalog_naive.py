@@ -12,7 +12,7 @@ import re
 
 import numpy as np
 
-_ATOM = r"([A-Za-z_?][\w?]*)\s*\(\s*([\w?]+)\s*,\s*([\w?]+)\s*\)"
+_ATOM = r"([A-Za-z_?][\w?]*)\s*\(\s*([\w?]+)\s*(?:,\s*([\w?]+)\s*)?\)"
 _CMP = {"=": lambda a, b: a == b, "!=": lambda a, b: a != b, "<": lambda a, b: a < b,
         "<=": lambda a, b: a <= b, ">": lambda a, b: a > b, ">=": lambda a, b: a >= b}
 
@@ -27,25 +27,29 @@ def parse(src):
         for n in re.findall(r"([\w?]+)\s*(?:\([^)]*\))?", names):
             (inputs.add(n) if kind == "input" else kept.append(n))
     text = re.sub(r"\.(decl|input|output|printsize)[^\n]*", " ", src)
-    rules = []
+    rules, facts = [], []
     for stmt in re.split(r"\.\s", text + " "):
         if ":-" not in stmt:
+            f = re.fullmatch(r"\s*" + _ATOM + r"\s*", stmt)
+            if f:  # a ground atom
+                facts.append((f.group(1), tuple(int(c) for c in f.groups()[1:] if c)))
             continue
         head, body = stmt.split(":-", 1)
         h = re.fullmatch(r"\s*" + _ATOM + r"\s*", head)
-        atoms = [(a, (x, y)) for a, x, y in re.findall(_ATOM, body)]
+        atoms = [(a, tuple(v for v in (x, y) if v)) for a, x, y in re.findall(_ATOM, body)]
         rest = re.sub(_ATOM, " ", body)
         cmps = re.findall(r"([\w?]+)\s*(!=|<=|>=|=|<|>)\s*([\w?]+)", rest)
-        rules.append(((h.group(1), (h.group(2), h.group(3))), atoms, cmps))
-    return decls, inputs, kept, rules
+        hv = tuple(v for v in (h.group(2), h.group(3)) if v)
+        rules.append(((h.group(1), hv), atoms, cmps))
+    return decls, inputs, kept, rules, facts
 
 
 def _matches(env, atom, tuples):
-    _, (a, b) = atom
-    for x, y in tuples:
+    _, args = atom
+    for t in tuples:
         e = dict(env)
         ok = True
-        for v, val in ((a, x), (b, y)):
+        for v, val in zip(args, t):
             if v == "_":
                 continue
             if v in e and e[v] != val:
@@ -57,7 +61,7 @@ def _matches(env, atom, tuples):
 
 
 def _fire(rule, sources):
-    (_, (h0, h1)), atoms, cmps = rule
+    (_, hv), atoms, cmps = rule
     envs = [{}]
     for atom, tuples in zip(atoms, sources):
         envs = [e2 for e in envs for e2 in _matches(e, atom, tuples)]
@@ -66,18 +70,24 @@ def _fire(rule, sources):
     out = set()
     for e in envs:
         if all(_CMP[op](e[l], e[r]) for l, op, r in cmps):
-            out.add((e[h0], e[h1]))
+            out.add(tuple(e[v] for v in hv))
     return out
 
 
 def evaluate(src, inputs):
     """inputs: {relation: packed uint64 keys}.  Returns {kept relation: sorted
-    packed keys} (every non-input relation when none is kept)."""
-    decls, in_names, kept, rules = parse(src)
+    packed keys} (every non-input relation when none is kept).  A unary
+    relation's tuples are the keys c0 << 32."""
+    decls, in_names, kept, rules, facts = parse(src)
+    arity = {n: len(a) for n, a in re.findall(r"\.decl\s+([\w?]+)\s*\(([^)]*)\)", src)
+             for a in [a.split(",")]}
     full = {n: set() for n in decls}
     for n, keys in inputs.items():
         k = np.asarray(keys, dtype=np.uint64)
-        full[n] = set(zip((k >> np.uint64(32)).tolist(), (k & np.uint64(0xffffffff)).tolist()))
+        hi, lo = (k >> np.uint64(32)).tolist(), (k & np.uint64(0xffffffff)).tolist()
+        full[n] = set(zip(hi, lo)) if arity[n] == 2 else {(x,) for x in hi}
+    for n, t in facts:
+        full[n].add(t)
     delta = {n: set(s) for n, s in full.items()}
     first = True
     while True:
@@ -103,5 +113,6 @@ def evaluate(src, inputs):
     out = {}
     for n in names:
         t = sorted(full[n])
-        out[n] = np.array([(x << 32) | y for x, y in t], dtype=np.uint64)
+        out[n] = np.array([(x[0] << 32) | (x[1] if len(x) > 1 else 0) for x in t],
+                          dtype=np.uint64)
     return out

@@ -143,3 +143,16 @@ def test_dl_reports_compile_errors(tmp_path):
     r = subprocess.run([os.path.join(BIN, "DL"), str(bad)], stdout=subprocess.PIPE,
                        stderr=subprocess.PIPE, text=True, timeout=60)
     assert r.returncode == 1 and "line 2: negation is not supported" in r.stderr
+
+
+def test_dl_unary_facts_and_results(tmp_path):
+    prog = tmp_path / "reach.dl"
+    prog.write_text(".decl edge(a: number, b: number)\n.input edge\n.decl start(a: number)\n"
+                    ".input start\n.decl reach(a: number)\n.output reach\n"
+                    "reach(x) :- start(x).\nreach(y) :- reach(x), edge(x, y).\n")
+    (tmp_path / "edge.facts").write_text("1\t2\n2\t3\n3\t1\n7\t8\n")
+    (tmp_path / "start.facts").write_text("2\n")
+    out = run("DL", prog, "-F", tmp_path, "-D", tmp_path)
+    assert (tmp_path / "reach.csv").read_text() == "1\n2\n3\n"
+    keys = np.array([1 << 32, 2 << 32, 3 << 32], dtype=np.uint64)
+    assert relation_lines(out) == {"reach": (3, gdl.tuple_checksum(keys))}

@@ -102,6 +102,28 @@ wide(x, y) :- e(x, z), sym(z, w), e(w, y), x < y.
 """
 
 
+# single-source reachability: unary relations, facts, a unary delta joined
+# with a binary inner, a reversed static read by a unary atom, a 3-atom join
+# starting at a unary atom, and copies between unary and binary relations
+REACH = """
+.decl edge(a: number, b: number)
+.input edge
+.decl source(a: number)
+source(0).
+source(7).
+.decl reach(a: number)
+.decl hop2(a: number, b: number)
+.decl entered(a: number)
+.decl loop(a: number, b: number)
+.output reach, hop2, entered, loop
+reach(x) :- source(x).
+reach(y) :- reach(x), edge(x, y).
+hop2(s, y) :- source(s), edge(s, z), edge(z, y), s != y.
+entered(x) :- reach(x), edge(_, x).
+loop(x, x) :- reach(x).
+"""
+
+
 def uniq(keys):
     return np.unique(np.asarray(keys, dtype=np.uint64))
 
@@ -136,6 +158,30 @@ def test_naive_evaluator_matches_the_oracle_cspa():
     assert np.array_equal(got["ValueAlias"], want["out"])
     assert np.array_equal(got["ValueFlow"], want["extra"][0])
     assert np.array_equal(got["MemoryAlias"], want["extra"][1])
+
+
+def test_naive_evaluator_matches_bfs_on_unary_relations():
+    e = uniq(gdl.gen_er(4, 300, 420))
+    src, dst = (e >> np.uint64(32)).astype(np.int64), (e & np.uint64(0xffffffff)).astype(np.int64)
+    seen, frontier = {0, 7}, [0, 7]
+    while frontier:
+        frontier = [int(d) for s_ in frontier for d in dst[src == s_] if int(d) not in seen]
+        frontier = sorted(set(frontier))
+        seen.update(frontier)
+    got = evaluate(REACH, {"edge": e})
+    assert np.array_equal(got["reach"], np.array(sorted(x << 32 for x in seen), dtype=np.uint64))
+    assert np.array_equal(got["loop"], np.array(sorted((x << 32) | x for x in seen), dtype=np.uint64))
+
+
+def test_unary_plan():
+    p = gdl.Datalog(REACH)
+    assert p.arity("reach") == 1 and p.arity("edge") == 2 and p.arity("nope") == 0
+    plan = p.plan
+    assert "join edge_2__1_2 FULL x reach_1__1 DELTA -> reach_1__1 (i1)" in plan
+    assert "copy source_1__1 FULL -> reach_1__1 (i0)" in plan
+    assert "copy reach_1__1 FULL -> loop_2__1_2 (i0, i0)" in plan
+    assert "join edge_2__2_1 FULL x reach_1__1 FULL -> entered_1__1 (o0)" in plan
+    assert p.kept() == ["reach", "hop2", "entered", "loop"]
 
 
 def lie_blocks(plan):
@@ -205,12 +251,16 @@ def test_kept_defaults_to_every_derived_relation():
 
 BAD = [
     ("path(x, y) :- edge(x, y).", gdl.GDL_ERR_INVALID_ARG, "relation path is not declared"),
-    (".decl r(a: number)", 4, "arity 1"),
+    (".decl r(a: number, b: number, c: number)", 4, "arity 3"),
     (".decl r(a: symbol, b: number)", 4, "attribute type symbol"),
     (".decl r(a: number, b: number)\n.decl r(a: number, b: number)", gdl.GDL_ERR_INVALID_ARG,
      "declared twice"),
-    (".decl r(a: number, b: number)\nr(1, 2).", 4, "constants in atoms"),
-    (".decl r(a: number, b: number)\nr(x, y).", 4, "facts"),
+    (".decl r(a: number, b: number)\nr(x, 1) :- r(x, y).", 4, "constants in rules"),
+    (".decl r(a: number, b: number)\nr(x, y) :- r(x, 1).", 4, "constants in rules"),
+    (".decl r(a: number, b: number)\nr(1, x).", 4, "constants in rules"),
+    (".decl r(a: number, b: number)\nr(x, y).", gdl.GDL_ERR_INVALID_ARG, "a fact with variables"),
+    (".decl r(a: number, b: number)\nr(4294967296, 1).", gdl.GDL_ERR_INVALID_ARG, "bad id"),
+    (".decl r(a: number)\nr(1, 2).", gdl.GDL_ERR_INVALID_ARG, "r takes 1 argument, not 2"),
     (".decl r(a: number, b: number)\nr(x, y) :- r(x, y), !r(y, x).", 4, "negation"),
     (".decl r(a: number, b: number)\nr(x, y) :- r(x, z), r(w, y).", 4, "cross product"),
     (".decl r(a: number, b: number)\nr(x, y) :- r(x, y), r(y, x).", 4, "two variables"),
@@ -218,7 +268,8 @@ BAD = [
     (".decl r(a: number, b: number)\nr(x, y) :- r(x, z)", gdl.GDL_ERR_INVALID_ARG, "line 2"),
     (".decl r(a: number, b: number)\nr(x, y) :- r(x, z).", gdl.GDL_ERR_INVALID_ARG,
      "head variable y is not bound"),
-    (".decl r(a: number, b: number)\nr(x, y) :- r(x, y), x != 3.", 4, "constants in comparisons"),
+    (".decl r(a: number, b: number)\nr(x, y) :- r(x, y), r(y, z), x != 3.", 4,
+     "constants in comparisons"),
     (".decl r(a: number, b: number)\nr(x, y) :- r(x, y), x != y.", 4, "one-atom rule"),
     (".decl r(a: number, b: number)\nr(x, y) :- r(x, z), r(z, y), z != y.", 4,
      "not a head variable"),
@@ -294,6 +345,31 @@ def test_mixed_program_matches_the_naive_evaluator(ctx, seed):
     for rel in want:
         assert np.array_equal(got[rel], want[rel]), rel
     assert got["even"].size > 0  # the empty inner did not starve the other join
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", [4, 5])
+def test_reach_program_matches_the_naive_evaluator(ctx, seed):
+    e = uniq(gdl.gen_er(seed, 300, 420))
+    want = evaluate(REACH, {"edge": e})
+    prog = gdl.Datalog(REACH)
+    got = prog.run(ctx, edge=e)
+    for rel in want:
+        assert np.array_equal(got[rel], want[rel]), rel
+    assert got["reach"].size > 2
+
+
+@pytest.mark.gpu
+def test_unary_inputs_and_facts_merge(ctx):
+    src = (".decl s(a: number)\n.input s\n.decl e(a: number, b: number)\n.input e\n"
+           ".decl r(a: number)\n.output r\ns(5).\nr(x) :- s(x).\nr(y) :- r(x), e(x, y).\n")
+    prog = gdl.Datalog(src)
+    e = gdl.pack(np.array([1, 2, 5, 9]), np.array([2, 3, 9, 9]))
+    got = prog.run(ctx, s=np.array([1 << 32], dtype=np.uint64), e=e)["r"]
+    assert (got >> np.uint64(32)).tolist() == [1, 2, 3, 5, 9]
+    with pytest.raises(gdl.GdlError) as err:
+        prog.run(ctx, s=np.array([(1 << 32) | 1], dtype=np.uint64), e=e)
+    assert "nonzero column 1" in str(err.value)
 
 
 @pytest.mark.gpu
