@@ -206,6 +206,8 @@ def load():
     _decl(prog, "gdl_prog_set_full_mode", c_int, [c_int])
     _decl(prog, "gdl_prog_set_checkpoint", c_int, [ctypes.c_char_p, c_int, c_int, c_int])
     _decl(prog, "gdl_datalog_compile", c_int, [c_char_p, POINTER(vp)])
+    _decl(prog, "gdl_datalog_compile_slog", c_int, [c_char_p, POINTER(vp)])
+    _decl(prog, "gdl_datalog_source", c_char_p, [vp])
     _decl(prog, "gdl_datalog_destroy", None, [vp])
     _decl(prog, "gdl_datalog_plan", c_char_p, [vp])
     _decl(prog, "gdl_datalog_run", c_int,
@@ -755,10 +757,13 @@ class Datalog:
         out = prog.run(ctx, edge=keys)   # {"path": sorted uint64 array, ...}
     """
 
-    def __init__(self, source):
+    def __init__(self, source, slog=False):
+        """source: Datalog text, or with slog=True a Slog compiler's RA plan
+        (an s-expression like the reference's test/souffle/cspa.slogc)."""
         _, self.prog = load()
         h = c_void_p()
-        st = self.prog.gdl_datalog_compile(source.encode(), ctypes.byref(h))
+        fn = self.prog.gdl_datalog_compile_slog if slog else self.prog.gdl_datalog_compile
+        st = fn(source.encode(), ctypes.byref(h))
         if st != GDL_OK:
             msg = (self.prog.gdl_datalog_last_error() or b"").decode(errors="replace")
             raise GdlError(st, f"datalog compile: {msg}")
@@ -767,6 +772,11 @@ class Datalog:
     @property
     def plan(self):
         return self.prog.gdl_datalog_plan(self.h).decode()
+
+    @property
+    def source(self):
+        """The Datalog text compiled (for a Slog plan, the rules read back)."""
+        return self.prog.gdl_datalog_source(self.h).decode()
 
     def run(self, ctx, inputs=None, **kw):
         """Runs on ctx with {relation: packed keys} for every .input relation;

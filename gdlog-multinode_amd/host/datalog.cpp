@@ -160,6 +160,7 @@ struct DatalogProgram::Impl {
     std::vector<int> base, other;
     std::vector<char> prim_rev, other_built;
     std::vector<Step> steps;
+    std::string source;  // the Datalog text compiled
     std::string plan_text;
     std::vector<std::string> kept;          // relations run() keeps, declaration order
     std::vector<std::string> input_names;   // .input relations, declaration order
@@ -967,13 +968,247 @@ void DatalogProgram::Impl::run(const std::map<std::string, std::vector<uint64_t>
     }
 }
 
+// ---- Slog RA plans ----------------------------------------------------------
+//
+// A Slog compiler plan (the shape of the reference's test/souffle/cspa.slogc)
+// is one s-expression:
+//   (slog-prog (<relation-decl>...) (<scc-decl>...) (<scc edge>...))
+//   (relation-decl ID NAME JCOLS CANONICAL ARITY (INDEX-COLUMNS...))
+//   (scc-decl NAME N RECURSIVE ((scc-rel ID DYNAMIC TMP)...) (OP...))
+//   OP: (copy DST SRC VER (P...)) | (acopy DST SRC VER (P...))
+//     | (join DST A VER B VER (P...))
+// A relation version stores a tuple as its index columns (1-based, in order),
+// then a provenance id, then the remaining columns.  A copy's or acopy's P
+// index the source's stored tuple (0-based); a join's P index [the JCOLS join
+// columns, A's stored columns after them, B's stored columns after them].
+// The plan is read back into rules -- each join or copy is one rule over the
+// relations' logical columns; acopies only maintain index versions and are
+// dropped; the DELTA / FULL variants of a join give the same rule -- and
+// those rules are lowered like Datalog source.  Provenance ids must not reach
+// a rule head.
+
+namespace {
+
+struct SExp {
+    std::string atom;  // empty for a list
+    std::vector<SExp> list;
+    int line = 0;
+    bool is_list() const { return atom.empty(); }
+};
+
+SExp sexp_parse(const std::string &src) {
+    size_t i = 0;
+    int line = 1;
+    std::function<SExp()> parse = [&]() -> SExp {
+        while (i < src.size() && std::isspace((unsigned char)src[i])) line += src[i++] == '\n';
+        if (i >= src.size()) fail(GDL_ERR_INVALID_ARG, line, "unexpected end of the plan");
+        SExp e;
+        e.line = line;
+        if (src[i] == '(') {
+            ++i;
+            for (;;) {
+                while (i < src.size() && std::isspace((unsigned char)src[i]))
+                    line += src[i++] == '\n';
+                if (i >= src.size()) fail(GDL_ERR_INVALID_ARG, e.line, "unbalanced parentheses");
+                if (src[i] == ')') {
+                    ++i;
+                    return e;
+                }
+                e.list.push_back(parse());
+            }
+        }
+        if (src[i] == ')') fail(GDL_ERR_INVALID_ARG, line, "unbalanced parentheses");
+        const size_t j = src.find_first_of("() \t\r\n", i);
+        e.atom = src.substr(i, j == std::string::npos ? std::string::npos : j - i);
+        i = j == std::string::npos ? src.size() : j;
+        return e;
+    };
+    SExp top = parse();
+    while (i < src.size() && std::isspace((unsigned char)src[i])) ++i;
+    if (i < src.size()) fail(GDL_ERR_INVALID_ARG, line, "text after the plan");
+    return top;
+}
+
+const SExp &item(const SExp &e, size_t k, const char *what) {
+    if (!e.is_list() || k >= e.list.size())
+        fail(GDL_ERR_INVALID_ARG, e.line, std::string("malformed ") + what);
+    return e.list[k];
+}
+
+int number(const SExp &e, const char *what) {
+    if (e.is_list() || e.atom.find_first_not_of("0123456789") != std::string::npos)
+        fail(GDL_ERR_INVALID_ARG, e.line, std::string("expected a number: ") + what);
+    return std::stoi(e.atom);
+}
+
+// Slog names (value-flow, $rule7-inter-body) as identifiers.
+std::string slog_ident(const std::string &name) {
+    std::string s;
+    for (char c : name) s += std::isalnum((unsigned char)c) ? c : '_';
+    if (s.empty() || std::isdigit((unsigned char)s[0])) s = "_" + s;
+    return s;
+}
+
+std::string slog_to_datalog(const std::string &plan) {
+    const SExp top = sexp_parse(plan);
+    if (!top.is_list() || top.list.size() < 3 || item(top, 0, "plan").atom != "slog-prog")
+        fail(GDL_ERR_INVALID_ARG, top.line, "not a slog-prog plan");
+    struct Version {
+        std::string rel;
+        int jcols = 0, arity = 0;
+        std::vector<int> stored;  // logical column (0-based) of each stored slot; -1: the id
+    };
+    std::map<std::string, Version> versions;
+    std::vector<std::string> order;  // logical relations, first declared first
+    std::map<std::string, int> arity;
+    for (const SExp &d : item(top, 1, "relation list").list) {
+        if (item(d, 0, "relation-decl").atom != "relation-decl" || d.list.size() != 7)
+            fail(GDL_ERR_INVALID_ARG, d.line, "malformed relation-decl");
+        Version v;
+        v.rel = slog_ident(item(d, 2, "relation-decl").atom);
+        v.jcols = number(item(d, 3, "relation-decl"), "join columns");
+        v.arity = number(item(d, 5, "relation-decl"), "arity");
+        if (v.arity < 1 || v.arity > 2)
+            fail(GDL_ERR_UNSUPPORTED, d.line,
+                 "relation " + v.rel + " has arity " + std::to_string(v.arity) +
+                     " (unary and binary relations only)");
+        std::vector<char> seen(v.arity, 0);
+        for (const SExp &c : item(d, 6, "index columns").list) {
+            const int col = number(c, "index column") - 1;
+            if (col < 0 || col >= v.arity || seen[col])
+                fail(GDL_ERR_INVALID_ARG, c.line, "bad index column");
+            seen[col] = 1;
+            v.stored.push_back(col);
+        }
+        if (v.jcols != (int)v.stored.size())
+            fail(GDL_ERR_INVALID_ARG, d.line, "join columns do not match the index");
+        v.stored.push_back(-1);
+        for (int c = 0; c < v.arity; ++c)
+            if (!seen[c]) v.stored.push_back(c);
+        auto ar = arity.find(v.rel);
+        if (ar == arity.end()) {
+            arity[v.rel] = v.arity;
+            order.push_back(v.rel);
+        } else if (ar->second != v.arity) {
+            fail(GDL_ERR_INVALID_ARG, d.line, "relation " + v.rel + " declared with two arities");
+        }
+        versions[item(d, 1, "relation-decl").atom] = v;
+    }
+    auto version = [&](const SExp &e) -> const Version & {
+        auto it = versions.find(e.atom);
+        if (it == versions.end())
+            fail(GDL_ERR_INVALID_ARG, e.line, "relation version " + e.atom + " is not declared");
+        return it->second;
+    };
+    std::set<std::string> tmp_rels, heads;
+    std::vector<std::string> rules;
+    std::set<std::string> rule_set;
+    auto atom_text = [](const Version &v, const std::vector<std::string> &slot_vars) {
+        std::vector<std::string> args(v.arity, "_");
+        for (size_t k = 0; k < v.stored.size(); ++k)
+            if (v.stored[k] >= 0) args[v.stored[k]] = slot_vars[k];
+        std::string t = v.rel + "(";
+        for (int c = 0; c < v.arity; ++c) t += (c ? ", " : "") + args[c];
+        return t + ")";
+    };
+    for (const SExp &scc : item(top, 2, "scc list").list) {
+        if (item(scc, 0, "scc-decl").atom != "scc-decl" || scc.list.size() != 6)
+            fail(GDL_ERR_INVALID_ARG, scc.line, "malformed scc-decl");
+        for (const SExp &r : item(scc, 4, "scc relations").list)
+            if (item(r, 3, "scc-rel").atom == "#t") tmp_rels.insert(version(item(r, 1, "scc-rel")).rel);
+        for (const SExp &op : item(scc, 5, "operators").list) {
+            const std::string kind = item(op, 0, "operator").atom;
+            if (kind == "acopy") {
+                if (version(item(op, 1, "acopy")).rel != version(item(op, 2, "acopy")).rel)
+                    fail(GDL_ERR_UNSUPPORTED, op.line, "an acopy between two relations");
+                continue;
+            }
+            const Version &dst = version(item(op, 1, "operator"));
+            if ((int)dst.stored.size() != dst.arity + 1 || dst.stored[dst.arity] != -1)
+                fail(GDL_ERR_UNSUPPORTED, op.line, "a " + kind + " into an index version");
+            std::vector<std::string> combined;  // variable of each addressable slot
+            std::string body;
+            int nv = 0;
+            auto fresh = [&] { return "v" + std::to_string(nv++); };
+            if (kind == "copy") {
+                if (op.list.size() != 5) fail(GDL_ERR_INVALID_ARG, op.line, "malformed copy");
+                const Version &src = version(item(op, 2, "copy"));
+                for (int slot : src.stored) combined.push_back(slot < 0 ? "#id" : fresh());
+                body = atom_text(src, combined);
+            } else if (kind == "join") {
+                if (op.list.size() != 7) fail(GDL_ERR_INVALID_ARG, op.line, "malformed join");
+                const Version &a = version(item(op, 2, "join")), &b = version(item(op, 4, "join"));
+                if (a.jcols != b.jcols || a.jcols < 1 || a.jcols >= a.arity + 1)
+                    fail(GDL_ERR_UNSUPPORTED, op.line, "join versions with different join columns");
+                std::vector<std::string> av(a.stored.size()), bv(b.stored.size());
+                for (int k = 0; k < a.jcols; ++k) {
+                    av[k] = bv[k] = fresh();
+                    combined.push_back(av[k]);
+                }
+                for (size_t k = a.jcols; k < a.stored.size(); ++k)
+                    combined.push_back(av[k] = a.stored[k] < 0 ? "#id" : fresh());
+                for (size_t k = b.jcols; k < b.stored.size(); ++k)
+                    combined.push_back(bv[k] = b.stored[k] < 0 ? "#id" : fresh());
+                body = atom_text(a, av) + ", " + atom_text(b, bv);
+            } else {
+                fail(GDL_ERR_UNSUPPORTED, op.line, "operator " + kind + " is not supported");
+            }
+            const SExp &proj = item(op, op.list.size() - 1, "projection");
+            if ((int)proj.list.size() != dst.arity)
+                fail(GDL_ERR_INVALID_ARG, proj.line, "the projection does not match " + dst.rel);
+            std::string head = dst.rel + "(";
+            for (int c = 0; c < dst.arity; ++c) {
+                const int p = number(item(proj, c, "projection"), "projection index");
+                if (p < 0 || p >= (int)combined.size())
+                    fail(GDL_ERR_INVALID_ARG, proj.line, "projection index out of range");
+                if (combined[p] == "#id")
+                    fail(GDL_ERR_UNSUPPORTED, proj.line, "a provenance id in a rule head");
+                head += (c ? ", " : "") + combined[p];
+            }
+            head += ")";
+            const std::string text = head + " :- " + body + ".";
+            heads.insert(dst.rel);
+            if (rule_set.insert(text).second) rules.push_back(text);
+        }
+    }
+    std::ostringstream os;
+    for (const std::string &rel : order) {
+        os << ".decl " << rel << "(c1: number" << (arity[rel] == 2 ? ", c2: number" : "") << ")\n";
+        if (!heads.count(rel)) os << ".input " << rel << "\n";
+        else if (!tmp_rels.count(rel)) os << ".output " << rel << "\n";
+    }
+    for (const std::string &r : rules) os << r << "\n";
+    return os.str();
+}
+
+}  // namespace
+
 // ---- public -----------------------------------------------------------------
 
 DatalogProgram::DatalogProgram() : impl_(new Impl) {}
 DatalogProgram::~DatalogProgram() = default;
 
+bool DatalogProgram::compile_slog(const std::string &plan, gdl_status *status,
+                                  std::string *err) {
+    std::string source;
+    try {
+        source = slog_to_datalog(plan);
+    } catch (const CompileError &e) {
+        if (status) *status = e.status;
+        if (err) *err = "slog plan: " + e.msg;
+        impl_.reset(new Impl);
+        return false;
+    }
+    if (!compile(source, status, err)) {
+        if (err) *err = "slog plan, rules read back: " + *err;
+        return false;
+    }
+    return true;
+}
+
 bool DatalogProgram::compile(const std::string &source, gdl_status *status, std::string *err) {
     std::unique_ptr<Impl> fresh(new Impl);
+    fresh->source = source;
     try {
         fresh->parse(lex(source));
         fresh->lower();
@@ -1014,6 +1249,7 @@ int DatalogProgram::arity(const std::string &relation) const {
 }
 
 const std::string &DatalogProgram::plan() const { return impl_->plan_text; }
+const std::string &DatalogProgram::source() const { return impl_->source; }
 const std::vector<int> &DatalogProgram::iterations() const { return impl_->iterations; }
 uint64_t DatalogProgram::derived() const { return impl_->derived; }
 
@@ -1045,7 +1281,25 @@ gdl_status gdl_datalog_compile(const char *source, gdl_datalog **out) {
     return GDL_OK;
 }
 
+gdl_status gdl_datalog_compile_slog(const char *plan, gdl_datalog **out) {
+    g_dl_error.clear();
+    if (!plan || !out) return GDL_ERR_INVALID_ARG;
+    *out = nullptr;
+    gdl_datalog *p = new gdl_datalog;
+    gdl_status st = GDL_OK;
+    if (!p->prog.compile_slog(plan, &st, &g_dl_error)) {
+        delete p;
+        return st;
+    }
+    *out = p;
+    return GDL_OK;
+}
+
 void gdl_datalog_destroy(gdl_datalog *prog) { delete prog; }
+
+const char *gdl_datalog_source(const gdl_datalog *prog) {
+    return prog ? prog->prog.source().c_str() : "";
+}
 
 const char *gdl_datalog_plan(const gdl_datalog *prog) {
     return prog ? prog->prog.plan().c_str() : "";

@@ -3,14 +3,17 @@
 // `souffle -F <facts> -D <out> <program.dl>` (the reference runs its Souffle
 // programs that way, test/souffle/*.dl).
 //
-//   DL <program.dl> [-F DIR] [-D DIR] [--plan]
+//   DL <program.dl | plan.slogc> [-F DIR] [-D DIR] [--plan]
 //                   [--checkpoint DIR [--every N] [--resume]]
+//   A .slogc file is a Slog compiler's RA plan (DatalogProgram::compile_slog);
+//   its relation names become identifiers (value-flow -> value_flow).
 //   -F DIR            each .input relation R is read from DIR/R.facts (rows of
 //                     two ids separated by a tab, space or comma, or one id
 //                     for a unary relation; default .)
 //   -D DIR            each kept relation R is written to DIR/R.csv, sorted
 //                     tab-separated rows
-//   --plan            print the lowered plan before running
+//   --plan            print the lowered plan before running (for a .slogc
+//                     plan, the rules read back from it first)
 //   --checkpoint ...  snapshot each LIE's fixpoint to DIR/lie<k>.ckpt, as the
 //                     TC / SG / CSPA drivers do
 // Prints each kept relation's size and checksum (sum of fmix64 over its keys),
@@ -82,11 +85,12 @@ int main(int argc, char *argv[]) {
     gdlog::DatalogProgram prog;
     gdl_status st = GDL_OK;
     std::string err;
-    if (!prog.compile(text.str(), &st, &err)) {
+    const bool slog = program.size() > 6 && program.compare(program.size() - 6, 6, ".slogc") == 0;
+    if (!(slog ? prog.compile_slog(text.str(), &st, &err) : prog.compile(text.str(), &st, &err))) {
         std::cerr << program << ": " << err << std::endl;
         return 1;
     }
-    if (show_plan) std::cout << prog.plan();
+    if (show_plan) std::cout << (slog ? prog.source() : "") << prog.plan();
 
     // each .input relation comes from its .facts file
     std::map<std::string, std::vector<uint64_t>> inputs;

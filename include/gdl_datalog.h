@@ -27,7 +27,16 @@ typedef struct gdl_datalog gdl_datalog;
  * (constants in rules, negation, arity above 2, ...).  The message, with
  * its line, is in gdl_datalog_last_error(). */
 gdl_status gdl_datalog_compile(const char *source, gdl_datalog **out);
+/* Same for a Slog compiler's RA plan (the s-expression format of the
+ * reference's test/souffle/cspa.slogc): its joins and copies are read back
+ * into rules (index-maintenance acopies dropped, DELTA / FULL variants
+ * merged), underived relations become inputs, derived ones not marked tmp are
+ * kept, and Slog names become identifiers (value-flow -> value_flow). */
+gdl_status gdl_datalog_compile_slog(const char *plan, gdl_datalog **out);
 void gdl_datalog_destroy(gdl_datalog *prog);
+/* The Datalog text the program was compiled from (for a Slog plan, the rules
+ * read back); valid until the program is destroyed. */
+const char *gdl_datalog_source(const gdl_datalog *prog);
 /* The lowered plan (LIEs, their relations and RA operators), one per line;
  * valid until the program is destroyed. */
 const char *gdl_datalog_plan(const gdl_datalog *prog);

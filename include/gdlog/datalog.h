@@ -60,6 +60,14 @@ class DatalogProgram {
     // Parses and plans the program; on failure returns false with *status
     // and *err set, and the program stays empty.
     bool compile(const std::string &source, gdl_status *status, std::string *err);
+    // Same for a Slog compiler's RA plan (the s-expression of the reference's
+    // test/souffle/cspa.slogc): the plan's joins and copies are read back
+    // into rules over the relations' logical columns (index-maintenance
+    // acopies dropped, DELTA / FULL variants merged), relations no rule
+    // derives become .input, derived ones not marked tmp become .output, and
+    // the rules are compiled as above.  Slog names become identifiers
+    // (value-flow -> value_flow).  source() returns the rules read back.
+    bool compile_slog(const std::string &plan, gdl_status *status, std::string *err);
     // Runs every stratum on the current context.  inputs: packed keys
     // ((c0 << 32) | c1; c0 << 32 for a unary relation), any order, duplicates
     // allowed, for each .input relation.  Throws gdlog::Error (GDL_ERR_INVALID_ARG: not compiled, a
@@ -76,6 +84,8 @@ class DatalogProgram {
     int arity(const std::string &relation) const;
     // The lowered plan: one line per step, LIE and operator.
     const std::string &plan() const;
+    // The Datalog text compiled (for compile_slog, the rules read back).
+    const std::string &source() const;
     // Iteration count of each LIE of the last run, in plan order.
     const std::vector<int> &iterations() const;
     // New tuples of the last run: sum of |delta| over LIEs, iterations and

@@ -156,3 +156,16 @@ def test_dl_unary_facts_and_results(tmp_path):
     assert (tmp_path / "reach.csv").read_text() == "1\n2\n3\n"
     keys = np.array([1 << 32, 2 << 32, 3 << 32], dtype=np.uint64)
     assert relation_lines(out) == {"reach": (3, gdl.tuple_checksum(keys))}
+
+
+def test_dl_runs_a_slog_plan(tmp_path):
+    from test_datalog import TC_SLOGC
+    plan = tmp_path / "tc.slogc"
+    plan.write_text(TC_SLOGC)
+    e = np.unique(gdl.gen_er(3, 500, 1500))
+    write_pairs(tmp_path / "edge.facts", e)
+    out = run("DL", plan, "-F", tmp_path, "-D", tmp_path, "--plan")
+    assert "path(v1, v2) :- path(v1, v0), edge(v0, v2)." in out
+    want = oracle.tc(e)["out"]
+    assert np.array_equal(read_tsv(tmp_path / "path.csv"), want)
+    assert relation_lines(out) == {"path": (want.size, gdl.tuple_checksum(want))}

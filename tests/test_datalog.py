@@ -13,6 +13,8 @@ oracle.  Inputs are duplicate-free: with duplicate rows the hand encodings
 seed their first delta from raw rows (the oracle restates that), while a
 Datalog program's result depends only on the set of input tuples.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -391,3 +393,88 @@ def test_runs_replace_results_and_reject_bad_inputs(ctx):
     assert "not an .input relation" in str(e.value)
     with pytest.raises(gdl.GdlError):
         prog.result("edge")
+
+
+# ---- Slog RA plans ---------------------------------------------------------------
+
+REF_CSPA_SLOGC = "/root/reference/test/souffle/cspa.slogc"
+
+# TC as a Slog compiler's plan, written for these tests in that format:
+# path(x, y) :- edge(x, y).   path(x, y) :- path(x, z), edge(z, y).
+# A version stores (index columns, id, other columns); the join projects
+# [z, path.id, x, edge.id, y] onto (2 4).
+TC_SLOGC = """
+(slog-prog
+ ((relation-decl rel__edge__2__1__2 edge 2 #t 2 (1 2))
+  (relation-decl rel__edge__2__1 edge 1 #f 2 (1))
+  (relation-decl rel__path__2__1__2 path 2 #t 2 (1 2))
+  (relation-decl rel__path__2__2 path 1 #f 2 (2)))
+ ((scc-decl scc0 0 #f
+   ((scc-rel rel__edge__2__1__2 #f #f) (scc-rel rel__path__2__1__2 #t #f))
+   ((copy rel__path__2__1__2 rel__edge__2__1__2 FULL (0 1))))
+  (scc-decl scc1 1 #f
+   ((scc-rel rel__edge__2__1 #t #f) (scc-rel rel__edge__2__1__2 #t #f))
+   ((acopy rel__edge__2__1 rel__edge__2__1__2 DELTA (0 2 1))))
+  (scc-decl scc2 2 #t
+   ((scc-rel rel__path__2__1__2 #t #f) (scc-rel rel__path__2__2 #t #f)
+    (scc-rel rel__edge__2__1 #f #f))
+   ((join rel__path__2__1__2 rel__path__2__2 DELTA rel__edge__2__1 FULL (2 4))
+    (acopy rel__path__2__2 rel__path__2__1__2 DELTA (1 2 0)))))
+ ((0 2) (1 2)))
+"""
+
+
+def test_slog_tc_plan_reads_back_to_tc():
+    p = gdl.Datalog(TC_SLOGC, slog=True)
+    assert "path(v1, v2) :- path(v1, v0), edge(v0, v2)." in p.source
+    assert ".input edge" in p.source and ".output path" in p.source
+    assert lie_blocks(p.plan)[1][1] == [
+        "join edge_2__1_2 FULL x path_2__2_1 DELTA -> path_2__2_1 (i1, o1)"]
+    e = uniq(gdl.gen_er(7, 80, 160))
+    assert np.array_equal(evaluate(p.source, {"edge": e})["path"], oracle.tc(e)["out"])
+
+
+@pytest.mark.skipif(not os.path.exists(REF_CSPA_SLOGC),
+                    reason="the reference's Slog plan is only present where it is checked out")
+def test_reference_cspa_slog_plan_reads_back_to_cspa():
+    with open(REF_CSPA_SLOGC) as f:
+        p = gdl.Datalog(f.read(), slog=True)
+    assert p.kept() == ["value_alias", "value_flow", "memory_alias"]
+    a, d = gdl.gen_cspa(9, 40, 1.1, 50, 25)
+    a, d = uniq(a), uniq(d)
+    got = evaluate(p.source, {"assign": a, "dereference": d})
+    want = oracle.cspa(a, d)
+    assert np.array_equal(got["value_alias"], want["out"])
+    assert np.array_equal(got["value_flow"], want["extra"][0])
+    assert np.array_equal(got["memory_alias"], want["extra"][1])
+
+
+SLOG_BAD = [
+    ("(not-a-plan)", gdl.GDL_ERR_INVALID_ARG, "not a slog-prog plan"),
+    ("(slog-prog (", gdl.GDL_ERR_INVALID_ARG, "unbalanced"),
+    (TC_SLOGC.replace("(2 4)", "(1 4)"), 4, "provenance id in a rule head"),
+    (TC_SLOGC.replace("(2 4)", "(2 9)"), gdl.GDL_ERR_INVALID_ARG, "out of range"),
+    (TC_SLOGC.replace("rel__edge__2__1 FULL", "rel__edge__2__1__2 FULL"), 4,
+     "different join columns"),
+    (TC_SLOGC.replace("(copy rel__path__2__1__2", "(copy rel__path__2__2"), 4,
+     "into an index version"),
+    (TC_SLOGC.replace("edge 2 #t 2 (1 2)", "edge 3 #t 3 (1 2 3)"), 4, "arity 3"),
+    (TC_SLOGC.replace("(acopy rel__edge__2__1 rel__edge__2__1__2",
+                      "(acopy rel__edge__2__1 rel__path__2__1__2"), 4, "acopy between two"),
+]
+
+
+@pytest.mark.parametrize("plan,status,fragment", SLOG_BAD)
+def test_bad_slog_plans_are_rejected(plan, status, fragment):
+    with pytest.raises(gdl.GdlError) as e:
+        gdl.Datalog(plan, slog=True)
+    assert e.value.status == status
+    assert fragment in str(e.value)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["er_500", "data_7035"])
+def test_slog_tc_plan_matches_the_oracle(ctx, name):
+    e = uniq(TC_INPUTS[name]())
+    got = gdl.Datalog(TC_SLOGC, slog=True).run(ctx, edge=e)
+    assert np.array_equal(got["path"], oracle.tc(e)["out"])
