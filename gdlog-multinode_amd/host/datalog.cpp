@@ -997,14 +997,16 @@ struct SExp {
 };
 
 SExp sexp_parse(const std::string &src) {
+    constexpr int kMaxDepth = 64;  // plans nest about 6 deep
     size_t i = 0;
-    int line = 1;
+    int line = 1, depth = 0;
     std::function<SExp()> parse = [&]() -> SExp {
         while (i < src.size() && std::isspace((unsigned char)src[i])) line += src[i++] == '\n';
         if (i >= src.size()) fail(GDL_ERR_INVALID_ARG, line, "unexpected end of the plan");
         SExp e;
         e.line = line;
         if (src[i] == '(') {
+            if (++depth > kMaxDepth) fail(GDL_ERR_INVALID_ARG, line, "the plan nests too deeply");
             ++i;
             for (;;) {
                 while (i < src.size() && std::isspace((unsigned char)src[i]))
@@ -1012,6 +1014,7 @@ SExp sexp_parse(const std::string &src) {
                 if (i >= src.size()) fail(GDL_ERR_INVALID_ARG, e.line, "unbalanced parentheses");
                 if (src[i] == ')') {
                     ++i;
+                    --depth;
                     return e;
                 }
                 e.list.push_back(parse());
@@ -1036,8 +1039,9 @@ const SExp &item(const SExp &e, size_t k, const char *what) {
 }
 
 int number(const SExp &e, const char *what) {
-    if (e.is_list() || e.atom.find_first_not_of("0123456789") != std::string::npos)
-        fail(GDL_ERR_INVALID_ARG, e.line, std::string("expected a number: ") + what);
+    if (e.is_list() || e.atom.empty() || e.atom.size() > 6 ||
+        e.atom.find_first_not_of("0123456789") != std::string::npos)
+        fail(GDL_ERR_INVALID_ARG, e.line, std::string("expected a small number: ") + what);
     return std::stoi(e.atom);
 }
 
@@ -1271,13 +1275,15 @@ gdl_status gdl_datalog_compile(const char *source, gdl_datalog **out) {
     g_dl_error.clear();
     if (!source || !out) return GDL_ERR_INVALID_ARG;
     *out = nullptr;
-    gdl_datalog *p = new gdl_datalog;
+    std::unique_ptr<gdl_datalog> p(new gdl_datalog);
     gdl_status st = GDL_OK;
-    if (!p->prog.compile(source, &st, &g_dl_error)) {
-        delete p;
-        return st;
+    try {
+        if (!p->prog.compile(source, &st, &g_dl_error)) return st;
+    } catch (const std::exception &e) {
+        g_dl_error = e.what();
+        return GDL_ERR_INVALID_ARG;
     }
-    *out = p;
+    *out = p.release();
     return GDL_OK;
 }
 
@@ -1285,13 +1291,15 @@ gdl_status gdl_datalog_compile_slog(const char *plan, gdl_datalog **out) {
     g_dl_error.clear();
     if (!plan || !out) return GDL_ERR_INVALID_ARG;
     *out = nullptr;
-    gdl_datalog *p = new gdl_datalog;
+    std::unique_ptr<gdl_datalog> p(new gdl_datalog);
     gdl_status st = GDL_OK;
-    if (!p->prog.compile_slog(plan, &st, &g_dl_error)) {
-        delete p;
-        return st;
+    try {
+        if (!p->prog.compile_slog(plan, &st, &g_dl_error)) return st;
+    } catch (const std::exception &e) {
+        g_dl_error = e.what();
+        return GDL_ERR_INVALID_ARG;
     }
-    *out = p;
+    *out = p.release();
     return GDL_OK;
 }
 

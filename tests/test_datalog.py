@@ -461,6 +461,9 @@ SLOG_BAD = [
     (TC_SLOGC.replace("edge 2 #t 2 (1 2)", "edge 3 #t 3 (1 2 3)"), 4, "arity 3"),
     (TC_SLOGC.replace("(acopy rel__edge__2__1 rel__edge__2__1__2",
                       "(acopy rel__edge__2__1 rel__path__2__1__2"), 4, "acopy between two"),
+    (TC_SLOGC.replace("(2 4)", "(2 99999999999999999999)"), gdl.GDL_ERR_INVALID_ARG,
+     "expected a small number"),
+    ("(" * 100000 + ")" * 100000, gdl.GDL_ERR_INVALID_ARG, "nests too deeply"),
 ]
 
 
