@@ -1,6 +1,6 @@
 # Build of the MI355X (gfx950) semi-naive Datalog engine.
 #   make            -- engine (libgdl_hip.so), C++ API + programs (libgdlog.so),
-#                      program executables (TC, SG, CSPA, DL), generators, CPU oracle
+#                      program executables (TC, SG, CSPA, PLEN, DL), generators, CPU oracle
 #   make clean
 HIPCC    ?= /opt/rocm/bin/hipcc
 CC       ?= gcc
@@ -19,7 +19,7 @@ API_OBJ    := $(patsubst $(PKG)/%,$(LIB)/obj/%.o,$(API_SRC))
 HEADERS    := $(wildcard include/*.h include/gdlog/*.h $(PKG)/csrc/*.hpp $(PKG)/programs/*.h)
 
 all: $(LIB)/libgdl_hip.so $(LIB)/libgdlog.so $(LIB)/libgdl_gen.so oracle/lib/libgdl_oracle.so \
-     $(LIB)/TC $(LIB)/SG $(LIB)/CSPA $(LIB)/DL
+     $(LIB)/TC $(LIB)/SG $(LIB)/CSPA $(LIB)/PLEN $(LIB)/DL
 
 $(LIB)/obj/engine_%.o: $(PKG)/csrc/%.hip $(HEADERS)
 	@mkdir -p $(dir $@)

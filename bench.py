@@ -24,7 +24,9 @@ Prints one JSON line.  Also reports:
                 server vs one-at-a-time service, p50 / p99 latency;
   datalog       tc.dl and cspa.dl through the Datalog front end vs the
                 hand-written RA encodings, whole runs from host inputs to
-                host results.
+                host results;
+  path_length   the hop-count program (a dependent column) beside TC on the
+                same tree, whole runs.
 """
 import argparse
 import importlib.util
@@ -374,6 +376,31 @@ def datalog_vs_hand(gdl, ctx, reps):
     return out
 
 
+def path_length_vs_tc(gdl, ctx, reps):
+    """The path_length program (hop count per pair, a dependent column) beside
+    TC on the 200k-node tree: whole runs from host edges to host results,
+    best of `reps` after a warm-up run each."""
+    tree = gdl.gen_tree(0, 200_000, 1000)
+
+    def best(f):
+        r = f()
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            r = f()
+            ts.append(time.perf_counter() - t)
+        return r, 1000.0 * min(ts)
+
+    plen, plen_ms = best(lambda: ctx.run_path_length(tree))
+    tc, tc_ms = best(lambda: ctx.run_tc(tree, want_trace=False))
+    out = {"tc_tree_200k": dict(path_length_ms=round(plen_ms, 2), tc_ms=round(tc_ms, 2),
+                                pairs=int(plen["n_out"]), longest=int(plen["longest"]),
+                                same_pairs=bool(plen["n_out"] == tc["n_out"] and
+                                                plen["checksum"] == tc["checksum"]))}
+    print(f"bench: path_length {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def workloads(gdl, nodes, window):
     """name -> (kind, make_input, description)."""
     return {
@@ -407,7 +434,8 @@ def main():
     ap.add_argument("--served-requests", type=int, default=1000,
                     help="requests of the served-mix measurement (0: skip it)")
     ap.add_argument("--datalog-reps", type=int, default=3,
-                    help="timed runs of the Datalog front-end comparison (0: skip it)")
+                    help="timed runs of the Datalog front-end and path-length "
+                         "comparisons (0: skip them)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device copy (traffic-counter calibration)")
     ap.add_argument("--phase", choices=["batched", "served"],
@@ -510,6 +538,7 @@ def main():
         d.free()
     gdl.set_full_mode(gdl.FULL_MODE_AUTO)
     dlog = datalog_vs_hand(gdl, ctx, args.datalog_reps) if args.datalog_reps > 0 else None
+    plen = path_length_vs_tc(gdl, ctx, args.datalog_reps) if args.datalog_reps > 0 else None
     # the batched and served measurements each run in a process of their own
     # (as a batch or a server would): device memory that one long run of
     # allocations and frees gave back comes back to the next one much more
@@ -551,6 +580,7 @@ def main():
         "batched": batch,
         "served": serve,
         "datalog": dlog,
+        "path_length": plen,
     }
     ctx.close()
     print(json.dumps(out), flush=True)

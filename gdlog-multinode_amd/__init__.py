@@ -203,6 +203,8 @@ def load():
     _decl(prog, "gdl_prog_sg", c_int, [vp, vp, c_uint64, c_int, rp, bp])
     _decl(prog, "gdl_prog_cspa", c_int, [vp, vp, c_uint64, vp, c_uint64, c_int, rp, bp])
     _decl(prog, "gdl_prog_last_error", c_char_p, [])
+    _decl(prog, "gdl_prog_path_length", c_int,
+          [vp, vp, c_uint64, c_int, rp, vp, vp, c_uint64])
     _decl(prog, "gdl_prog_set_full_mode", c_int, [c_int])
     _decl(prog, "gdl_prog_set_checkpoint", c_int, [ctypes.c_char_p, c_int, c_int, c_int])
     _decl(prog, "gdl_datalog_compile", c_int, [c_char_p, POINTER(vp)])
@@ -718,6 +720,27 @@ class Context:
         finally:
             if isinstance(keep, DeviceArray):
                 keep.free()
+
+    def run_path_length(self, edges, on_device=False, out_cap=None):
+        """All-pairs hop counts (the reference's path_length program): returns
+        the pairs (sorted keys), their hop counts, and the run's statistics."""
+        p, n, keep = self._edges(edges, on_device)
+        cap = out_cap if out_cap is not None else 60_000_000
+        keys = np.empty(max(cap, 1), dtype=np.uint64)
+        lens = np.empty(max(cap, 1), dtype=np.uint32)
+        res = ProgResult()
+        try:
+            st = self.prog.gdl_prog_path_length(self.h, p, n, int(on_device), ctypes.byref(res),
+                                                keys.ctypes.data, lens.ctypes.data, cap)
+        finally:
+            if isinstance(keep, DeviceArray):
+                keep.free()
+        if st != GDL_OK:
+            msg = (self.prog.gdl_prog_last_error() or b"").decode(errors="replace")
+            raise GdlError(st, msg)
+        return dict(out=keys[:res.n_out].copy(), lens=lens[:res.n_out].copy(),
+                    n_out=res.n_out, longest=res.sizes[0], iterations=res.iterations,
+                    derived=res.derived, checksum=res.checksum, fixpoint_ms=res.fixpoint_ms)
 
     def run_sg(self, edges, on_device=False, want_keys=True, want_trace=True, out_cap=None):
         p, n, keep = self._edges(edges, on_device)

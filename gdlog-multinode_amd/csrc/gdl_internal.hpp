@@ -77,6 +77,8 @@ enum KernelId {
     K_SEG_HIST,
     K_MASK_COLS,
     K_LINEAR,
+    K_LABEL,
+    K_SCATTER,
     K_NUM
 };
 extern const char *kKernelNames[K_NUM];

@@ -80,11 +80,12 @@ void enter_append_mode(Relation *rel) {
     rel->append_mode = true;
 }
 
-// FULL back to one sorted array; drops the hash set.
-void leave_append_mode(Relation *rel) {
+// FULL back to one sorted array (unless keep_log: FULL stays the append log,
+// its iterations' deltas in order); drops the hash set.
+void leave_append_mode(Relation *rel, bool keep_log) {
     gdl_ctx *ctx = context();
     const tuple_size_t n = rel->current_full_size;
-    if (n && !rel->full->sorted) {
+    if (n && !rel->full->sorted && !keep_log) {
         uint64_t *sorted = device_alloc_keys(n);
         const int b = id_bits();
         gdl_status st = gdl_sort_u64(ctx, rel->full->keys, n, b, b, sorted);
@@ -510,7 +511,7 @@ void LIE::fixpoint_loop() {
     iteration_count = iteration_counter;
 
     for (Relation *rel : update_relations) {
-        if (rel->append_mode) leave_append_mode(rel);
+        if (rel->append_mode) leave_append_mode(rel, keep_full_log);
         if (reload_full_flag) {
             // lie.cu:219-254: full reloaded from the merged tuples; its rows
             // (data_raw) become the sorted full.
@@ -525,7 +526,7 @@ void LIE::fixpoint_loop() {
                       "reload full");
                 rel->raw_rows = n;
             }
-            rel->raw_sorted = true;
+            rel->raw_sorted = rel->full->sorted;
             container_free(rel->delta);
             container_free(rel->newt);
         }

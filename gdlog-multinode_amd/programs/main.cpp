@@ -1,4 +1,4 @@
-// main.cpp -- command-line drivers TC, SG and CSPA (the reference's
+// main.cpp -- command-line drivers TC, SG, CSPA and PLEN (the reference's
 // executables of the same names, CMakeLists.txt:58-91): read a tab-separated
 // edge file (CSPA: a directory holding assign.facts and dereference.facts),
 // run the program and print the result size and times.
@@ -6,7 +6,8 @@
 // Options (extensions; the reference prints sizes only):
 //   --out PATH        the result as sorted tab-separated rows, one per tuple
 //                     (CSPA: a directory receiving ValueFlow.csv,
-//                     MemoryAlias.csv and ValueAlias.csv), for parity diffs
+//                     MemoryAlias.csv and ValueAlias.csv; PLEN: rows
+//                     from, to, hop count), for parity diffs
 //   --out-bin PATH    the main result as sorted little-endian u64 keys
 //                     (c0 << 32) | c1
 //   --checkpoint DIR  snapshot each fixpoint to DIR/lie<k>.ckpt (at the end,
@@ -91,6 +92,24 @@ int main(int argc, char *argv[]) {
         std::cout << "Path counts " << out.n_main << std::endl;
         std::cout << "TC time: " << out.fixpoint_ms / 1000.0 << std::endl;
         if (!out_path.empty()) write_tsv(out_path, out.main_keys);
+#elif defined(GDL_PROGRAM_PLEN)
+        std::vector<column_type> g = read_pairs(dataset);
+        std::cout << "Input graph rows: " << g.size() / 2 << std::endl;
+        std::vector<uint32_t> lens;
+        out.want_keys = true;
+        path_length_program(host_input(g), grid_size, block_size, &out,
+                            out_path.empty() ? nullptr : &lens);
+        std::cout << "Path counts " << out.n_main << std::endl;
+        std::cout << "longest path " << out.sizes[0] << std::endl;
+        std::cout << "PLEN time: " << out.fixpoint_ms / 1000.0 << std::endl;
+        if (!out_path.empty()) {
+            FILE *f = std::fopen(out_path.c_str(), "w");
+            if (!f) throw gdlog::Error(GDL_ERR_INVALID_ARG, "cannot write " + out_path);
+            for (size_t i = 0; i < lens.size(); ++i)
+                std::fprintf(f, "%u\t%u\t%u\n", (unsigned)(out.main_keys[i] >> 32),
+                             (unsigned)(out.main_keys[i] & 0xffffffffu), lens[i]);
+            if (std::fclose(f) != 0) throw gdlog::Error(GDL_ERR_INVALID_ARG, "cannot write " + out_path);
+        }
 #elif defined(GDL_PROGRAM_SG)
         std::vector<column_type> g = read_pairs(dataset);
         std::cout << "Input graph rows: " << g.size() / 2 << std::endl;

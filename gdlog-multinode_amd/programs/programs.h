@@ -38,6 +38,11 @@ uint64_t sum_delta(const std::vector<IterationRecord> &t);
 
 void tc_program(const EdgeInput &graph, int max_iteration, int grid_size, int block_size,
                 ProgramOutput *out);
+// All-pairs hop counts (test/path_length.cu): out as TC's, plus *lens[i] the
+// hop count of out->main_keys[i] (filled when lens is given); out->sizes[0]
+// is the longest hop count.
+void path_length_program(const EdgeInput &graph, int grid_size, int block_size,
+                         ProgramOutput *out, std::vector<uint32_t> *lens);
 
 // One program run (kind 0 TC, 1 SG, 2 CSPA) from host packed keys on ctx,
 // which is this thread's engine context for the run; output keys are copied

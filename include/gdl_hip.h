@@ -317,6 +317,22 @@ gdl_status gdl_checksum_u64(gdl_ctx *ctx, const uint64_t *d_keys, uint64_t n, ui
 gdl_status gdl_mask_columns_u64(gdl_ctx *ctx, const uint64_t *d_in, uint64_t n,
                                 uint32_t col_mask, uint64_t *d_out);
 
+/* ---- labels of an append log ------------------------------------------
+ * No counterpart in the reference's code; they produce the dependent column
+ * of its path_length program (test/path_length.cu:49-124: the hop count
+ * newt[2] = outer[2] + 1, left out of the set difference), which equals one
+ * plus the log segment (iteration) that first derived the pair.
+ * gdl_segment_labels_u32: d_out[i] = first + s for bounds[s] <= i <
+ * bounds[s + 1] (d_bounds: n_seg + 1 nondecreasing device words, the last n).
+ * gdl_scatter_by_key_u32: for each d_keys[i], its position p in the sorted,
+ * duplicate-free d_sorted gets d_out[p] = d_vals[i]; the keys must be
+ * distinct and all present (GDL_ERR_DEVICE otherwise). */
+gdl_status gdl_segment_labels_u32(gdl_ctx *ctx, const uint64_t *d_bounds, uint32_t n_seg,
+                                  uint32_t first, uint32_t *d_out, uint64_t n);
+gdl_status gdl_scatter_by_key_u32(gdl_ctx *ctx, const uint64_t *d_sorted, uint64_t n_sorted,
+                                  const uint64_t *d_keys, const uint32_t *d_vals, uint64_t n,
+                                  uint32_t *d_out);
+
 /* ---- projection / filter ----------------------------------------------
  * gdl_project_u64 replaces get_copy_result (relation.cu:390-403);
  * gdl_filter_u64 keeps keys whose tuple satisfies pred. */

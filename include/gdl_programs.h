@@ -60,6 +60,16 @@ gdl_status gdl_prog_tc(gdl_ctx *ctx, const uint64_t *edges, uint64_t n, int on_d
                        int max_iteration, gdl_prog_result *res, const gdl_prog_buffers *bufs);
 gdl_status gdl_prog_sg(gdl_ctx *ctx, const uint64_t *edges, uint64_t n, int on_device,
                        gdl_prog_result *res, const gdl_prog_buffers *bufs);
+/* All-pairs hop counts: the reference's path_length program
+ * (test/path_length.cu:49-124, the PLEN executable), path(a, c, l + 1) :-
+ * edge(a, b), path(b, c, l) with l a dependent column (edges load with l = 1,
+ * a pair keeps the l of the iteration that first derives it).  res as for
+ * TC, with sizes[0] the longest hop count.  keys / lens (host, cap entries,
+ * either may be NULL): the pairs in sorted order and each pair's hop count;
+ * GDL_ERR_OVERFLOW when cap < res->n_out. */
+gdl_status gdl_prog_path_length(gdl_ctx *ctx, const uint64_t *edges, uint64_t n, int on_device,
+                                gdl_prog_result *res, uint64_t *keys, uint32_t *lens,
+                                uint64_t cap);
 gdl_status gdl_prog_cspa(gdl_ctx *ctx, const uint64_t *assign, uint64_t n_assign,
                          const uint64_t *dereference, uint64_t n_dereference, int on_device,
                          gdl_prog_result *res, const gdl_prog_buffers *bufs);

@@ -30,6 +30,12 @@ struct LIE {
 
     bool reload_full_flag = true;
     int max_iteration = INT_MAX;
+    // An extension: an update relation kept as an append log (FULL not read as
+    // a join inner) ends the fixpoint with FULL in log order -- its loaded
+    // tuples, then each iteration's delta -- instead of sorted
+    // (full->sorted == false), so a tuple's position tells the iteration that
+    // first derived it.
+    bool keep_full_log = false;
 
     // Checkpoint / resume (an extension: the reference has none).
     // checkpoint_file: the loop writes a snapshot there (every update

@@ -129,6 +129,44 @@ def tc(edges, max_iteration=-1):
     return _collect(res)
 
 
+def path_length(edges):
+    """All-pairs hop counts: restates test/path_length.cu:49-124 with numpy.
+
+    path(a, c, l + 1) :- edge(a, b), path(b, c, l), path loaded from the edges
+    with l = 1 (path_length.cu:83-87, 104-105); the set difference compares
+    (from, to) only (lie.cu:131-138 over arity - dependent columns), so a
+    pair keeps the l of the first iteration deriving it.  The first delta is
+    the first |FULL| raw rows, as in TC (relation.cu:764-795 via lie.cu:56).
+    Returns (sorted keys, hop counts aligned with them)."""
+    e = _arr(edges)
+    mask = np.uint64(0xffffffff)
+    full = np.unique(e)
+    # edges by target b: the inner edge_2__2_1 indexed on b
+    by_b = full[np.argsort(full & mask, kind="stable")]
+    tgt = by_b & mask
+    src = by_b >> np.uint64(32)
+    segments = [(full, 1)]
+    delta = e[:full.size]
+    length = 1
+    while delta.size:
+        b, c = delta >> np.uint64(32), delta & mask
+        lo, hi = np.searchsorted(tgt, b, "left"), np.searchsorted(tgt, b, "right")
+        cnt = hi - lo
+        rows = np.repeat(np.arange(delta.size), cnt)
+        pos = np.repeat(lo - np.cumsum(cnt) + cnt, cnt) + np.arange(rows.size)
+        newt = np.unique((src[pos] << np.uint64(32)) | c[rows])
+        fresh = np.setdiff1d(newt, full, assume_unique=True)
+        length += 1
+        if fresh.size:
+            segments.append((fresh, length))
+            full = np.union1d(full, fresh)
+        delta = fresh
+    keys = np.concatenate([s for s, _ in segments])
+    lens = np.concatenate([np.full(s.size, l, dtype=np.uint32) for s, l in segments])
+    order = np.argsort(keys, kind="stable")
+    return keys[order], lens[order]
+
+
 def sg(edges):
     e = _arr(edges)
     res = Result()
