@@ -249,7 +249,17 @@ struct Parser {
             if (!is(",")) break;
             next();
         }
+        const int close_line = peek().line;
         expect(")");
+        // a relation qualifier after the declaration, on its line
+        static const char *quals[] = {"brie", "btree", "btree_delete", "eqrel", "choice",
+                                      "inline", "no_inline", "magic", "no_magic", "override",
+                                      "overridable", "nullary"};
+        if (peek().kind == Tok::Ident && peek().line == close_line)
+            for (const char *q : quals)
+                if (peek().s == q)
+                    fail(GDL_ERR_UNSUPPORTED, close_line,
+                         "relation qualifier " + peek().s + " is not supported");
         if (arity != 1 && arity != 2)
             fail(GDL_ERR_UNSUPPORTED, line,
                  "relation " + name + " has arity " + std::to_string(arity) +

@@ -481,3 +481,34 @@ def test_slog_tc_plan_matches_the_oracle(ctx, name):
     e = uniq(TC_INPUTS[name]())
     got = gdl.Datalog(TC_SLOGC, slog=True).run(ctx, edge=e)
     assert np.array_equal(got["path"], oracle.tc(e)["out"])
+
+
+REF_SOUFFLE = "/root/reference/test/souffle"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_SOUFFLE),
+                    reason="the reference's Souffle programs are only present where it is checked out")
+def test_the_reference_souffle_programs_compile_as_written():
+    def prog(name):
+        with open(os.path.join(REF_SOUFFLE, name)) as f:
+            return f.read()
+    e = uniq(gdl.gen_er(7, 80, 160))
+    tc = gdl.Datalog(prog("tc.dl"))
+    assert tc.kept() == ["path"]
+    assert np.array_equal(evaluate(tc.source, {"edge": e})["path"], oracle.tc(e)["out"])
+    sg = gdl.Datalog(prog("sg.dl"))
+    g = uniq(gdl.gen_rmat(3, 6, 120))
+    assert np.array_equal(evaluate(sg.source, {"edge": g})["sg"], oracle.sg(g)["out"])
+    cspa = gdl.Datalog(prog("cspa.dl"))
+    assert cspa.kept() == ["ValueFlow", "ValueAlias", "MemoryAlias", "tmp"]
+    a, d = gdl.gen_cspa(9, 40, 1.1, 50, 25)
+    a, d = uniq(a), uniq(d)
+    got = evaluate(cspa.source, {"assign": a, "dereference": d})
+    want = oracle.cspa(a, d)
+    assert np.array_equal(got["ValueAlias"], want["out"])
+    assert np.array_equal(got["ValueFlow"], want["extra"][0])
+    # the programs outside the subset (symbols, choice domains, arity 3) are refused
+    for name in ["bip.dl", "spanning.dl", "choice_total.dl", "path_length.dl"]:
+        with pytest.raises(gdl.GdlError) as err:
+            gdl.Datalog(prog(name))
+        assert err.value.status == 4, name
