@@ -153,6 +153,7 @@ struct DatalogProgram::Impl {
     std::vector<int> arity;                   // 1 or 2
     std::vector<std::vector<uint64_t>> facts;  // ground atoms of the program, packed
     std::vector<char> is_input, is_kept;
+    std::vector<char> hidden;  // made by the front end (constants), never kept
     bool any_kept = false;
     std::vector<Rule> rules;
     // plan
@@ -264,12 +265,18 @@ struct Parser {
             fail(GDL_ERR_UNSUPPORTED, line,
                  "relation " + name + " has arity " + std::to_string(arity) +
                      " (unary and binary relations only)");
+        add_relation(name, arity, false);
+    }
+
+    int add_relation(const std::string &name, int arity, bool hidden) {
         m.rel_id[name] = (int)m.rel_names.size();
         m.rel_names.push_back(name);
         m.arity.push_back(arity);
         m.facts.emplace_back();
         m.is_input.push_back(0);
         m.is_kept.push_back(0);
+        m.hidden.push_back(hidden);
+        return (int)m.rel_names.size() - 1;
     }
 
     void skip_parens() {
@@ -307,8 +314,9 @@ struct Parser {
         return next().s;
     }
 
-    // An atom; with consts, a ground atom's numbers go there instead (a fact).
-    Atom atom(std::vector<uint64_t> *consts = nullptr) {
+    // An atom.  Its numbers go to consts (with their argument positions in
+    // const_pos); without consts an atom may not hold numbers.
+    Atom atom(std::vector<uint64_t> *consts = nullptr, std::vector<int> *const_pos = nullptr) {
         Atom a;
         const int line = peek().line;
         const std::string name = ident("a relation name");
@@ -322,6 +330,8 @@ struct Parser {
                     std::stoull(x.s) > 0xffffffffull)
                     fail(GDL_ERR_INVALID_ARG, x.line, "bad id " + x.s + " (decimal, below 2^32)");
                 consts->push_back(std::stoull(x.s));
+                if (const_pos) const_pos->push_back(k);
+                if (k < 2) a.v[k] = kAnon + std::to_string(++anon);
                 next();
                 ++nconst;
             } else {
@@ -341,10 +351,59 @@ struct Parser {
             fail(GDL_ERR_INVALID_ARG, line,
                  name + " takes " + std::to_string(want) + " argument" + (want > 1 ? "s" : "") +
                      ", not " + std::to_string(k));
-        if (nconst && nconst != k)
-            fail(GDL_ERR_UNSUPPORTED, line, "constants in rules are not supported");
+        if (nconst && nconst != k && !const_pos)
+            fail(GDL_ERR_UNSUPPORTED, line, "constants in rule heads are not supported");
         if (want == 1) a.v[1] = a.v[0];
         return a;
+    }
+
+    // A body atom R(.., c, ..) with one constant becomes #s(v), where
+    //   #s(v) :- #c(w), R(.., w, ..).     #c(c).
+    // a selection in its own rule, so the constant is matched by a join on
+    // the constant's column before the atom meets the rest of the body.
+    Atom select_constant(const Atom &a, const std::vector<uint64_t> &c,
+                         const std::vector<int> &pos, int line) {
+        if (m.arity[a.rel] == 1 || c.size() > 1)
+            fail(GDL_ERR_UNSUPPORTED, line, "a ground atom in a rule body");
+        const int p = pos[0], q = 1 - p;
+        if (is_anon(a.v[q]))
+            fail(GDL_ERR_UNSUPPORTED, line, "an atom holding a constant and no variable");
+        const std::string cname = "#c" + std::to_string(c[0]);
+        auto it = m.rel_id.find(cname);
+        int crel;
+        if (it == m.rel_id.end()) {
+            crel = add_relation(cname, 1, true);
+            m.facts[crel].push_back(c[0] << 32);
+        } else {
+            crel = it->second;
+        }
+        const std::string sname =
+            "#s_" + m.rel_names[a.rel] + "_" + std::to_string(p) + "_" + std::to_string(c[0]);
+        auto st = m.rel_id.find(sname);
+        int srel;
+        if (st == m.rel_id.end()) {
+            srel = add_relation(sname, 1, true);
+            Rule sel;
+            sel.line = line;
+            sel.head = srel;
+            sel.head_arity = 1;
+            sel.hv[0] = sel.hv[1] = "v";
+            Atom src;
+            src.rel = a.rel;
+            src.v[q] = "v";
+            src.v[p] = "w";
+            Atom cw;
+            cw.rel = crel;
+            cw.v[0] = cw.v[1] = "w";
+            sel.body = {cw, src};  // the one-tuple #c first: the outer of an init join
+            m.rules.push_back(sel);
+        } else {
+            srel = st->second;
+        }
+        Atom out;
+        out.rel = srel;
+        out.v[0] = out.v[1] = a.v[q];
+        return out;
     }
 
     void rule() {
@@ -354,7 +413,7 @@ struct Parser {
         const Atom h = atom(&ground);
         if (!ground.empty()) {  // a fact
             if (!is("."))
-                fail(GDL_ERR_UNSUPPORTED, r.line, "constants in rules are not supported");
+                fail(GDL_ERR_UNSUPPORTED, r.line, "constants in rule heads are not supported");
             next();
             m.facts[h.rel].push_back((ground[0] << 32) | (ground.size() > 1 ? ground[1] : 0));
             return;
@@ -366,7 +425,10 @@ struct Parser {
             const int line = peek().line;
             if (is("!")) fail(GDL_ERR_UNSUPPORTED, line, "negation is not supported");
             if (peek().kind == Tok::Ident && is("(", 1)) {
-                r.body.push_back(atom());
+                std::vector<uint64_t> c;
+                std::vector<int> pos;
+                const Atom a = atom(&c, &pos);
+                r.body.push_back(c.empty() ? a : select_constant(a, c, pos, line));
             } else {
                 const std::string lhs = variable("in comparisons");
                 static const std::pair<const char *, int> ops[] = {
@@ -401,7 +463,7 @@ struct Parser {
 
     void validate(const Rule &r) const {
         if (r.body.empty()) fail(GDL_ERR_UNSUPPORTED, r.line, "a rule needs a body atom");
-        if (r.body.size() > 3) fail(GDL_ERR_UNSUPPORTED, r.line, "more than three body atoms");
+        if (r.body.size() > 4) fail(GDL_ERR_UNSUPPORTED, r.line, "more than four body atoms");
         for (const Atom &a : r.body)
             if (m.arity[a.rel] == 2 && a.v[0] == a.v[1])
                 fail(GDL_ERR_UNSUPPORTED, r.line, "variable " + a.v[0] + " twice in one atom");
@@ -465,10 +527,26 @@ void plan_chain(const Rule &r, int s, RelationVersion ver, std::vector<AOp> *ops
     used[s] = 1;
     std::string cur[2] = {r.body[s].v[0], r.body[s].v[1]};
     int cur_tmp = -1;
+    // The next atom: one sharing a variable with c, the one that leaves the
+    // fewest variables live after its join (a filter on bound variables
+    // first), then the first in body order.
     auto pick = [&](const std::string c[2]) {
-        for (int b = 0; b < k; ++b)
-            if (!used[b] && !shared_vars(c, r.body[b]).empty()) return b;
-        return -1;
+        int best = -1;
+        size_t best_live = 0;
+        for (int b = 0; b < k; ++b) {
+            if (used[b] || shared_vars(c, r.body[b]).empty()) continue;
+            std::set<std::string> need(r.hv, r.hv + 2);
+            for (int x = 0; x < k; ++x)
+                if (!used[x] && x != b) need.insert(r.body[x].v, r.body[x].v + 2);
+            std::set<std::string> live;
+            for (const std::string &v : {c[0], c[1], r.body[b].v[0], r.body[b].v[1]})
+                if (need.count(v)) live.insert(v);
+            if (best < 0 || live.size() < best_live) {
+                best = b;
+                best_live = live.size();
+            }
+        }
+        return best;
     };
     auto join_var = [&](const std::string c[2], int b) {
         if (b < 0)
@@ -1234,7 +1312,7 @@ bool DatalogProgram::compile(const std::string &source, gdl_status *status, std:
         return false;
     }
     for (size_t rel = 0; rel < fresh->rel_names.size(); ++rel) {
-        if (fresh->any_kept ? fresh->is_kept[rel] : !fresh->is_input[rel])
+        if (fresh->any_kept ? fresh->is_kept[rel] : !fresh->is_input[rel] && !fresh->hidden[rel])
             fresh->kept.push_back(fresh->rel_names[rel]);
         if (fresh->is_input[rel]) fresh->input_names.push_back(fresh->rel_names[rel]);
     }

@@ -14,14 +14,15 @@
 //                                       program names none)
 //   R(1, 2).  S(0).                     facts: ground atoms, added to the
 //                                       relation's starting tuples
-//   H(x, y) :- A(x, z), B(z, y), x != y.
-//     1 to 3 body atoms over variables (_ for an unused column); every head
+//   H(x, y) :- A(x, z), B(z, y), x != y.      R(y) :- E(0, y).
+//     1 to 4 body atoms over variables (_ for an unused column) and
+//     constants (one per binary atom, next to a variable); every head
 //     variable bound by the body; no variable twice in one atom; each atom
 //     after the first shares exactly one variable with what is joined so
 //     far, and at most two variables are live between two joins; at most one
 //     comparison (= != < <= > >=) between two head variables, in rules of two
 //     or more atoms.  Line and block comments.
-// Constants in rules, negation, aggregates, functors, other arities and other
+// Constants in heads, negation, aggregates, functors, other arities and other
 // directives are rejected (GDL_ERR_UNSUPPORTED), syntax errors and unknown
 // relations with GDL_ERR_INVALID_ARG; the message names the line.  A tuple of
 // a unary relation is the key c0 << 32.
@@ -37,7 +38,10 @@
 // is stored in the column order its stratum reads most (a linear rule then
 // needs no second copy), and a read in the other order goes to a reversed
 // copy: a mirror kept by ACopy inside the relation's own recursive LIE, or a
-// static copy built from FULL once the relation is final.
+// static copy built from FULL once the relation is final.  A body atom
+// with a constant, E(0, y), becomes a selection relation #s(y) :- #c(w),
+// E(w, y) over a one-fact relation #c(0); such relations are never kept.
+// A chain joins next the atom that leaves the fewest variables live.
 #pragma once
 
 #include <cstdint>

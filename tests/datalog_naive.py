@@ -52,6 +52,11 @@ def _matches(env, atom, tuples):
         for v, val in zip(args, t):
             if v == "_":
                 continue
+            if v.isdigit():  # a constant
+                if int(v) != val:
+                    ok = False
+                    break
+                continue
             if v in e and e[v] != val:
                 ok = False
                 break

@@ -126,6 +126,26 @@ loop(x, x) :- reach(x).
 """
 
 
+# constants in rule bodies: each atom with a constant becomes a selection
+# relation (#s) joined from a one-fact relation (#c); `via` selects inside
+# its own recursive stratum
+CONSTS = """
+.decl edge(a: number, b: number)
+.input edge
+.decl reach(a: number)
+.decl hub(a: number, b: number)
+.decl via(a: number, b: number)
+.decl hop4(a: number, b: number)
+.output reach, hub, via, hop4
+reach(y) :- edge(0, y).
+reach(y) :- reach(x), edge(x, y).
+hub(x, y) :- edge(x, 3), edge(x, y).
+via(x, y) :- edge(x, y).
+via(x, y) :- via(x, z), via(z, y), via(z, 1).
+hop4(x, y) :- edge(x, a), edge(a, b), edge(b, c), edge(c, y).
+"""
+
+
 def uniq(keys):
     return np.unique(np.asarray(keys, dtype=np.uint64))
 
@@ -184,6 +204,21 @@ def test_unary_plan():
     assert "copy reach_1__1 FULL -> loop_2__1_2 (i0, i0)" in plan
     assert "join edge_2__2_1 FULL x reach_1__1 FULL -> entered_1__1 (o0)" in plan
     assert p.kept() == ["reach", "hop2", "entered", "loop"]
+
+
+def test_body_constants_become_selections():
+    p = gdl.Datalog(CONSTS)
+    assert p.kept() == ["reach", "hub", "via", "hop4"]
+    plan = p.plan
+    assert "join edge_2__1_2 FULL x #c0_1__1 FULL -> #s_edge_0_0_1__1 (i1)" in plan
+    assert "copy #s_edge_0_0_1__1 FULL -> reach_1__1 (i0)" in plan
+    # via(z, 1) reads via: the selection joins via's recursive stratum
+    assert any(l.startswith("lie via, #s_via_1_1") or l.startswith("lie #s_via_1_1, via")
+               for l in plan.splitlines())
+    e = uniq(gdl.gen_er(5, 25, 45))
+    got = evaluate(CONSTS, {"edge": e})
+    k = e[(e >> np.uint64(32)) == 0] & np.uint64(0xffffffff)
+    assert set(k.tolist()) <= set((got["reach"] >> np.uint64(32)).tolist())
 
 
 def lie_blocks(plan):
@@ -257,9 +292,12 @@ BAD = [
     (".decl r(a: symbol, b: number)", 4, "attribute type symbol"),
     (".decl r(a: number, b: number)\n.decl r(a: number, b: number)", gdl.GDL_ERR_INVALID_ARG,
      "declared twice"),
-    (".decl r(a: number, b: number)\nr(x, 1) :- r(x, y).", 4, "constants in rules"),
-    (".decl r(a: number, b: number)\nr(x, y) :- r(x, 1).", 4, "constants in rules"),
-    (".decl r(a: number, b: number)\nr(1, x).", 4, "constants in rules"),
+    (".decl r(a: number, b: number)\nr(x, 1) :- r(x, y).", 4, "constants in rule heads"),
+    (".decl r(a: number, b: number)\nr(x, y) :- r(1, 2), r(x, y).", 4, "a ground atom in a rule body"),
+    (".decl r(a: number, b: number)\nr(x, y) :- r(x, y), r(1, _).", 4,
+     "holding a constant and no variable"),
+    (".decl r(a: number)\nr(x) :- r(x), r(1).", 4, "a ground atom in a rule body"),
+    (".decl r(a: number, b: number)\nr(1, x).", 4, "constants in rule heads"),
     (".decl r(a: number, b: number)\nr(x, y).", gdl.GDL_ERR_INVALID_ARG, "a fact with variables"),
     (".decl r(a: number, b: number)\nr(4294967296, 1).", gdl.GDL_ERR_INVALID_ARG, "bad id"),
     (".decl r(a: number)\nr(1, 2).", gdl.GDL_ERR_INVALID_ARG, "r takes 1 argument, not 2"),
@@ -275,11 +313,11 @@ BAD = [
     (".decl r(a: number, b: number)\nr(x, y) :- r(x, y), x != y.", 4, "one-atom rule"),
     (".decl r(a: number, b: number)\nr(x, y) :- r(x, z), r(z, y), z != y.", 4,
      "not a head variable"),
-    (".decl r(a: number, b: number)\nr(x, y) :- r(x, a), r(a, b), r(b, c), r(c, y).", 4,
-     "more than three"),
+    (".decl r(a: number, b: number)\nr(x, y) :- r(x, a), r(a, b), r(b, c), r(c, d), r(d, y).",
+     4, "more than four"),
     (".decl r(a: number, b: number)\nr(x, y) :- r(x, z), r(z, y); r(y, x).", 4, "disjunction"),
     (".decl r(a: number, b: number)\n.type T = number", 4, "directive .type"),
-    (".decl r(a: number, b: number)\nr(x, y) :- r(x, z), r(z, y), r(z, w).", 4,
+    (".decl r(a: number, b: number)\nr(x, y) :- r(x, z), r(z, w), r(w, y), r(z, y).", 4,
      "three variables live"),
     ("/* open", gdl.GDL_ERR_INVALID_ARG, "unterminated comment"),
 ]
@@ -359,6 +397,17 @@ def test_reach_program_matches_the_naive_evaluator(ctx, seed):
     for rel in want:
         assert np.array_equal(got[rel], want[rel]), rel
     assert got["reach"].size > 2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", [6, 9])
+def test_constants_program_matches_the_naive_evaluator(ctx, seed):
+    e = uniq(gdl.gen_er(seed, 25, 45))
+    want = evaluate(CONSTS, {"edge": e})
+    got = gdl.Datalog(CONSTS).run(ctx, edge=e)
+    for rel in want:
+        assert np.array_equal(got[rel], want[rel]), rel
+    assert got["reach"].size > 0 and got["via"].size > 0
 
 
 @pytest.mark.gpu
