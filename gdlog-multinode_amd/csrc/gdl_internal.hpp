@@ -114,6 +114,15 @@ struct gdl_ctx {
 
 namespace gdl {
 
+// A comparison op (optionally against a constant) over columns < arity.
+inline bool valid_pred(const gdl_pred &p, int arity) {
+    const int op = p.op & ~GDL_PRED_CONST;
+    if (op < GDL_PRED_NONE || op > GDL_PRED_FALSE) return false;
+    if (op == GDL_PRED_NONE || op == GDL_PRED_FALSE) return !(p.op & GDL_PRED_CONST);
+    if (p.lhs < 0 || p.lhs >= arity) return false;
+    return (p.op & GDL_PRED_CONST) || (p.rhs >= 0 && p.rhs < arity);
+}
+
 // Grid cap of a grid-stride kernel: the tuned blocks per CU (x mult).
 inline uint64_t grid_cap(const gdl_ctx *ctx, uint64_t mult = 1) {
     return (uint64_t)(ctx->cu_count > 0 ? ctx->cu_count : 256) * ctx->tune.blocks_per_cu * mult;
@@ -267,8 +276,9 @@ __device__ __forceinline__ u64 apply_map(u64 ik, u64 ok, int arity, int s0, int 
 }
 
 __device__ __forceinline__ bool eval_pred(u64 k, int op, int lhs, int rhs) {
-    u32 a = key_col(k, lhs), b = key_col(k, rhs);
-    switch (op) {
+    const u32 a = key_col(k, lhs);
+    const u32 b = (op & GDL_PRED_CONST) ? (u32)rhs : key_col(k, rhs);
+    switch (op & ~GDL_PRED_CONST) {
         case GDL_PRED_EQ: return a == b;
         case GDL_PRED_NE: return a != b;
         case GDL_PRED_LT: return a < b;

@@ -698,11 +698,8 @@ gdl_status gdl_join(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n_inner,
     if (!ctx || !inner_idx || !d_out || !n_out || (n_outer && !d_outer) || (n_inner && !d_inner))
         return GDL_ERR_INVALID_ARG;
     if (!valid_map(map, true)) return fail(ctx, GDL_ERR_INVALID_ARG, "bad join column map");
-    if (pred.op < GDL_PRED_NONE || pred.op > GDL_PRED_FALSE)
+    if (!valid_pred(pred, map.arity))
         return fail(ctx, GDL_ERR_INVALID_ARG, "bad join predicate");
-    if (pred.op != GDL_PRED_NONE && pred.op != GDL_PRED_FALSE &&
-        (pred.lhs < 0 || pred.lhs >= map.arity || pred.rhs < 0 || pred.rhs >= map.arity))
-        return fail(ctx, GDL_ERR_INVALID_ARG, "bad join predicate columns");
     *d_out = nullptr;
     *n_out = 0;
     if (n_outer == 0 || n_inner == 0) return GDL_OK;
@@ -880,7 +877,7 @@ gdl_status gdl_join_insert(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n_inn
         (n_outer && !d_outer) || (n_inner && !d_inner))
         return GDL_ERR_INVALID_ARG;
     if (!valid_map(map, true)) return fail(ctx, GDL_ERR_INVALID_ARG, "bad join column map");
-    if (pred.op < GDL_PRED_NONE || pred.op > GDL_PRED_FALSE)
+    if (!valid_pred(pred, map.arity))
         return fail(ctx, GDL_ERR_INVALID_ARG, "bad join predicate");
     return join_insert_impl<false>(ctx, d_inner, n_inner, inner_idx, d_outer, n_outer, map, pred,
                                    hs, HsSel{nullptr, 0, nullptr, 0, 0}, d_new, new_cap, n_new,
@@ -899,7 +896,7 @@ gdl_status gdl_join_insert_seg(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
     if (n_segs == 0 || n_segs > kMaxSegs || seg_shift < 1 || seg_shift > 63)
         return fail(ctx, GDL_ERR_INVALID_ARG, "segmented join insert: 1..256 sets, shift 1..63");
     if (!valid_map(map, true)) return fail(ctx, GDL_ERR_INVALID_ARG, "bad join column map");
-    if (pred.op < GDL_PRED_NONE || pred.op > GDL_PRED_FALSE)
+    if (!valid_pred(pred, map.arity))
         return fail(ctx, GDL_ERR_INVALID_ARG, "bad join predicate");
     *n_out = 0;
     u64 *out = d_out;
@@ -919,7 +916,7 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
         (*delta_end && !*d_log) || *delta_end > *log_cap)
         return GDL_ERR_INVALID_ARG;
     if (!valid_map(map, true)) return fail(ctx, GDL_ERR_INVALID_ARG, "bad join column map");
-    if (pred.op < GDL_PRED_NONE || pred.op > GDL_PRED_FALSE)
+    if (!valid_pred(pred, map.arity))
         return fail(ctx, GDL_ERR_INVALID_ARG, "bad join predicate");
     *bodies = 0;
     if (max_bodies <= 0) return GDL_OK;

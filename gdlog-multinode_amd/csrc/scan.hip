@@ -285,9 +285,7 @@ gdl_status gdl_unique_u64(gdl_ctx *ctx, const uint64_t *d_sorted, uint64_t n, ui
 gdl_status gdl_filter_u64(gdl_ctx *ctx, const uint64_t *d_in, uint64_t n, gdl_pred pred,
                           uint64_t *d_out, uint64_t *n_out) {
     if (!ctx || !n_out || (n && (!d_in || !d_out))) return GDL_ERR_INVALID_ARG;
-    if (pred.op < GDL_PRED_NONE || pred.op > GDL_PRED_FALSE || pred.lhs < 0 || pred.lhs > 1 ||
-        pred.rhs < 0 || pred.rhs > 1)
-        return fail(ctx, GDL_ERR_INVALID_ARG, "gdl_filter_u64: bad predicate");
+    if (!valid_pred(pred, 2)) return fail(ctx, GDL_ERR_INVALID_ARG, "gdl_filter_u64: bad predicate");
     return filter_keys(ctx, d_in, n, pred, d_out, n_out);
 }
 

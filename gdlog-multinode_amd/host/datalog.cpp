@@ -111,8 +111,10 @@ struct Rule {
     int head_arity = 2;
     std::string hv[2];
     std::vector<Atom> body;
-    int cmp_op = GDL_PRED_NONE;  // one comparison between head variables
+    int cmp_op = GDL_PRED_NONE;  // one comparison: cl <op> cr, or cl <op> cmp_value
     std::string cl, cr;
+    bool cmp_const = false;
+    uint32_t cmp_value = 0;
     int line = 0;
 };
 
@@ -306,6 +308,13 @@ struct Parser {
         }
     }
 
+    uint32_t id_value(const Tok &x) {
+        if (x.s.size() > 10 || x.s.find_first_not_of("0123456789") != std::string::npos ||
+            std::stoull(x.s) > 0xffffffffull)
+            fail(GDL_ERR_INVALID_ARG, x.line, "bad id " + x.s + " (decimal, below 2^32)");
+        return (uint32_t)std::stoull(x.s);
+    }
+
     std::string variable(const char *where) {
         const Tok &x = peek();
         if (x.kind == Tok::Num || x.kind == Tok::Str)
@@ -326,10 +335,7 @@ struct Parser {
         while (!is(")")) {
             const Tok &x = peek();
             if (x.kind == Tok::Num && consts) {
-                if (x.s.size() > 10 || x.s.find_first_not_of("0123456789") != std::string::npos ||
-                    std::stoull(x.s) > 0xffffffffull)
-                    fail(GDL_ERR_INVALID_ARG, x.line, "bad id " + x.s + " (decimal, below 2^32)");
-                consts->push_back(std::stoull(x.s));
+                consts->push_back(id_value(x));
                 if (const_pos) const_pos->push_back(k);
                 if (k < 2) a.v[k] = kAnon + std::to_string(++anon);
                 next();
@@ -430,7 +436,22 @@ struct Parser {
                 const Atom a = atom(&c, &pos);
                 r.body.push_back(c.empty() ? a : select_constant(a, c, pos, line));
             } else {
-                const std::string lhs = variable("in comparisons");
+                // a comparison of two variables, or of a variable and an id
+                auto operand = [&](std::string *var, uint32_t *val) {
+                    const Tok &x = peek();
+                    if (x.kind == Tok::Num) {
+                        *val = id_value(x);
+                        next();
+                        return false;
+                    }
+                    if (x.kind != Tok::Ident)
+                        fail(GDL_ERR_INVALID_ARG, x.line, "expected a variable or an id" + found());
+                    *var = next().s;
+                    return true;
+                };
+                std::string lhs, rhs;
+                uint32_t lc = 0, rc = 0;
+                const bool lvar = operand(&lhs, &lc);
                 static const std::pair<const char *, int> ops[] = {
                     {"=", GDL_PRED_EQ}, {"!=", GDL_PRED_NE}, {"<", GDL_PRED_LT},
                     {"<=", GDL_PRED_LE}, {">", GDL_PRED_GT}, {">=", GDL_PRED_GE}};
@@ -440,13 +461,27 @@ struct Parser {
                 if (code == GDL_PRED_NONE)
                     fail(GDL_ERR_INVALID_ARG, peek().line, "expected an atom or a comparison" + found());
                 next();
-                const std::string rhs = variable("in comparisons");
+                const bool rvar = operand(&rhs, &rc);
                 if (r.cmp_op != GDL_PRED_NONE)
                     fail(GDL_ERR_UNSUPPORTED, line, "more than one comparison in a rule");
+                if (!lvar && !rvar) fail(GDL_ERR_UNSUPPORTED, line, "a comparison of two ids");
                 if (lhs == "_" || rhs == "_") fail(GDL_ERR_INVALID_ARG, line, "_ in a comparison");
+                if (!lvar) {  // id <op> x  ==  x <flipped op> id
+                    static const int flip[] = {GDL_PRED_NONE, GDL_PRED_EQ, GDL_PRED_NE, GDL_PRED_GT,
+                                               GDL_PRED_GE, GDL_PRED_LT, GDL_PRED_LE};
+                    code = flip[code];
+                    r.cl = rhs;
+                    r.cmp_const = true;
+                    r.cmp_value = lc;
+                } else if (!rvar) {
+                    r.cl = lhs;
+                    r.cmp_const = true;
+                    r.cmp_value = rc;
+                } else {
+                    r.cl = lhs;
+                    r.cr = rhs;
+                }
                 r.cmp_op = code;
-                r.cl = lhs;
-                r.cr = rhs;
             }
             if (is(";")) fail(GDL_ERR_UNSUPPORTED, peek().line, "disjunction is not supported");
             if (!is(",")) break;
@@ -478,9 +513,7 @@ struct Parser {
                 fail(GDL_ERR_INVALID_ARG, r.line, "head variable " + v + " is not bound by the body");
         }
         if (r.cmp_op != GDL_PRED_NONE) {
-            if (r.body.size() < 2)
-                fail(GDL_ERR_UNSUPPORTED, r.line, "a comparison in a one-atom rule");
-            for (const std::string &v : {r.cl, r.cr}) {
+            for (const std::string &v : {r.cl, r.cmp_const ? r.cl : r.cr}) {
                 if (!bound(v))
                     fail(GDL_ERR_INVALID_ARG, r.line, "variable " + v + " is not bound by the body");
                 if (v != r.hv[0] && v != r.hv[1])
@@ -507,6 +540,14 @@ struct AOp {
     gdl_colmap map{};
     gdl_pred pred{GDL_PRED_NONE, 0, 0};
 };
+
+// The rule's comparison over its head's columns.
+gdl_pred rule_pred(const Rule &r) {
+    if (r.cmp_op == GDL_PRED_NONE) return gdl_pred{GDL_PRED_NONE, 0, 0};
+    const int l = r.cl == r.hv[0] ? 0 : 1;
+    if (r.cmp_const) return gdl_pred{r.cmp_op | GDL_PRED_CONST, l, (int32_t)r.cmp_value};
+    return gdl_pred{r.cmp_op, l, r.cr == r.hv[0] ? 0 : 1};
+}
 
 std::vector<std::string> shared_vars(const std::string cur[2], const Atom &a) {
     std::vector<std::string> s;
@@ -585,8 +626,7 @@ void plan_chain(const Rule &r, int s, RelationVersion ver, std::vector<AOp> *ops
         if (step == k - 1) {
             op.head = r.head;
             op.map = {r.head_arity, {src_of(r.hv[0]), src_of(r.hv[1])}};
-            if (r.cmp_op != GDL_PRED_NONE)
-                op.pred = {r.cmp_op, r.cl == r.hv[0] ? 0 : 1, r.cr == r.hv[0] ? 0 : 1};
+            op.pred = rule_pred(r);
         } else {
             std::set<std::string> need(r.hv, r.hv + 2);
             for (int x = 0; x < k; ++x)
@@ -615,6 +655,7 @@ void plan_chain(const Rule &r, int s, RelationVersion ver, std::vector<AOp> *ops
 AOp plan_copy(const Rule &r, RelationVersion ver) {
     AOp op;
     op.kind = 1;
+    op.pred = rule_pred(r);
     op.src_rel = r.body[0].rel;
     op.src_ver = ver;
     op.head = r.head;
@@ -810,7 +851,7 @@ void DatalogProgram::Impl::lower() {
                         op.map = swap_outputs(op.map);
                         if (op.pred.op != GDL_PRED_NONE) {
                             op.pred.lhs = 1 - op.pred.lhs;
-                            op.pred.rhs = 1 - op.pred.rhs;
+                            if (!(op.pred.op & GDL_PRED_CONST)) op.pred.rhs = 1 - op.pred.rhs;
                         }
                     }
                 }
@@ -876,7 +917,10 @@ void DatalogProgram::Impl::describe() {
             os << " -> " << phys[op.dst].name << " (" << col_name(op.map.src[0]);
             if (op.map.arity == 2) os << ", " << col_name(op.map.src[1]);
             os << ")";
-            if (op.pred.op != GDL_PRED_NONE)
+            if (op.pred.op & GDL_PRED_CONST)
+                os << " if c" << op.pred.lhs << " " << pred_name(op.pred.op & ~GDL_PRED_CONST) << " "
+                   << (uint32_t)op.pred.rhs;
+            else if (op.pred.op != GDL_PRED_NONE)
                 os << " if c" << op.pred.lhs << " " << pred_name(op.pred.op) << " c" << op.pred.rhs;
             os << "\n";
         }
@@ -1012,7 +1056,7 @@ void DatalogProgram::Impl::run(const std::map<std::string, std::vector<uint64_t>
                 lie.add_ra(RelationalJoin(R.v[op.inner], FULL, R.v[op.src], op.src_ver, R.v[op.dst],
                                           op.map, op.pred));
             else if (op.kind == 1)
-                lie.add_ra(RelationalCopy(R.v[op.src], op.src_ver, R.v[op.dst], op.map));
+                lie.add_ra(RelationalCopy(R.v[op.src], op.src_ver, R.v[op.dst], op.map, op.pred));
             else
                 lie.add_ra(RelationalACopy(R.v[op.src], R.v[op.dst], op.map));
         }

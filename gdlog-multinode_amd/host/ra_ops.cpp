@@ -208,9 +208,23 @@ void RelationalCopy::operator()() {
     if (!copy_map.ok)
         throw Error(GDL_ERR_UNSUPPORTED, "RelationalCopy into " + dest_rel->name +
                                              ": the copy hook is not a column selection");
-    const tuple_size_t n = src->tuple_counts;
+    tuple_size_t n = src->tuple_counts;
     uint64_t *copied = device_alloc_keys(n);
     check(gdl_project_u64(context(), src->keys, n, copy_map.map, copied), "gdl_project_u64");
+    if (filter.op != GDL_PRED_NONE) {
+        uint64_t *kept = device_alloc_keys(n);
+        uint64_t m = 0;
+        gdl_status st = gdl_filter_u64(context(), copied, n, filter, kept, &m);
+        device_free(copied);
+        if (st != GDL_OK) device_free(kept);
+        check(st, "gdl_filter_u64");
+        copied = kept;
+        n = m;
+        if (n == 0) {
+            device_free(copied);
+            return;
+        }
+    }
     if (dest_rel->append_mode) append_raw(dest_rel->newt, copied, n);
     else accumulate_sorted(dest_rel->newt, copied, n, nullptr);
 }

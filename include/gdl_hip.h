@@ -221,7 +221,10 @@ typedef struct gdl_colmap {
 
 enum { GDL_PRED_NONE = 0, GDL_PRED_EQ, GDL_PRED_NE, GDL_PRED_LT, GDL_PRED_LE, GDL_PRED_GT,
        GDL_PRED_GE, GDL_PRED_FALSE };
-/* Keeps output tuple t iff t[lhs] <op> t[rhs] (columns of the generated tuple). */
+/* Flag on a comparison op: rhs is a constant id (its 32 bits), not a column. */
+#define GDL_PRED_CONST 0x100
+/* Keeps output tuple t iff t[lhs] <op> t[rhs] (columns of the generated tuple;
+ * with op | GDL_PRED_CONST, iff t[lhs] <op> (uint32_t)rhs). */
 typedef struct gdl_pred {
     int32_t op;
     int32_t lhs;

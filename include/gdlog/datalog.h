@@ -20,8 +20,8 @@
 //     variable bound by the body; no variable twice in one atom; each atom
 //     after the first shares exactly one variable with what is joined so
 //     far, and at most two variables are live between two joins; at most one
-//     comparison (= != < <= > >=) between two head variables, in rules of two
-//     or more atoms.  Line and block comments.
+//     comparison (= != < <= > >=) of a head variable with another or with an
+//     id.  Line and block comments.
 // Constants in heads, negation, aggregates, functors, other arities and other
 // directives are rejected (GDL_ERR_UNSUPPORTED), syntax errors and unknown
 // relations with GDL_ERR_INVALID_ARG; the message names the line.  A tuple of

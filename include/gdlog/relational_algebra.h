@@ -100,6 +100,10 @@ struct RelationalCopy {
     // destination's newt count (copy.cu:23-26), dropping what earlier
     // operators put there in that iteration.  Off in the column-map form.
     bool empty_source_zeroes_dest = true;
+    // Column-map form only: copied tuples must pass this comparison (over the
+    // destination's columns); the hook form ignores predicates, as the
+    // reference's copy does.
+    gdl_pred filter{GDL_PRED_NONE, 0, 0};
 
     RelationalCopy(Relation *src, RelationVersion src_ver, Relation *dest,
                    tuple_copy_hook tuple_generator, tuple_predicate tuple_pred, int grid_size,
@@ -109,10 +113,13 @@ struct RelationalCopy {
         copy_map = gdlog::probe_copy(tuple_generator, dest ? dest->arity : 0);
     }
 
-    // Column-map form (source columns as GDL_COL_INNER0 / GDL_COL_INNER1).
-    RelationalCopy(Relation *src, RelationVersion src_ver, Relation *dest, gdl_colmap map)
+    // Column-map form (source columns as GDL_COL_INNER0 / GDL_COL_INNER1),
+    // with an optional filter on the copied tuples.
+    RelationalCopy(Relation *src, RelationVersion src_ver, Relation *dest, gdl_colmap map,
+                   gdl_pred filter = gdl_pred{GDL_PRED_NONE, 0, 0})
         : src_rel(src), src_ver(src_ver), dest_rel(dest), tuple_generator(nullptr),
-          tuple_pred(nullptr), grid_size(0), block_size(0), empty_source_zeroes_dest(false) {
+          tuple_pred(nullptr), grid_size(0), block_size(0), empty_source_zeroes_dest(false),
+          filter(filter) {
         copy_map.ok = true;
         copy_map.map = map;
     }

@@ -74,7 +74,8 @@ def _fire(rule, sources):
             return set()
     out = set()
     for e in envs:
-        if all(_CMP[op](e[l], e[r]) for l, op, r in cmps):
+        val = lambda v: int(v) if v.isdigit() else e[v]  # noqa: E731 (an id or a variable)
+        if all(_CMP[op](val(l), val(r)) for l, op, r in cmps):
             out.add(tuple(e[v] for v in hv))
     return out
 

@@ -146,6 +146,24 @@ hop4(x, y) :- edge(x, a), edge(a, b), edge(b, c), edge(c, y).
 """
 
 
+# comparisons with ids and in one-atom rules (a copy's filter); `far`
+# runs in the linear loop with its comparison against an id
+COMPARE = """
+.decl edge(a: number, b: number)
+.input edge
+.decl low(a: number, b: number)
+.decl loop(a: number, b: number)
+.decl far(a: number, b: number)
+.decl mid(a: number)
+.output low, loop, far, mid
+low(x, y) :- edge(x, y), y < 30.
+loop(x, y) :- edge(x, y), x = y.
+far(x, y) :- edge(x, y).
+far(x, y) :- far(x, z), edge(z, y), 20 <= y.
+mid(y) :- edge(x, y), far(y, z), 10 > y.
+"""
+
+
 def uniq(keys):
     return np.unique(np.asarray(keys, dtype=np.uint64))
 
@@ -308,9 +326,7 @@ BAD = [
     (".decl r(a: number, b: number)\nr(x, y) :- r(x, z)", gdl.GDL_ERR_INVALID_ARG, "line 2"),
     (".decl r(a: number, b: number)\nr(x, y) :- r(x, z).", gdl.GDL_ERR_INVALID_ARG,
      "head variable y is not bound"),
-    (".decl r(a: number, b: number)\nr(x, y) :- r(x, y), r(y, z), x != 3.", 4,
-     "constants in comparisons"),
-    (".decl r(a: number, b: number)\nr(x, y) :- r(x, y), x != y.", 4, "one-atom rule"),
+    (".decl r(a: number, b: number)\nr(x, y) :- r(x, y), 2 != 3.", 4, "a comparison of two ids"),
     (".decl r(a: number, b: number)\nr(x, y) :- r(x, z), r(z, y), z != y.", 4,
      "not a head variable"),
     (".decl r(a: number, b: number)\nr(x, y) :- r(x, a), r(a, b), r(b, c), r(c, d), r(d, y).",
@@ -397,6 +413,24 @@ def test_reach_program_matches_the_naive_evaluator(ctx, seed):
     for rel in want:
         assert np.array_equal(got[rel], want[rel]), rel
     assert got["reach"].size > 2
+
+
+def test_comparison_plans():
+    plan = gdl.Datalog(COMPARE).plan
+    assert "copy edge_2__1_2 FULL -> low_2__1_2 (i0, i1) if c1 < 30" in plan
+    assert "copy edge_2__1_2 FULL -> loop_2__1_2 (i0, i1) if c0 = c1" in plan
+    assert "-> far_2__2_1 (i1, o1) if c0 >= 20" in plan  # y <= ... flipped, far stored reversed
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", [3, 4])
+def test_comparison_program_matches_the_naive_evaluator(ctx, seed):
+    e = uniq(np.concatenate([gdl.gen_er(seed, 50, 140), gdl.pack(np.arange(5), np.arange(5))]))
+    want = evaluate(COMPARE, {"edge": e})
+    got = gdl.Datalog(COMPARE).run(ctx, edge=e)
+    for rel in want:
+        assert np.array_equal(got[rel], want[rel]), rel
+    assert all(got[r].size > 0 for r in want)
 
 
 @pytest.mark.gpu
