@@ -595,3 +595,31 @@ def test_the_reference_souffle_programs_compile_as_written():
         with pytest.raises(gdl.GdlError) as err:
             gdl.Datalog(prog(name))
         assert err.value.status == 4, name
+
+
+# ---- the reference's data files through the front end ------------------------------
+
+def read_dl(name):
+    path = os.path.join(os.path.dirname(gdl.__file__), "programs", "dl", name)
+    with open(path) as f:
+        return f.read()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fname,size", [("data_48232.txt", 14_503_742),
+                                        ("data_26013.txt", 62_056_583)])
+def test_sg_dl_sizes_match_cudf(ctx, fname, size):
+    # the sizes cuDF published for these graphs (test/cuDF/sg.json: cti, p2p-Gnutella09),
+    # an implementation independent of the reference's engine
+    got = gdl.Datalog(read_dl("sg.dl")).run(ctx, edge=read_edges(fname))
+    assert got["sg"].size == size
+    assert np.all(got["sg"][1:] > got["sg"][:-1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fname", ["data_5.txt", "hpc_talk.txt", "data_7035.txt",
+                                   "data_21693.txt", "data_23874.txt"])
+def test_tc_dl_on_the_data_files_matches_the_oracle(ctx, fname):
+    e = uniq(read_edges(fname))
+    got = gdl.Datalog(read_dl("tc.dl")).run(ctx, edge=e)
+    assert np.array_equal(got["path"], oracle.tc(e)["out"])
