@@ -824,7 +824,7 @@ void DatalogProgram::Impl::lower() {
             };
             std::vector<int> tmp_phys(ntmps);
             for (int i = 0; i < ntmps; ++i) {
-                tmp_phys[i] = new_phys("tmp_" + std::to_string(steps.size()) + "_" + std::to_string(i),
+                tmp_phys[i] = new_phys("#tmp_" + std::to_string(steps.size()) + "_" + std::to_string(i),
                                        -1, false, kTmp);
                 lie.tmps.push_back(tmp_phys[i]);
             }
