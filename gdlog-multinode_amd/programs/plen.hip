@@ -9,9 +9,11 @@
 // delta of iteration i holds len i + 1), so len = 1 + the append-log segment
 // of the pair: the loaded edges, then each iteration's delta.  The program
 // runs the TC encoding with FULL kept as the append log
-// (LIE::keep_full_log), labels the log by segment on the device, sorts the
-// keys and carries each label to its sorted position (include/gdl_hip.h,
-// gdl_segment_labels_u32 / gdl_scatter_by_key_u32).
+// (LIE::keep_full_log), labels the log by segment on the device and sorts it
+// with the labels carried in the ids' spare low bits
+// (gdl_segment_labels_sorted), or, when the ids leave too few bits, sorts the
+// keys and carries each label to its sorted position by binary search
+// (gdl_segment_labels_u32 / gdl_scatter_by_key_u32).
 #include "gdlog/exception.h"
 #include "programs.h"
 
@@ -73,20 +75,25 @@ void path_length_program(const EdgeInput &graph, int grid_size, int block_size,
         check(gdl_memcpy_h2d(ctx, d_bounds, bounds.data(), bounds.size() * sizeof(uint64_t)),
               "segment bounds");
         if (n) {
-            labels = reinterpret_cast<uint32_t *>(device_alloc_keys((n + 1) / 2));
+            const uint32_t n_seg = (uint32_t)(bounds.size() - 1);
             sorted_labels = reinterpret_cast<uint32_t *>(device_alloc_keys((n + 1) / 2));
             sorted = device_alloc_keys(n);
-            check(gdl_segment_labels_u32(ctx, d_bounds, (uint32_t)(bounds.size() - 1), 1, labels,
-                                         n),
-                  "gdl_segment_labels_u32");
-            if (full->sorted) {
-                check(gdl_memcpy_d2d(ctx, sorted, full->keys, n * sizeof(uint64_t)), "copy FULL");
-            } else {
+            // the labels ride through the sort in the ids' spare bits when
+            // there are enough; otherwise label, sort, and binary-search each
+            // key's sorted position
+            gdl_status st = gdl_segment_labels_sorted(ctx, full->keys, n, d_bounds, n_seg, 1,
+                                                      id_bits(), sorted, sorted_labels);
+            if (st == GDL_ERR_UNSUPPORTED) {
+                labels = reinterpret_cast<uint32_t *>(device_alloc_keys((n + 1) / 2));
+                check(gdl_segment_labels_u32(ctx, d_bounds, n_seg, 1, labels, n),
+                      "gdl_segment_labels_u32");
                 const int b = id_bits();
                 check(gdl_sort_u64(ctx, full->keys, n, b, b, sorted), "gdl_sort_u64");
+                check(gdl_scatter_by_key_u32(ctx, sorted, n, full->keys, labels, n, sorted_labels),
+                      "gdl_scatter_by_key_u32");
+            } else {
+                check(st, "gdl_segment_labels_sorted");
             }
-            check(gdl_scatter_by_key_u32(ctx, sorted, n, full->keys, labels, n, sorted_labels),
-                  "gdl_scatter_by_key_u32");
             if (lens) {
                 host_lens.resize(n);
                 check(gdl_memcpy_d2h(ctx, host_lens.data(), sorted_labels, n * sizeof(uint32_t)),

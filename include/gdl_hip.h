@@ -335,6 +335,14 @@ gdl_status gdl_segment_labels_u32(gdl_ctx *ctx, const uint64_t *d_bounds, uint32
 gdl_status gdl_scatter_by_key_u32(gdl_ctx *ctx, const uint64_t *d_sorted, uint64_t n_sorted,
                                   const uint64_t *d_keys, const uint32_t *d_vals, uint64_t n,
                                   uint32_t *d_out);
+/* Both steps at once for a log of distinct keys whose ids have at most
+ * id_bits bits: d_keys gets the log sorted, d_labels[p] the segment label of
+ * d_keys[p] (first + segment), the label carried through the sort in the low
+ * bits of the key.  GDL_ERR_UNSUPPORTED when id_bits plus the label's bit
+ * length exceed 32 (then use the two calls above). */
+gdl_status gdl_segment_labels_sorted(gdl_ctx *ctx, const uint64_t *d_log, uint64_t n,
+                                     const uint64_t *d_bounds, uint32_t n_seg, uint32_t first,
+                                     int id_bits, uint64_t *d_keys, uint32_t *d_labels);
 
 /* ---- projection / filter ----------------------------------------------
  * gdl_project_u64 replaces get_copy_result (relation.cu:390-403);

@@ -59,6 +59,8 @@ INPUTS = {
     "tree_5k": lambda: gdl.gen_tree(11, 5000, 100),
     "data_7035_dups": lambda: read_edges("data_7035.txt"),
     "rmat_10": lambda: gdl.gen_rmat(6, 10, 3000),
+    # ids near 2^31 leave no spare bits: the labels take the binary-search path
+    "chain_high_ids": lambda: gdl.pack(np.arange(60) + (1 << 31), np.arange(1, 61) + (1 << 31)),
 }
 
 
