@@ -623,3 +623,20 @@ def test_tc_dl_on_the_data_files_matches_the_oracle(ctx, fname):
     e = uniq(read_edges(fname))
     got = gdl.Datalog(read_dl("tc.dl")).run(ctx, edge=e)
     assert np.array_equal(got["path"], oracle.tc(e)["out"])
+
+
+@pytest.mark.gpu
+def test_raw_inputs_with_duplicate_rows_where_the_inputs_are_static(ctx):
+    # SG's edge and CSPA's assign / dereference are static relations, so the
+    # hand encodings' raw-row seeding of the first delta does not touch them:
+    # the front end gives the oracle's relations on the raw rows as well
+    e = read_edges("data_7035.txt")  # 6 duplicate rows
+    assert e.size > np.unique(e).size
+    assert np.array_equal(gdl.Datalog(SG).run(ctx, edge=e)["sg"], oracle.sg(e)["out"])
+    a, d = gdl.gen_cspa(5, 150, 1.1, 200, 100)
+    a, d = np.concatenate([a, a[:20]]), np.concatenate([d, d[:10]])  # repeated rows
+    got = gdl.Datalog(CSPA).run(ctx, assign=a, dereference=d)
+    want = oracle.cspa(a, d)
+    assert np.array_equal(got["ValueAlias"], want["out"])
+    assert np.array_equal(got["ValueFlow"], want["extra"][0])
+    assert np.array_equal(got["MemoryAlias"], want["extra"][1])
