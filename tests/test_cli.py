@@ -169,3 +169,21 @@ def test_dl_runs_a_slog_plan(tmp_path):
     want = oracle.tc(e)["out"]
     assert np.array_equal(read_tsv(tmp_path / "path.csv"), want)
     assert relation_lines(out) == {"path": (want.size, gdl.tuple_checksum(want))}
+
+
+def test_dl_cspa_checkpoint_then_resume(tmp_path):
+    # a plan with mirrors and tmp relations: each LIE's snapshot restores it
+    facts, ck = tmp_path / "facts", tmp_path / "ck"
+    facts.mkdir()
+    ck.mkdir()
+    a, d = gdl.gen_cspa(5, 150, 1.1, 200, 100)
+    write_pairs(facts / "assign.facts", a)
+    write_pairs(facts / "dereference.facts", d)
+    prog = os.path.join(DL_DIR, "cspa.dl")
+    first = run("DL", prog, "-F", facts, "--checkpoint", ck, "--every", 2)
+    assert sorted(os.listdir(ck)) == ["lie0.ckpt", "lie1.ckpt"]
+    again = run("DL", prog, "-F", facts, "--checkpoint", ck, "--resume")
+    want = oracle.cspa(a, d)
+    lines = relation_lines(again)
+    assert lines == relation_lines(first)
+    assert lines["ValueAlias"] == (want["out"].size, gdl.tuple_checksum(want["out"]))

@@ -640,3 +640,25 @@ def test_raw_inputs_with_duplicate_rows_where_the_inputs_are_static(ctx):
     assert np.array_equal(got["ValueAlias"], want["out"])
     assert np.array_equal(got["ValueFlow"], want["extra"][0])
     assert np.array_equal(got["MemoryAlias"], want["extra"][1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("stop", [1, 4])
+def test_a_stopped_datalog_run_resumes_to_the_same_relations(ctx, tmp_path, stop):
+    # the recursive CSPA LIE (mirrors, tmps) stops right after its snapshot at
+    # iteration `stop`; the next run resumes every LIE from its snapshot
+    a, d = gdl.gen_cspa(8, 400, 1.1, 600, 300)
+    a, d = uniq(a), uniq(d)
+    prog = gdl.Datalog(CSPA)
+    want = prog.run(ctx, assign=a, dereference=d)
+    try:
+        gdl.set_checkpoint(str(tmp_path), every=0, resume=False, stop_after=stop)
+        with pytest.raises(gdl.GdlError) as err:
+            prog.run(ctx, assign=a, dereference=d)
+        assert err.value.status == gdl.GDL_ERR_BUSY
+        gdl.set_checkpoint(str(tmp_path), every=0, resume=True, stop_after=-1)
+        got = prog.run(ctx, assign=a, dereference=d)
+    finally:
+        gdl.set_checkpoint(None)
+    for rel in want:
+        assert np.array_equal(got[rel], want[rel]), rel
