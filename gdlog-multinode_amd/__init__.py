@@ -236,6 +236,11 @@ def load():
            vp, c_uint64, _U64P, _U64P])
     _decl(eng, "gdl_key_histogram_u64", c_int, [vp, vp, c_uint64, c_int, c_uint32, vp])
     _decl(eng, "gdl_mask_columns_u64", c_int, [vp, vp, c_uint64, c_uint32, vp])
+    _decl(eng, "gdl_segment_labels_u32", c_int, [vp, vp, c_uint32, c_uint32, vp, c_uint64])
+    _decl(eng, "gdl_graph_begin", c_int, [vp])
+    _decl(eng, "gdl_graph_end", c_int, [vp, POINTER(vp)])
+    _decl(eng, "gdl_graph_launch", c_int, [vp, vp])
+    _decl(eng, "gdl_graph_destroy", c_int, [vp])
     _decl(eng, "gdl_tcb_config_default", None, [POINTER(TcbConfig)])
     _decl(eng, "gdl_tcb_create", c_int, [vp, POINTER(TcbConfig), POINTER(vp)])
     _decl(eng, "gdl_tcb_destroy", c_int, [vp])
@@ -614,6 +619,23 @@ class Context:
         r = c.numpy()
         d.free(); c.free()
         return r
+
+    # graph capture of the context's stream (include/gdl_hip.h): between
+    # graph_begin() and graph_end() only kernel-only entry points on
+    # preallocated device buffers may be called
+    def graph_begin(self):
+        self._check(self.eng.gdl_graph_begin(self.h), "gdl_graph_begin")
+
+    def graph_end(self):
+        g = c_void_p()
+        self._check(self.eng.gdl_graph_end(self.h, ctypes.byref(g)), "gdl_graph_end")
+        return g
+
+    def graph_launch(self, g):
+        self._check(self.eng.gdl_graph_launch(self.h, g), "gdl_graph_launch")
+
+    def graph_destroy(self, g):
+        self.eng.gdl_graph_destroy(g)
 
     def mask_columns(self, keys, col_mask):
         d = self.to_device(keys)

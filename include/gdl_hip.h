@@ -320,6 +320,24 @@ gdl_status gdl_checksum_u64(gdl_ctx *ctx, const uint64_t *d_keys, uint64_t n, ui
 gdl_status gdl_mask_columns_u64(gdl_ctx *ctx, const uint64_t *d_in, uint64_t n,
                                 uint32_t col_mask, uint64_t *d_out);
 
+/* ---- graph capture -----------------------------------------------------
+ * Records the context's stream into a HIP graph and replays it (the build's
+ * replacement for launching a fixed sequence of calls one by one; no
+ * counterpart in the reference, whose calls synchronize after every kernel).
+ * Between gdl_graph_begin and gdl_graph_end, only entry points that launch
+ * kernels on caller-owned buffers may be called: gdl_project_u64,
+ * gdl_mask_columns_u64, gdl_segment_labels_u32 (and async copies between
+ * device buffers).  Any call that synchronizes, reads a size back or
+ * allocates fails during the capture, and gdl_graph_end then reports the
+ * error.  Kernel statistics must be off.  A graph replays on the context it
+ * was captured on, ordered on its stream; the buffers it names must stay
+ * allocated. */
+typedef struct gdl_graph gdl_graph;
+gdl_status gdl_graph_begin(gdl_ctx *ctx);
+gdl_status gdl_graph_end(gdl_ctx *ctx, gdl_graph **out);
+gdl_status gdl_graph_launch(gdl_ctx *ctx, const gdl_graph *graph);
+gdl_status gdl_graph_destroy(gdl_graph *graph);
+
 /* ---- labels of an append log ------------------------------------------
  * No counterpart in the reference's code; they produce the dependent column
  * of its path_length program (test/path_length.cu:49-124: the hop count

@@ -288,3 +288,53 @@ def test_join_insert_is_join_then_set_difference(ctx, colmap, pred, inner_kind):
             seen |= want_new
     finally:
         ctx.hashset_free(hs)
+
+
+def test_graph_capture_replays_kernel_only_calls(ctx):
+    """A captured projection, column mask and segment labelling replayed three
+    times, with the outputs overwritten and other eager work in between; every
+    replay must equal the eager run."""
+    ctx.stats_enable(False)  # kernel timing events cannot be captured
+    keys = rand_keys(200_000)
+    n = keys.size
+    d_in = ctx.to_device(keys)
+    d_proj, d_mask = ctx.alloc(n), ctx.alloc(n)
+    d_lab = ctx.alloc(n, np.uint32)
+    bounds = np.array([0, 1000, 50_000, n], dtype=np.uint64)
+    d_b = ctx.to_device(bounds)
+    cm = gdl.ColMap(2, (gdl.COL_INNER1, gdl.COL_INNER0))
+
+    def calls():
+        ctx._check(ctx.eng.gdl_project_u64(ctx.h, d_in.ptr, n, cm, d_proj.ptr), "project")
+        ctx._check(ctx.eng.gdl_mask_columns_u64(ctx.h, d_proj.ptr, n, 1, d_mask.ptr), "mask")
+        ctx._check(ctx.eng.gdl_segment_labels_u32(ctx.h, d_b.ptr, 3, 7, d_lab.ptr, n), "labels")
+
+    calls()
+    ctx.sync()
+    want = (d_proj.numpy(), d_mask.numpy(), d_lab.numpy())
+    swapped = (keys << np.uint64(32)) | (keys >> np.uint64(32))
+    assert np.array_equal(want[0], swapped)
+    assert np.array_equal(want[2], np.repeat(np.array([7, 8, 9], dtype=np.uint32),
+                                             np.diff(bounds).astype(np.int64)))
+    ctx.graph_begin()
+    with pytest.raises(gdl.GdlError):
+        ctx.graph_begin()  # one capture at a time
+    calls()
+    g = ctx.graph_end()
+    try:
+        junk = np.full(n, 0xDEADBEEF, dtype=np.uint64)
+        for rep in range(3):
+            for d in (d_proj, d_mask):
+                ctx._check(ctx.eng.gdl_memcpy_h2d(ctx.h, d.ptr, junk.ctypes.data, junk.nbytes), "h2d")
+            ctx._check(ctx.eng.gdl_memcpy_h2d(ctx.h, d_lab.ptr, junk.ctypes.data, n * 4), "h2d")
+            assert np.array_equal(ctx.sort(keys[: 1000 * (rep + 1)]),
+                                  np.sort(keys[: 1000 * (rep + 1)]))  # eager work between
+            ctx.graph_launch(g)
+            ctx.sync()
+            got = (d_proj.numpy(), d_mask.numpy(), d_lab.numpy())
+            for a, b in zip(got, want):
+                assert np.array_equal(a, b), rep
+    finally:
+        ctx.graph_destroy(g)
+        for d in (d_in, d_proj, d_mask, d_lab, d_b):
+            d.free()
