@@ -156,6 +156,7 @@ struct DatalogProgram::Impl {
     std::vector<std::vector<uint64_t>> facts;  // ground atoms of the program, packed
     std::vector<char> is_input, is_kept;
     std::vector<char> hidden;  // made by the front end (constants), never kept
+    std::map<int, std::string> input_file;  // an .input's filename attribute
     bool any_kept = false;
     std::vector<Rule> rules;
     // plan
@@ -281,15 +282,23 @@ struct Parser {
         return (int)m.rel_names.size() - 1;
     }
 
-    void skip_parens() {
-        const int line = peek().line;
-        int depth = 0;
-        do {
-            if (peek().kind == Tok::End) fail(GDL_ERR_INVALID_ARG, line, "unbalanced parentheses");
-            if (is("(")) ++depth;
-            else if (is(")")) --depth;
+    // (key = value, ...) after an I/O directive's relation; values are
+    // identifiers, strings or numbers.  Only an .input's filename is used.
+    std::map<std::string, std::string> attributes() {
+        std::map<std::string, std::string> a;
+        expect("(");
+        while (!is(")")) {
+            const std::string key = ident("an attribute name");
+            expect("=");
+            const Tok &v = peek();
+            if (v.kind != Tok::Ident && v.kind != Tok::Str && v.kind != Tok::Num)
+                fail(GDL_ERR_INVALID_ARG, v.line, "expected an attribute value" + found());
+            a[key] = next().s;
+            if (!is(",")) break;
             next();
-        } while (depth > 0);
+        }
+        expect(")");
+        return a;
     }
 
     void io(const std::string &d) {
@@ -302,7 +311,11 @@ struct Parser {
                 m.is_kept[rel] = 1;
                 m.any_kept = true;
             }
-            if (is("(")) skip_parens();
+            if (is("(")) {
+                const std::map<std::string, std::string> a = attributes();
+                auto f = a.find("filename");
+                if (d == "input" && f != a.end()) m.input_file[rel] = f->second;
+            }
             if (!is(",")) break;
             next();
         }
@@ -1378,6 +1391,14 @@ const std::vector<uint64_t> *DatalogProgram::result(const std::string &relation)
 
 const std::vector<std::string> &DatalogProgram::kept() const { return impl_->kept; }
 const std::vector<std::string> &DatalogProgram::inputs() const { return impl_->input_names; }
+
+const std::string &DatalogProgram::input_file(const std::string &relation) const {
+    static const std::string none;
+    auto it = impl_->rel_id.find(relation);
+    if (it == impl_->rel_id.end()) return none;
+    auto f = impl_->input_file.find(it->second);
+    return f == impl_->input_file.end() ? none : f->second;
+}
 
 int DatalogProgram::arity(const std::string &relation) const {
     auto it = impl_->rel_id.find(relation);

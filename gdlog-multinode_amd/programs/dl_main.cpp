@@ -7,9 +7,10 @@
 //                   [--checkpoint DIR [--every N] [--resume]]
 //   A .slogc file is a Slog compiler's RA plan (DatalogProgram::compile_slog);
 //   its relation names become identifiers (value-flow -> value_flow).
-//   -F DIR            each .input relation R is read from DIR/R.facts (rows of
-//                     two ids separated by a tab, space or comma, or one id
-//                     for a unary relation; default .)
+//   -F DIR            each .input relation R is read from DIR/R.facts, or from
+//                     its filename attribute (relative to DIR unless absolute);
+//                     rows of two ids separated by a tab, space or comma, or
+//                     one id for a unary relation; default .
 //   -D DIR            each kept relation R is written to DIR/R.csv, sorted
 //                     tab-separated rows
 //   --plan            print the lowered plan before running (for a .slogc
@@ -95,7 +96,12 @@ int main(int argc, char *argv[]) {
     // each .input relation comes from its .facts file
     std::map<std::string, std::vector<uint64_t>> inputs;
     for (const std::string &rel : prog.inputs()) {
-        const std::string fn = facts + "/" + rel + ".facts";
+        // the .input's filename attribute (relative to -F unless absolute), else
+        // <relation>.facts, as Souffle resolves them
+        const std::string &attr = prog.input_file(rel);
+        const std::string fn = attr.empty() ? facts + "/" + rel + ".facts"
+                               : attr[0] == '/' ? attr
+                                                : facts + "/" + attr;
         const int ar = prog.arity(rel);
         const std::vector<column_type> rows =
             ar == 1 ? gdlog_cli::read_ids(fn) : gdlog_cli::read_pairs(fn);

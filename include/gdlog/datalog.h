@@ -7,8 +7,9 @@
 // Accepted (a subset of Souffle's syntax):
 //   .decl R(a: number, b: number)       unary and binary relations of
 //   .decl S(a: number)                  number / unsigned ids below 2^32
-//   .input R, S(IO=file, ...)           relations given to run() (attributes
-//                                       in parentheses are ignored)
+//   .input R, S(IO=file, ...)           relations given to run() (a filename
+//                                       attribute is kept for drivers; the
+//                                       other attributes are ignored)
 //   .output R   .printsize R            relations whose tuples run() keeps
 //                                       (every derived relation when a
 //                                       program names none)
@@ -86,6 +87,8 @@ class DatalogProgram {
     const std::vector<std::string> &inputs() const;
     // Declared arity of a relation (1 or 2; 0 if not declared).
     int arity(const std::string &relation) const;
+    // The filename attribute of a relation's .input directive ("" if none).
+    const std::string &input_file(const std::string &relation) const;
     // The lowered plan: one line per step, LIE and operator.
     const std::string &plan() const;
     // The Datalog text compiled (for compile_slog, the rules read back).

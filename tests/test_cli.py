@@ -187,3 +187,19 @@ def test_dl_cspa_checkpoint_then_resume(tmp_path):
     lines = relation_lines(again)
     assert lines == relation_lines(first)
     assert lines["ValueAlias"] == (want["out"].size, gdl.tuple_checksum(want["out"]))
+
+
+def test_dl_reads_the_input_filename_attribute(tmp_path):
+    # Souffle's .input filename attribute: relative to -F unless absolute
+    e = np.unique(gdl.gen_er(4, 300, 700))
+    (tmp_path / "sub").mkdir()
+    write_pairs(tmp_path / "sub" / "graph.tsv", e)
+    want = oracle.tc(e)["out"]
+    for name in ["sub/graph.tsv", str(tmp_path / "sub" / "graph.tsv")]:
+        prog = tmp_path / "tc.dl"
+        prog.write_text(".decl edge(a: number, b: number)\n"
+                        f'.input edge(IO=file, filename="{name}", deliminator="\\t")\n'
+                        ".decl path(a: number, b: number)\n.printsize path\n"
+                        "path(x, y) :- edge(x, y).\npath(x, y) :- path(x, z), edge(z, y).\n")
+        out = run("DL", prog, "-F", tmp_path)
+        assert relation_lines(out) == {"path": (want.size, gdl.tuple_checksum(want))}, name
