@@ -237,6 +237,8 @@ def load():
     _decl(eng, "gdl_key_histogram_u64", c_int, [vp, vp, c_uint64, c_int, c_uint32, vp])
     _decl(eng, "gdl_mask_columns_u64", c_int, [vp, vp, c_uint64, c_uint32, vp])
     _decl(eng, "gdl_segment_labels_u32", c_int, [vp, vp, c_uint32, c_uint32, vp, c_uint64])
+    _decl(eng, "gdl_segment_labels_sorted", c_int,
+          [vp, vp, c_uint64, vp, c_uint32, c_uint32, c_int, vp, vp])
     _decl(eng, "gdl_graph_begin", c_int, [vp])
     _decl(eng, "gdl_graph_end", c_int, [vp, POINTER(vp)])
     _decl(eng, "gdl_graph_launch", c_int, [vp, vp])

@@ -338,3 +338,41 @@ def test_graph_capture_replays_kernel_only_calls(ctx):
         ctx.graph_destroy(g)
         for d in (d_in, d_proj, d_mask, d_lab, d_b):
             d.free()
+
+
+@pytest.mark.parametrize("bits,n_seg,first", [(16, 1000, 1), (31, 1, 1), (20, 3, 5), (8, 40, 1)])
+def test_segment_labels_ride_through_the_sort(ctx, bits, n_seg, first):
+    """Distinct keys in log order, labelled by segment, come out sorted with
+    each key's label; ids of `bits` bits plus the label's bits fill at most 32."""
+    rng = np.random.default_rng(bits * 1000 + n_seg)
+    n = min(60_000, (1 << bits) * 4)
+    keys = np.unique(rand_keys(n, bits, bits, rng))
+    rng.shuffle(keys)
+    n = keys.size
+    cuts = np.sort(rng.choice(np.arange(1, n), size=n_seg - 1, replace=False)) if n_seg > 1 \
+        else np.zeros(0, dtype=np.int64)
+    bounds = np.concatenate([[0], cuts, [n]]).astype(np.uint64)
+    labels = (first + np.repeat(np.arange(n_seg), np.diff(bounds).astype(np.int64))).astype(np.uint32)
+    order = np.argsort(keys, kind="stable")
+    d_log, d_b = ctx.to_device(keys), ctx.to_device(bounds)
+    d_k, d_l = ctx.alloc(n), ctx.alloc(n, np.uint32)
+    try:
+        ctx._check(ctx.eng.gdl_segment_labels_sorted(ctx.h, d_log.ptr, n, d_b.ptr, n_seg, first,
+                                                      bits, d_k.ptr, d_l.ptr), "labels_sorted")
+        assert np.array_equal(d_k.numpy(), keys[order])
+        assert np.array_equal(d_l.numpy(), labels[order])
+    finally:
+        for d in (d_log, d_b, d_k, d_l):
+            d.free()
+
+
+def test_segment_labels_too_wide_for_the_ids_are_refused(ctx):
+    d = ctx.to_device(np.array([1, 2], dtype=np.uint64))
+    b = ctx.to_device(np.array([0, 1, 2], dtype=np.uint64))
+    o, lab = ctx.alloc(2), ctx.alloc(2, np.uint32)
+    try:  # 31-bit ids and labels up to 2: 2 label bits make 33
+        st = ctx.eng.gdl_segment_labels_sorted(ctx.h, d.ptr, 2, b.ptr, 2, 1, 31, o.ptr, lab.ptr)
+        assert st == 4
+    finally:
+        for x in (d, b, o, lab):
+            x.free()
