@@ -2,7 +2,9 @@
 
 ctypes wrapper of oracle/lib/libgdl_oracle.so, the plain-C restatement of the
 reference engine's semi-naive algorithm (see gdl_oracle.h for what each
-function restates and how it is pinned).  Only tests/, __graft_entry__.smoke()
+function restates and how it is pinned).  path_length() is a numpy
+restatement of test/path_length.cu's loop; tests/test_path_length.py pins it
+against breadth-first distances (scipy) and its pairs against tc().  Only tests/, __graft_entry__.smoke()
 and bench.py's cpu_baseline leg may use it, and only as the checker / the CPU
 baseline -- never as the product path.
 """
