@@ -26,7 +26,10 @@ Prints one JSON line.  Also reports:
                 hand-written RA encodings, whole runs from host inputs to
                 host results;
   path_length   the hop-count program (a dependent column) beside TC on the
-                same tree, whole runs.
+                same tree, whole runs;
+  micro         the primitives (radix sort, merge + set difference, set
+                difference, prefix index build, join without load) on
+                device-resident data, algorithmic GB/s against 8 TB/s.
 """
 import argparse
 import importlib.util
@@ -401,6 +404,94 @@ def path_length_vs_tc(gdl, ctx, reps):
     return out
 
 
+def micro(gdl, ctx, reps):
+    """The hot path's primitives on device-resident data at the headline's
+    sizes (the reference's micro-benchmark shapes: test/datastructure.cu index
+    build and join without load, test/merge.cu set difference and merge), each
+    with its algorithmic bytes (w = 8 B per tuple) against the 8 TB/s peak."""
+    import ctypes
+    import numpy as np
+    rng = np.random.default_rng(0)
+    U = np.uint64
+    eng, h = ctx.eng, ctx.h
+
+    def rand(n, bits):
+        return np.unique((rng.integers(0, 1 << bits, n, dtype=U) << U(32)) |
+                         rng.integers(0, 1 << bits, n, dtype=U))
+
+    def timed(f):
+        f()
+        ctx.sync()
+        t = time.perf_counter()
+        for _ in range(reps):
+            f()
+        ctx.sync()
+        return (time.perf_counter() - t) / reps
+
+    def entry(sec, nbytes, **kw):
+        gbs = nbytes / sec / 1e9
+        return dict(ms=round(sec * 1e3, 3), alg_bytes=int(nbytes), gbs=round(gbs, 1),
+                    frac=round(gbs / HBM_PEAK_GBS, 4), **kw)
+
+    out = {}
+    # radix sort: 40.8M keys over 18-bit ids (the tree's output sort)
+    keys = rand(41_000_000, 18)
+    rng.shuffle(keys)
+    n = keys.size
+    d, o = ctx.to_device(keys), ctx.alloc(n)
+    sec = timed(lambda: ctx._check(eng.gdl_sort_u64(h, d.ptr, n, 18, 18, o.ptr), "sort"))
+    out["sort_u64"] = entry(sec, 16 * n, keys=n)
+    d.free()
+    # fused merge + set difference: FULL 40M sorted, newt 2M sorted (half new)
+    full = o  # sorted above
+    newt = np.unique(np.concatenate([keys[:1_000_000], rand(1_000_000, 18)]))
+    dn = ctx.to_device(newt)
+    u, dl = ctx.alloc(n + newt.size), ctx.alloc(newt.size)
+    nu, nd = ctypes.c_uint64(), ctypes.c_uint64()
+    sec = timed(lambda: ctx._check(eng.gdl_merge_diff_u64(
+        h, full.ptr, n, dn.ptr, newt.size, u.ptr, ctypes.byref(nu), dl.ptr, ctypes.byref(nd)),
+        "merge_diff"))
+    out["merge_diff_u64"] = entry(sec, 8 * (n + newt.size + nu.value + nd.value), full=n,
+                                  newt=int(newt.size), new=int(nd.value))
+    sec = timed(lambda: ctx._check(eng.gdl_set_difference_u64(
+        h, dn.ptr, newt.size, full.ptr, n, dl.ptr, ctypes.byref(nd)), "set_difference"))
+    out["set_difference_u64"] = entry(sec, 8 * (n + newt.size + nd.value), full=n,
+                                      newt=int(newt.size))
+    for x in (u, dl):
+        x.free()
+    # prefix index over 40M sorted keys (one-column prefixes), then a join of 2M
+    # outer tuples against it with the result dropped (disable_load)
+    idx = gdl.Index()
+
+    def build():
+        if idx.slots:
+            ctx.index_free(idx)
+        ctx._check(eng.gdl_index_build(h, full.ptr, n, 1, ctypes.byref(idx)), "index_build")
+    sec = timed(build)
+    # read the keys; write one run word per slot (8 B dense, 16 B hash)
+    out["index_build"] = entry(sec, 8 * n + (8 if idx.dense else 16) * idx.n_slots, keys=n,
+                               dense=bool(idx.dense))
+    outer = rng.integers(0, 1 << 18, 2_000_000, dtype=U) << U(32)
+    do = ctx.to_device(outer)
+    cm = gdl.ColMap(2, (gdl.COL_OUTER0, gdl.COL_INNER1))
+    m = ctypes.c_uint64()
+
+    def join():
+        res = ctypes.c_void_p()
+        ctx._check(eng.gdl_join(h, full.ptr, n, ctypes.byref(idx), do.ptr, do.n, cm,
+                                gdl.Pred(gdl.PRED_NONE, 0, 1), ctypes.byref(res),
+                                ctypes.byref(m)), "join")
+        if res.value:
+            eng.gdl_free(h, res.value)
+    sec = timed(join)
+    out["join"] = entry(sec, 8 * (do.n + 2 * m.value), outer=int(do.n), matches=int(m.value))
+    ctx.index_free(idx)
+    for x in (do, dn, full):
+        x.free()
+    print(f"bench: micro {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def workloads(gdl, nodes, window):
     """name -> (kind, make_input, description)."""
     return {
@@ -436,6 +527,8 @@ def main():
     ap.add_argument("--datalog-reps", type=int, default=3,
                     help="timed runs of the Datalog front-end and path-length "
                          "comparisons (0: skip them)")
+    ap.add_argument("--micro-reps", type=int, default=10,
+                    help="timed repeats of each primitive micro-benchmark (0: skip them)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device copy (traffic-counter calibration)")
     ap.add_argument("--phase", choices=["batched", "served"],
@@ -539,6 +632,7 @@ def main():
     gdl.set_full_mode(gdl.FULL_MODE_AUTO)
     dlog = datalog_vs_hand(gdl, ctx, args.datalog_reps) if args.datalog_reps > 0 else None
     plen = path_length_vs_tc(gdl, ctx, args.datalog_reps) if args.datalog_reps > 0 else None
+    micro_out = micro(gdl, ctx, args.micro_reps) if args.micro_reps > 0 else None
     # the batched and served measurements each run in a process of their own
     # (as a batch or a server would): device memory that one long run of
     # allocations and frees gave back comes back to the next one much more
@@ -581,6 +675,7 @@ def main():
         "served": serve,
         "datalog": dlog,
         "path_length": plen,
+        "micro": micro_out,
     }
     ctx.close()
     print(json.dumps(out), flush=True)
