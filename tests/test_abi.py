@@ -49,6 +49,25 @@ def test_null_arguments_are_rejected_without_a_device():
     assert eng.gdl_sort_u64(None, None, 0, 32, 32, None) == 1
     res = gdl.ProgResult()
     assert prog.gdl_prog_tc(None, None, 0, 0, -1, ctypes.byref(res), None) == 1
+    assert prog.gdl_prog_path_length(None, None, 0, 0, ctypes.byref(res), None, None, 0) == 1
+    assert eng.gdl_segment_labels_u32(None, None, 1, 0, None, 0) == 1
+    assert eng.gdl_segment_labels_sorted(None, None, 0, None, 1, 0, 8, None, None) == 1
+    assert eng.gdl_graph_begin(None) == 1
+    assert eng.gdl_graph_end(None, None) == 1
+    assert eng.gdl_graph_launch(None, None) == 1
+    assert eng.gdl_graph_destroy(None) == 0  # destroying nothing is fine
+    # the Datalog front end compiles without a device; running needs a context
+    p = ctypes.c_void_p()
+    assert prog.gdl_datalog_compile(None, ctypes.byref(p)) == 1
+    assert prog.gdl_datalog_compile(b".decl r(a: number, b: number)", ctypes.byref(p)) == 0
+    try:
+        assert prog.gdl_datalog_run(None, p, 0, None, None, None) == 1
+        n = ctypes.c_uint64()
+        assert prog.gdl_datalog_result(p, b"r", None, 0, ctypes.byref(n)) == 1  # not run yet
+        assert prog.gdl_datalog_kept(p, 0) == b"r" and prog.gdl_datalog_kept(p, 1) is None
+        assert prog.gdl_datalog_arity(p, b"r") == 2 and prog.gdl_datalog_arity(p, b"x") == 0
+    finally:
+        prog.gdl_datalog_destroy(p)
 
 
 def test_generator_library_is_deterministic():
