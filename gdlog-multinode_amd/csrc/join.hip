@@ -222,10 +222,11 @@ __device__ __forceinline__ void block_final_flush(u64 (*s_stage)[kStage], u32 st
         }
         s_base = tot ? atomicAdd(&counters[0], (unsigned long long)tot) : 0ull;
         if (prod) atomicAdd(&counters[1], (unsigned long long)prod);
-        // both counter updates complete (acknowledged at the device's
-        // coherence point) before this thread goes on: a caller may count
-        // finished blocks after this function and read the counters then
-        __builtin_amdgcn_s_waitcnt(0);
+        // release: both counter updates are ordered (device scope) before
+        // whatever this thread does next, so a caller that counts finished
+        // blocks after this function and reads the counters after an acquire
+        // fence sees them
+        __threadfence();
     }
     __syncthreads();
     u64 off = s_base;
@@ -366,8 +367,9 @@ enum : u32 { kLinRun = 0, kLinFixpoint = 1, kLinGrow = 2, kLinLimit = 3, kLinOve
 
 // End of a body (thread 0 of the last block to finish): record its counts,
 // move the delta window to the tuples it appended, and decide whether the
-// next enqueued body runs.  Every block's cursor / produced updates are
-// complete before it counts itself finished, so the atomic reads below see
+// next enqueued body runs.  Every block releases its cursor / produced
+// updates (a device-scope fence) before it counts itself finished, and the
+// last block fences again before reading them, so the atomic reads below see
 // the final values; the plain stores reach the next body at its launch.
 __device__ void linear_epilogue(LinState *st, u64 *counts, u64 d1) {
     const u64 end = atomicAdd(&st->cursor, 0ull);
@@ -411,7 +413,10 @@ __global__ __launch_bounds__(B) void k_linear_body(const IdxSlot *slots, u64 n_s
     // (thread 0, before a barrier)
     if (threadIdx.x == 0) s_last = atomicAdd(&st->blocks_done, 1u) == gridDim.x - 1;
     __syncthreads();
-    if (s_last && threadIdx.x == 0) linear_epilogue(st, counts, d1);
+    if (s_last && threadIdx.x == 0) {
+        __threadfence();  // acquire: every block's counter updates precede its blocks_done add
+        linear_epilogue(st, counts, d1);
+    }
 }
 
 // Load-balanced variant for inner relations with long runs: outputs are

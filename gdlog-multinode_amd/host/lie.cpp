@@ -65,6 +65,31 @@ bool output_fusable(const std::vector<ra_op> &ops, const Relation *rel) {
     return true;
 }
 
+// A snapshot written in merge mode holds FULL sorted, with the delta inside
+// it at sorted positions; the append log needs the delta as its tail (the
+// linear loop reads the delta there).  FULL := (FULL \ delta) then delta.
+// A log kept for its segment order (keep_full_log) cannot be rebuilt from a
+// sorted FULL.
+void log_with_delta_tail(Relation *rel, bool keep_log) {
+    if (keep_log)
+        throw Error(GDL_ERR_UNSUPPORTED, "resume of " + rel->name +
+                                             ": the snapshot's FULL is sorted (merge mode), "
+                                             "not the append log this LIE keeps");
+    container_normalize(rel->delta);
+    const tuple_size_t nf = rel->current_full_size, nd = rel->delta->tuple_counts;
+    uint64_t *log = device_alloc_keys(nf + nd);
+    uint64_t rest = 0;
+    gdl_status st = gdl_set_difference_u64(context(), rel->full->keys, nf, rel->delta->keys, nd,
+                                           log, &rest);
+    if (st == GDL_OK && rest + nd != nf)
+        st = GDL_ERR_INVALID_ARG;  // the delta must be part of FULL
+    if (st == GDL_OK)
+        st = gdl_memcpy_d2d(context(), log + rest, rel->delta->keys, nd * sizeof(uint64_t));
+    if (st != GDL_OK) device_free(log);
+    check(st, "resume: append log from a sorted FULL");
+    container_take(rel->full, log, nf, nf + nd, false);
+}
+
 void enter_append_mode(Relation *rel) {
     gdl_ctx *ctx = context();
     const tuple_size_t n = rel->current_full_size;
@@ -320,12 +345,16 @@ void LIE::fixpoint_loop() {
             if (auto *c = std::get_if<RelationalCopy>(&op))
                 if (c->src_ver == FULL) c->copied = true;
     }
+    // keep_full_log needs the append log whatever the thread's FULL mode
+    const bool auto_mode = keep_full_log || full_mode() == FullMode::Auto;
     for (Relation *rel : update_relations) {
         if (!resumed) {
             rel->current_full_size = rel->full->tuple_counts;
             seed_delta(rel);
         }
-        if (full_mode() == FullMode::Auto && !full_is_inner(ra_ops, rel)) {
+        if (auto_mode && !full_is_inner(ra_ops, rel)) {
+            if (resumed && rel->full->sorted && rel->delta->keys && rel->delta->tuple_counts)
+                log_with_delta_tail(rel, keep_full_log);
             enter_append_mode(rel);
             rel->fused_output = output_fusable(ra_ops, rel);
         } else if (resumed && !rel->full->sorted) {

@@ -153,9 +153,18 @@ bool RelationalJoin::run_linear(int max_bodies, std::vector<uint64_t> *new_tuple
     if (!inner->keys || inner->tuple_counts == 0) return false;  // the quirk path
     const int k = (int)delta->index_column_size;
     if (k < 1 || k > inner->arity || k > delta->arity) return false;
-    // the delta is the log's tail (the fused output appends it to FULL)
+    // the delta is the log's tail (the fused output appends it to FULL); a
+    // log that does not end with the delta (a FULL restored some other way)
+    // leaves the bodies to the per-iteration path
     const tuple_size_t nd = delta->tuple_counts, nf = out->current_full_size;
-    if (nd == 0 || nd > nf || !full->keys) return false;
+    if (nd == 0 || nd > nf || !full->keys || !delta->keys) return false;
+    {
+        uint64_t sum_delta = 0, sum_tail = 0;
+        check(gdl_checksum_u64(context(), delta->keys, nd, &sum_delta), "linear delta checksum");
+        check(gdl_checksum_u64(context(), full->keys + (nf - nd), nd, &sum_tail),
+              "linear log checksum");
+        if (sum_delta != sum_tail) return false;
+    }
     const gdl_index *idx = container_index(inner, k);
     if (idx->max_run > 256) return false;  // hub runs: the per-body path balances them
     uint64_t *log = full->keys;

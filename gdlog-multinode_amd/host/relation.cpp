@@ -63,7 +63,9 @@ void set_phase_timing(bool on) { g_phase_timing = on; }
 bool phase_timing() { return g_phase_timing; }
 
 namespace {
-FullMode g_full_mode = [] {
+// Per thread, like the context: a program on one thread never switches the
+// mode of programs running on others.
+thread_local FullMode g_full_mode = [] {
     const char *e = std::getenv("GDLOG_FULL_MODE");
     return (e && std::string(e) == "merge") ? FullMode::Merge : FullMode::Auto;
 }();

@@ -213,27 +213,12 @@ struct gdl_server {
         cv_done.notify_all();
     }
 
-    // Executor e of n gets CU i for every i with i mod n == e.
-    gdl_status partition(gdl_ctx *ctx, int e) {
-        const int n = cfg.workers + (cfg.batch_tc ? 1 : 0);
-        if (!cfg.partition_cus || n < 2) return GDL_OK;
-        int cus = 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, cfg.device) !=
-                hipSuccess ||
-            cus < n)
-            return GDL_OK;  // too few CUs to split: share them
-        std::vector<uint32_t> mask((cus + 31) / 32, 0u);
-        for (int i = e; i < cus; i += n) mask[i / 32] |= 1u << (i % 32);
-        return gdl_set_cu_mask(ctx, mask.data(), (uint32_t)mask.size());
-    }
-
     void worker_main(int e) {
         gdl_config c;
         gdl_config_default(&c);
         c.device = cfg.device;
         gdl_ctx *ctx = nullptr;
         gdl_status st = gdl_ctx_create(&c, &ctx);
-        if (st == GDL_OK) st = partition(ctx, e);
         if (st == GDL_OK) {
             // keep freed blocks for the next query, up to this worker's share
             // of the budget: a block returned to HIP synchronises the whole
@@ -314,7 +299,6 @@ struct gdl_server {
         gdl_ctx *ctx = nullptr;
         gdl_tcb *b = nullptr;
         gdl_status st = gdl_ctx_create(&c, &ctx);
-        if (st == GDL_OK) st = partition(ctx, cfg.workers);
         if (st == GDL_OK) {
             // the same cache limit as a worker: instances of every size come
             // and go, and an unbounded cache of their freed sets fills the
@@ -459,6 +443,7 @@ gdl_status gdl_server_create(const gdl_server_config *cfg_in, gdl_server **out) 
     if (cfg.workers < 1 || cfg.workers > 16 || cfg.batch_slots < 1 || cfg.batch_slots > 256 ||
         cfg.node_bits < 8 || cfg.node_bits > 24)
         return GDL_ERR_INVALID_ARG;
+    if (cfg.partition_cus) return GDL_ERR_UNSUPPORTED;  // no CU-masked streams
     auto s = std::make_unique<gdl_server>();
     s->cfg = cfg;
     s->t0 = clock_ms();

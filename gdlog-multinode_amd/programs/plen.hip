@@ -19,21 +19,8 @@
 
 using namespace gdlog;
 
-namespace {
-
-// The FULL-maintenance mode for one scope (the hop counts need the append
-// log, which the merge mode does not keep).
-struct AutoFullMode {
-    FullMode saved = full_mode();
-    AutoFullMode() { set_full_mode(FullMode::Auto); }
-    ~AutoFullMode() { set_full_mode(saved); }
-};
-
-}  // namespace
-
 void path_length_program(const EdgeInput &graph, int grid_size, int block_size,
                          ProgramOutput *out, std::vector<uint32_t> *lens) {
-    AutoFullMode mode;
     double t0 = now_ms();
     Relation *edge_2__2_1 = new Relation();
     Relation *path_2__1_2 = new Relation();

@@ -106,12 +106,10 @@ gdl_status gdl_mem_info(gdl_ctx *ctx, uint64_t *free_bytes, uint64_t *total_byte
 gdl_status gdl_release_cached(gdl_ctx *ctx);
 /* Changes the byte cap (0 = none). */
 gdl_status gdl_set_mem_cap(gdl_ctx *ctx, uint64_t cap_bytes);
-/* Restricts the context's stream to the CUs set in mask (words x 32 bits,
- * bit i = CU i; words = 0: all CUs again), so concurrent contexts on one
- * device can run on disjoint CU sets instead of queueing behind each other's
- * kernels.  Waits for the context's pending work first; grids are then sized
- * for the selected CU count.  Experimental: no test exercises it on a device
- * yet, and the server does not use it unless asked (partition_cus). */
+/* CU masks (a stream restricted to a subset of the CUs) are not supported:
+ * returns GDL_ERR_UNSUPPORTED for any context (GDL_ERR_INVALID_ARG for a null
+ * one).  Kept in the ABI so callers probing for it get a status, not a
+ * missing symbol. */
 gdl_status gdl_set_cu_mask(gdl_ctx *ctx, const uint32_t *mask, uint32_t words);
 /* Bytes of freed blocks the allocator keeps for reuse (0 = no limit); past
  * it the largest cached blocks go back to HIP.  Freed blocks are reused best

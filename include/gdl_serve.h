@@ -47,11 +47,9 @@ typedef struct gdl_server_config {
     uint64_t mem_budget;  /* device bytes the admitted estimates may take
                              (0: 80% of the device memory free at create) */
     int keep_keys;        /* 1: results keep the sorted output tuples */
-    int partition_cus;    /* 1: each worker and the batch get their own CUs
-                             (gdl_set_cu_mask, CU i to executor i mod executors),
-                             so one query's kernels never queue behind another's;
-                             0: all share every CU (default 0: the partitioned
-                             configuration has not been validated on the device) */
+    int partition_cus;    /* must be 0: per-executor CU masks (gdl_set_cu_mask) are
+                             not supported, and gdl_server_create returns
+                             GDL_ERR_UNSUPPORTED for 1 */
     int take_turns;       /* 1: the TC batch and the workers take turns on the device
                              (one at a time, in the order they asked; the batch's
                              turn is a slice of about 20 ms of bodies), so no

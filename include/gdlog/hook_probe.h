@@ -167,6 +167,10 @@ static inline PredMap probe_predicate_now(tuple_predicate p, int arity) {
         column_type b = (i % 4 == 0) ? a : (column_type)(s >> 32);
         if (i == 3) { a = 0; b = 0xffffffffu; }
         if (i == 4) { a = 0xffffffffu; b = 0; }
+        if (i >= 8 && i < 32) {  // small ids, where constants in hooks tend to be
+            a = (column_type)(i - 8);
+            b = (column_type)((i * 7) % 24);
+        }
         put(i, a, arity == 2 ? b : 0);
     }
     int *dout = nullptr;
@@ -215,6 +219,9 @@ static inline PredMap probe_predicate_now(tuple_predicate p, int arity) {
         }
         if ((res[i] != 0) != want) return r;
     }
+    // A non-null predicate that kept every probe is not a comparison of two
+    // columns (t[1] != 42 is true on all of them): it is not dropped.
+    if (pr.op == GDL_PRED_NONE) return r;
     r.ok = true;
     r.pred = pr;
     return r;

@@ -84,3 +84,16 @@ def test_generator_library_is_deterministic():
     rs, rd = gdl.unpack(r)
     assert r.size == 40000 and len(set(r.tolist())) == 40000 and (rs != rd).all()
     assert rs.max() < 32768 and rd.max() < 32768
+
+
+def test_cu_masks_are_refused_without_a_device():
+    eng, _ = gdl.load()
+    prog = gdl._serve_lib()
+    assert eng.gdl_set_cu_mask(None, None, 0) == 1  # null context
+    # a server asked for per-executor CU masks is refused before any device call
+    cfg = gdl.ServerConfig()
+    prog.gdl_server_config_default(ctypes.byref(cfg))
+    cfg.partition_cus = 1
+    h = ctypes.c_void_p()
+    assert prog.gdl_server_create(ctypes.byref(cfg), ctypes.byref(h)) == 4
+    assert not h.value

@@ -147,3 +147,64 @@ def test_snapshot_of_other_inputs_is_refused(ctx, tmp_path):
         ctx.run_tc(other)
     gdl.set_checkpoint(None)
     assert err.value.status == gdl.GDL_ERR_INVALID_ARG
+
+
+def same_modulo_newt(got, want):
+    # the trace's newt field counts raw rows in append mode and deduplicated
+    # rows in merge mode; everything else is mode-independent
+    assert np.array_equal(got["out"], want["out"])
+    for k in ("n_out", "iterations", "init_iterations", "derived"):
+        assert got[k] == want[k], k
+    for key in ("trace", "trace_init"):
+        assert [(t[0], t[1], t[3], t[4]) for t in got[key]] == \
+            [(t[0], t[1], t[3], t[4]) for t in want[key]], key
+
+
+CROSS = [("merge", "auto"), ("auto", "merge")]
+
+
+@pytest.mark.parametrize("first,then", CROSS)
+@pytest.mark.parametrize("name,stop", [("tree", 7), ("er", 3), ("chain", 20)])
+def test_tc_resumes_across_full_modes(ctx, tmp_path, first, then, name, stop):
+    # a snapshot written in one FULL mode resumes in the other: a sorted FULL
+    # from merge mode becomes an append log whose tail is the delta, which the
+    # stream-ordered linear loop reads
+    e = TC[name]()
+    gdl.set_full_mode(gdl.FULL_MODE_AUTO)
+    gdl.set_checkpoint(None)
+    want = ctx.run_tc(e)
+    try:
+        gdl.set_full_mode(MODES[first])
+        gdl.set_checkpoint(str(tmp_path), stop_after=stop)
+        stops(ctx.run_tc, e)
+        gdl.set_full_mode(MODES[then])
+        gdl.set_checkpoint(str(tmp_path), resume=True)
+        got = ctx.run_tc(e)
+    finally:
+        gdl.set_checkpoint(None)
+        gdl.set_full_mode(gdl.FULL_MODE_AUTO)
+    same_modulo_newt(got, want)
+
+
+@pytest.mark.parametrize("first,then", CROSS)
+def test_sg_and_cspa_resume_across_full_modes(ctx, tmp_path, first, then):
+    cases = [("sg", ctx.run_sg, (gdl.gen_rmat(1, 11, 4000),)),
+             ("cspa", ctx.run_cspa, gdl.gen_cspa(5, 150, 1.1, 200, 100))]
+    for label, run, args in cases:
+        gdl.set_full_mode(gdl.FULL_MODE_AUTO)
+        gdl.set_checkpoint(None)
+        want = run(*args)
+        for i, stop in enumerate(_two_lie_stops(want)):
+            d = tmp_path / f"{label}{i}"
+            d.mkdir()
+            try:
+                gdl.set_full_mode(MODES[first])
+                gdl.set_checkpoint(str(d), stop_after=stop)
+                stops(run, *args)
+                gdl.set_full_mode(MODES[then])
+                gdl.set_checkpoint(str(d), resume=True)
+                got = run(*args)
+            finally:
+                gdl.set_checkpoint(None)
+                gdl.set_full_mode(gdl.FULL_MODE_AUTO)
+            same_modulo_newt(got, want)
