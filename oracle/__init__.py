@@ -25,12 +25,18 @@ class TraceRec(Structure):
                 ("full", c_uint64)]
 
 
+class OpRec(Structure):
+    _fields_ = [("iteration", c_int), ("op", c_int), ("ran", c_int), ("raw", c_uint64),
+                ("out_newt", c_uint64)]
+
+
 class Result(Structure):
     _fields_ = [("iterations", c_int), ("init_iterations", c_int), ("derived", c_uint64),
                 ("n_out", c_size_t), ("out", _U64P), ("n_trace", c_size_t),
                 ("trace", POINTER(TraceRec)), ("n_trace_init", c_size_t),
                 ("trace_init", POINTER(TraceRec)), ("sizes", c_uint64 * 4),
-                ("extra", _U64P * 3), ("n_extra", c_size_t * 3)]
+                ("extra", _U64P * 3), ("n_extra", c_size_t * 3),
+                ("n_oplog", c_size_t), ("oplog", POINTER(OpRec))]
 
 
 _lib = None
@@ -117,9 +123,11 @@ def _collect(res):
         n = res.n_extra[k]
         extra.append(np.ctypeslib.as_array(res.extra[k], shape=(n,)).copy() if n else
                      np.zeros(0, dtype=np.uint64))
+    ops = [(res.oplog[i].iteration, res.oplog[i].op, res.oplog[i].ran, res.oplog[i].raw,
+            res.oplog[i].out_newt) for i in range(res.n_oplog)]
     r = dict(iterations=res.iterations, init_iterations=res.init_iterations,
              derived=res.derived, n_out=res.n_out, out=out, trace=tr, trace_init=ti,
-             sizes=list(res.sizes), extra=extra)
+             sizes=list(res.sizes), extra=extra, oplog=ops)
     lib().orc_result_free(ctypes.byref(res))
     return r
 

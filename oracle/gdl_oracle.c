@@ -299,6 +299,8 @@ struct orc_lie {
     int reload_full_flag; /* lie.cuh:29 */
     orc_trace_rec *trace;
     size_t n_trace, cap_trace;
+    orc_op_rec *oplog;
+    size_t n_oplog, cap_oplog;
 };
 
 orc_lie *orc_lie_new(void) {
@@ -315,6 +317,7 @@ void orc_lie_free(orc_lie *l) {
     free(l->stat);
     free(l->tmp);
     free(l->trace);
+    free(l->oplog);
     free(l);
 }
 
@@ -404,7 +407,7 @@ static void newt_append_raw(orc_rel *r, orc_key *fresh, size_t n, size_t cap) {
 }
 
 /* RelationalJoin::operator() (join.cu:12-270) */
-static void run_join(orc_op *op) {
+static uint64_t run_join(orc_op *op) {
     orc_rel *in = op->inner, *ou = op->outer;
     /* version select (join.cu:15-29): inner DELTA, otherwise FULL */
     const orc_key *inner;
@@ -417,8 +420,8 @@ static void run_join(orc_op *op) {
     else if (op->outer_ver == ORC_FULL) { outer = ou->full.data; outer_n = &ou->full_view; }
     else { outer = ou->newt.data; outer_n = &ou->newt.n; }
     /* early outs (join.cu:38-45): an empty inner zeroes the outer's count */
-    if (*outer_n == 0) return;
-    if (ni == 0) { *outer_n = 0; return; }
+    if (*outer_n == 0) return 0;
+    if (ni == 0) { *outer_n = 0; return 0; }
     size_t no = *outer_n;
     int k = ou->index_cols; /* join_column_counts = outer->index_column_size */
     size_t m = orc_join(inner, ni, in->arity, outer, no, ou->arity, k, op->gen, op->pred,
@@ -427,30 +430,47 @@ static void run_join(orc_op *op) {
     orc_join(inner, ni, in->arity, outer, no, ou->arity, k, op->gen, op->pred, op->out->arity, res);
     if (op->out->tmp_flag) newt_append_raw(op->out, res, m, m);
     else newt_accumulate_sorted(op->out, res, m, m);
+    return m;
 }
 
 /* RelationalCopy::operator() (copy.cu:12-95) */
-static void run_copy(orc_op *op) {
+static uint64_t run_copy(orc_op *op) {
     orc_rel *s = op->src;
     const orc_key *src;
     size_t n;
     if (op->src_ver == ORC_DELTA) { src = s->delta.data; n = s->delta.n; }
     else { src = s->full.data; n = s->full_view; }
-    if (n == 0) { op->dst->newt.n = 0; return; } /* copy.cu:23-26 */
+    if (n == 0) { op->dst->newt.n = 0; return 0; } /* copy.cu:23-26 */
     orc_key *res = (orc_key *)xmalloc(n * sizeof(orc_key));
     orc_project(src, n, s->arity, op->cp, op->dst->arity, res);
     newt_accumulate_sorted(op->dst, res, n, n);
+    return n;
 }
 
 /* RelationalACopy::operator() (acopy.cu:12-50): dest newt is overwritten. */
-static void run_acopy(orc_op *op) {
+static uint64_t run_acopy(orc_op *op) {
     orc_rel *s = op->src;
-    if (s->newt.n == 0) { op->dst->newt.n = 0; return; }
+    if (s->newt.n == 0) { op->dst->newt.n = 0; return 0; }
     size_t n = s->newt.n;
     orc_key *res = (orc_key *)xmalloc(n * sizeof(orc_key));
     orc_project(s->newt.data, n, s->arity, op->cp, op->dst->arity, res);
+    size_t raw = n;
     n = orc_sort_unique(res, n);
     vec_take(&op->dst->newt, res, n, s->newt.n);
+    return raw;
+}
+
+static void oplog_push(orc_lie *l, int it, int op, int ran, uint64_t raw, uint64_t newt) {
+    if (l->n_oplog == l->cap_oplog) {
+        l->cap_oplog = l->cap_oplog ? 2 * l->cap_oplog : 64;
+        l->oplog = (orc_op_rec *)xrealloc(l->oplog, l->cap_oplog * sizeof(orc_op_rec));
+    }
+    orc_op_rec *o = &l->oplog[l->n_oplog++];
+    o->iteration = it;
+    o->op = op;
+    o->ran = ran;
+    o->raw = raw;
+    o->out_newt = newt;
 }
 
 static void trace_push(orc_lie *l, int it, int rel, uint64_t newt, uint64_t delta, uint64_t full) {
@@ -469,6 +489,7 @@ static void trace_push(orc_lie *l, int it, int rel, uint64_t newt, uint64_t delt
 /* LIE::fixpoint_loop (lie.cu:26-274) */
 int orc_lie_run(orc_lie *l) {
     l->n_trace = 0;
+    l->n_oplog = 0;
     /* init (lie.cu:49-59): current_full_size = full count; delta is
      * copy_relation_container(delta, full) (relation.cu:764-795), which copies
      * the first tuple_counts rows of full's data_raw -- the rows as loaded,
@@ -484,11 +505,16 @@ int orc_lie_run(orc_lie *l) {
     for (;;) {
         for (size_t i = 0; i < l->n_ops; i++) {
             orc_op *op = &l->ops[i];
-            if (op->kind == OP_JOIN) run_join(op);
-            else if (op->kind == OP_ACOPY) run_acopy(op);
+            int ran = 1;
+            uint64_t raw = 0;
+            if (op->kind == OP_JOIN) raw = run_join(op);
+            else if (op->kind == OP_ACOPY) raw = run_acopy(op);
             else if (op->src_ver == ORC_FULL) {
-                if (!op->copied) { run_copy(op); op->copied = 1; }
-            } else run_copy(op);
+                if (!op->copied) { raw = run_copy(op); op->copied = 1; }
+                else ran = 0;
+            } else raw = run_copy(op);
+            orc_rel *dst = op->kind == OP_JOIN ? op->out : op->dst;
+            oplog_push(l, iteration, (int)i, ran, raw, dst->newt.n);
         }
         for (size_t t = 0; t < l->n_tmp; t++) vec_clear(&l->tmp[t]->newt); /* lie.cu:85-87 */
         int fixpoint = 1;
@@ -538,6 +564,11 @@ size_t orc_lie_trace(const orc_lie *l, const orc_trace_rec **recs) {
     return l->n_trace;
 }
 
+size_t orc_lie_oplog(const orc_lie *l, const orc_op_rec **recs) {
+    *recs = l->oplog;
+    return l->n_oplog;
+}
+
 /* ---- program encodings -------------------------------------------------- */
 
 static orc_key *reverse_pairs(const orc_key *e, size_t n) {
@@ -557,6 +588,14 @@ static orc_trace_rec *copy_trace(const orc_lie *l, size_t *n) {
     *n = orc_lie_trace(l, &t);
     orc_trace_rec *r = (orc_trace_rec *)xmalloc(*n * sizeof(orc_trace_rec));
     if (*n) memcpy(r, t, *n * sizeof(orc_trace_rec));
+    return r;
+}
+
+static orc_op_rec *copy_oplog(const orc_lie *l, size_t *n) {
+    const orc_op_rec *t;
+    *n = orc_lie_oplog(l, &t);
+    orc_op_rec *r = (orc_op_rec *)xmalloc((*n ? *n : 1) * sizeof(orc_op_rec));
+    if (*n) memcpy(r, t, *n * sizeof(orc_op_rec));
     return r;
 }
 
@@ -601,6 +640,7 @@ int orc_tc(const orc_key *edges, size_t n, int max_iteration, orc_result *res) {
     res->n_out = path->full.n;
     res->out = copy_keys(path->full.data, path->full.n);
     res->trace = copy_trace(lie, &res->n_trace);
+    res->oplog = copy_oplog(lie, &res->n_oplog);
     orc_lie_free(lie);
     orc_rel_free(edge);
     orc_rel_free(path);
@@ -643,6 +683,7 @@ int orc_sg(const orc_key *edges, size_t n, orc_result *res) {
     res->n_out = sg->full.n;
     res->out = copy_keys(sg->full.data, sg->full.n);
     res->trace = copy_trace(lie, &res->n_trace);
+    res->oplog = copy_oplog(lie, &res->n_oplog);
     orc_lie_free(init);
     orc_lie_free(lie);
     orc_rel_free(edge);
@@ -756,6 +797,7 @@ int orc_cspa(const orc_key *assign, size_t na, const orc_key *deref, size_t nd, 
     res->iterations = orc_lie_run(a);
     res->derived = derived + sum_delta(a);
     res->trace = copy_trace(a, &res->n_trace);
+    res->oplog = copy_oplog(a, &res->n_oplog);
     res->n_out = va->full.n;
     res->out = copy_keys(va->full.data, va->full.n);
     res->extra[0] = copy_keys(vf12->full.data, vf12->full.n);
@@ -779,6 +821,7 @@ void orc_result_free(orc_result *res) {
     free(res->out);
     free(res->trace);
     free(res->trace_init);
+    free(res->oplog);
     for (int i = 0; i < 3; i++) free(res->extra[i]);
     memset(res, 0, sizeof(*res));
 }

@@ -24,8 +24,11 @@
  *     cuDF SG result sizes (test/cuDF/sg.json), a scipy sparse-matrix SG
  *     fixpoint, and a naive Datalog evaluation of test/souffle/cspa.dl
  *     written in Python in the tests.
- *   The per-iteration CSPA delta split is pinned only by the restatement of
- *     src/lie.cu itself plus conservation (sum of deltas == final size).
+ *   The per-iteration CSPA trace (newt, delta, full of all five update
+ *     relations in both LIEs) and the per-operator raw join sizes are pinned
+ *     by a known-answer case derived by hand from src/lie.cu, src/join.cu,
+ *     src/copy.cu and src/acopy.cu (tests/test_cspa_kat.py), which also pins
+ *     the empty-source and empty-inner zeroing on LIEs built for them.
  */
 #ifndef GDL_ORACLE_H
 #define GDL_ORACLE_H
@@ -96,6 +99,22 @@ typedef struct {
 } orc_trace_rec;
 size_t orc_lie_trace(const orc_lie *l, const orc_trace_rec **recs);
 
+/* Per-operator log: one record per RA op per iteration, in execution order.
+ * raw is the operator's output before deduplication -- for a join the
+ * "join result size(non dedup)" the reference prints (join.cu:86-88), for a
+ * Copy/ACopy the source count -- and 0 when the op returned early (empty
+ * outer or inner, join.cu:38-45; empty source, copy.cu:23-26, acopy.cu:18-22).
+ * ran is 0 for a FULL-source Copy skipped after its first run (lie.cu:69-77).
+ * out_newt is the output relation's newt count after the op. */
+typedef struct {
+    int iteration;
+    int op;              /* index in add_ra order */
+    int ran;
+    uint64_t raw;
+    uint64_t out_newt;
+} orc_op_rec;
+size_t orc_lie_oplog(const orc_lie *l, const orc_op_rec **recs);
+
 /* ---- program encodings (test/tc.cu, test/sg.cu, test/cspa.cu) ----------- */
 typedef struct {
     int iterations;          /* iteration counter of the (last) LIE */
@@ -111,6 +130,8 @@ typedef struct {
     uint64_t sizes[4];
     orc_key *extra[3];       /* CSPA: VF, MA sorted (malloc'd), VA == out */
     size_t n_extra[3];
+    size_t n_oplog;
+    orc_op_rec *oplog;       /* operator log of the (last) LIE (malloc'd) */
 } orc_result;
 
 /* edges: n pairs (src,dst) as u64 keys (src << 32 | dst). */
