@@ -40,9 +40,11 @@ $(LIB)/libgdl_gen.so: $(PKG)/gen/gdl_gen.c $(PKG)/gen/gdl_gen.h
 	@mkdir -p $(LIB)
 	$(CC) $(CFLAGS) -shared $< -o $@ -lm
 
-oracle/lib/libgdl_oracle.so: oracle/gdl_oracle.c oracle/cpu_tc_omp.c oracle/gdl_oracle.h
+ORACLE_SRC := oracle/gdl_oracle.c oracle/cpu_tc_omp.c oracle/cpu_souffle.c
+
+oracle/lib/libgdl_oracle.so: $(ORACLE_SRC) oracle/gdl_oracle.h
 	@mkdir -p oracle/lib
-	$(CC) $(CFLAGS) -fopenmp -shared oracle/gdl_oracle.c oracle/cpu_tc_omp.c -o $@
+	$(CC) $(CFLAGS) -fopenmp -shared $(ORACLE_SRC) -o $@
 
 $(LIB)/obj/main_%.o: $(PKG)/programs/main.cpp $(HEADERS)
 	@mkdir -p $(dir $@)

@@ -62,6 +62,10 @@ def lib():
         L.orc_cspa.argtypes = [_U64P, c_size_t, _U64P, c_size_t, POINTER(Result)]
         L.orc_result_free.restype = None
         L.orc_result_free.argtypes = [POINTER(Result)]
+        for fn in (L.souffle_sg, L.souffle_tc):
+            fn.restype = c_int
+            fn.argtypes = [_U64P, c_uint64, c_int, c_int, _U64P, _U64P, POINTER(c_int), _U64P,
+                           c_int]
         L.cpu_tc_omp.restype = c_int
         L.cpu_tc_omp.argtypes = [_U64P, c_uint64, c_int, c_int, _U64P, _U64P,
                                  POINTER(c_int), _U64P, c_int]
@@ -81,6 +85,33 @@ def tc_omp(edges, threads, max_iteration=-1, trace_cap=1 << 20):
         raise MemoryError("cpu_tc_omp")
     return dict(n_out=n_out.value, derived=der.value, iterations=it.value,
                 deltas=[int(x) for x in tr[:min(it.value, trace_cap)]])
+
+
+def _souffle(fn, edges, threads, max_iteration, trace_cap):
+    e = _arr(edges)
+    tr = np.zeros(trace_cap, dtype=np.uint64)
+    n_out, cs = c_uint64(), c_uint64()
+    it = c_int()
+    if fn(_p(e), e.size, threads, max_iteration, ctypes.byref(n_out), ctypes.byref(cs),
+          ctypes.byref(it), _p(tr), trace_cap) != 0:
+        raise MemoryError(fn.__name__)
+    return dict(n_out=n_out.value, checksum=cs.value, iterations=it.value, trace=tr)
+
+
+def souffle_sg(edges, threads, max_iteration=-1, trace_cap=1 << 16):
+    """Multi-core restatement of Souffle's SG plan (cpu_souffle.c): |sg|,
+    checksum, recursive bodies, per-body |new| (entry 0: the x != y rule)."""
+    r = _souffle(lib().souffle_sg, edges, threads, max_iteration, trace_cap)
+    r["trace"] = [int(x) for x in r["trace"][:min(r["iterations"] + 1, trace_cap)]]
+    return r
+
+
+def souffle_tc(edges, threads, max_iteration=-1, trace_cap=1 << 16):
+    """Multi-core restatement of Souffle's non-linear TC plan
+    (cpu_souffle.c): |path|, checksum, recursive bodies, per-body |new|."""
+    r = _souffle(lib().souffle_tc, edges, threads, max_iteration, trace_cap)
+    r["trace"] = [int(x) for x in r["trace"][:min(r["iterations"], trace_cap)]]
+    return r
 
 
 def _p(a):

@@ -149,6 +149,22 @@ int cpu_tc_omp(const uint64_t *edges, uint64_t n, int threads, int max_iteration
                uint64_t *n_out, uint64_t *derived, int *iterations, uint64_t *delta_trace,
                int trace_cap);
 
+/* Multi-core CPU baselines restating Souffle's generated plans
+ * (cpu_souffle.c): SG (test/souffle/sg.cpp:474-645) and TC with tc.dl's
+ * non-linear rule (test/souffle/tc.cpp:379-489), split over `threads`
+ * OpenMP threads.  Return |result|, its checksum (sum of fmix64 of the
+ * keys), the number of recursive loop bodies run (the last one derives
+ * nothing; at most max_iteration when >= 0) and the per-body |new| (SG: entry
+ * 0 is the non-recursive rule's |sg|, entry i body i; TC: entry i-1 body i).
+ * 0 on success, -1 when memory ran out.  Checked against orc_sg / orc_tc in
+ * tests/test_oracle.py. */
+int souffle_sg(const uint64_t *edges, uint64_t n, int threads, int max_iteration,
+               uint64_t *n_out, uint64_t *checksum, int *iterations, uint64_t *delta_trace,
+               int trace_cap);
+int souffle_tc(const uint64_t *edges, uint64_t n, int threads, int max_iteration,
+               uint64_t *n_out, uint64_t *checksum, int *iterations, uint64_t *delta_trace,
+               int trace_cap);
+
 #ifdef __cplusplus
 }
 #endif

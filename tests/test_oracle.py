@@ -283,3 +283,36 @@ def test_cspa_delta_conservation():
         tot = sum(t[3] for t in r["trace_init"] if t[1] == rel) if rel < 4 else 0
         tot += sum(t[3] for t in r["trace"] if t[1] == rel)
         assert tot == final
+
+
+@pytest.mark.parametrize("name", ["hpc_talk", "data_5", "er", "tree", "chain", "rmat_small"])
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_souffle_plan_baselines_give_the_oracle_s_sets(name, threads):
+    # oracle/cpu_souffle.c restates Souffle's SG plan and its non-linear TC
+    # plan (different loops from the gdlog encodings): same final sets
+    edges = {"hpc_talk": lambda: read_edges("hpc_talk.txt"),
+             "data_5": lambda: read_edges("data_5.txt"),
+             "er": lambda: gdl.gen_er(0, 300, 900),
+             "tree": lambda: gdl.gen_tree(0, 3000, 100),
+             "chain": lambda: gdl.pack(np.arange(99), np.arange(1, 100)),
+             "rmat_small": lambda: gdl.gen_rmat(1, 10, 3000)}[name]()
+    tc, sg = oracle.tc(edges), oracle.sg(edges)
+    r = oracle.souffle_tc(edges, threads)
+    assert r["n_out"] == tc["n_out"] and r["checksum"] == gdl.tuple_checksum(tc["out"])
+    assert sum(r["trace"]) == tc["n_out"] - np.unique(edges).size
+    s = oracle.souffle_sg(edges, threads)
+    assert s["n_out"] == sg["n_out"] and s["checksum"] == gdl.tuple_checksum(sg["out"])
+    # the non-recursive rule and the recursive bodies: the same per-body sets as
+    # the gdlog encoding's init LIE and sg LIE (both are semi-naive over delta sg)
+    want = [sum(t[3] for t in sg["trace_init"])] + [t[3] for t in sg["trace"]]
+    strip = lambda xs: xs[:max([i + 1 for i, x in enumerate(xs) if x] or [0])]
+    assert strip(s["trace"]) == strip(want)
+
+
+def test_souffle_tc_plan_is_non_linear():
+    # chain of 65 nodes: the linear loop needs 64 bodies, path o path doubles
+    # the covered distance each body: 1 + log2(64) bodies derive, one more ends
+    e = gdl.pack(np.arange(64), np.arange(1, 65))
+    r = oracle.souffle_tc(e, 4)
+    assert r["n_out"] == 65 * 64 // 2 and r["iterations"] == 7
+    assert oracle.tc(e)["iterations"] == 64
