@@ -66,3 +66,34 @@ clean:
 .PHONY: all clean
 
 .SECONDARY:
+
+# ---- AddressSanitizer + UndefinedBehaviorSanitizer builds of the host code
+# (CPU only): the oracle and the CPU baselines under gcc, and the C++ API's host
+# code (Datalog front end, LIE, relations, RA operators) under hipcc with the
+# sanitizers on the host side only.  `make sanitize` builds and runs both.
+SAN      := oracle/lib/asan
+SANFLAGS := -fsanitize=address,undefined -fno-sanitize-recover=all -fno-omit-frame-pointer -g -O1
+HOSTSAN  := -Xarch_host -fsanitize=address -Xarch_host -fsanitize=undefined \
+            -Xarch_host -fno-sanitize-recover=all -Xarch_host -fno-omit-frame-pointer
+SAN_HOST := $(patsubst %,$(SAN)/host_%.o,datalog lie relation ra_ops)
+
+$(SAN)/oracle_asan: oracle/sanitize/oracle_asan_main.c $(ORACLE_SRC) oracle/gdl_oracle.h
+	@mkdir -p $(SAN)
+	$(CC) $(SANFLAGS) -fopenmp oracle/sanitize/oracle_asan_main.c $(ORACLE_SRC) -o $@ -lm
+
+$(SAN)/host_%.o: $(PKG)/host/%.cpp $(HEADERS)
+	@mkdir -p $(SAN)
+	$(HIPCC) --offload-arch=$(ARCH) -O1 -g -std=c++17 -fPIC -Iinclude \
+	    -Xclang -target-feature -Xclang -packed-fp32-ops $(HOSTSAN) -x hip -c $< -o $@
+
+$(SAN)/frontend_asan: oracle/sanitize/frontend_asan_main.cpp $(SAN_HOST) $(LIB)/libgdl_hip.so
+	$(HIPCC) -O1 -g -std=c++17 -Iinclude $(HOSTSAN) -c oracle/sanitize/frontend_asan_main.cpp \
+	    -o $(SAN)/frontend_main.o
+	$(HIPCC) --offload-arch=$(ARCH) $(HOSTSAN) $(SAN)/frontend_main.o $(SAN_HOST) -L$(LIB) \
+	    -lgdl_hip -Wl,-rpath,'$$ORIGIN/../../../$(LIB)' -o $@
+
+sanitize: $(SAN)/oracle_asan $(SAN)/frontend_asan
+	ASAN_OPTIONS=detect_leaks=1 $(SAN)/oracle_asan
+	ASAN_OPTIONS=detect_leaks=0 $(SAN)/frontend_asan $(PKG)/programs/dl/*.dl
+
+.PHONY: sanitize

@@ -1,12 +1,13 @@
-"""Benchmark: transitive closure on a deep sparse tree, one MI355X.
+"""Benchmark: same generation (SG) on an R-MAT graph, one MI355X.
 
-Workload: TC (path(x,z) :- path(x,y), edge(y,z), semi-naive, the reference's
-TC program encoding) on a child -> parent tree of 200,000 u32 node ids whose
-node i has its parent uniform in [max(0, i - 1000), i - 1] (SplitMix64 seed 0):
-199,999 edges, about 41M path tuples, about 400 fixpoint iterations.
+Workload: SG (sg(x,y) :- edge(p,x), edge(p,y), x != y;
+sg(x,y) :- edge(a,x), sg(a,b), edge(b,y); the reference's SG program
+encoding) on an R-MAT graph of scale 15 (32,768 u32 node ids), 40,000
+distinct directed edges, a/b/c/d = 0.57/0.19/0.19/0.05, self-loops removed,
+SplitMix64 seed 1: 67,916,723 sg tuples over 2 + 6 fixpoint iterations.
 
 A step is one whole run of the program from device-resident edges: loading
-the two relations (sort, unique, index) and the fixpoint.  `value` is derived
+the relation (sort, unique, index) and both fixpoints.  `value` is derived
 tuples per second (new unique tuples summed over iterations / step time).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -15,11 +16,17 @@ Prints one JSON line.  Also reports:
   roofline      the dominant kernel's algorithmic GB/s against 8 TB/s, from
                 hipEvents on the engine stream over the same K steps run
                 again (the timed steps run without per-kernel events);
-  cpu_baseline  the same semi-naive TC loop on the host's threads
-                (oracle/cpu_tc_omp.c), with the one-thread oracle beside it;
+  phases        per kernel: launches, ms, algorithmic GB and GB/s per step;
+  cpu_baseline  a restatement of Souffle's SG plan on the host's threads
+                (oracle/cpu_souffle.c), with a one-thread sample beside it;
+  beside        TC on a 200,000-node child->parent tree (parent window 1,000,
+                ~41M tuples, 425 iterations) measured the same way, with
+                gdlog's linear loop and Souffle's non-linear TC plan on the
+                host beside it;
   batched       256 TC instances (log-uniform 1k-50k edges, ER or tree) in
                 one continuous batch vs one program run at a time, with the
-                autotuned launch / layout parameters;
+                autotuned launch / layout parameters, every instance checked
+                against the oracle's goldens;
   served        a Poisson stream of TC / SG / CSPA queries through the query
                 server vs one-at-a-time service, p50 / p99 latency;
   datalog       tc.dl and cspa.dl through the Datalog front end vs the
@@ -98,48 +105,87 @@ def cpu_threads():
     return max(1, min(16, n))
 
 
+def host_info():
+    """The host the CPU baselines ran on: CPU share and model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        share = len(os.sched_getaffinity(0))
+    except AttributeError:
+        share = None
+    return dict(nproc=os.cpu_count(), affinity=share, model=model)
+
+
+def _best(fn, reps):
+    best, r = None, None
+    for _ in range(reps):
+        t = time.perf_counter()
+        r = fn()
+        dt = time.perf_counter() - t
+        best = dt if best is None else min(best, dt)
+    return r, best
+
+
 def cpu_baseline(kind, inp, max_iteration):
-    """CPU baseline.  TC: the same semi-naive loop on all the host threads
-    (oracle/cpu_tc_omp.c, partitioned by the path's second column), on the
-    whole workload, best of 3 after a warm-up run, with the one-thread oracle
-    on a bounded sample beside it.  SG / CSPA: the one-thread oracle on the
-    whole workload."""
+    """CPU baselines on the host's threads (this process's CPU share, at most
+    16).  SG: the restatement of Souffle's SG plan (oracle/cpu_souffle.c,
+    test/souffle/sg.cpp:474-645) on the whole workload, best of 2 after a
+    warm-up on its first body, and on one thread over a bounded sample (the
+    non-recursive rule and the first recursive body).  TC: gdlog's linear
+    loop (oracle/cpu_tc_omp.c) on the whole workload, best of 3 after a
+    warm-up; Souffle's non-linear TC plan (oracle/cpu_souffle.c,
+    test/souffle/tc.cpp:379-489) on the whole workload once; the one-thread
+    oracle on a bounded sample.  CSPA: the one-thread oracle."""
     if REPO not in sys.path:
         sys.path.insert(0, REPO)
     import oracle
+    threads = cpu_threads()
+    host = host_info()
+    if kind == "sg":
+        oracle.souffle_sg(inp, threads, max_iteration=0)  # warm-up: page-in, thread pool
+        r, best = _best(lambda: oracle.souffle_sg(inp, threads), 2)
+        s, dt1 = _best(lambda: oracle.souffle_sg(inp, 1, max_iteration=1), 1)
+        return dict(value=round(r["n_out"] / best, 1), unit="derived tuples/s", cores=threads,
+                    kind="port", host=host,
+                    sample=f"oracle/cpu_souffle.c (Souffle's SG plan) on the whole workload "
+                           f"({r['n_out']} derived tuples, {r['iterations']} recursive bodies, "
+                           f"best of 2: {best:.2f} s, {threads} threads)",
+                    single_thread=dict(value=round(s["n_out"] / dt1, 1), cores=1,
+                                       sample=f"the same plan, non-recursive rule + first "
+                                              f"recursive body ({s['n_out']} tuples, "
+                                              f"{dt1:.1f} s)"))
     if kind == "tc":
-        threads = cpu_threads()
         oracle.tc_omp(inp, threads)  # warm-up: page-in, thread pool
-        best = None
-        for _ in range(3):
-            t = time.perf_counter()
-            r = oracle.tc_omp(inp, threads)
-            dt = time.perf_counter() - t
-            best = dt if best is None else min(best, dt)
-        t = time.perf_counter()
-        s = oracle.tc(inp, max_iteration=max_iteration)
-        dt1 = time.perf_counter() - t
+        r, best = _best(lambda: oracle.tc_omp(inp, threads), 3)
+        q, dtq = _best(lambda: oracle.souffle_tc(inp, threads), 1)
+        s, dt1 = _best(lambda: oracle.tc(inp, max_iteration=max_iteration), 1)
         return dict(value=round(r["derived"] / best, 1), unit="derived tuples/s", cores=threads,
-                    kind="port",
-                    sample=f"oracle/cpu_tc_omp.c on the whole workload ({r['derived']} derived "
-                           f"tuples, {r['iterations']} iterations, best of 3: {best:.2f} s, "
-                           f"{threads} threads)",
+                    kind="port", host=host,
+                    sample=f"oracle/cpu_tc_omp.c (gdlog's linear loop) on the whole workload "
+                           f"({r['derived']} derived tuples, {r['iterations']} iterations, "
+                           f"best of 3: {best:.2f} s, {threads} threads)",
+                    souffle_plan=dict(value=round((q["n_out"] - int(inp.size)) / dtq, 1),
+                                      cores=threads,
+                                      sample=f"oracle/cpu_souffle.c (Souffle's non-linear TC "
+                                             f"plan) on the whole workload ({q['iterations']} "
+                                             f"recursive bodies, {dtq:.2f} s)"),
                     single_thread=dict(value=round(s["derived"] / dt1, 1), cores=1,
                                        sample=f"oracle, first {s['iterations']} iterations "
                                               f"({s['derived']} derived tuples, {dt1:.1f} s)"))
     t = time.perf_counter()
-    if kind == "tc":
-        r = oracle.tc(inp, max_iteration=max_iteration)
-        what = f"first {r['iterations']} fixpoint iterations of the same TC workload"
-    elif kind == "sg":
-        r = oracle.sg(inp)
-        what = "the whole SG workload"
-    else:
-        r = oracle.cspa(*inp)
-        what = "the whole CSPA workload"
+    r = oracle.cspa(*inp)
     dt = time.perf_counter() - t
     return dict(value=r["derived"] / dt, unit="derived tuples/s", cores=1, kind="port",
-                sample=f"oracle on {what} ({r['derived']} derived tuples, {dt:.1f} s)")
+                host=host,
+                sample=f"oracle on the whole CSPA workload ({r['derived']} derived tuples, "
+                       f"{dt:.1f} s)")
 
 
 _last_note = [0.0]
@@ -219,6 +265,18 @@ def batched(gdl, ctx, n_inst, min_edges=1000, max_edges=50000, window=1000):
     batch_s = time.perf_counter() - t0
     mism = [q for q, _, _ in inst
             if (done[q]["n_out"], done[q]["iterations"], done[q]["derived"]) != alone[q]]
+    vs_oracle = None
+    gold = os.path.join(REPO, "tests", "golden", "batch256.json")
+    if n_inst == 256 and (min_edges, max_edges, window) == (1000, 50000, 1000) and \
+            os.path.exists(gold):
+        with open(gold) as f:
+            g = {r["qid"]: r for r in json.load(f)["instances"]}
+        vs_oracle = sum(1 for q, _, _ in inst
+                        if (done[q]["n_out"], done[q]["iterations"], done[q]["derived"],
+                            str(done[q]["checksum"]),
+                            [[int(t[3]), int(t[4])] for t in done[q]["trace"]]) !=
+                        (g[q]["n_out"], g[q]["iterations"], g[q]["derived"], g[q]["checksum"],
+                         g[q]["trace"]))
     derived = sum(v[2] for v in alone.values())
     return {
         "workload": f"tc_batched_{n_inst}: {n_inst} TC instances, edges log-uniform "
@@ -234,6 +292,7 @@ def batched(gdl, ctx, n_inst, min_edges=1000, max_edges=50000, window=1000):
                    "derived_tuples_per_s": round(derived / serial_s, 1)},
         "speedup_vs_serial": round(serial_s / batch_s, 3),
         "bodies": info["bodies"], "mismatches_vs_serial": len(mism),
+        "mismatches_vs_oracle": vs_oracle,
         "tuning": dict(tuned["tuning"], calibration_s=tuned["seconds"],
                        default_s=tuned["default_seconds"], trials=tuned["trials"],
                        sample="16 instances of the same generator, seeds 100000-100015"),
@@ -507,12 +566,93 @@ def workloads(gdl, nodes, window):
     }
 
 
+def measure(gdl, ctx, kind, inp, desc, steps, warmup):
+    """Times `steps` whole program runs from device-resident inputs (after
+    `warmup` untimed ones), then runs the same steps again with a hipEvent
+    pair around every launch on the engine stream: per-kernel times, their
+    algorithmic bytes (GB/s per phase) and the dominant kernel's roofline."""
+    if kind == "cspa":
+        dev = (ctx.to_device(inp[0]), ctx.to_device(inp[1]))
+
+        def step():
+            return ctx.run_cspa(dev[0], dev[1], on_device=True, want_keys=False,
+                                want_trace=False)
+        n_in = int(inp[0].size + inp[1].size)
+    else:
+        dev = (ctx.to_device(inp),)
+        run = ctx.run_tc if kind == "tc" else ctx.run_sg
+
+        def step():
+            return run(dev[0], on_device=True, want_keys=False, want_trace=False)
+        n_in = int(inp.size)
+
+    res = None
+    for _ in range(warmup):
+        res = step()
+    # the timed steps run without per-kernel events (two hipEventRecords per
+    # launch would add host time to ~1,000 launches a step)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = step()
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1000.0 * elapsed / steps
+    ctx.stats_reset()
+    ctx.stats_enable(True)
+    ctx.sync()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    ctx.sync()
+    ms_instrumented = 1000.0 * (time.perf_counter() - t1) / steps
+    ctx.stats_enable(False)
+    stats = ctx.stats()
+    for d in dev:
+        d.free()
+
+    timed = {k: v for k, v in stats.items() if v["launches"] and v["total_ms"] > 0}
+    dom = max(timed.items(), key=lambda kv: kv[1]["total_ms"]) if timed else None
+    roofline = None
+    if dom:
+        name, s = dom
+        avg_ms = s["total_ms"] / s["launches"]
+        avg_bytes = s["alg_bytes"] / s["launches"]
+        achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+        traffic = profiled_traffic(name, desc.split(":")[0])
+        roofline = dict(bound="hbm", kernel=name, achieved=round(achieved, 2),
+                        peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 5),
+                        traffic=traffic, avg_launch_us=round(avg_ms * 1e3, 3),
+                        avg_alg_bytes=round(avg_bytes))
+        if traffic is not None:
+            roofline["traffic_unit"] = "HBM bytes per launch (rocprofv3 --pmc, profiles/)"
+            roofline["traffic_gbs"] = round(traffic / (avg_ms * 1e-3) / 1e9, 2)
+    tot_bytes = sum(v["alg_bytes"] for v in stats.values()) / steps
+    kernel_ms = sum(v["total_ms"] for v in stats.values()) / steps
+    # per phase: launches, hipEvent ms, algorithmic bytes over that time
+    phases = {}
+    for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["total_ms"]):
+        if not v["launches"]:
+            continue
+        ms = v["total_ms"] / steps
+        gbs = v["alg_bytes"] / steps / (ms * 1e-3) / 1e9 if ms > 0 else None
+        phases[k] = dict(launches=v["launches"] // steps, ms=round(ms, 3),
+                         gb=round(v["alg_bytes"] / steps / 1e9, 4),
+                         gbs=None if gbs is None else round(gbs, 1),
+                         frac=None if gbs is None else round(gbs / HBM_PEAK_GBS, 4))
+    return dict(res=res, n_in=n_in, ms_per_step=ms_per_step,
+                derived_per_s=res["derived"] / (elapsed / steps),
+                ms_instrumented=round(ms_instrumented, 3), roofline=roofline, phases=phases,
+                step_hbm_frac=round(tot_bytes / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9), 5),
+                kernel_busy_frac=round(min(1.0, kernel_ms / ms_per_step), 4))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="tree", choices=["tree", "er", "sg", "cspa"])
+    ap.add_argument("--workload", default="sg", choices=["tree", "er", "sg", "cspa"])
     ap.add_argument("--mode", default="auto", choices=["auto", "merge"],
                     help="FULL maintenance (merge = the reference's per-iteration merge)")
     ap.add_argument("--nodes", type=int, default=200_000)
@@ -520,6 +660,8 @@ def main():
     ap.add_argument("--cpu-iterations", type=int, default=100,
                     help="fixpoint iterations of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-beside", action="store_true",
+                    help="skip the tree-TC measurement beside the SG headline")
     ap.add_argument("--batched-instances", type=int, default=256,
                     help="instances of the batched-TC measurement (0: skip it)")
     ap.add_argument("--served-requests", type=int, default=1000,
@@ -559,76 +701,28 @@ def main():
         print(json.dumps(out), flush=True)
         return
     gdl.set_full_mode(gdl.FULL_MODE_MERGE if args.mode == "merge" else gdl.FULL_MODE_AUTO)
-    kind, make, desc = workloads(gdl, args.nodes, args.window)[args.workload]
+    wl = workloads(gdl, args.nodes, args.window)
+    kind, make, desc = wl[args.workload]
     inp = make()
     ctx = gdl.Context(device=0)
-    if kind == "cspa":
-        dev = (ctx.to_device(inp[0]), ctx.to_device(inp[1]))
-
-        def step():
-            return ctx.run_cspa(dev[0], dev[1], on_device=True, want_keys=False,
-                                want_trace=False)
-        n_in = int(inp[0].size + inp[1].size)
-    else:
-        dev = (ctx.to_device(inp),)
-        run = ctx.run_tc if kind == "tc" else ctx.run_sg
-
-        def step():
-            return run(dev[0], on_device=True, want_keys=False, want_trace=False)
-        n_in = int(inp.size)
-
-    res = None
-    for _ in range(args.warmup):
-        res = step()
-    # the timed steps run without per-kernel events (two hipEventRecords per
-    # launch would add host time to ~1,000 launches a step); the same steps
-    # then run again with hipEvents on the engine stream for the roofline
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        res = step()
-    ctx.sync()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = 1000.0 * elapsed / args.steps
-    derived_per_s = res["derived"] / (elapsed / args.steps)
-    ctx.stats_reset()
-    ctx.stats_enable(True)
-    ctx.sync()
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    ctx.sync()
-    ms_instrumented = 1000.0 * (time.perf_counter() - t1) / args.steps
-    ctx.stats_enable(False)
-    stats = ctx.stats()
-
-    # dominant kernel by hipEvent time
-    timed = {k: v for k, v in stats.items() if v["launches"] and v["total_ms"] > 0}
-    dom = max(timed.items(), key=lambda kv: kv[1]["total_ms"]) if timed else None
-    roofline = None
-    if dom:
-        name, s = dom
-        avg_ms = s["total_ms"] / s["launches"]
-        avg_bytes = s["alg_bytes"] / s["launches"]
-        achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
-        traffic = profiled_traffic(name, desc.split(":")[0])
-        roofline = dict(bound="hbm", kernel=name, achieved=round(achieved, 2),
-                        peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 5),
-                        traffic=traffic, avg_launch_us=round(avg_ms * 1e3, 3),
-                        avg_alg_bytes=round(avg_bytes))
-        if traffic is not None:
-            roofline["traffic_unit"] = "HBM bytes per launch (rocprofv3 --pmc, profiles/)"
-            roofline["traffic_gbs"] = round(traffic / (avg_ms * 1e-3) / 1e9, 2)
-    # whole-step HBM fraction: all algorithmic bytes over the step time
-    tot_bytes = sum(v["alg_bytes"] for v in stats.values()) / args.steps
-    kernel_ms = sum(v["total_ms"] for v in stats.values()) / args.steps
-    phases = {k: dict(launches=v["launches"] // args.steps,
-                      ms=round(v["total_ms"] / args.steps, 3))
-              for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["total_ms"])
-              if v["launches"]}
-
-    for d in dev:
-        d.free()
+    head = measure(gdl, ctx, kind, inp, desc, args.steps, args.warmup)
+    res, ms_per_step = head["res"], head["ms_per_step"]
+    derived_per_s = head["derived_per_s"]
+    # the deep-tree TC workload beside the headline (its own roofline: the
+    # stream-ordered linear loop of small bodies)
+    beside = None
+    if args.workload == "sg" and not args.no_beside:
+        bk, bmake, bdesc = wl["tree"]
+        binp = bmake()
+        b = measure(gdl, ctx, bk, binp, bdesc, args.steps, args.warmup)
+        beside = {"workload": bdesc, "value": round(b["derived_per_s"], 1),
+                  "unit": "derived tuples/s", "ms_per_step": round(b["ms_per_step"], 3),
+                  "output_tuples": b["res"]["n_out"], "iterations": b["res"]["iterations"],
+                  "derived_tuples": b["res"]["derived"], "roofline": b["roofline"],
+                  "step_hbm_frac": b["step_hbm_frac"], "phases": b["phases"],
+                  "cpu_baseline": None if args.no_cpu else
+                  cpu_baseline(bk, binp, args.cpu_iterations)}
+    roofline, phases = head["roofline"], head["phases"]
     gdl.set_full_mode(gdl.FULL_MODE_AUTO)
     dlog = datalog_vs_hand(gdl, ctx, args.datalog_reps) if args.datalog_reps > 0 else None
     plen = path_length_vs_tc(gdl, ctx, args.datalog_reps) if args.datalog_reps > 0 else None
@@ -660,19 +754,20 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64",
-        "data": "synthetic (SplitMix64 seed 0)",
-        "config": {"workload": desc, "full_mode": args.mode, "input_tuples": n_in,
+        "data": "synthetic (SplitMix64 generators; R-MAT seed 1, tree seed 0)",
+        "config": {"workload": desc, "full_mode": args.mode, "input_tuples": head["n_in"],
                    "output_tuples": res["n_out"], "iterations": res["iterations"],
                    "init_iterations": res["init_iterations"],
                    "derived_tuples": res["derived"]},
         "roofline": roofline,
-        "step_hbm_frac": round(tot_bytes / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9), 5),
-        "kernel_busy_frac": round(min(1.0, kernel_ms / ms_per_step), 4),
-        "ms_per_step_with_kernel_events": round(ms_instrumented, 3),
+        "step_hbm_frac": head["step_hbm_frac"],
+        "kernel_busy_frac": head["kernel_busy_frac"],
+        "ms_per_step_with_kernel_events": head["ms_instrumented"],
         "phases": phases,
         "cpu_baseline": cpu,
         "batched": batch,
         "served": serve,
+        "beside": beside,
         "datalog": dlog,
         "path_length": plen,
         "micro": micro_out,

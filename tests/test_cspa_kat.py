@@ -264,11 +264,11 @@ def test_empty_inner_zeroes_the_outer_delta():
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["merge", "auto"])
 def test_gpu_cspa_trace_is_the_hand_derived_one(ctx, mode):
-    gdl.set_full_mode(mode)
+    gdl.set_full_mode(gdl.FULL_MODE_MERGE if mode == "merge" else gdl.FULL_MODE_AUTO)
     try:
         got = ctx.run_cspa(ASSIGN, DEREF, out_cap=1000, extra_cap=1000)
     finally:
-        gdl.set_full_mode("auto")
+        gdl.set_full_mode(gdl.FULL_MODE_AUTO)
     assert got["init_iterations"] == 3 and got["iterations"] == 5
     assert [tuple(map(int, t)) for t in got["trace_init"]] == INIT_TRACE
     assert [tuple(map(int, t)) for t in got["trace"]] == ANALYSIS_TRACE
