@@ -22,6 +22,7 @@ INCLUDE_DIR = os.path.join(REPO, "include")
 
 GDL_OK = 0
 GDL_ERR_INVALID_ARG = 1
+GDL_ERR_OVERFLOW = 5
 STATUS_NAMES = {0: "ok", 1: "invalid argument", 2: "out of device memory", 3: "HIP error",
                 4: "unsupported", 5: "size limit exceeded", 6: "device-side error",
                 7: "busy"}
@@ -47,7 +48,8 @@ class Index(Structure):
 
 
 class HashSet(Structure):
-    _fields_ = [("slots", c_void_p), ("capacity", c_uint64), ("size", c_uint64)]
+    _fields_ = [("slots", c_void_p), ("capacity", c_uint64), ("size", c_uint64),
+                ("key_bits", c_uint32), ("pad_", c_uint32)]
 
 
 FULL_MODE_AUTO, FULL_MODE_MERGE = 0, 1
@@ -178,6 +180,8 @@ def load():
     _decl(eng, "gdl_unique_u64", c_int, [vp, vp, c_uint64, vp, _U64P])
     _decl(eng, "gdl_key_bits", c_int, [vp, vp, c_uint64, POINTER(c_int), POINTER(c_int)])
     _decl(eng, "gdl_sort_unique_u64", c_int, [vp, vp, c_uint64, c_int, c_int, vp, _U64P])
+    _decl(eng, "gdl_hashset_create_bitmap", c_int, [vp, c_uint32, vp])
+    _decl(eng, "gdl_hashset_check", c_int, [vp, vp])
     _decl(eng, "gdl_merge_union_u64", c_int, [vp, vp, c_uint64, vp, c_uint64, vp, _U64P])
     _decl(eng, "gdl_set_difference_u64", c_int, [vp, vp, c_uint64, vp, c_uint64, vp, _U64P])
     _decl(eng, "gdl_merge_diff_u64", c_int,
@@ -204,6 +208,8 @@ def load():
     _decl(prog, "gdl_prog_cspa", c_int, [vp, vp, c_uint64, vp, c_uint64, c_int, rp, bp])
     _decl(prog, "gdl_prog_last_error", c_char_p, [])
     _decl(prog, "gdl_prog_path_length", c_int,
+          [vp, vp, c_uint64, c_int, rp, vp, vp, c_uint64])
+    _decl(prog, "gdl_prog_path_length_hooks", c_int,
           [vp, vp, c_uint64, c_int, rp, vp, vp, c_uint64])
     _decl(prog, "gdl_prog_set_full_mode", c_int, [c_int])
     _decl(prog, "gdl_prog_set_checkpoint", c_int, [ctypes.c_char_p, c_int, c_int, c_int])
@@ -544,6 +550,17 @@ class Context:
                     "hashset_create")
         return hs
 
+    def hashset_create_bitmap(self, key_bits):
+        hs = HashSet()
+        self._check(self.eng.gdl_hashset_create_bitmap(self.h, key_bits, ctypes.byref(hs)),
+                    "hashset_create_bitmap")
+        return hs
+
+    def hashset_check(self, hs):
+        """Status of gdl_hashset_check (0, or the overflow status of a bitmap
+        set that was offered a key outside its domain)."""
+        return self.eng.gdl_hashset_check(self.h, ctypes.byref(hs))
+
     def hashset_free(self, hs):
         self._check(self.eng.gdl_hashset_free(self.h, ctypes.byref(hs)), "hashset_free")
 
@@ -745,17 +762,21 @@ class Context:
             if isinstance(keep, DeviceArray):
                 keep.free()
 
-    def run_path_length(self, edges, on_device=False, out_cap=None):
+    def run_path_length(self, edges, on_device=False, out_cap=None, hooks=False):
         """All-pairs hop counts (the reference's path_length program): returns
-        the pairs (sorted keys), their hop counts, and the run's statistics."""
+        the pairs (sorted keys), their hop counts, and the run's statistics.
+        hooks=True runs the reference's own encoding (an arity-3 relation with
+        a dependent column and an arithmetic generator hook run on the device)
+        instead of the TC-based program."""
         p, n, keep = self._edges(edges, on_device)
         cap = out_cap if out_cap is not None else 60_000_000
         keys = np.empty(max(cap, 1), dtype=np.uint64)
         lens = np.empty(max(cap, 1), dtype=np.uint32)
         res = ProgResult()
         try:
-            st = self.prog.gdl_prog_path_length(self.h, p, n, int(on_device), ctypes.byref(res),
-                                                keys.ctypes.data, lens.ctypes.data, cap)
+            fn = self.prog.gdl_prog_path_length_hooks if hooks else self.prog.gdl_prog_path_length
+            st = fn(self.h, p, n, int(on_device), ctypes.byref(res), keys.ctypes.data,
+                    lens.ctypes.data, cap)
         finally:
             if isinstance(keep, DeviceArray):
                 keep.free()

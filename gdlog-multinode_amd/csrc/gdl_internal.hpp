@@ -372,11 +372,41 @@ __device__ __forceinline__ bool hs_probe_from(u64 *slots, u64 mask, u64 k, u64 s
     return false;  // unreachable: the table is never full
 }
 
+// A set argument with this bit set is a bitmap over a dense key domain: the
+// low 6 bits are key_bits k, bit (c0 << k) | c1 of the word array stands for
+// the key (c0 << 32) | c1, and the word after the 2^(2k) bits is a flag set
+// when a key outside the domain was offered (it is not inserted).
+constexpr u64 kHsBitmap = 1ull << 63;
+
+__device__ __forceinline__ u64 bm_bit(u64 k, int kb) {
+    return ((k >> 32) << kb) | (k & 0xffffffffull);
+}
+
+__device__ __forceinline__ bool bm_in_domain(u64 k, int kb) {
+    return ((k >> 32) >> kb) == 0 && ((k & 0xffffffffull) >> kb) == 0;
+}
+
+__device__ __forceinline__ u64 bm_flag_word(int kb) {
+    return kb >= 3 ? (1ull << (2 * kb - 6)) : 1ull;
+}
+
+// Inserts k into a bitmap set; true when k was not a member.
+__device__ __forceinline__ bool bm_insert_key(u64 *words, int kb, u64 k) {
+    if (!bm_in_domain(k, kb)) {
+        atomicOr((unsigned long long *)&words[bm_flag_word(kb)], 1ull);
+        return false;
+    }
+    const u64 b = bm_bit(k, kb);
+    const u64 m = 1ull << (b & 63);
+    return (atomicOr((unsigned long long *)&words[b >> 6], (unsigned long long)m) & m) == 0;
+}
+
 // CAS the home slot first: one round trip decides most inserts (reading
 // first instead costs a second round trip for every key that is new, and
 // was slower even where most keys are already members); past the home slot
 // the probe reads kHsScan slots at a time.
 __device__ __forceinline__ bool hs_insert_key(u64 *slots, u64 mask, u64 k) {
+    if (mask & kHsBitmap) return bm_insert_key(slots, (int)(mask & 63), k);
     if (k == kHsEmpty) return atomicExch((unsigned int *)&slots[mask + 1], 1u) == 0u;
     const u64 s = mix64(k) & mask;
     const unsigned long long old = atomicCAS(
@@ -397,6 +427,12 @@ constexpr u64 kBucket = 4;
 // or not (a hash probe for an absent prefix walks to a bucket with room).
 constexpr u64 kIdxDense = 1ull << 63;
 inline u64 idx_arg(const gdl_index *idx) { return idx->n_slots | (idx->dense ? kIdxDense : 0); }
+
+// The set argument of a kernel: capacity - 1 of a hash table, or the bitmap
+// flag with key_bits.
+inline u64 hs_arg(const gdl_hashset *hs) {
+    return hs->key_bits ? (kHsBitmap | hs->key_bits) : hs->capacity - 1;
+}
 
 __device__ __forceinline__ bool idx_probe(const IdxSlot *slots, u64 n_slots, u64 p, u64 *begin,
                                           u64 *count) {

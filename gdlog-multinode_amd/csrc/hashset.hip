@@ -86,6 +86,41 @@ __global__ __launch_bounds__(HB) void k_hs_fill(u64 *slots, u64 cap) {
         slots[i] = i < cap - 1 ? kEmpty : 0ull;  // last word: the ~0 flag (cleared)
 }
 
+__global__ __launch_bounds__(HB) void k_bm_clear(u64 *words, u64 n) {
+    for (u64 i = (u64)blockIdx.x * HB + threadIdx.x; i < n; i += (u64)gridDim.x * HB)
+        words[i] = 0ull;
+}
+
+// Members of a bitmap set: each lane takes one word and emits the key of
+// every set bit, one atomic per wave for the positions.
+__global__ __launch_bounds__(HB) void k_bm_extract(const u64 *words, u64 n_words, int kb, u64 *out,
+                                                   u64 out_cap, unsigned long long *counter) {
+    const u64 lo_mask = (1ull << kb) - 1;
+    for (u64 base = (u64)blockIdx.x * HB; base < n_words; base += (u64)gridDim.x * HB) {
+        const u64 i = base + threadIdx.x;
+        u64 w = i < n_words ? words[i] : 0ull;
+        const u32 c = (u32)__popcll(w);
+        // wave prefix sum of the per-lane bit counts
+        u64 incl = c;
+        for (int d = 1; d < 64; d <<= 1) {
+            const u64 v = __shfl_up(incl, d, 64);
+            if ((int)(threadIdx.x & 63) >= d) incl += v;
+        }
+        const u64 total = __shfl(incl, 63, 64);
+        unsigned long long pos = 0;
+        if ((threadIdx.x & 63) == 63 && total) pos = atomicAdd(counter, (unsigned long long)total);
+        pos = __shfl(pos, 63, 64);
+        u64 at = pos + incl - c;
+        while (w) {
+            const int b = __ffsll((unsigned long long)w) - 1;
+            w &= w - 1;
+            const u64 bit = i * 64 + (u64)b;
+            if (at < out_cap) out[at] = ((bit >> kb) << 32) | (bit & lo_mask);
+            ++at;
+        }
+    }
+}
+
 static unsigned hs_grid(gdl_ctx *ctx, u64 n) {
     const u64 cap = grid_cap(ctx, 2);
     u64 g = (n + HB - 1) / HB;
@@ -130,6 +165,7 @@ gdl_status hashset_reserve(gdl_ctx *ctx, gdl_hashset *hs, u64 incoming) {
 // growing is rehashed every few multiples of its size, not at every doubling.
 gdl_status hashset_reserve2(gdl_ctx *ctx, gdl_hashset *hs, u64 expected, u64 worst, double load,
                             u64 grow_to) {
+    if (hs->key_bits) return GDL_OK;  // a bitmap holds its whole domain
     if (!(load > 0.0)) load = ctx->cfg.load_factor > 0.f ? (double)ctx->cfg.load_factor : 0.5;
     const double lf = std::min(0.8, load);
     expected = std::min(expected, worst);
@@ -167,7 +203,7 @@ gdl_status hashset_insert_new(gdl_ctx *ctx, gdl_hashset *hs, const u64 *keys, u6
     {
         KTimer kt(ctx, K_HS_INSERT, 24.0 * n);
         hipLaunchKernelGGL(k_hs_insert_new, dim3(hs_grid(ctx, n)), dim3(HB), 0, ctx->stream,
-                           (u64 *)hs->slots, hs->capacity - 1, keys, n, d_new,
+                           (u64 *)hs->slots, hs_arg(hs), keys, n, d_new,
                            (unsigned long long *)counter);
         GDL_TRY(check_launch(ctx, "k_hs_insert_new"));
     }
@@ -187,7 +223,52 @@ gdl_status gdl_hashset_create(gdl_ctx *ctx, uint64_t expected, gdl_hashset *out)
     out->slots = nullptr;
     out->capacity = 0;
     out->size = 0;
+    out->key_bits = 0;
+    out->pad_ = 0;
     return hashset_reserve(ctx, out, expected);
+}
+
+gdl_status gdl_hashset_create_bitmap(gdl_ctx *ctx, uint32_t key_bits, gdl_hashset *out) {
+    if (!ctx || !out) return GDL_ERR_INVALID_ARG;
+    if (key_bits < 1 || key_bits > 16)
+        return fail(ctx, GDL_ERR_INVALID_ARG, "bitmap set: key_bits must be 1..16");
+    out->slots = nullptr;
+    out->capacity = 0;
+    out->size = 0;
+    out->key_bits = key_bits;
+    out->pad_ = 0;
+    const u64 bits = 1ull << (2 * key_bits);
+    const u64 words = (bits + 63) / 64 + 1;  // + the out-of-domain flag word
+    void *p = nullptr;
+    std::string err;
+    gdl_status st = ctx->alloc->alloc(words * sizeof(u64), &p, &err);
+    if (st != GDL_OK) return fail(ctx, st, err);
+    {
+        KTimer kt(ctx, K_FILL, 8.0 * words);
+        hipLaunchKernelGGL(k_bm_clear, dim3(hs_grid(ctx, words)), dim3(HB), 0, ctx->stream,
+                           (u64 *)p, words);
+        gdl_status ls = check_launch(ctx, "k_bm_clear");
+        if (ls != GDL_OK) {
+            ctx->alloc->free(p, nullptr);
+            return ls;
+        }
+    }
+    out->slots = p;
+    out->capacity = bits;
+    return GDL_OK;
+}
+
+gdl_status gdl_hashset_check(gdl_ctx *ctx, const gdl_hashset *hs) {
+    if (!ctx || !hs) return GDL_ERR_INVALID_ARG;
+    if (!hs->key_bits || !hs->slots) return GDL_OK;
+    const u64 flag_at = hs->key_bits >= 3 ? (1ull << (2 * hs->key_bits - 6)) : 1ull;
+    u64 flag = 0;
+    GDL_TRY(read_u64(ctx, (const u64 *)hs->slots + flag_at, &flag));
+    if (flag)
+        return fail(ctx, GDL_ERR_OVERFLOW, "bitmap set: a key outside its " +
+                                               std::to_string(hs->key_bits) +
+                                               "-bit id domain was offered");
+    return GDL_OK;
 }
 
 gdl_status gdl_hashset_free(gdl_ctx *ctx, gdl_hashset *hs) {
@@ -200,6 +281,7 @@ gdl_status gdl_hashset_free(gdl_ctx *ctx, gdl_hashset *hs) {
     hs->slots = nullptr;
     hs->capacity = 0;
     hs->size = 0;
+    hs->key_bits = 0;
     return GDL_OK;
 }
 
@@ -223,7 +305,14 @@ gdl_status gdl_hashset_extract(gdl_ctx *ctx, const gdl_hashset *hs, uint64_t *d_
     u64 *counter;
     GDL_TRY(scr.get(1, &counter));
     GDL_HIP(ctx, hipMemsetAsync(counter, 0, sizeof(u64), ctx->stream));
-    {
+    if (hs->key_bits) {
+        const u64 words = (hs->capacity + 63) / 64;
+        KTimer kt(ctx, K_HS_EXTRACT, 8.0 * words + 8.0 * hs->size);
+        hipLaunchKernelGGL(k_bm_extract, dim3(hs_grid(ctx, words)), dim3(HB), 0, ctx->stream,
+                           (const u64 *)hs->slots, words, (int)hs->key_bits, d_out, out_cap,
+                           (unsigned long long *)counter);
+        GDL_TRY(check_launch(ctx, "k_bm_extract"));
+    } else {
         KTimer kt(ctx, K_HS_EXTRACT, 8.0 * hs->capacity + 8.0 * hs->size);
         hipLaunchKernelGGL(k_hs_extract, dim3(hs_grid(ctx, hs->capacity + 1)), dim3(HB), 0,
                            ctx->stream, (const u64 *)hs->slots, hs->capacity, d_out, out_cap,

@@ -467,26 +467,42 @@ __global__ __launch_bounds__(JB) void k_join_insert_lb(
             }
         }
         produced += __popc(live);
-        // home-slot CAS of every key first (independent round trips in flight)
+        // home-slot CAS (bitmap: the OR of the key's bit) of every key first,
+        // independent round trips in flight
         unsigned long long old[JM_I];
+        u32 bm = 0;  // bit r: key r went to a bitmap set (old[r]: its bit before)
 #pragma unroll
         for (int r = 0; r < JM_I; ++r) {
             old[r] = 0;
             if ((live >> r) & 1u) {
                 const gdl_hs_seg hs = sel_set<SEG>(sel, s_segs, keys[r]);
-                if (!hs.slots) live &= ~(1u << r);
-                else if (keys[r] != kHsEmpty)
+                if (!hs.slots) {
+                    live &= ~(1u << r);
+                } else if (hs.mask & kHsBitmap) {
+                    const int kb = (int)(hs.mask & 63);
+                    if (bm_in_domain(keys[r], kb)) {
+                        const u64 b = bm_bit(keys[r], kb);
+                        const u64 m = 1ull << (b & 63);
+                        old[r] = atomicOr((unsigned long long *)((u64 *)hs.slots + (b >> 6)),
+                                          (unsigned long long)m) & m;
+                        bm |= 1u << r;
+                    }
+                } else if (keys[r] != kHsEmpty) {
                     old[r] = atomicCAS(
                         (unsigned long long *)((u64 *)hs.slots + (mix64(keys[r]) & hs.mask)),
                         (unsigned long long)kHsEmpty, (unsigned long long)keys[r]);
+                }
             }
         }
 #pragma unroll
         for (int r = 0; r < JM_I; ++r) {
             bool fresh = false;
-            if ((live >> r) & 1u) {
+            if ((bm >> r) & 1u) {
+                fresh = old[r] == 0;
+            } else if ((live >> r) & 1u) {
                 if (keys[r] == kHsEmpty || (old[r] != (unsigned long long)kHsEmpty &&
-                                            old[r] != (unsigned long long)keys[r])) {
+                                            old[r] != (unsigned long long)keys[r]) ||
+                    (sel_set<SEG>(sel, s_segs, keys[r]).mask & kHsBitmap)) {
                     // collision (or the key ~0): the full probe
                     const gdl_hs_seg hs = sel_set<SEG>(sel, s_segs, keys[r]);
                     fresh = hs_insert_key((u64 *)hs.slots, hs.mask, keys[r]);
@@ -809,7 +825,7 @@ static gdl_status join_insert_impl(gdl_ctx *ctx, const u64 *d_inner, u64 n_inner
         // only has to fit
         GDL_TRY(hashset_reserve2(ctx, hs, std::max<u64>(2 * n_outer, 1024), bound));
         sel.slots = (u64 *)hs->slots;
-        sel.mask = hs->capacity - 1;
+        sel.mask = hs_arg(hs);
         if (*n_new + bound > *new_cap || !*d_new) {
             const u64 cap = *n_new + bound + (*n_new + bound) / 4 + 1024;
             void *p = nullptr;
@@ -981,7 +997,7 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
         h.delta_begin = d0;
         h.delta_end = d1;
         h.size = hs->size;
-        h.hs_limit = (u64)(kHsHardLoad * (double)hs->capacity);
+        h.hs_limit = hs->key_bits ? ~0ull : (u64)(kHsHardLoad * (double)hs->capacity);
         h.log_cap = *log_cap;
         h.max_run = mr;
         h.max_bodies = (u32)k;
@@ -998,7 +1014,7 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
                                (const IdxSlot *)inner_idx->slots, idx_arg(inner_idx), d_inner,
                                n_inner, shift, map.arity, map.src[0],
                                map.arity == 2 ? map.src[1] : 0, pred.op, pred.lhs, pred.rhs,
-                               (u64 *)hs->slots, hs->capacity - 1, *d_log, st, counts);
+                               (u64 *)hs->slots, hs_arg(hs), *d_log, st, counts);
             GDL_TRY(check_launch(ctx, "k_linear_body"));
         }
         LinState r{};

@@ -108,6 +108,41 @@ struct PredFlag {
         return eval_pred(k, op, lhs, rhs);
     }
 };
+// keeps in[i] where flags[i] != 0
+struct ByteFlag {
+    const uint8_t *flags;
+    __device__ __forceinline__ bool operator()(const u64 *, u64 i, u64) const {
+        return flags[i] != 0;
+    }
+};
+
+// First index in [lo, hi) of sorted keys whose prefix (key >> shift) is >= p.
+__device__ __forceinline__ u64 prefix_lower_bound(const u64 *keys, u64 lo, u64 hi, u64 p,
+                                                  int shift) {
+    while (lo < hi) {
+        const u64 mid = lo + ((hi - lo) >> 1);
+        if ((keys[mid] >> shift) < p) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// std::set_difference(b, a) under the order of the prefix key >> shift, over
+// arrays sorted by key (hence by prefix): of the m elements of b sharing a
+// prefix that n elements of a have, the first min(m, n) are dropped and the
+// rest kept, in order.
+struct PrefixDiffFlag {
+    const u64 *a;
+    u64 na;
+    int shift;
+    __device__ __forceinline__ bool operator()(const u64 *b, u64 i, u64 k) const {
+        const u64 p = k >> shift;
+        const u64 first_b = prefix_lower_bound(b, 0, i, p, shift);
+        const u64 lo = prefix_lower_bound(a, 0, na, p, shift);
+        const u64 n = prefix_lower_bound(a, lo, na, p + 1, shift) - lo;
+        return i - first_b >= n;
+    }
+};
 
 template <typename F>
 __global__ __launch_bounds__(TB) void k_compact_count(const u64 *in, u64 n, F flag,
@@ -265,6 +300,11 @@ gdl_status unique_keys(gdl_ctx *ctx, const u64 *d_in, u64 n, u64 *d_out, u64 *n_
     return compact_impl(ctx, d_in, n, UniqueFlag{}, d_out, n_out);
 }
 
+gdl_status compact_flags(gdl_ctx *ctx, const u64 *d_in, const uint8_t *flags, u64 n, u64 *d_out,
+                         u64 *n_out) {
+    return compact_impl(ctx, d_in, n, ByteFlag{flags}, d_out, n_out);
+}
+
 gdl_status filter_keys(gdl_ctx *ctx, const u64 *d_in, u64 n, gdl_pred pred, u64 *d_out,
                        u64 *n_out) {
     return compact_impl(ctx, d_in, n, PredFlag{pred.op, pred.lhs, pred.rhs}, d_out, n_out);
@@ -287,6 +327,21 @@ gdl_status gdl_filter_u64(gdl_ctx *ctx, const uint64_t *d_in, uint64_t n, gdl_pr
     if (!ctx || !n_out || (n && (!d_in || !d_out))) return GDL_ERR_INVALID_ARG;
     if (!valid_pred(pred, 2)) return fail(ctx, GDL_ERR_INVALID_ARG, "gdl_filter_u64: bad predicate");
     return filter_keys(ctx, d_in, n, pred, d_out, n_out);
+}
+
+gdl_status gdl_compact_flags_u64(gdl_ctx *ctx, const uint64_t *d_in, const uint8_t *d_flags,
+                                 uint64_t n, uint64_t *d_out, uint64_t *n_out) {
+    if (!ctx || !n_out || (n && (!d_in || !d_flags || !d_out))) return GDL_ERR_INVALID_ARG;
+    return compact_flags(ctx, d_in, d_flags, n, d_out, n_out);
+}
+
+gdl_status gdl_set_difference_prefix_u64(gdl_ctx *ctx, const uint64_t *d_b, uint64_t nb,
+                                         const uint64_t *d_a, uint64_t na, int shift,
+                                         uint64_t *d_out, uint64_t *n_out) {
+    if (!ctx || !n_out || (nb && (!d_b || !d_out)) || (na && !d_a)) return GDL_ERR_INVALID_ARG;
+    if (shift < 0 || shift > 63)
+        return fail(ctx, GDL_ERR_INVALID_ARG, "gdl_set_difference_prefix_u64: shift 0..63");
+    return compact_impl(ctx, d_b, nb, PrefixDiffFlag{d_a, na, shift}, d_out, n_out);
 }
 
 gdl_status gdl_join_scan(gdl_ctx *ctx, const uint32_t *d_counts, uint64_t n, uint64_t *d_offsets,

@@ -14,6 +14,9 @@
 #include "gdlog/exception.h"
 #include "gdlog/lie.h"
 
+#include <cstdlib>
+#include <string>
+
 using namespace gdlog;
 
 void LIE::add_ra(ra_op op) { ra_ops.push_back(op); }
@@ -49,6 +52,37 @@ bool full_is_inner(const std::vector<ra_op> &ops, const Relation *rel) {
 
 // True when only joins write rel and no operator reads rel's FULL or NEWT
 // (reading DELTA is fine: the pending delta is a separate buffer).
+// True when an operator runs its device hooks (not a column map) or moves
+// tuples of arity 3 or 4.
+bool op_uses_hooks(const ra_op &op) {
+    if (const auto *j = std::get_if<RelationalJoin>(&op))
+        return j->uses_hooks() || j->inner_rel->arity > 2 || j->outer_rel->arity > 2 ||
+               j->output_rel->arity > 2;
+    if (const auto *c = std::get_if<RelationalCopy>(&op))
+        return !c->copy_map.ok || c->src_rel->arity > 2 || c->dest_rel->arity > 2;
+    if (const auto *a = std::get_if<RelationalACopy>(&op))
+        return !a->copy_map.ok || a->src_rel->arity > 2 || a->dest_rel->arity > 2;
+    return false;
+}
+
+// A relation whose FULL stays one sorted array merged every iteration (the
+// reference's structure): dependent columns (a prefix set difference), keys
+// of arity 3 or 4, or an output of a hook-running operator.
+bool needs_sorted_full(const std::vector<ra_op> &ops, const Relation *rel) {
+    if (rel->dependent_column_size > 0 || rel->arity > 2) return true;
+    for (const auto &op : ops) {
+        if (!op_uses_hooks(op)) continue;
+        if (const auto *j = std::get_if<RelationalJoin>(&op)) {
+            if (j->output_rel == rel) return true;
+        } else if (const auto *c = std::get_if<RelationalCopy>(&op)) {
+            if (c->dest_rel == rel) return true;
+        } else if (const auto *a = std::get_if<RelationalACopy>(&op)) {
+            if (a->dest_rel == rel) return true;
+        }
+    }
+    return false;
+}
+
 bool output_fusable(const std::vector<ra_op> &ops, const Relation *rel) {
     for (const auto &op : ops) {
         if (const auto *j = std::get_if<RelationalJoin>(&op)) {
@@ -90,10 +124,26 @@ void log_with_delta_tail(Relation *rel, bool keep_log) {
     container_take(rel->full, log, nf, nf + nd, false);
 }
 
-void enter_append_mode(Relation *rel) {
+// FULL's membership set: a bitmap when every id of the program fits 16 bits
+// (one bit per possible pair, at most 512 MiB; an insert is one atomic OR and
+// the set never grows), else the hash table.  GDLOG_FULL_SET=hash forces the
+// hash table.
+static bool bitmap_full_set(bool hooks) {
+    const int b = id_bits();
+    // hooks may produce ids beyond those loaded so far
+    if (hooks || b < 1 || b > 16) return false;
+    const char *e = std::getenv("GDLOG_FULL_SET");
+    return !(e && std::string(e) == "hash");
+}
+
+void enter_append_mode(Relation *rel, bool hooks) {
     gdl_ctx *ctx = context();
     const tuple_size_t n = rel->current_full_size;
-    check(gdl_hashset_create(ctx, n + 1024, &rel->full_set), "gdl_hashset_create");
+    if (bitmap_full_set(hooks))
+        check(gdl_hashset_create_bitmap(ctx, (uint32_t)id_bits(), &rel->full_set),
+              "gdl_hashset_create_bitmap");
+    else
+        check(gdl_hashset_create(ctx, n + 1024, &rel->full_set), "gdl_hashset_create");
     if (n) {
         uint64_t *scratch = device_alloc_keys(n);
         uint64_t added = 0;
@@ -118,6 +168,7 @@ void leave_append_mode(Relation *rel, bool keep_log) {
         check(st, "gdl_sort_u64");
         container_take(rel->full, sorted, n, n, true);
     }
+    check(gdl_hashset_check(ctx, &rel->full_set), "gdl_hashset_check");
     check(gdl_hashset_free(ctx, &rel->full_set), "gdl_hashset_free");
     device_free(rel->pend_keys);
     rel->pend_keys = nullptr;
@@ -347,15 +398,17 @@ void LIE::fixpoint_loop() {
     }
     // keep_full_log needs the append log whatever the thread's FULL mode
     const bool auto_mode = keep_full_log || full_mode() == FullMode::Auto;
+    bool lie_uses_hooks = false;
+    for (const auto &op : ra_ops) lie_uses_hooks = lie_uses_hooks || op_uses_hooks(op);
     for (Relation *rel : update_relations) {
         if (!resumed) {
             rel->current_full_size = rel->full->tuple_counts;
             seed_delta(rel);
         }
-        if (auto_mode && !full_is_inner(ra_ops, rel)) {
+        if (auto_mode && !full_is_inner(ra_ops, rel) && !needs_sorted_full(ra_ops, rel)) {
             if (resumed && rel->full->sorted && rel->delta->keys && rel->delta->tuple_counts)
                 log_with_delta_tail(rel, keep_full_log);
-            enter_append_mode(rel);
+            enter_append_mode(rel, lie_uses_hooks);
             rel->fused_output = output_fusable(ra_ops, rel);
         } else if (resumed && !rel->full->sorted) {
             container_normalize(rel->full);  // the merge needs a sorted FULL
@@ -508,8 +561,19 @@ void LIE::fixpoint_loop() {
             uint64_t *new_full = device_alloc_keys(nf + nn);
             uint64_t *delta = device_alloc_keys(nn);
             uint64_t nu = 0, nd = 0;
-            gdl_status st = gdl_merge_diff_u64(ctx, rel->full->keys, nf, newt->keys, nn, new_full,
-                                               &nu, delta, &nd);
+            gdl_status st;
+            if (rel->dependent_column_size > 0) {
+                // set difference over the first arity - dependent columns
+                // (lie.cu:131-138), then the merge of the delta into FULL
+                const int last = rel->arity - rel->dependent_column_size - 1;
+                st = gdl_set_difference_prefix_u64(ctx, newt->keys, nn, rel->full->keys, nf,
+                                                   rel->layout.shift[last], delta, &nd);
+                if (st == GDL_OK)
+                    st = gdl_merge_union_u64(ctx, rel->full->keys, nf, delta, nd, new_full, &nu);
+            } else {
+                st = gdl_merge_diff_u64(ctx, rel->full->keys, nf, newt->keys, nn, new_full, &nu,
+                                        delta, &nd);
+            }
             if (st != GDL_OK) {
                 device_free(new_full);
                 device_free(delta);

@@ -76,7 +76,165 @@ void append_raw(GHashRelContainer *newt, uint64_t *fresh, tuple_size_t n) {
     device_free(fresh);
 }
 
+gdlog::HookLayout hook_layout(const Relation *r) {
+    gdlog::HookLayout h{};
+    h.arity = r->layout.arity;
+    for (int c = 0; c < 4; ++c) {
+        h.bits[c] = r->layout.bits[c];
+        h.shift[c] = r->layout.shift[c];
+    }
+    return h;
+}
+
+int bits_of(uint64_t v) { return v ? 64 - __builtin_clzll((unsigned long long)v) : 0; }
+
+// After a hook kernel: the overflow flag, and the produced keys' bit bound
+// for the radix sorts that follow.
+void finish_hook_stats(gdl_ctx *ctx, const uint64_t *d_stats, const std::string &what) {
+    uint64_t st[3] = {0, 0, 0};
+    check(gdl_memcpy_d2h(ctx, st, d_stats, sizeof(st)), "hook stats");
+    if (st[2])
+        throw Error(GDL_ERR_OVERFLOW, what + ": a hook produced a value that does not fit its "
+                                             "column of the output relation's key layout");
+    note_id_bits(bits_of(st[0] | st[1]));
+}
+
+// Stable compaction of the kept outputs; frees out and keep.
+tuple_size_t compact_kept(gdl_ctx *ctx, uint64_t *out, uint64_t *keep, tuple_size_t n,
+                          uint64_t **kept) {
+    uint64_t *k = device_alloc_keys(n);
+    uint64_t m = 0;
+    gdl_status st = gdl_compact_flags_u64(ctx, out, (const uint8_t *)keep, n, k, &m);
+    device_free(out);
+    device_free(keep);
+    if (st != GDL_OK) device_free(k);
+    check(st, "gdl_compact_flags_u64");
+    if (m == 0) {
+        device_free(k);
+        k = nullptr;
+    }
+    *kept = k;
+    return m;
+}
+
+uint64_t *zeroed_stats(gdl_ctx *ctx) {
+    uint64_t *d = device_alloc_keys(3);
+    const uint64_t z[3] = {0, 0, 0};
+    check(gdl_memcpy_h2d(ctx, d, z, sizeof(z)), "hook stats");
+    return d;
+}
+
 }  // namespace
+
+// The join through its device hooks (gdlog/hook_exec.h): the runs are
+// counted and scanned on the engine, the hook kernel of the translation unit
+// that built the operator generates and filters every produced tuple, and the
+// kept ones come back in production order.
+static tuple_size_t run_hook_join(RelationalJoin &op, GHashRelContainer *inner,
+                              GHashRelContainer *outer, const gdl_index *idx, uint64_t **res) {
+    gdl_ctx *ctx = context();
+    *res = nullptr;
+    if (!op.hook_join)
+        throw Error(GDL_ERR_UNSUPPORTED, "RelationalJoin into " + op.output_rel->name +
+                                             ": no hook kernel for a non-column-map operator");
+    const tuple_size_t no = outer->tuple_counts;
+    uint64_t *counts = device_alloc_keys((no + 1) / 2);
+    uint64_t *begins = device_alloc_keys(no);
+    uint64_t *offsets = device_alloc_keys(no + 1);
+    uint64_t total = 0;
+    gdl_status st = gdl_join_count(ctx, inner->keys, inner->tuple_counts, idx, outer->keys, no,
+                                   (uint32_t *)counts, begins);
+    if (st == GDL_OK) st = gdl_join_scan(ctx, (const uint32_t *)counts, no, offsets, &total);
+    device_free(counts);
+    if (st != GDL_OK || total == 0) {
+        device_free(begins);
+        device_free(offsets);
+        check(st, "hook join count");
+        return 0;
+    }
+    uint64_t *out = device_alloc_keys(total);
+    uint64_t *keep = device_alloc_keys((total + 7) / 8);
+    uint64_t *stats = zeroed_stats(ctx);
+    gdlog::HookJoinArgs a{};
+    a.stream = (hipStream_t)gdl_stream(ctx);
+    a.gen = op.tuple_generator;
+    a.pred = op.tuple_pred;
+    a.inner = inner->keys;
+    a.inner_layout = hook_layout(op.inner_rel);
+    a.outer = outer->keys;
+    a.outer_layout = hook_layout(op.outer_rel);
+    a.n_outer = no;
+    a.offsets = offsets;
+    a.begins = begins;
+    a.total = total;
+    a.out_layout = hook_layout(op.output_rel);
+    a.out = out;
+    a.keep = (uint8_t *)keep;
+    a.stats = (unsigned long long *)stats;
+    const hipError_t e = op.hook_join(&a);
+    device_free(begins);
+    device_free(offsets);
+    if (e != hipSuccess) {
+        device_free(out);
+        device_free(keep);
+        device_free(stats);
+        throw Error(GDL_ERR_HIP, "RelationalJoin into " + op.output_rel->name +
+                                     ": hook kernel launch: " + hipGetErrorString(e));
+    }
+    try {
+        finish_hook_stats(ctx, stats, "RelationalJoin into " + op.output_rel->name);
+    } catch (...) {
+        device_free(out);
+        device_free(keep);
+        device_free(stats);
+        throw;
+    }
+    device_free(stats);
+    return compact_kept(ctx, out, keep, total, res);
+}
+
+// A copy through its device hook: one output per source tuple.
+static tuple_size_t run_hook_copy(gdlog::HookCopyLaunch launch, tuple_copy_hook cp,
+                              const Relation *src_rel, const GHashRelContainer *src,
+                              const Relation *dest_rel, uint64_t **res) {
+    gdl_ctx *ctx = context();
+    *res = nullptr;
+    if (!launch)
+        throw Error(GDL_ERR_UNSUPPORTED, "copy into " + dest_rel->name +
+                                             ": no hook kernel for a non-column-map operator");
+    const tuple_size_t n = src->tuple_counts;
+    uint64_t *out = device_alloc_keys(n);
+    uint64_t *keep = device_alloc_keys((n + 7) / 8);
+    uint64_t *stats = zeroed_stats(ctx);
+    gdlog::HookCopyArgs a{};
+    a.stream = (hipStream_t)gdl_stream(ctx);
+    a.cp = cp;
+    a.src = src->keys;
+    a.src_layout = hook_layout(src_rel);
+    a.n = n;
+    a.out_layout = hook_layout(dest_rel);
+    a.out = out;
+    a.keep = (uint8_t *)keep;
+    a.stats = (unsigned long long *)stats;
+    const hipError_t e = launch(&a);
+    if (e != hipSuccess) {
+        device_free(out);
+        device_free(keep);
+        device_free(stats);
+        throw Error(GDL_ERR_HIP, "copy into " + dest_rel->name + ": hook kernel launch: " +
+                                     hipGetErrorString(e));
+    }
+    try {
+        finish_hook_stats(ctx, stats, "copy into " + dest_rel->name);
+    } catch (...) {
+        device_free(out);
+        device_free(keep);
+        device_free(stats);
+        throw;
+    }
+    device_free(stats);
+    return compact_kept(ctx, out, keep, n, res);
+}
 
 void RelationalJoin::operator()() {
     const bool output_is_tmp = output_rel->tmp_flag;
@@ -91,12 +249,6 @@ void RelationalJoin::operator()() {
         if (empty_inner_zeroes_outer) outer->tuple_counts = 0;
         return;
     }
-    if (!gen_map.ok)
-        throw Error(GDL_ERR_UNSUPPORTED, "RelationalJoin into " + output_rel->name +
-                                             ": the generator hook is not a column selection");
-    if (!pred_map.ok)
-        throw Error(GDL_ERR_UNSUPPORTED, "RelationalJoin into " + output_rel->name +
-                                             ": the predicate is not a column comparison");
     const int k = (int)outer->index_column_size;
     if (k < 1 || k > inner->arity || k > outer->arity)
         throw Error(GDL_ERR_INVALID_ARG, "RelationalJoin: bad join column count");
@@ -104,6 +256,27 @@ void RelationalJoin::operator()() {
     {
         Phase ph(detail_time, 5);
         idx = container_index(inner, k);
+    }
+    if (uses_hooks() || inner_rel->arity > 2 || outer_rel->arity > 2 || output_rel->arity > 2) {
+        if (k != 1)
+            throw Error(GDL_ERR_UNSUPPORTED, "RelationalJoin into " + output_rel->name +
+                                                 ": a hook join matches on one column");
+        uint64_t *res = nullptr;
+        tuple_size_t m;
+        {
+            Phase ph(detail_time, 2);
+            m = run_hook_join(*this, inner, outer, idx, &res);
+        }
+        if (verbose())
+            std::cout << output_rel->name << "   " << k << " join result size(non dedup) " << m
+                      << std::endl;
+        if (disable_load || m == 0) {
+            device_free(res);
+            return;
+        }
+        if (output_is_tmp || output_rel->append_mode) append_raw(output_rel->newt, res, m);
+        else accumulate_sorted(output_rel->newt, res, m, detail_time);
+        return;
     }
     if (output_rel->fused_output && !output_is_tmp && !disable_load) {
         uint64_t m = 0;
@@ -214,12 +387,15 @@ void RelationalCopy::operator()() {
         if (empty_source_zeroes_dest) dest_rel->newt->tuple_counts = 0;  // copy.cu:23-26
         return;
     }
-    if (!copy_map.ok)
-        throw Error(GDL_ERR_UNSUPPORTED, "RelationalCopy into " + dest_rel->name +
-                                             ": the copy hook is not a column selection");
     tuple_size_t n = src->tuple_counts;
-    uint64_t *copied = device_alloc_keys(n);
-    check(gdl_project_u64(context(), src->keys, n, copy_map.map, copied), "gdl_project_u64");
+    uint64_t *copied = nullptr;
+    if (!copy_map.ok || src_rel->arity > 2 || dest_rel->arity > 2) {
+        n = run_hook_copy(hook_copy, tuple_generator, src_rel, src, dest_rel, &copied);
+        if (n == 0) return;
+    } else {
+        copied = device_alloc_keys(n);
+        check(gdl_project_u64(context(), src->keys, n, copy_map.map, copied), "gdl_project_u64");
+    }
     if (filter.op != GDL_PRED_NONE) {
         uint64_t *kept = device_alloc_keys(n);
         uint64_t m = 0;
@@ -246,12 +422,18 @@ void RelationalACopy::operator()() {
         container_free(dest);  // acopy.cu:19-23
         return;
     }
-    if (!copy_map.ok)
-        throw Error(GDL_ERR_UNSUPPORTED, "RelationalACopy into " + dest_rel->name +
-                                             ": the copy hook is not a column selection");
-    const tuple_size_t n = src->tuple_counts;
-    uint64_t *copied = device_alloc_keys(n);
-    check(gdl_project_u64(context(), src->keys, n, copy_map.map, copied), "gdl_project_u64");
+    tuple_size_t n = src->tuple_counts;
+    uint64_t *copied = nullptr;
+    if (!copy_map.ok || src_rel->arity > 2 || dest_rel->arity > 2) {
+        n = run_hook_copy(hook_copy, tuple_generator, src_rel, src, dest_rel, &copied);
+        if (n == 0) {
+            container_free(dest);
+            return;
+        }
+    } else {
+        copied = device_alloc_keys(n);
+        check(gdl_project_u64(context(), src->keys, n, copy_map.map, copied), "gdl_project_u64");
+    }
     if (dest_rel->append_mode) {
         container_take(dest, copied, n, n, false);
         return;

@@ -172,34 +172,101 @@ using namespace gdlog;
 
 namespace {
 
-// Packs host rows of 8-byte columns into keys, checking the u32 id range.
-std::vector<uint64_t> pack_rows(const column_type *data, tuple_size_t rows, int arity,
-                                const std::string &name) {
+int bit_length(uint64_t v) { return v ? 64 - __builtin_clzll((unsigned long long)v) : 0; }
+
+KeyLayout layout_from_bits(int arity, const int *low_bits) {
+    KeyLayout L;
+    L.arity = arity;
+    for (int c = 0; c < 4; ++c) L.bits[c] = L.shift[c] = 0;
+    L.bits[0] = 32;
+    L.shift[0] = 32;
+    if (arity == 2) L.bits[1] = 32;
+    if (arity >= 3) {
+        int at = 32;
+        for (int c = 1; c < arity; ++c) {
+            L.bits[c] = low_bits[c - 1];
+            at -= L.bits[c];
+            L.shift[c] = at;
+        }
+    }
+    return L;
+}
+
+// Key layout of a relation being loaded with `rows` rows of host columns.
+KeyLayout choose_layout(const Relation *r, const column_type *data, tuple_size_t rows) {
+    const int a = r->arity;
+    if (a <= 2) {
+        KeyLayout L;
+        L.arity = a;
+        if (a == 1) L.bits[1] = 0;
+        return L;
+    }
+    int low[3] = {0, 0, 0};
+    if (!r->column_bits.empty()) {
+        if ((int)r->column_bits.size() != a - 1)
+            throw Error(GDL_ERR_INVALID_ARG, "relation " + r->name +
+                                                 ": column_bits needs one width per column 1.." +
+                                                 std::to_string(a - 1));
+        int sum = 0;
+        for (int c = 0; c < a - 1; ++c) {
+            low[c] = r->column_bits[c];
+            if (low[c] < 1) throw Error(GDL_ERR_INVALID_ARG, "relation " + r->name + ": width < 1");
+            sum += low[c];
+        }
+        if (sum != 32)
+            throw Error(GDL_ERR_INVALID_ARG,
+                        "relation " + r->name + ": column_bits must sum to 32");
+        return layout_from_bits(a, low);
+    }
+    uint64_t mx[4] = {0, 0, 0, 0};
+    for (tuple_size_t i = 0; i < rows; ++i)
+        for (int c = 1; c < a; ++c) mx[c] |= data[i * a + c];
+    int used = 0;
+    for (int c = 1; c < a - 1; ++c) {
+        low[c - 1] = std::max({bit_length(mx[c]), id_bits(), 1});
+        used += low[c - 1];
+    }
+    low[a - 2] = 32 - used;
+    if (low[a - 2] < std::max(bit_length(mx[a - 1]), 1))
+        throw Error(GDL_ERR_UNSUPPORTED,
+                    "relation " + r->name + ": columns 1.." + std::to_string(a - 1) +
+                        " do not fit the 32 bits beside column 0 (set column_bits)");
+    return layout_from_bits(a, low);
+}
+
+// Packs host rows of 8-byte columns into keys under the relation's layout,
+// checking every value against its field.
+std::vector<uint64_t> pack_rows(const column_type *data, tuple_size_t rows, const Relation *r) {
+    const KeyLayout &L = r->layout;
+    const int arity = L.arity;
     std::vector<uint64_t> keys(rows);
     column_type mx = 0;
     for (tuple_size_t i = 0; i < rows; ++i) {
-        const column_type c0 = data[i * arity];
-        const column_type c1 = arity > 1 ? data[i * arity + 1] : 0;
-        mx |= c0 | c1;
-        keys[i] = ((uint64_t)c0 << 32) | (uint64_t)c1;
+        uint64_t k;
+        if (!pack_key(L, data + i * arity, &k))
+            throw Error(GDL_ERR_OVERFLOW, "load_relation " + r->name + ": row " +
+                                              std::to_string(i) + " does not fit its key layout");
+        keys[i] = k;
+        mx |= (k >> 32) | (k & 0xffffffffull);
     }
-    if (mx > 0xffffffffull)
-        throw Error(GDL_ERR_OVERFLOW, "load_relation " + name + ": an id does not fit in 32 bits");
-    note_id_bits(mx ? 64 - __builtin_clzll((unsigned long long)mx) : 0);
+    note_id_bits(bit_length(mx));
     return keys;
 }
 
 void check_shape(const std::string &name, int arity, tuple_size_t index_column_size,
                  int dependent_column_size) {
-    if (arity < 1 || arity > 2)
+    if (arity < 1 || arity > 4)
         throw Error(GDL_ERR_UNSUPPORTED,
                     "relation " + name + ": arity " + std::to_string(arity) +
-                        " (this engine stores arity 1 and 2)");
+                        " (this engine stores arity 1 to 4)");
     if (index_column_size < 1 || index_column_size > (tuple_size_t)arity)
         throw Error(GDL_ERR_INVALID_ARG, "relation " + name + ": bad index_column_size");
-    if (dependent_column_size != 0)
+    if (arity >= 3 && index_column_size != 1)
         throw Error(GDL_ERR_UNSUPPORTED,
-                    "relation " + name + ": dependent columns are not supported");
+                    "relation " + name + ": arity 3 and 4 are indexed on column 0 only");
+    if (dependent_column_size < 0 || dependent_column_size >= arity ||
+        (tuple_size_t)(arity - dependent_column_size) < index_column_size)
+        throw Error(GDL_ERR_INVALID_ARG, "relation " + name + ": bad dependent_column_size");
 }
 
 void init_relation(Relation *target, const std::string &name, int arity,
@@ -216,6 +283,7 @@ void init_relation(Relation *target, const std::string &name, int arity,
     target->raw_keys = nullptr;
     target->raw_rows = 0;
     target->raw_sorted = false;
+    target->layout = choose_layout(target, nullptr, 0);
 }
 
 // full := sort + unique(raw)
@@ -235,7 +303,8 @@ void load_relation(Relation *target, std::string name, int arity, column_type *d
     (void)block_size;
     init_relation(target, name, arity, index_column_size, dependent_column_size, tmp_flag);
     if (data_row_size == 0 || !data) return;
-    std::vector<uint64_t> keys = pack_rows(data, data_row_size, arity, name);
+    target->layout = choose_layout(target, data, data_row_size);
+    std::vector<uint64_t> keys = pack_rows(data, data_row_size, target);
     target->raw_keys = device_alloc_keys(data_row_size);
     target->raw_rows = data_row_size;
     check(gdl_memcpy_h2d(context(), target->raw_keys, keys.data(), keys.size() * sizeof(uint64_t)),
@@ -287,7 +356,13 @@ void load_relation_container(GHashRelContainer *target, int arity, column_type *
         check(gdl_key_bits(context(), raw, data_row_size, &hi, &lo), "gdl_key_bits");
         note_id_bits(std::max(hi, lo));
     } else {
-        std::vector<uint64_t> keys = pack_rows(data, data_row_size, arity, "container");
+        // a container has no relation: arity 1-2 as (c0 << 32) | c1, arity
+        // 3-4 with the layout of an unloaded relation of that arity
+        Relation shape;
+        shape.name = "container";
+        shape.arity = arity;
+        shape.layout = choose_layout(&shape, data, data_row_size);
+        std::vector<uint64_t> keys = pack_rows(data, data_row_size, &shape);
         check(gdl_memcpy_h2d(context(), raw, keys.data(), keys.size() * sizeof(uint64_t)),
               "load_relation_container H2D");
     }
@@ -354,6 +429,21 @@ uint64_t relation_checksum(GHashRelContainer *c) {
     uint64_t s = 0;
     check(gdl_checksum_u64(context(), c->keys, c->tuple_counts, &s), "relation checksum");
     return s;
+}
+
+void unpack_key(const KeyLayout &L, uint64_t key, column_type *cols) {
+    for (int c = 0; c < L.arity; ++c)
+        cols[c] = (key >> L.shift[c]) & (L.bits[c] >= 64 ? ~0ull : ((1ull << L.bits[c]) - 1));
+}
+
+bool pack_key(const KeyLayout &L, const column_type *cols, uint64_t *key) {
+    uint64_t k = 0;
+    for (int c = 0; c < L.arity; ++c) {
+        if (L.bits[c] < 64 && (cols[c] >> L.bits[c]) != 0) return false;
+        k |= (uint64_t)cols[c] << L.shift[c];
+    }
+    *key = k;
+    return true;
 }
 
 std::vector<uint64_t> relation_keys(GHashRelContainer *c) {

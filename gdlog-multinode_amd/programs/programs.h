@@ -43,6 +43,11 @@ void tc_program(const EdgeInput &graph, int max_iteration, int grid_size, int bl
 // is the longest hop count.
 void path_length_program(const EdgeInput &graph, int grid_size, int block_size,
                          ProgramOutput *out, std::vector<uint32_t> *lens);
+// The same result from the reference's own encoding (test/path_length.cu:
+// arity-3 path with a dependent column, the hook computing newt[2] =
+// outer[2] + 1 run on the device); out->main_keys are the (from, to) pairs.
+void path_length_hooks_program(const EdgeInput &graph, int grid_size, int block_size,
+                               ProgramOutput *out, std::vector<uint32_t> *lens);
 
 // One program run (kind 0 TC, 1 SG, 2 CSPA) from host packed keys on ctx,
 // which is this thread's engine context for the run; output keys are copied

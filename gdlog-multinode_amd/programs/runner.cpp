@@ -234,6 +234,29 @@ gdl_status gdl_prog_path_length(gdl_ctx *ctx, const uint64_t *edges, uint64_t n,
     return GDL_OK;
 }
 
+gdl_status gdl_prog_path_length_hooks(gdl_ctx *ctx, const uint64_t *edges, uint64_t n,
+                                      int on_device, gdl_prog_result *res, uint64_t *keys,
+                                      uint32_t *lens, uint64_t cap) {
+    if (n && !edges) return GDL_ERR_INVALID_ARG;
+    HostRows hr;
+    EdgeInput in = hr.input(edges, n, on_device);
+    std::vector<uint32_t> l;
+    std::vector<uint64_t> k;
+    gdl_status st = run_program(ctx, res, nullptr, [&](ProgramOutput *out) {
+        out->want_keys = keys != nullptr;
+        path_length_hooks_program(in, grid_for_ctx(ctx), kBlock, out, lens ? &l : nullptr);
+        k = std::move(out->main_keys);
+    });
+    if (st != GDL_OK) return st;
+    if ((keys || lens) && res->n_out > cap) {
+        g_last_error = "an output buffer is too small";
+        return GDL_ERR_OVERFLOW;
+    }
+    if (keys && !k.empty()) std::memcpy(keys, k.data(), k.size() * sizeof(uint64_t));
+    if (lens && !l.empty()) std::memcpy(lens, l.data(), l.size() * sizeof(uint32_t));
+    return GDL_OK;
+}
+
 const char *gdl_prog_last_error(void) { return g_last_error.c_str(); }
 
 gdl_status gdl_prog_set_checkpoint(const char *dir, int every, int resume, int stop_after) {

@@ -154,6 +154,19 @@ gdl_status gdl_merge_diff_u64(gdl_ctx *ctx, const uint64_t *d_full, uint64_t nf,
                               const uint64_t *d_newt, uint64_t nn, uint64_t *d_full_out,
                               uint64_t *n_full_out, uint64_t *d_delta, uint64_t *n_delta);
 
+/* Stable compaction: d_out gets, in order, the d_in[i] with d_flags[i] != 0. */
+gdl_status gdl_compact_flags_u64(gdl_ctx *ctx, const uint64_t *d_in, const uint8_t *d_flags,
+                                 uint64_t n, uint64_t *d_out, uint64_t *n_out);
+/* thrust::set_difference(newt, full, tuple_indexed_less(k, arity - dependent))
+ * of the fixpoint's set difference when a relation has dependent columns
+ * (lie.cu:131-138): d_b and d_a sorted by key; tuples compare by the prefix
+ * key >> shift (the non-dependent columns).  Of the m elements of b sharing a
+ * prefix that n elements of a have, the first min(m, n) are dropped, the
+ * rest kept in order.  d_out holds nb slots. */
+gdl_status gdl_set_difference_prefix_u64(gdl_ctx *ctx, const uint64_t *d_b, uint64_t nb,
+                                         const uint64_t *d_a, uint64_t na, int shift,
+                                         uint64_t *d_out, uint64_t *n_out);
+
 /* ---- hash index over the distinct join prefixes of a sorted array ------
  * Replaces init_index_map + calculate_index_hash (relation.cu:12-80,124-137):
  * open addressing with linear probing; each distinct prefix maps to the
@@ -186,10 +199,22 @@ gdl_status gdl_index_lookup(gdl_ctx *ctx, const gdl_index *idx, const uint64_t *
  * load factor (at most 0.8), grown by rehashing. */
 typedef struct gdl_hashset {
     void *slots;       /* device; owned by the context allocator */
-    uint64_t capacity; /* slots, a power of two */
+    uint64_t capacity; /* slots, a power of two (bitmap: bits) */
     uint64_t size;     /* members */
+    uint32_t key_bits; /* 0: hash table; k > 0: bitmap over keys whose two
+                          columns are both below 2^k (bit (c0 << k) | c1) */
+    uint32_t pad_;
 } gdl_hashset;
 gdl_status gdl_hashset_create(gdl_ctx *ctx, uint64_t expected, gdl_hashset *out);
+/* A bitmap set for a dense id domain: every key (c0 << 32) | c1 with c0, c1 <
+ * 2^key_bits (1..16; 2^(2 key_bits) bits, at most 512 MiB) has its own bit, so
+ * an insert is one atomic OR on one word and the set never grows.  Same
+ * operations as the hash table.  A key outside the domain is not inserted; it
+ * sets a flag that gdl_hashset_check reports. */
+gdl_status gdl_hashset_create_bitmap(gdl_ctx *ctx, uint32_t key_bits, gdl_hashset *out);
+/* GDL_ERR_OVERFLOW when a key outside a bitmap set's domain was offered
+ * (always GDL_OK for a hash table). */
+gdl_status gdl_hashset_check(gdl_ctx *ctx, const gdl_hashset *hs);
 gdl_status gdl_hashset_free(gdl_ctx *ctx, gdl_hashset *hs);
 /* Inserts n keys (any order, repeats allowed); every key not yet a member is
  * written once to d_new (n slots), in no particular order; *n_new = count. */

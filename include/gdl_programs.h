@@ -70,6 +70,14 @@ gdl_status gdl_prog_sg(gdl_ctx *ctx, const uint64_t *edges, uint64_t n, int on_d
 gdl_status gdl_prog_path_length(gdl_ctx *ctx, const uint64_t *edges, uint64_t n, int on_device,
                                 gdl_prog_result *res, uint64_t *keys, uint32_t *lens,
                                 uint64_t cap);
+/* The same result from the reference's own encoding of path_length
+ * (test/path_length.cu:49-124 unchanged: an arity-3 path relation with a
+ * dependent column, load_relation(..., 3, ..., 1, 1, ...), and the generator
+ * hook newt[2] = outer[2] + 1, which runs on the device for every produced
+ * tuple).  Arguments and outputs as gdl_prog_path_length. */
+gdl_status gdl_prog_path_length_hooks(gdl_ctx *ctx, const uint64_t *edges, uint64_t n,
+                                      int on_device, gdl_prog_result *res, uint64_t *keys,
+                                      uint32_t *lens, uint64_t cap);
 gdl_status gdl_prog_cspa(gdl_ctx *ctx, const uint64_t *assign, uint64_t n_assign,
                          const uint64_t *dereference, uint64_t n_dereference, int on_device,
                          gdl_prog_result *res, const gdl_prog_buffers *bufs);

@@ -16,6 +16,19 @@
 #include "gdlog/tuple.h"
 
 enum RelationVersion { DELTA, FULL, NEWT };
+
+// Bit layout of a relation's u64 tuple keys.  Column 0 holds the top 32 bits;
+// columns 1..arity-1 share the low 32 bits (arity 2: column 1 alone), each in
+// its own field in column order, so integer order is the reference's
+// lexicographic order over all columns (tuple.cuh:94-127) and the one-column
+// join prefix is key >> 32 for every arity.  Arity 3 and 4 (up to 4 columns;
+// the reference allows 20, tuple.cuh:5-10) split the low 32 bits by the
+// widths chosen when the relation is loaded.
+struct KeyLayout {
+    int arity = 2;
+    int bits[4] = {32, 32, 0, 0};   // field widths
+    int shift[4] = {32, 0, 0, 0};   // field offsets
+};
 enum JoinDirection { LEFT, RIGHT };
 enum MonotonicOrder { DESC, ASC, UNSPEC };
 
@@ -48,6 +61,13 @@ struct Relation {
     int index_column_size = 1;
     std::string name;
     int dependent_column_size = 0;
+    // Key layout (set by load_relation).  For arity 3 and 4, column_bits may
+    // be set before loading to choose the widths of columns 1..arity-1 (they
+    // must sum to 32); otherwise every column but the last gets the bit
+    // length of the largest id loaded so far (at least that of its own loaded
+    // values) and the last column the remaining bits.
+    KeyLayout layout;
+    std::vector<int> column_bits;
     bool index_flag = true;
     bool tmp_flag = false;
 
@@ -119,8 +139,13 @@ void copy_relation_container(GHashRelContainer *dst, GHashRelContainer *src, int
 // Frees a Relation's three versions and its raw rows (the reference leaks them).
 void free_relation(Relation *target);
 
-// Copies a version's tuples to the host as sorted keys ((c0 << 32) | c1).
+// Copies a version's tuples to the host as sorted keys ((c0 << 32) | c1;
+// arity 3 and 4: the relation's KeyLayout).
 std::vector<uint64_t> relation_keys(GHashRelContainer *c);
+// Columns of a key under a layout (cols has layout.arity entries), and back
+// (false, with *key untouched, when a value does not fit its field).
+void unpack_key(const KeyLayout &layout, uint64_t key, column_type *cols);
+bool pack_key(const KeyLayout &layout, const column_type *cols, uint64_t *key);
 // Order-independent checksum of a version's tuples (gdl_checksum_u64).
 uint64_t relation_checksum(GHashRelContainer *c);
 

@@ -5,12 +5,15 @@
 // into output_rel->newt), RelationalCopy (projection into dest newt) and
 // RelationalACopy (re-indexing copy of one relation's newt into another's).
 // The hooks are turned into column maps when the operator is constructed
-// (gdlog/hook_probe.h), so constructors must be compiled with hipcc in the
-// translation unit that defines the hooks.
+// (gdlog/hook_probe.h); hooks that are not column selections run through
+// kernels of the same translation unit (gdlog/hook_exec.h).  Constructors
+// must therefore be compiled with hipcc in the translation unit that defines
+// the hooks.
 #pragma once
 
 #include <variant>
 
+#include "gdlog/hook_exec.h"
 #include "gdlog/hook_probe.h"
 #include "gdlog/relation.h"
 
@@ -39,6 +42,10 @@ struct RelationalJoin {
 
     gdlog::HookMap gen_map;
     gdlog::PredMap pred_map;
+    // Runs the hooks themselves when they are not a column selection and a
+    // column comparison (gdlog/hook_exec.h); set by the hook constructor.
+    gdlog::HookJoinLaunch hook_join = nullptr;
+    bool uses_hooks() const { return !gen_map.ok || !pred_map.ok; }
     // The reference's handling of an empty inner: it zeroes the outer's tuple
     // count (join.cu:42-45), so later operators reading the same outer version
     // in that iteration see nothing.  The column-map constructor turns it off:
@@ -55,6 +62,7 @@ struct RelationalJoin {
           detail_time(detail_time) {
         gen_map = gdlog::probe_generator(tp_gen, output_rel ? output_rel->arity : 0);
         pred_map = gdlog::probe_predicate(tp_pred, output_rel ? output_rel->arity : 0);
+        hook_join = &gdlog::hook_exec_detail::launch_join;
     }
 
     // Column-map form (an extension for generated plans, gdlog/datalog.h): the
@@ -96,6 +104,7 @@ struct RelationalCopy {
     bool copied = false;
 
     gdlog::HookMap copy_map;
+    gdlog::HookCopyLaunch hook_copy = nullptr;  // a copy hook that is not a column selection
     // The reference's handling of an empty source: it zeroes the
     // destination's newt count (copy.cu:23-26), dropping what earlier
     // operators put there in that iteration.  Off in the column-map form.
@@ -111,6 +120,7 @@ struct RelationalCopy {
         : src_rel(src), src_ver(src_ver), dest_rel(dest), tuple_generator(tuple_generator),
           tuple_pred(tuple_pred), grid_size(grid_size), block_size(block_size) {
         copy_map = gdlog::probe_copy(tuple_generator, dest ? dest->arity : 0);
+        hook_copy = &gdlog::hook_exec_detail::launch_copy;
     }
 
     // Column-map form (source columns as GDL_COL_INNER0 / GDL_COL_INNER1),
@@ -136,12 +146,14 @@ struct RelationalACopy {
     int block_size;
 
     gdlog::HookMap copy_map;
+    gdlog::HookCopyLaunch hook_copy = nullptr;  // a copy hook that is not a column selection
 
     RelationalACopy(Relation *src, Relation *dest, tuple_copy_hook tuple_generator,
                     tuple_predicate tuple_pred, int grid_size, int block_size)
         : src_rel(src), dest_rel(dest), tuple_generator(tuple_generator), tuple_pred(tuple_pred),
           grid_size(grid_size), block_size(block_size) {
         copy_map = gdlog::probe_copy(tuple_generator, dest ? dest->arity : 0);
+        hook_copy = &gdlog::hook_exec_detail::launch_copy;
     }
 
     // Column-map form (source columns as GDL_COL_INNER0 / GDL_COL_INNER1).

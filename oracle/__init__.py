@@ -66,6 +66,9 @@ def lib():
             fn.restype = c_int
             fn.argtypes = [_U64P, c_uint64, c_int, c_int, _U64P, _U64P, POINTER(c_int), _U64P,
                            c_int]
+        L.souffle_cspa.restype = c_int
+        L.souffle_cspa.argtypes = [_U64P, c_uint64, _U64P, c_uint64, c_int, c_int, _U64P, _U64P,
+                                   POINTER(c_int)]
         L.cpu_tc_omp.restype = c_int
         L.cpu_tc_omp.argtypes = [_U64P, c_uint64, c_int, c_int, _U64P, _U64P,
                                  POINTER(c_int), _U64P, c_int]
@@ -112,6 +115,20 @@ def souffle_tc(edges, threads, max_iteration=-1, trace_cap=1 << 16):
     r = _souffle(lib().souffle_tc, edges, threads, max_iteration, trace_cap)
     r["trace"] = [int(x) for x in r["trace"][:min(r["iterations"], trace_cap)]]
     return r
+
+
+def souffle_cspa(assign, deref, threads, max_iteration=-1):
+    """Multi-core restatement of Souffle's CSPA plan (cpu_souffle.c): sizes
+    and checksums of ValueFlow, MemoryAlias, ValueAlias; recursive bodies."""
+    a, d = _arr(assign), _arr(deref)
+    sz = np.zeros(3, dtype=np.uint64)
+    cs = np.zeros(3, dtype=np.uint64)
+    it = c_int()
+    if lib().souffle_cspa(_p(a), a.size, _p(d), d.size, threads, max_iteration, _p(sz), _p(cs),
+                          ctypes.byref(it)) != 0:
+        raise MemoryError("souffle_cspa")
+    return dict(sizes=[int(x) for x in sz], checksums=[int(x) for x in cs],
+                iterations=it.value)
 
 
 def _p(a):

@@ -7,7 +7,7 @@
  * product path never links or calls them.
  *
  * Souffle itself is not available here (no binary, no runtime headers), so the
- * generated C++ (test/souffle/{tc,sg}.cpp) cannot be compiled.  These functions
+ * generated C++ (test/souffle/{tc,sg,cspa}.cpp) cannot be compiled.  These functions
  * run the same semi-naive plans -- the same rule versions, loop nests, range
  * lookups and contains() filters, with delta / new relations swapped at the end
  * of each iteration -- over OpenMP threads:
@@ -24,10 +24,12 @@
  *         if !path.contains(x,z) && !delta.contains(y,z): new.insert(x,z)
  *       for path(x,y): for delta(y,z):
  *         if !path.contains(x,z): new.insert(x,z)
+ *   souffle_cspa  test/souffle/cspa.cpp:344-815 (cspa.dl's ten rules; the
+ *     rule versions are listed above souffle_cspa below)
  *
  * Souffle's pfor splits the outer relation of each loop nest into chunks.  Here
  * the outer tuples are split by the first column of the tuple they derive (x
- * in both programs), so each thread owns every relation's tuples whose first
+ * in every rule), so each thread owns every relation's tuples whose first
  * column x has x mod T == t: its contains() and insert() calls touch only its
  * own hash sets, and the range lookups read other threads' adjacency lists,
  * which change only between iterations.  Souffle's B-trees become
@@ -421,5 +423,211 @@ int souffle_tc(const uint64_t *edges, uint64_t n, int threads, int max_iteration
     free(dset);
     vecs_free(delta, nodes + 1);
     vecs_free(path, nodes + 1);
+    return oom ? -1 : 0;
+}
+
+/* ---- CSPA: test/souffle/cspa.cpp:344-815 ---------------------------------
+ * Non-recursive rules (cspa.dl:34-39) seed ValueFlow and MemoryAlias from
+ * assign; then one loop over the mutually recursive ValueFlow / MemoryAlias /
+ * ValueAlias with these rule versions (outer relation first), all reading the
+ * previous iteration's deltas:
+ *   MA(x,w)  :- deref(y,x), dVA(y,z), deref(z,w)             !MA(x,w)
+ *   VA(x,y)  :- dVF(z,x), VF(z,y)                             !VA, !dVF(z,y)
+ *   VA(x,y)  :- VF(z,x), dVF(z,y)                             !VA
+ *   VA(x,y)  :- dVF(z,x), MA(z,w), VF(w,y)   !dMA(z,w)        !VA, !dVF(w,y)
+ *   VA(x,y)  :- VF(z,x), dMA(z,w), VF(w,y)                    !VA, !dVF(w,y)
+ *   VA(x,y)  :- VF(z,x), MA(z,w), dVF(w,y)                    !VA
+ *   VF(x,y)  :- dVF(x,z), VF(z,y)                             !VF, !dVF(z,y)
+ *   VF(x,y)  :- VF(x,z), dVF(z,y)                             !VF
+ *   VF(x,y)  :- assign(x,z), dMA(z,y)                         !VF
+ * Each thread derives the tuples whose first column x has x mod T == t. */
+
+typedef struct {
+    part *p;       /* per thread: full set, new set, new list */
+    vec32 *full;   /* adjacency of the full relation by column 0 (NULL: not kept) */
+    vec32 *delta;  /* adjacency of the delta by column 0 */
+    hset *dset;    /* per thread: the delta's tuples (first column owned by t) */
+    uint64_t size;
+} crel;
+
+static int crel_init(crel *r, int T, uint32_t nodes, int keep_full) {
+    memset(r, 0, sizeof(*r));
+    r->p = (part *)calloc((size_t)T, sizeof(part));
+    r->delta = (vec32 *)calloc((size_t)nodes + 1, sizeof(vec32));
+    r->dset = (hset *)calloc((size_t)T, sizeof(hset));
+    if (keep_full) r->full = (vec32 *)calloc((size_t)nodes + 1, sizeof(vec32));
+    if (!r->p || !r->delta || !r->dset || (keep_full && !r->full) || parts_init(r->p, T, 1024))
+        return -1;
+    for (int t = 0; t < T; ++t)
+        if (hset_init(&r->dset[t], 16)) return -1;
+    return 0;
+}
+
+static void crel_free(crel *r, int T, uint32_t nodes) {
+    if (r->p) parts_free(r->p, T);
+    free(r->p);
+    if (r->dset)
+        for (int t = 0; t < T; ++t) hset_free(&r->dset[t]);
+    free(r->dset);
+    vecs_free(r->delta, nodes + 1);
+    vecs_free(r->full, nodes + 1);
+}
+
+/* thread t's candidate (x,y): kept when not in full and not yet new */
+static void crel_offer(crel *r, int t, uint64_t key) {
+    part *me = &r->p[t];
+    if (hset_contains(&me->full, key)) return;
+    const int ins = hset_insert(&me->fresh, key);
+    if (ins < 0 || (ins > 0 && vec64_push(&me->fresh_list, key))) me->oom = 1;
+}
+
+/* end of an iteration: new -> full, delta, delta set; returns |new| */
+static uint64_t crel_commit(crel *r, uint32_t nodes, int T, int *oom) {
+#pragma omp parallel num_threads(T)
+    {
+        const int t = omp_get_thread_num();
+        hset_free(&r->dset[t]);
+        if (hset_init(&r->dset[t], 2 * r->p[t].fresh_list.n + 16)) r->p[t].oom = 1;
+        for (uint64_t i = 0; i < r->p[t].fresh_list.n; ++i)
+            if (hset_insert(&r->dset[t], r->p[t].fresh_list.v[i]) < 0) r->p[t].oom = 1;
+    }
+    const uint64_t nd = commit_new(r->p, r->delta, r->full, nodes, T, oom);
+    r->size += nd;
+    return nd;
+}
+
+static int dcontains(const crel *r, int T, uint64_t a, uint64_t b) {
+    return hset_contains(&r->dset[a % (uint64_t)T], pk(a, b));
+}
+
+int souffle_cspa(const uint64_t *assign, uint64_t na, const uint64_t *deref, uint64_t nd_,
+                 int threads, int max_iteration, uint64_t *sizes, uint64_t *checksums,
+                 int *iterations) {
+    const int T = threads > 0 ? threads : 1;
+    uint32_t nodes = node_count(assign, na), n2 = node_count(deref, nd_);
+    if (n2 > nodes) nodes = n2;
+    crel VF, MA, VA;
+    csr dg;
+    int oom = 0;
+    if (crel_init(&VF, T, nodes, 1) || crel_init(&MA, T, nodes, 1) || crel_init(&VA, T, nodes, 0) ||
+        csr_build(&dg, deref, nd_, nodes))
+        return -1;
+    /* cspa.dl:34-39 */
+#pragma omp parallel num_threads(T)
+    {
+        const int t = omp_get_thread_num();
+        for (uint64_t i = 0; i < na; ++i) {
+            const uint64_t a0 = assign[i] >> 32, a1 = assign[i] & 0xffffffffull;
+            if ((int)(a1 % T) == t) crel_offer(&MA, t, pk(a1, a1));
+            if ((int)(a0 % T) == t) {
+                crel_offer(&MA, t, pk(a0, a0));
+                crel_offer(&VF, t, pk(a0, a1));
+                crel_offer(&VF, t, pk(a0, a0));
+            }
+            if ((int)(a1 % T) == t) crel_offer(&VF, t, pk(a1, a1));
+        }
+    }
+    crel_commit(&VF, nodes, T, &oom);
+    crel_commit(&MA, nodes, T, &oom);
+    crel_commit(&VA, nodes, T, &oom);
+    int it = 0;
+    while (!oom && (max_iteration < 0 || it < max_iteration)) {
+#pragma omp parallel num_threads(T)
+        {
+            const int t = omp_get_thread_num();
+            /* MA(x,w) :- deref(y,x), dVA(y,z), deref(z,w) */
+            for (uint32_t y = 0; y < nodes; ++y) {
+                const vec32 *dv = &VA.delta[y];
+                if (!dv->n) continue;
+                for (uint64_t i = dg.off[y]; i < dg.off[y + 1]; ++i) {
+                    const uint32_t x = dg.dst[i];
+                    if ((int)(x % (uint32_t)T) != t) continue;
+                    for (uint32_t j = 0; j < dv->n; ++j) {
+                        const uint32_t z = dv->v[j];
+                        for (uint64_t k = dg.off[z]; k < dg.off[z + 1]; ++k)
+                            crel_offer(&MA, t, pk(x, dg.dst[k]));
+                    }
+                }
+            }
+            for (uint32_t z = 0; z < nodes; ++z) {
+                const vec32 *dz = &VF.delta[z], *fz = &VF.full[z];
+                const vec32 *mz = &MA.full[z], *dmz = &MA.delta[z];
+                /* VA(x,y) :- dVF(z,x), VF(z,y) and VF(z,x), dVF(z,y) */
+                for (uint32_t i = 0; i < dz->n; ++i) {
+                    const uint32_t x = dz->v[i];
+                    if ((int)(x % (uint32_t)T) != t) continue;
+                    for (uint32_t j = 0; j < fz->n; ++j)
+                        if (!dcontains(&VF, T, z, fz->v[j])) crel_offer(&VA, t, pk(x, fz->v[j]));
+                    /* VA(x,y) :- dVF(z,x), MA(z,w), VF(w,y) */
+                    for (uint32_t j = 0; j < mz->n; ++j) {
+                        const uint32_t w = mz->v[j];
+                        if (dcontains(&MA, T, z, w)) continue;
+                        const vec32 *fw = &VF.full[w];
+                        for (uint32_t k = 0; k < fw->n; ++k)
+                            if (!dcontains(&VF, T, w, fw->v[k]))
+                                crel_offer(&VA, t, pk(x, fw->v[k]));
+                    }
+                }
+                for (uint32_t i = 0; i < fz->n; ++i) {
+                    const uint32_t x = fz->v[i];
+                    if ((int)(x % (uint32_t)T) != t) continue;
+                    for (uint32_t j = 0; j < dz->n; ++j) crel_offer(&VA, t, pk(x, dz->v[j]));
+                    /* VA(x,y) :- VF(z,x), dMA(z,w), VF(w,y) */
+                    for (uint32_t j = 0; j < dmz->n; ++j) {
+                        const vec32 *fw = &VF.full[dmz->v[j]];
+                        for (uint32_t k = 0; k < fw->n; ++k)
+                            if (!dcontains(&VF, T, dmz->v[j], fw->v[k]))
+                                crel_offer(&VA, t, pk(x, fw->v[k]));
+                    }
+                    /* VA(x,y) :- VF(z,x), MA(z,w), dVF(w,y) */
+                    for (uint32_t j = 0; j < mz->n; ++j) {
+                        const vec32 *dw = &VF.delta[mz->v[j]];
+                        for (uint32_t k = 0; k < dw->n; ++k) crel_offer(&VA, t, pk(x, dw->v[k]));
+                    }
+                }
+            }
+            /* VF(x,y) :- dVF(x,z), VF(z,y) and VF(x,z), dVF(z,y) */
+            for (uint32_t x = (uint32_t)t; x < nodes; x += (uint32_t)T) {
+                const vec32 *dx = &VF.delta[x], *fx = &VF.full[x];
+                for (uint32_t i = 0; i < dx->n; ++i) {
+                    const uint32_t z = dx->v[i];
+                    const vec32 *fz = &VF.full[z];
+                    for (uint32_t j = 0; j < fz->n; ++j)
+                        if (!dcontains(&VF, T, z, fz->v[j])) crel_offer(&VF, t, pk(x, fz->v[j]));
+                }
+                for (uint32_t i = 0; i < fx->n; ++i) {
+                    const vec32 *dz = &VF.delta[fx->v[i]];
+                    for (uint32_t j = 0; j < dz->n; ++j) crel_offer(&VF, t, pk(x, dz->v[j]));
+                }
+            }
+            /* VF(x,y) :- assign(x,z), dMA(z,y) */
+            for (uint64_t i = 0; i < na; ++i) {
+                const uint32_t x = (uint32_t)(assign[i] >> 32), z = (uint32_t)assign[i];
+                if ((int)(x % (uint32_t)T) != t) continue;
+                const vec32 *dz = &MA.delta[z];
+                for (uint32_t j = 0; j < dz->n; ++j) crel_offer(&VF, t, pk(x, dz->v[j]));
+            }
+        }
+        ++it;
+        const uint64_t a = crel_commit(&MA, nodes, T, &oom);
+        const uint64_t b = crel_commit(&VA, nodes, T, &oom);
+        const uint64_t c = crel_commit(&VF, nodes, T, &oom);
+        if (!a && !b && !c) break;
+    }
+    for (int r = 0; r < 3; ++r) {
+        crel *R = r == 0 ? &VF : r == 1 ? &MA : &VA;
+        uint64_t cs = 0;
+        for (int t = 0; t < T; ++t) {
+            cs += R->p[t].checksum;
+            oom |= R->p[t].oom;
+        }
+        sizes[r] = R->size;
+        checksums[r] = cs;
+    }
+    *iterations = it;
+    crel_free(&VF, T, nodes);
+    crel_free(&MA, T, nodes);
+    crel_free(&VA, T, nodes);
+    csr_free(&dg);
     return oom ? -1 : 0;
 }

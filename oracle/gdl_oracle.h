@@ -164,6 +164,12 @@ int souffle_sg(const uint64_t *edges, uint64_t n, int threads, int max_iteration
 int souffle_tc(const uint64_t *edges, uint64_t n, int threads, int max_iteration,
                uint64_t *n_out, uint64_t *checksum, int *iterations, uint64_t *delta_trace,
                int trace_cap);
+/* CSPA (test/souffle/cspa.cpp:344-815, cspa.dl's ten rules): sizes and
+ * checksums of ValueFlow, MemoryAlias, ValueAlias (in that order) and the
+ * recursive loop bodies run.  Checked against orc_cspa. */
+int souffle_cspa(const uint64_t *assign, uint64_t na, const uint64_t *deref, uint64_t nd,
+                 int threads, int max_iteration, uint64_t *sizes, uint64_t *checksums,
+                 int *iterations);
 
 #ifdef __cplusplus
 }

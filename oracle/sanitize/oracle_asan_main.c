@@ -58,6 +58,9 @@ int main(void) {
             bad += check(n_out == tc.n_out, "souffle_tc vs orc_tc");
             if (souffle_sg(e, m, threads, -1, &n_out, &cksum, &it, trace, 256) != 0) bad++;
             bad += check(n_out == sg.n_out, "souffle_sg vs orc_sg");
+            uint64_t sz[3], cks[3];
+            if (souffle_cspa(e, m / 2, d, md, threads, -1, sz, cks, &it) != 0) bad++;
+            bad += check(sz[2] == cs.n_out && sz[0] == cs.sizes[0], "souffle_cspa vs orc_cspa");
         }
         orc_result_free(&tc);
         orc_result_free(&sg);

@@ -207,13 +207,14 @@ def test_project_and_filter(ctx):
     assert np.array_equal(f, kk[(kk >> np.uint64(32)) != (kk & M32)])
 
 
-def test_hashset_insert_new_is_set_difference(ctx):
-    hs = ctx.hashset_create(0)
+@pytest.mark.parametrize("set_kind", ["hash", "bitmap"])
+def test_hashset_insert_new_is_set_difference(ctx, set_kind):
+    hs = ctx.hashset_create(0) if set_kind == "hash" else ctx.hashset_create_bitmap(10)
     seen = set()
     try:
-        for batch in range(6):  # grows and rehashes across batches
+        for batch in range(6):  # the hash table grows and rehashes across batches
             k = rand_keys(150_000 * (batch + 1), 10, 10)  # heavy repeats
-            if batch == 2:
+            if batch == 2 and set_kind == "hash":
                 k = np.concatenate([k, np.array([0xffffffffffffffff] * 3, dtype=np.uint64)])
             new = ctx.hashset_insert_new(hs, k)
             want = set(k.tolist()) - seen
@@ -221,8 +222,26 @@ def test_hashset_insert_new_is_set_difference(ctx):
             assert set(new.tolist()) == want
             seen |= want
         assert hs.size == len(seen)
+        assert np.array_equal(np.sort(ctx.hashset_extract(hs)), np.array(sorted(seen), np.uint64))
+        assert ctx.hashset_check(hs) == 0
     finally:
         ctx.hashset_free(hs)
+
+
+def test_bitmap_set_refuses_keys_outside_its_domain(ctx):
+    hs = ctx.hashset_create_bitmap(8)  # ids 0..255 in both columns
+    try:
+        inside = gdl.pack([0, 255, 17], [255, 0, 17])
+        assert set(ctx.hashset_insert_new(hs, inside).tolist()) == set(inside.tolist())
+        assert ctx.hashset_check(hs) == 0
+        outside = gdl.pack([256, 3], [1, 1 << 20])
+        assert ctx.hashset_insert_new(hs, outside).size == 0  # not inserted ...
+        assert ctx.hashset_check(hs) == gdl.GDL_ERR_OVERFLOW  # ... and reported
+        assert np.array_equal(np.sort(ctx.hashset_extract(hs)), np.sort(inside))
+    finally:
+        ctx.hashset_free(hs)
+    with pytest.raises(gdl.GdlError):
+        ctx.hashset_create_bitmap(17)  # above 512 MiB
 
 
 def test_allocator_cap_reports_out_of_memory(ctx):
@@ -265,13 +284,14 @@ def test_allocator_cache_limit_and_best_fit_reuse():
 @pytest.mark.parametrize("colmap,pred", [((1, 3), (gdl.PRED_NONE, 0, 1)),
                                          ((3, 1), (gdl.PRED_NE, 0, 1))])
 @pytest.mark.parametrize("inner_kind", ["uniform", "hub"])
-def test_join_insert_is_join_then_set_difference(ctx, colmap, pred, inner_kind):
+@pytest.mark.parametrize("set_kind", ["hash", "bitmap"])
+def test_join_insert_is_join_then_set_difference(ctx, colmap, pred, inner_kind, set_kind):
     if inner_kind == "hub":  # runs above the wave-cooperative threshold
         inner = np.unique(np.concatenate([gdl.pack(np.full(5000, 7), np.arange(5000)),
                                           rand_keys(20_000, 9, 9)]))
     else:
         inner = np.unique(rand_keys(50_000, 10, 12))
-    hs = ctx.hashset_create(0)
+    hs = ctx.hashset_create(0) if set_kind == "hash" else ctx.hashset_create_bitmap(13)
     try:
         seen = set()
         for step in range(3):

@@ -19,13 +19,20 @@ def sha(keys):
     return hashlib.sha256(np.ascontiguousarray(keys, dtype="<u8").tobytes()).hexdigest()
 
 
-MODES = {"auto": gdl.FULL_MODE_AUTO, "merge": gdl.FULL_MODE_MERGE}
+# auto: FULL as an append log + a membership set (a bitmap when the ids fit
+# 16 bits, as on every input here); auto_hash: the same with the hash-table
+# set forced (GDLOG_FULL_SET=hash); merge: the reference's sorted FULL
+MODES = {"auto": gdl.FULL_MODE_AUTO, "auto_hash": gdl.FULL_MODE_AUTO,
+         "merge": gdl.FULL_MODE_MERGE}
 
 
 @pytest.fixture(params=sorted(MODES))
 def mode(request):
     gdl.set_full_mode(MODES[request.param])
+    if request.param == "auto_hash":
+        os.environ["GDLOG_FULL_SET"] = "hash"
     yield request.param
+    os.environ.pop("GDLOG_FULL_SET", None)
     gdl.set_full_mode(gdl.FULL_MODE_AUTO)
 
 

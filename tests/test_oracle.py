@@ -316,3 +316,14 @@ def test_souffle_tc_plan_is_non_linear():
     r = oracle.souffle_tc(e, 4)
     assert r["n_out"] == 65 * 64 // 2 and r["iterations"] == 7
     assert oracle.tc(e)["iterations"] == 64
+
+
+@pytest.mark.parametrize("seed", range(5))
+@pytest.mark.parametrize("threads", [1, 4])
+def test_souffle_cspa_plan_gives_the_oracle_s_sets(seed, threads):
+    a, d = gdl.gen_cspa(seed, 60 + 40 * seed, 1.1, 90 + 60 * seed, 45 + 30 * seed)
+    w = oracle.cspa(a, d)
+    r = oracle.souffle_cspa(a, d, threads)
+    assert r["sizes"] == [int(w["sizes"][0]), int(w["sizes"][1]), int(w["n_out"])]
+    assert r["checksums"] == [gdl.tuple_checksum(w["extra"][0]),
+                              gdl.tuple_checksum(w["extra"][1]), gdl.tuple_checksum(w["out"])]

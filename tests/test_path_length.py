@@ -77,6 +77,33 @@ def test_path_length_matches_the_oracle(ctx, name):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(n for n in INPUTS if n != "chain_high_ids"))
+@pytest.mark.parametrize("on_device", [0, 1])
+def test_reference_encoding_with_hooks_matches_the_oracle(ctx, name, on_device):
+    # test/path_length.cu as written: path_3__1_2_3 of arity 3 with one
+    # dependent column, the generator hook newt[2] = outer[2] + 1 run on the
+    # device, the set difference over (from, to) only
+    e = INPUTS[name]()
+    got = ctx.run_path_length(e, on_device=bool(on_device), hooks=True)
+    keys, lens = oracle.path_length(e)
+    assert np.array_equal(got["out"], keys)
+    assert np.array_equal(got["lens"], lens)
+    assert got["longest"] == lens.max()
+    assert got["iterations"] == oracle.tc(e)["iterations"]
+    assert got["derived"] == oracle.tc(e)["derived"]
+
+
+@pytest.mark.gpu
+def test_reference_encoding_refuses_ids_its_key_layout_cannot_hold(ctx):
+    # from and to take 32 bits each beside the hop count: nothing is left for it
+    with pytest.raises(gdl.GdlError) as err:
+        ctx.run_path_length(INPUTS["chain_high_ids"](), hooks=True)
+    assert err.value.status == 4  # unsupported, before anything runs
+    small = ctx.run_path_length(gdl.pack([4], [4]), hooks=True)
+    assert small["out"].tolist() == [(4 << 32) | 4] and small["lens"].tolist() == [1]
+
+
+@pytest.mark.gpu
 def test_path_length_edge_cases(ctx):
     empty = ctx.run_path_length(np.zeros(0, dtype=np.uint64))
     assert empty["n_out"] == 0
