@@ -230,8 +230,8 @@ gdl_status gdl_hashset_create(gdl_ctx *ctx, uint64_t expected, gdl_hashset *out)
 
 gdl_status gdl_hashset_create_bitmap(gdl_ctx *ctx, uint32_t key_bits, gdl_hashset *out) {
     if (!ctx || !out) return GDL_ERR_INVALID_ARG;
-    if (key_bits < 1 || key_bits > 16)
-        return fail(ctx, GDL_ERR_INVALID_ARG, "bitmap set: key_bits must be 1..16");
+    if (key_bits < 1 || key_bits > 18)
+        return fail(ctx, GDL_ERR_INVALID_ARG, "bitmap set: key_bits must be 1..18");
     out->slots = nullptr;
     out->capacity = 0;
     out->size = 0;

@@ -821,13 +821,24 @@ static gdl_status join_insert_impl(gdl_ctx *ctx, const u64 *d_inner, u64 n_inner
         if (bound == 0) return GDL_OK;
     }
     if constexpr (!SEG) {
+        // new members are distinct keys of the id domain the set does not
+        // hold yet: at most the domain's free keys, however many tuples the
+        // join produces
+        u64 room = ~0ull;
+        if (hs->key_bits) {
+            room = hs->capacity - std::min(hs->capacity, hs->size);
+        } else if (ctx->id_bits > 0 && ctx->id_bits < 32) {
+            const u64 dom = 1ull << (2 * ctx->id_bits);
+            room = dom - std::min(dom, hs->size);
+        }
+        const u64 new_bound = std::min(bound, room);
         // size for up to 2 |outer| new members at the load factor; the bound
         // only has to fit
-        GDL_TRY(hashset_reserve2(ctx, hs, std::max<u64>(2 * n_outer, 1024), bound));
+        GDL_TRY(hashset_reserve2(ctx, hs, std::max<u64>(2 * n_outer, 1024), new_bound));
         sel.slots = (u64 *)hs->slots;
         sel.mask = hs_arg(hs);
-        if (*n_new + bound > *new_cap || !*d_new) {
-            const u64 cap = *n_new + bound + (*n_new + bound) / 4 + 1024;
+        if (*n_new + new_bound > *new_cap || !*d_new) {
+            const u64 cap = *n_new + new_bound + (*n_new + new_bound) / 4 + 1024;
             void *p = nullptr;
             std::string err;
             gdl_status st = ctx->alloc->alloc(cap * sizeof(u64), &p, &err);

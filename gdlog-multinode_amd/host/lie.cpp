@@ -124,14 +124,14 @@ void log_with_delta_tail(Relation *rel, bool keep_log) {
     container_take(rel->full, log, nf, nf + nd, false);
 }
 
-// FULL's membership set: a bitmap when every id of the program fits 16 bits
-// (one bit per possible pair, at most 512 MiB; an insert is one atomic OR and
-// the set never grows), else the hash table.  GDLOG_FULL_SET=hash forces the
-// hash table.
+// FULL's membership set: a bitmap when every id of the program fits 17 bits
+// (one bit per possible pair, at most 2 GiB, the size of a hash table of 90M
+// members; an insert is one atomic OR and the set never grows), else the hash
+// table.  GDLOG_FULL_SET=hash forces the hash table.
 static bool bitmap_full_set(bool hooks) {
     const int b = id_bits();
     // hooks may produce ids beyond those loaded so far
-    if (hooks || b < 1 || b > 16) return false;
+    if (hooks || b < 1 || b > 17) return false;
     const char *e = std::getenv("GDLOG_FULL_SET");
     return !(e && std::string(e) == "hash");
 }

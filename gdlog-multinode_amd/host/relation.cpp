@@ -39,17 +39,27 @@ gdl_ctx *context() {
         cfg.device = dev;
         cfg.verbose = verbose() ? 1 : 0;
         check(gdl_ctx_create(&cfg, &g_ctx), "gdl_ctx_create");
+        (void)gdl_set_id_bits(g_ctx, g_id_bits);
     }
     return g_ctx;
 }
 
 gdl_ctx *context_or_null() { return g_ctx; }
 
-void set_context(gdl_ctx *ctx) { g_ctx = ctx; }
+void set_context(gdl_ctx *ctx) {
+    g_ctx = ctx;
+    if (g_ctx) (void)gdl_set_id_bits(g_ctx, g_id_bits);
+}
 
 int id_bits() { return g_id_bits; }
-void note_id_bits(int bits) { g_id_bits = std::max(g_id_bits, std::min(bits, 32)); }
-void reset_id_bits() { g_id_bits = 0; }
+void note_id_bits(int bits) {
+    g_id_bits = std::max(g_id_bits, std::min(bits, 32));
+    if (g_ctx) (void)gdl_set_id_bits(g_ctx, g_id_bits);
+}
+void reset_id_bits() {
+    g_id_bits = 0;
+    if (g_ctx) (void)gdl_set_id_bits(g_ctx, 0);
+}
 
 bool verbose() {
     if (g_verbose < 0) {

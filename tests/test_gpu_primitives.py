@@ -241,7 +241,7 @@ def test_bitmap_set_refuses_keys_outside_its_domain(ctx):
     finally:
         ctx.hashset_free(hs)
     with pytest.raises(gdl.GdlError):
-        ctx.hashset_create_bitmap(17)  # above 512 MiB
+        ctx.hashset_create_bitmap(19)  # above 8 GiB
 
 
 def test_allocator_cap_reports_out_of_memory(ctx):
