@@ -358,6 +358,12 @@ gdl_status gdl_memcpy_d2d(gdl_ctx *ctx, void *dst, const void *src, size_t bytes
     return GDL_OK;
 }
 
+gdl_status gdl_set_id_bits(gdl_ctx *ctx, int bits) {
+    if (!ctx || bits < 0 || bits > 32) return GDL_ERR_INVALID_ARG;
+    ctx->id_bits = bits;
+    return GDL_OK;
+}
+
 gdl_status gdl_mem_info(gdl_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes,
                         uint64_t *in_use, uint64_t *peak, uint64_t *cached) {
     if (!ctx) return GDL_ERR_INVALID_ARG;

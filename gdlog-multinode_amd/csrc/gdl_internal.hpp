@@ -107,6 +107,7 @@ struct gdl_ctx {
     gdl_tuning tune{GDL_DEFAULT_LOAD_FACTOR, 8, 8, 256};  // load_factor mirrors cfg.load_factor
     std::string last_error;
     bool stats_on = false;
+    int id_bits = 0;  // gdl_set_id_bits: bound of every id's bit length, 0 = unknown
     bool capturing = false;  // between gdl_graph_begin and gdl_graph_end
     gdl::KernelStat stats[gdl::K_NUM];
     std::vector<gdl::PendingTiming> pending;

@@ -89,6 +89,13 @@ typedef struct gdl_tuning {
                                default 256 */
 } gdl_tuning;
 gdl_status gdl_get_tuning(gdl_ctx *ctx, gdl_tuning *out);
+/* Upper bound of the bit length of every id (both columns of a key) that the
+ * context's joins read and produce; 0 (the default) = unknown.  The fused
+ * join (gdl_join_insert) then sizes its hash set and new-tuple buffer by the
+ * keys the id domain still has room for, not only by the produced count
+ * (which a join with duplicate-heavy output can exceed by orders of
+ * magnitude).  A bitmap set always sizes by its own domain. */
+gdl_status gdl_set_id_bits(gdl_ctx *ctx, int bits);
 gdl_status gdl_set_tuning(gdl_ctx *ctx, const gdl_tuning *t);
 
 /* ---- device memory (replaces the per-op cudaMalloc/cudaFree/cudaMemset of
@@ -207,7 +214,7 @@ typedef struct gdl_hashset {
 } gdl_hashset;
 gdl_status gdl_hashset_create(gdl_ctx *ctx, uint64_t expected, gdl_hashset *out);
 /* A bitmap set for a dense id domain: every key (c0 << 32) | c1 with c0, c1 <
- * 2^key_bits (1..16; 2^(2 key_bits) bits, at most 512 MiB) has its own bit, so
+ * 2^key_bits (1..18; 2^(2 key_bits) bits, at most 8 GiB) has its own bit, so
  * an insert is one atomic OR on one word and the set never grows.  Same
  * operations as the hash table.  A key outside the domain is not inserted; it
  * sets a flag that gdl_hashset_check reports. */
