@@ -206,6 +206,7 @@ def load():
     rp, bp = POINTER(ProgResult), POINTER(ProgBuffers)
     _decl(prog, "gdl_prog_tc", c_int, [vp, vp, c_uint64, c_int, c_int, rp, bp])
     _decl(prog, "gdl_prog_sg", c_int, [vp, vp, c_uint64, c_int, rp, bp])
+    _decl(prog, "gdl_prog_tc_hooks", c_int, [vp, vp, c_uint64, c_int, c_int, rp, bp])
     _decl(prog, "gdl_prog_cspa", c_int, [vp, vp, c_uint64, vp, c_uint64, c_int, rp, bp])
     _decl(prog, "gdl_prog_last_error", c_char_p, [])
     _decl(prog, "gdl_prog_path_length", c_int,
@@ -758,6 +759,19 @@ class Context:
         cap = out_cap if out_cap is not None else (60_000_000 if want_keys else 0)
         try:
             return self._run(self.prog.gdl_prog_tc, (p, n, int(on_device), max_iteration),
+                             want_keys, want_trace, cap)
+        finally:
+            if isinstance(keep, DeviceArray):
+                keep.free()
+
+    def run_tc_hooks(self, edges, variant, on_device=False, want_keys=True, want_trace=True,
+                     out_cap=None):
+        """TC with device-run hooks (gdl_prog_tc_hooks): variant 0 predicate
+        t[1] != 42, 1 arithmetic generator, 2 arithmetic copy hook."""
+        p, n, keep = self._edges(edges, on_device)
+        cap = out_cap if out_cap is not None else (20_000_000 if want_keys else 0)
+        try:
+            return self._run(self.prog.gdl_prog_tc_hooks, (p, n, int(on_device), int(variant)),
                              want_keys, want_trace, cap)
         finally:
             if isinstance(keep, DeviceArray):

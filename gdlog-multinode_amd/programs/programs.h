@@ -46,6 +46,11 @@ void path_length_program(const EdgeInput &graph, int grid_size, int block_size,
 // The same result from the reference's own encoding (test/path_length.cu:
 // arity-3 path with a dependent column, the hook computing newt[2] =
 // outer[2] + 1 run on the device); out->main_keys are the (from, to) pairs.
+// TC with hooks that run on the device (variant 0: predicate t[1] != 42;
+// 1: generator (inner[1], (outer[1] + 1) mod 1000); 2: a copy hook seeding
+// path with (src[0], (src[1] * 7) mod 1000)).
+void tc_hooks_program(const EdgeInput &graph, int variant, int grid_size, int block_size,
+                      ProgramOutput *out);
 void path_length_hooks_program(const EdgeInput &graph, int grid_size, int block_size,
                                ProgramOutput *out, std::vector<uint32_t> *lens);
 

@@ -189,6 +189,16 @@ gdl_status gdl_prog_tc(gdl_ctx *ctx, const uint64_t *edges, uint64_t n, int on_d
     });
 }
 
+gdl_status gdl_prog_tc_hooks(gdl_ctx *ctx, const uint64_t *edges, uint64_t n, int on_device,
+                             int variant, gdl_prog_result *res, const gdl_prog_buffers *bufs) {
+    if ((n && !edges) || variant < 0 || variant > 2) return GDL_ERR_INVALID_ARG;
+    HostRows hr;
+    EdgeInput in = hr.input(edges, n, on_device);
+    return run_program(ctx, res, bufs, [&](ProgramOutput *out) {
+        tc_hooks_program(in, variant, grid_for_ctx(ctx), kBlock, out);
+    });
+}
+
 gdl_status gdl_prog_sg(gdl_ctx *ctx, const uint64_t *edges, uint64_t n, int on_device,
                        gdl_prog_result *res, const gdl_prog_buffers *bufs) {
     if (n && !edges) return GDL_ERR_INVALID_ARG;

@@ -58,6 +58,15 @@ typedef struct gdl_prog_buffers {
 /* max_iteration < 0: unbounded (INT_MAX, lie.cuh:30). */
 gdl_status gdl_prog_tc(gdl_ctx *ctx, const uint64_t *edges, uint64_t n, int on_device,
                        int max_iteration, gdl_prog_result *res, const gdl_prog_buffers *bufs);
+/* TC with device hooks that are not column selections (an extension that
+ * exercises the reference's per-tuple hook calls, relation.cu:348-358 and
+ * 390-403): variant 0 keeps test/tc.cu's generator and adds the predicate
+ * t[1] != 42; variant 1 uses the generator (inner[1], (outer[1] + 1) mod
+ * 1000); variant 2 seeds path with a RelationalCopy whose hook is
+ * (src[0], (src[1] * 7) mod 1000), then runs test/tc.cu's join (its trace in
+ * trace_init).  Outputs as gdl_prog_tc. */
+gdl_status gdl_prog_tc_hooks(gdl_ctx *ctx, const uint64_t *edges, uint64_t n, int on_device,
+                             int variant, gdl_prog_result *res, const gdl_prog_buffers *bufs);
 gdl_status gdl_prog_sg(gdl_ctx *ctx, const uint64_t *edges, uint64_t n, int on_device,
                        gdl_prog_result *res, const gdl_prog_buffers *bufs);
 /* All-pairs hop counts: the reference's path_length program
