@@ -244,6 +244,10 @@ struct HsSel {
     const gdl_hs_seg *segs;
     u32 n_segs;
     int seg_shift;
+    // > 0: the caller sized the set and the output by this id bound; a
+    // produced key with an id at or above 2^dom_bits is not inserted and
+    // sets counters[2]
+    int dom_bits;
 };
 
 // The set a key belongs to (slots == nullptr: a segment with no set).
@@ -319,7 +323,10 @@ __device__ __forceinline__ void join_insert_blocks(
                 if (eval_pred(key, pop, plhs, prhs)) {
                     ++produced;
                     const gdl_hs_seg hs = sel_set<SEG>(sel, s_segs, key);
-                    if (hs.slots) fresh = hs_insert_key((u64 *)hs.slots, hs.mask, key);
+                    if (sel.dom_bits && !bm_in_domain(key, sel.dom_bits))
+                        atomicOr(&counters[2], 1ull);
+                    else if (hs.slots)
+                        fresh = hs_insert_key((u64 *)hs.slots, hs.mask, key);
                 }
             }
             const u64 bal = __ballot(fresh);
@@ -476,7 +483,10 @@ __global__ __launch_bounds__(JB) void k_join_insert_lb(
             old[r] = 0;
             if ((live >> r) & 1u) {
                 const gdl_hs_seg hs = sel_set<SEG>(sel, s_segs, keys[r]);
-                if (!hs.slots) {
+                if (sel.dom_bits && !bm_in_domain(keys[r], sel.dom_bits)) {
+                    atomicOr(&counters[2], 1ull);
+                    live &= ~(1u << r);
+                } else if (!hs.slots) {
                     live &= ~(1u << r);
                 } else if (hs.mask & kHsBitmap) {
                     const int kb = (int)(hs.mask & 63);
@@ -528,6 +538,12 @@ __global__ __launch_bounds__(JB) void k_join_insert_lb(
 __global__ void k_set2(u64 *p, u64 a, u64 b) {
     p[0] = a;
     p[1] = b;
+}
+
+__global__ void k_set3(u64 *p, u64 a, u64 b, u64 c) {
+    p[0] = a;
+    p[1] = b;
+    p[2] = c;
 }
 
 __global__ __launch_bounds__(JB) void k_project(const u64 *in, u64 n, int arity, int s0, int s1,
@@ -830,6 +846,7 @@ static gdl_status join_insert_impl(gdl_ctx *ctx, const u64 *d_inner, u64 n_inner
         } else if (ctx->id_bits > 0 && ctx->id_bits < 32) {
             const u64 dom = 1ull << (2 * ctx->id_bits);
             room = dom - std::min(dom, hs->size);
+            sel.dom_bits = ctx->id_bits;  // the kernel checks every key against it
         }
         const u64 new_bound = std::min(bound, room);
         // size for up to 2 |outer| new members at the load factor; the bound
@@ -853,9 +870,10 @@ static gdl_status join_insert_impl(gdl_ctx *ctx, const u64 *d_inner, u64 n_inner
     }
     Scratch scr(ctx);
     u64 *counters;
-    GDL_TRY(scr.get(2, &counters));
-    hipLaunchKernelGGL(k_set2, dim3(1), dim3(1), 0, ctx->stream, counters, (u64)*n_new, (u64)0);
-    GDL_TRY(check_launch(ctx, "k_set2"));
+    GDL_TRY(scr.get(3, &counters));
+    hipLaunchKernelGGL(k_set3, dim3(1), dim3(1), 0, ctx->stream, counters, (u64)*n_new, (u64)0,
+                       (u64)0);
+    GDL_TRY(check_launch(ctx, "k_set3"));
     if (balanced) {
         KTimer kt(ctx, K_JOIN_INSERT, 24.0 * n_outer);
         const unsigned g = (unsigned)std::max<u64>(
@@ -884,8 +902,12 @@ static gdl_status join_insert_impl(gdl_ctx *ctx, const u64 *d_inner, u64 n_inner
         }
         GDL_TRY(check_launch(ctx, "k_join_insert"));
     }
-    u64 h[2];
-    GDL_TRY(read_u64(ctx, counters, h, 2));
+    u64 h[3];
+    GDL_TRY(read_u64(ctx, counters, h, 3));
+    if (h[2])
+        return fail(ctx, GDL_ERR_OVERFLOW,
+                    "fused join: a produced key has an id at or above 2^" +
+                        std::to_string(sel.dom_bits) + ", the context's id bound (gdl_set_id_bits)");
     if (h[0] > *new_cap)
         return fail(ctx, GDL_ERR_OVERFLOW, "fused join: new tuples exceed the output capacity (" +
                                                std::to_string(h[0]) + " > " +

@@ -83,6 +83,28 @@ bool needs_sorted_full(const std::vector<ra_op> &ops, const Relation *rel) {
     return false;
 }
 
+// A tmp relation whose newt can be deduplicated by a set as the joins produce
+// it: not a raw (tmp_flag) relation, written only by column-map joins, never
+// read as a join inner (which would need it sorted).
+bool tmp_dedup_ok(const std::vector<ra_op> &ops, const Relation *rel) {
+    if (rel->tmp_flag || rel->arity > 2) return false;
+    bool written = false;
+    for (const auto &op : ops) {
+        if (const auto *j = std::get_if<RelationalJoin>(&op)) {
+            if (j->inner_rel == rel) return false;
+            if (j->output_rel == rel) {
+                if (op_uses_hooks(op) || j->disable_load) return false;
+                written = true;
+            }
+        } else if (const auto *c = std::get_if<RelationalCopy>(&op)) {
+            if (c->dest_rel == rel || c->src_rel == rel) return false;
+        } else if (const auto *a = std::get_if<RelationalACopy>(&op)) {
+            if (a->dest_rel == rel || a->src_rel == rel) return false;
+        }
+    }
+    return written;
+}
+
 bool output_fusable(const std::vector<ra_op> &ops, const Relation *rel) {
     for (const auto &op : ops) {
         if (const auto *j = std::get_if<RelationalJoin>(&op)) {
@@ -376,6 +398,10 @@ void reset_checkpoint_sequence() { g_ckpt.next = 0; }
 void LIE::fixpoint_loop() {
     trace.clear();
     gdl_ctx *ctx = context();
+    struct Scope {
+        Scope() { enter_lie_scope(); }
+        ~Scope() { leave_lie_scope(); }
+    } id_scope;
     if (verbose()) std::cout << "start lie .... " << std::endl;
     // checkpoint files: the LIE's own, else the thread's defaults
     std::string ck_file = checkpoint_file, rs_file = resume_file;
@@ -400,6 +426,9 @@ void LIE::fixpoint_loop() {
     const bool auto_mode = keep_full_log || full_mode() == FullMode::Auto;
     bool lie_uses_hooks = false;
     for (const auto &op : ra_ops) lie_uses_hooks = lie_uses_hooks || op_uses_hooks(op);
+    // tmp relations whose newt is a set: deduplicated by a per-iteration set
+    for (Relation *rel : tmp_relations)
+        rel->tmp_dedup = auto_mode && tmp_dedup_ok(ra_ops, rel);
     for (Relation *rel : update_relations) {
         if (!resumed) {
             rel->current_full_size = rel->full->tuple_counts;
@@ -488,7 +517,10 @@ void LIE::fixpoint_loop() {
                 }
             }
         }
-        for (Relation *rel : tmp_relations) container_free(rel->newt);
+        for (Relation *rel : tmp_relations) {
+            container_free(rel->newt);
+            if (rel->tmp_set.slots) check(gdl_hashset_free(ctx, &rel->tmp_set), "gdl_hashset_free");
+        }
 
         if (verbose())
             std::cout << "Iteration " << iteration_counter << " popluating new tuple" << std::endl;
@@ -602,6 +634,10 @@ void LIE::fixpoint_loop() {
         if (fixpoint_flag || iteration_counter > max_iteration) break;
     }
     iteration_count = iteration_counter;
+    for (Relation *rel : tmp_relations) {
+        rel->tmp_dedup = false;
+        if (rel->tmp_set.slots) check(gdl_hashset_free(ctx, &rel->tmp_set), "gdl_hashset_free");
+    }
 
     for (Relation *rel : update_relations) {
         if (rel->append_mode) leave_append_mode(rel, keep_full_log);

@@ -278,6 +278,37 @@ void RelationalJoin::operator()() {
         else accumulate_sorted(output_rel->newt, res, m, detail_time);
         return;
     }
+    if (output_rel->tmp_dedup && !disable_load) {
+        // a tmp relation's newt is the set of this iteration's outputs: the
+        // join inserts into the iteration's set and appends what is new
+        gdl_ctx *ctx = context();
+        if (!output_rel->tmp_set.slots) {
+            const int b = id_bits();
+            if (b >= 1 && b <= 17)
+                check(gdl_hashset_create_bitmap(ctx, (uint32_t)b, &output_rel->tmp_set),
+                      "gdl_hashset_create_bitmap");
+            else
+                check(gdl_hashset_create(ctx, outer->tuple_counts + 1024, &output_rel->tmp_set),
+                      "gdl_hashset_create");
+        }
+        GHashRelContainer *nw = output_rel->newt;
+        uint64_t n_new = nw->keys ? nw->tuple_counts : 0, cap = nw->capacity, m = 0;
+        {
+            Phase ph(detail_time, 2);
+            check(gdl_join_insert(ctx, inner->keys, inner->tuple_counts, idx, outer->keys,
+                                  outer->tuple_counts, gen_map.map, pred_map.pred,
+                                  &output_rel->tmp_set, &nw->keys, &cap, &n_new, &m),
+                  "gdl_join_insert");
+        }
+        nw->capacity = cap;
+        nw->tuple_counts = n_new;
+        nw->data_raw_row_size = n_new;
+        nw->sorted = n_new <= 1;
+        if (verbose())
+            std::cout << output_rel->name << "   " << k << " join result size(non dedup) " << m
+                      << std::endl;
+        return;
+    }
     if (output_rel->fused_output && !output_is_tmp && !disable_load) {
         uint64_t m = 0;
         {

@@ -39,26 +39,29 @@ gdl_ctx *context() {
         cfg.device = dev;
         cfg.verbose = verbose() ? 1 : 0;
         check(gdl_ctx_create(&cfg, &g_ctx), "gdl_ctx_create");
-        (void)gdl_set_id_bits(g_ctx, g_id_bits);
     }
     return g_ctx;
 }
 
 gdl_ctx *context_or_null() { return g_ctx; }
 
-void set_context(gdl_ctx *ctx) {
-    g_ctx = ctx;
-    if (g_ctx) (void)gdl_set_id_bits(g_ctx, g_id_bits);
-}
+void set_context(gdl_ctx *ctx) { g_ctx = ctx; }
 
 int id_bits() { return g_id_bits; }
+// The engine context's id bound (gdl_set_id_bits) is set only while a
+// fixpoint of this thread runs, and follows note_id_bits meanwhile: a direct
+// engine call on the same context outside a fixpoint is never sized by it.
+thread_local int g_lie_depth = 0;
 void note_id_bits(int bits) {
     g_id_bits = std::max(g_id_bits, std::min(bits, 32));
-    if (g_ctx) (void)gdl_set_id_bits(g_ctx, g_id_bits);
+    if (g_ctx && g_lie_depth > 0) (void)gdl_set_id_bits(g_ctx, g_id_bits);
 }
-void reset_id_bits() {
-    g_id_bits = 0;
-    if (g_ctx) (void)gdl_set_id_bits(g_ctx, 0);
+void reset_id_bits() { g_id_bits = 0; }
+void enter_lie_scope() {
+    if (g_lie_depth++ == 0 && g_ctx) (void)gdl_set_id_bits(g_ctx, g_id_bits);
+}
+void leave_lie_scope() {
+    if (--g_lie_depth == 0 && g_ctx) (void)gdl_set_id_bits(g_ctx, 0);
 }
 
 bool verbose() {

@@ -97,6 +97,12 @@ struct Relation {
     // (gdl_join_insert) and the new tuples collect in pend_keys, which becomes
     // the next delta.
     bool fused_output = false;
+    // A LIE tmp relation loaded without tmp_flag (its newt is a set, dropped
+    // after every iteration) whose writers are all column-map joins: joins
+    // insert into tmp_set (a bitmap or hash set, cleared every iteration) and
+    // append the new keys to newt, instead of sorting and merging.
+    bool tmp_dedup = false;
+    gdl_hashset tmp_set{};
     uint64_t *pend_keys = nullptr;
     uint64_t pend_cap = 0, pend_n = 0, pend_raw = 0;
 
@@ -160,6 +166,10 @@ void set_context(gdl_ctx *ctx);
 int id_bits();
 void note_id_bits(int bits);
 void reset_id_bits();
+// Bracket a running fixpoint: the engine context's id bound
+// (gdl_set_id_bits) holds id_bits() inside and 0 outside.
+void enter_lie_scope();
+void leave_lie_scope();
 
 // Internal helpers shared by the RA operators and the fixpoint loop.
 void container_free(GHashRelContainer *c);
