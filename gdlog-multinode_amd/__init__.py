@@ -183,6 +183,8 @@ def load():
     _decl(eng, "gdl_hashset_create_bitmap", c_int, [vp, c_uint32, vp])
     _decl(eng, "gdl_set_id_bits", c_int, [vp, c_int])
     _decl(eng, "gdl_hashset_check", c_int, [vp, vp])
+    _decl(eng, "gdl_sort_unique_u64_noalloc", c_int,
+          [vp, vp, c_uint64, c_int, c_int, vp, _U64P])
     _decl(eng, "gdl_merge_union_u64", c_int, [vp, vp, c_uint64, vp, c_uint64, vp, _U64P])
     _decl(eng, "gdl_set_difference_u64", c_int, [vp, vp, c_uint64, vp, c_uint64, vp, _U64P])
     _decl(eng, "gdl_merge_diff_u64", c_int,

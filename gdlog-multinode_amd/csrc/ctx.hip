@@ -259,6 +259,10 @@ gdl_status gdl_ctx_create(const gdl_config *cfg_in, gdl_ctx **out) {
         gdl_ctx_destroy(ctx);
         return GDL_ERR_HIP;
     }
+    if (noalloc_setup(ctx) != GDL_OK) {
+        gdl_ctx_destroy(ctx);
+        return GDL_ERR_HIP;
+    }
     *out = ctx;
     return GDL_OK;
 }

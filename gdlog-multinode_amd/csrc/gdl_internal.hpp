@@ -112,6 +112,10 @@ struct gdl_ctx {
     gdl::KernelStat stats[gdl::K_NUM];
     std::vector<gdl::PendingTiming> pending;
     std::vector<hipEvent_t> event_pool;
+    // Fixed scratch arena: while set, Scratch carves from it instead of the
+    // allocator (the allocation-free entry points set it for one call).
+    char *arena = nullptr;
+    size_t arena_size = 0, arena_used = 0;
 };
 
 namespace gdl {
@@ -146,7 +150,9 @@ gdl_status hip_fail(gdl_ctx *ctx, hipError_t e, const char *what);
         if (_s != GDL_OK) return _s;          \
     } while (0)
 
-// Device scratch owned by the context allocator, freed on scope exit.
+// Device scratch owned by the context allocator, freed on scope exit; or,
+// while the context has a fixed arena, 256-byte aligned pieces of the arena
+// (bump allocation, released when the arena is cleared).
 class Scratch {
   public:
     Scratch(gdl_ctx *ctx) : ctx_(ctx) {}
@@ -156,6 +162,15 @@ class Scratch {
     template <typename T>
     gdl_status get(size_t count, T **out) {
         void *p = nullptr;
+        if (ctx_->arena) {
+            const size_t at = (ctx_->arena_used + 255) & ~size_t(255);
+            const size_t bytes = count * sizeof(T) + 16;
+            if (at > ctx_->arena_size || bytes > ctx_->arena_size - at)
+                return fail(ctx_, GDL_ERR_OUT_OF_MEMORY, "scratch arena exhausted");
+            ctx_->arena_used = at + bytes;
+            *out = reinterpret_cast<T *>(ctx_->arena + at);
+            return GDL_OK;
+        }
         std::string err;
         gdl_status st = ctx_->alloc->alloc(count * sizeof(T) + 16, &p, &err);
         if (st != GDL_OK) return fail(ctx_, st, err);
@@ -183,6 +198,9 @@ class KTimer {
 };
 
 gdl_status read_u64(gdl_ctx *ctx, const u64 *d, u64 *h, int count = 1);
+// Looks up this device's fixed scratch array (once per device; the device
+// must be current).  Called by gdl_ctx_create.
+gdl_status noalloc_setup(gdl_ctx *ctx);
 gdl_status check_launch(gdl_ctx *ctx, const char *what);
 gdl_status check_device_error(gdl_ctx *ctx);
 

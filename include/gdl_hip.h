@@ -142,6 +142,20 @@ gdl_status gdl_key_bits(gdl_ctx *ctx, const uint64_t *d_keys, uint64_t n, int *h
 gdl_status gdl_sort_unique_u64(gdl_ctx *ctx, const uint64_t *d_in, uint64_t n, int hi_bits,
                                int lo_bits, uint64_t *d_out, uint64_t *n_out);
 
+/* gdl_sort_unique_u64 without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP-graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * scratch is one fixed device array per GPU (18 MiB, held until the process
+ * ends), so n is limited to GDL_NOALLOC_MAX_KEYS (larger n: INVALID_ARG, and
+ * nothing runs).  The caller still passes the input and output buffers.
+ * Every call on one GPU uses that GPU's array: calls that overlap on one GPU
+ * (other threads, or other contexts' streams) race, and the caller must
+ * serialise them.  Kernel timing events are not recorded for the call. */
+#define GDL_NOALLOC_MAX_KEYS (1u << 20)
+gdl_status gdl_sort_unique_u64_noalloc(gdl_ctx *ctx, const uint64_t *d_in, uint64_t n,
+                                       int hi_bits, int lo_bits, uint64_t *d_out,
+                                       uint64_t *n_out);
+
 /* ---- merge / set difference over sorted duplicate-free arrays ----------
  * gdl_merge_union_u64: thrust::merge + thrust::unique of newt accumulation
  *   (join.cu:181-195, copy.cu:63-72); d_out holds na + nb slots.
