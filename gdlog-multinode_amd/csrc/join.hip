@@ -491,10 +491,9 @@ __global__ __launch_bounds__(JB) void k_join_insert_lb(
                 } else if (hs.mask & kHsBitmap) {
                     const int kb = (int)(hs.mask & 63);
                     if (bm_in_domain(keys[r], kb)) {
+                        // a plain load first: a set bit proves membership
                         const u64 b = bm_bit(keys[r], kb);
-                        const u64 m = 1ull << (b & 63);
-                        old[r] = atomicOr((unsigned long long *)((u64 *)hs.slots + (b >> 6)),
-                                          (unsigned long long)m) & m;
+                        old[r] = ((const u64 *)hs.slots)[b >> 6] & (1ull << (b & 63));
                         bm |= 1u << r;
                     }
                 } else if (keys[r] != kHsEmpty) {
@@ -502,6 +501,17 @@ __global__ __launch_bounds__(JB) void k_join_insert_lb(
                         (unsigned long long *)((u64 *)hs.slots + (mix64(keys[r]) & hs.mask)),
                         (unsigned long long)kHsEmpty, (unsigned long long)keys[r]);
                 }
+            }
+        }
+        // bitmap keys whose bit read clear: the atomic OR decides, all in flight
+#pragma unroll
+        for (int r = 0; r < JM_I; ++r) {
+            if (((bm >> r) & 1u) && old[r] == 0) {
+                const gdl_hs_seg hs = sel_set<SEG>(sel, s_segs, keys[r]);
+                const u64 b = bm_bit(keys[r], (int)(hs.mask & 63));
+                const u64 m = 1ull << (b & 63);
+                old[r] = atomicOr((unsigned long long *)((u64 *)hs.slots + (b >> 6)),
+                                  (unsigned long long)m) & m;
             }
         }
 #pragma unroll
