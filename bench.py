@@ -718,6 +718,8 @@ def main():
                     help="instances of the batched-TC measurement (0: skip it)")
     ap.add_argument("--served-requests", type=int, default=1000,
                     help="requests of the served-mix measurement (0: skip it)")
+    ap.add_argument("--served-workers", type=int, default=1,
+                    help="program workers of the query server beside its TC batch")
     ap.add_argument("--datalog-reps", type=int, default=3,
                     help="timed runs of the Datalog front-end and path-length "
                          "comparisons (0: skip them)")
@@ -748,7 +750,7 @@ def main():
     if args.phase:
         ctx = gdl.Context(device=0)
         out = batched(gdl, ctx, args.batched_instances) if args.phase == "batched" else \
-            served(gdl, ctx, args.served_requests)
+            served(gdl, ctx, args.served_requests, workers=args.served_workers)
         ctx.close()
         print(json.dumps(out), flush=True)
         return
@@ -788,7 +790,8 @@ def main():
     if args.batched_instances > 0:
         batch = phase_child("batched", ["--batched-instances", str(args.batched_instances)])
     if args.served_requests > 0:
-        serve = phase_child("served", ["--served-requests", str(args.served_requests)])
+        serve = phase_child("served", ["--served-requests", str(args.served_requests),
+                                       "--served-workers", str(args.served_workers)])
 
     cpu = None
     if not args.no_cpu:
