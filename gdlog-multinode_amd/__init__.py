@@ -106,6 +106,14 @@ class ProgBuffers(Structure):
                 ("trace_init", POINTER(TraceRec)), ("trace_init_cap", c_uint64)]
 
 
+class DenseRule(Structure):
+    _fields_ = [("shape", c_int), ("r1", c_void_p), ("n1", c_uint64), ("r2", c_void_p),
+                ("n2", c_uint64)]
+
+
+DENSE_TC, DENSE_SG = 1, 2
+
+
 class HsSeg(Structure):
     _fields_ = [("slots", c_void_p), ("mask", c_uint64)]
 
@@ -241,6 +249,11 @@ def load():
     _decl(eng, "gdl_set_tuning", c_int, [vp, POINTER(Tuning)])
     _decl(eng, "gdl_hashset_reserve", c_int, [vp, POINTER(HashSet), c_uint64])
     _decl(eng, "gdl_hashset_extract", c_int, [vp, POINTER(HashSet), vp, c_uint64, _U64P])
+    _decl(eng, "gdl_hashset_extract_sorted", c_int, [vp, POINTER(HashSet), vp, c_uint64, _U64P])
+    _decl(eng, "gdl_set_dense_mode", c_int, [vp, c_int])
+    _decl(eng, "gdl_fixpoint_dense", c_int,
+          [vp, POINTER(DenseRule), POINTER(HashSet), vp, c_uint64, POINTER(vp), _U64P, _U64P,
+           c_int, _U64P, _U64P, POINTER(c_int)])
     _decl(eng, "gdl_join_insert_seg", c_int,
           [vp, vp, c_uint64, POINTER(Index), vp, c_uint64, ColMap, Pred, vp, c_uint32, c_int,
            vp, c_uint64, _U64P, _U64P])
@@ -596,6 +609,74 @@ class Context:
         self.index_free(idx)
         di.free(); do.free()
         return new, raw.value
+
+    def set_dense_mode(self, mode):
+        """0: no bit-matrix bodies; 1: when the cost model prefers them (default);
+        2: whenever they apply."""
+        self._check(self.eng.gdl_set_dense_mode(self.h, int(mode)), "set_dense_mode")
+
+    def hashset_extract_sorted(self, hs):
+        """Members of a set in key order."""
+        o = self.alloc(max(hs.size, 1))
+        m = c_uint64()
+        self._check(self.eng.gdl_hashset_extract_sorted(
+            self.h, ctypes.byref(hs), o.ptr, max(hs.size, 1), ctypes.byref(m)),
+            "hashset_extract_sorted")
+        r = o.numpy(m.value)
+        o.free()
+        return r
+
+    def fixpoint_dense(self, shape, key_bits, full, delta, r1, r2=None, max_bodies=1 << 20):
+        """gdl_fixpoint_dense from host arrays: FULL (a set; also the log's start), the
+        delta, the static relations.  Returns dict(new, raw, log, members) with the
+        per-body counts, the log (FULL then each body's new tuples) and the bitmap set's
+        members in key order afterwards."""
+        hs = self.hashset_create_bitmap(key_bits)
+        df = self.to_device(full)
+        log = c_void_p(None)
+        cap = c_uint64(0)
+        end = c_uint64(int(df.n))
+        bufs = []
+        try:
+            if df.n:
+                scratch = self.alloc(df.n)
+                added = c_uint64()
+                self._check(self.eng.gdl_hashset_insert_new(
+                    self.h, ctypes.byref(hs), df.ptr, df.n, scratch.ptr, ctypes.byref(added)),
+                    "hashset_insert_new")
+                scratch.free()
+                p = c_void_p()
+                self._check(self.eng.gdl_alloc(self.h, df.n * 8, ctypes.byref(p)), "alloc")
+                self._check(self.eng.gdl_memcpy_d2d(self.h, p, df.ptr, df.n * 8), "d2d")
+                log, cap = p, c_uint64(int(df.n))
+            dd, d1 = self.to_device(delta), self.to_device(r1)
+            bufs += [dd, d1]
+            d2 = self.to_device(r2) if r2 is not None else None
+            if d2 is not None:
+                bufs.append(d2)
+            rule = DenseRule(shape, d1.ptr, d1.n, d2.ptr if d2 is not None else None,
+                             d2.n if d2 is not None else 0)
+            hn = np.zeros(max_bodies, dtype=np.uint64)
+            hr = np.zeros(max_bodies, dtype=np.uint64)
+            nb = c_int()
+            self._check(self.eng.gdl_fixpoint_dense(
+                self.h, ctypes.byref(rule), ctypes.byref(hs), dd.ptr, dd.n, ctypes.byref(log),
+                ctypes.byref(cap), ctypes.byref(end), max_bodies, _p64(hn), _p64(hr),
+                ctypes.byref(nb)), "fixpoint_dense")
+            out = np.zeros(end.value, dtype=np.uint64)
+            if end.value:
+                self._check(self.eng.gdl_memcpy_d2h(self.h, _p64(out), log, end.value * 8),
+                            "d2h")
+            members = self.hashset_extract_sorted(hs)
+            return dict(new=[int(x) for x in hn[:nb.value]], raw=[int(x) for x in hr[:nb.value]],
+                        log=out, members=members, size=int(hs.size))
+        finally:
+            if log.value:
+                self.eng.gdl_free(self.h, log)
+            for b in bufs:
+                b.free()
+            df.free()
+            self.hashset_free(hs)
 
     def hashset_extract(self, hs):
         """Members of a set (any order)."""

@@ -1,5 +1,6 @@
 // ctx.hip -- context, caching allocator, statistics, memory entry points.
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -14,7 +15,8 @@ const char *kKernelNames[K_NUM] = {
     "merge_count",  "merge_write",  "idx_count",    "idx_build",     "idx_lookup",
     "join_count",   "join_materialize", "project",  "fill",          "hs_insert_new",
     "hs_rehash",    "join_insert",  "hs_extract",   "seg_hist",      "mask_cols",
-    "linear_body",  "segment_labels", "scatter_by_key"};
+    "linear_body",  "segment_labels", "scatter_by_key", "dense_setup", "dense_body",
+    "bm_extract_sorted"};
 
 // ---------------------------------------------------------------- allocator
 
@@ -246,6 +248,10 @@ gdl_status gdl_ctx_create(const gdl_config *cfg_in, gdl_ctx **out) {
     ctx->cfg = cfg;
     ctx->tune.load_factor = cfg.load_factor;
     ctx->stats_on = cfg.collect_stats != 0;
+    if (const char *e = getenv("GDLOG_DENSE")) {
+        const int m = atoi(e);
+        ctx->dense_mode = m < 0 ? 0 : (m > 2 ? 2 : m);
+    }
     ctx->alloc = new Allocator(cfg.device);
     ctx->alloc->set_cap(cfg.mem_cap_bytes);
     hipDeviceProp_t prop;

@@ -79,6 +79,9 @@ enum KernelId {
     K_LINEAR,
     K_LABEL,
     K_SCATTER,
+    K_DENSE_SETUP,
+    K_DENSE_BODY,
+    K_BM_EXTRACT,
     K_NUM
 };
 extern const char *kKernelNames[K_NUM];
@@ -109,6 +112,7 @@ struct gdl_ctx {
     bool stats_on = false;
     int id_bits = 0;  // gdl_set_id_bits: bound of every id's bit length, 0 = unknown
     bool capturing = false;  // between gdl_graph_begin and gdl_graph_end
+    int dense_mode = 1;      // gdl_set_dense_mode
     gdl::KernelStat stats[gdl::K_NUM];
     std::vector<gdl::PendingTiming> pending;
     std::vector<hipEvent_t> event_pool;
@@ -229,6 +233,13 @@ gdl_status join_count(gdl_ctx *ctx, const u64 *inner, u64 n_inner, const gdl_ind
                       const u64 *outer, u64 n_outer, u32 *counts, u64 *begins);
 // Per-instance hash sets of a segmented fused join (one per batch slot).
 constexpr u32 kMaxSegs = 256;
+// Bodies on bit matrices (dense.hip): the smallest delta for which the
+// context's dense mode runs them (~0: never), and the run itself
+// (gdl_fixpoint_dense's contract).
+u64 dense_threshold_for(const gdl_ctx *ctx, const gdl_hashset *hs, int shape, u64 n1, u64 n2);
+gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
+                     const u64 *d_delta, u64 n_delta, u64 **d_log, u64 *log_cap, u64 *log_end,
+                     int max_bodies, u64 *h_new, u64 *h_raw, int *bodies);
 // Exclusive scans of two arrays in place in one launch (v[n] = total each).
 gdl_status scan_inplace2(gdl_ctx *ctx, u64 *v0, u64 *v1, u64 n, u64 *totals /* 2 */);
 
@@ -241,6 +252,13 @@ inline unsigned grid_for(u64 n, u64 per_block) {
 
 // ----------------------------------------------------------- device helpers
 namespace gdl {
+
+// Orders a wave's LDS accesses: what one lane wrote is visible to the others
+// after it.
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
 
 __device__ __forceinline__ u64 lane_mask_lt() {
     return (1ull << (threadIdx.x & 63)) - 1ull;

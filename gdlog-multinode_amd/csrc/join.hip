@@ -180,11 +180,6 @@ __global__ __launch_bounds__(JB) void k_join_materialize(const u64 *inner, const
 constexpr u32 kStage = 256;
 constexpr u64 kHubRun = 256;  // longer inner runs use the load-balanced variant
 
-__device__ __forceinline__ void wave_sync_lds() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
 // Writes stay below out_cap; a cursor past it tells the host the caller's
 // output bound was wrong.
 __device__ __forceinline__ void flush_stage(const u64 *stage, u32 n, u64 *out, u64 out_cap,
@@ -1007,9 +1002,31 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
     GDL_TRY(scr.get(2 * kChunk, &counts));
     LinState *st = (LinState *)st_words;
     u64 d0 = *delta_begin, d1 = *delta_end;
+    // TC's map over a bitmap FULL: bodies whose delta is dense run on bit
+    // matrices (dense.hip); the delta size where they start
+    const bool tc_map = map.arity == 2 && map.src[0] == GDL_COL_INNER1 &&
+                        map.src[1] == GDL_COL_OUTER1 && pred.op == GDL_PRED_NONE &&
+                        inner_idx->prefix_cols == 1;
+    const gdl_dense_rule rule{GDL_DENSE_TC, d_inner, n_inner, nullptr, 0};
+    u64 dense_at = tc_map ? dense_threshold_for(ctx, hs, GDL_DENSE_TC, n_inner, 0) : ~0ull;
     int done = 0, ramp = 4;
     while (done < max_bodies) {
         const u64 nd = d1 - d0;
+        if (nd >= dense_at) {
+            int nb = 0;
+            u64 end = d1;
+            GDL_TRY(dense_run(ctx, &rule, hs, *d_log + d0, nd, d_log, log_cap, &end,
+                              max_bodies - done, h_new + done, h_raw + done, &nb));
+            if (nb > 0) {
+                done += nb;
+                const u64 last = h_new[done - 1];
+                d0 = end - last;
+                d1 = end;
+                if (last == 0) break;  // the fixpoint
+                continue;
+            }
+            dense_at = ~0ull;  // declined (ids beyond the set's domain): tuple bodies only
+        }
         const u64 worst = nd > (~0ull >> 20) / mr ? (~0ull >> 20) : nd * mr;
         // the set: the next chunk's bodies (about |delta| each) at the load
         // factor and the worst body at 0.9; when it has to grow, it grows to
@@ -1050,6 +1067,8 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
         const u64 want = (nd + LB - 1) / LB;
         const unsigned g = (unsigned)std::min<u64>(grid_cap(ctx), std::max<u64>(want, 16));
         h.delta_limit = g >= grid_cap(ctx) ? ~0ull : (u64)g * LB * 8;
+        // a delta that turns dense ends the chunk too (bit-matrix bodies next)
+        if (dense_at != ~0ull) h.delta_limit = std::min<u64>(h.delta_limit, dense_at - 1);
         GDL_HIP(ctx, hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
         for (int i = 0; i < k; ++i) {
             KTimer kt(ctx, K_LINEAR, 0.0);

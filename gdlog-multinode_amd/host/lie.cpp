@@ -184,10 +184,26 @@ void leave_append_mode(Relation *rel, bool keep_log) {
     const tuple_size_t n = rel->current_full_size;
     if (n && !rel->full->sorted && !keep_log) {
         uint64_t *sorted = device_alloc_keys(n);
-        const int b = id_bits();
-        gdl_status st = gdl_sort_u64(ctx, rel->full->keys, n, b, b, sorted);
+        const int kb = (int)rel->full_set.key_bits;
+        gdl_status st;
+        if (kb && (1ull << (2 * kb)) / 8 <= 32ull * n) {
+            // a bitmap set is FULL in key order: two streaming passes over it
+            // instead of a sort of the log (when the bitmap is not much larger
+            // than the keys)
+            uint64_t got = 0;
+            st = gdl_hashset_extract_sorted(ctx, &rel->full_set, sorted, n, &got);
+            if (st == GDL_OK && got != n) {
+                device_free(sorted);
+                throw Error(GDL_ERR_DEVICE, rel->name + ": the FULL set holds " +
+                                                std::to_string(got) + " tuples, its log " +
+                                                std::to_string(n));
+            }
+        } else {
+            const int b = id_bits();
+            st = gdl_sort_u64(ctx, rel->full->keys, n, b, b, sorted);
+        }
         if (st != GDL_OK) device_free(sorted);
-        check(st, "gdl_sort_u64");
+        check(st, "sorted FULL");
         container_take(rel->full, sorted, n, n, true);
     }
     check(gdl_hashset_check(ctx, &rel->full_set), "gdl_hashset_check");
@@ -382,6 +398,99 @@ bool read_snapshot(LIE &lie, const std::string &path, int *iteration, bool *fini
     return true;
 }
 
+// An SG-shaped LIE (gdl_fixpoint_dense's GDL_DENSE_SG): out(x, y) :-
+// r1(a, x), delta(a, b), r2(b, y) as a join of a static r1 with out's delta
+// into a deduplicated tmp (b, x), then a join of a static r2 with the tmp
+// into out, both with the (outer column 1, inner column 1) map and no
+// predicate.  Null when the LIE has another shape.
+struct Chain {
+    RelationalJoin *j1 = nullptr, *j2 = nullptr;
+};
+
+Chain sg_chain(std::vector<ra_op> &ops, const std::vector<Relation *> &updates,
+               const std::vector<Relation *> &tmps, const std::vector<Relation *> &statics) {
+    Chain c;
+    if (ops.size() != 2 || updates.size() != 1 || tmps.size() != 1) return c;
+    auto *j1 = std::get_if<RelationalJoin>(&ops[0]);
+    auto *j2 = std::get_if<RelationalJoin>(&ops[1]);
+    Relation *out = updates[0], *tmp = tmps[0];
+    if (!j1 || !j2 || !out->fused_output || !out->full_set.key_bits || !tmp->tmp_dedup) return c;
+    auto is_static = [&](const Relation *r) {
+        return std::find(statics.begin(), statics.end(), r) != statics.end();
+    };
+    auto swap_map = [](const RelationalJoin *j) {
+        return j->gen_map.ok && j->pred_map.ok && j->pred_map.pred.op == GDL_PRED_NONE &&
+               j->gen_map.map.arity == 2 && j->gen_map.map.src[0] == GDL_COL_OUTER1 &&
+               j->gen_map.map.src[1] == GDL_COL_INNER1 && !j->disable_load;
+    };
+    auto binary = [](const Relation *r) { return r->arity == 2 && r->index_column_size == 1; };
+    if (swap_map(j1) && swap_map(j2) && j1->output_rel == tmp && j1->outer_rel == out &&
+        j1->outer_ver == DELTA && j1->inner_ver == FULL && is_static(j1->inner_rel) &&
+        j2->outer_rel == tmp && j2->outer_ver == NEWT && j2->inner_ver == FULL &&
+        is_static(j2->inner_rel) && j2->output_rel == out && binary(out) && binary(tmp) &&
+        binary(j1->inner_rel) && binary(j2->inner_rel)) {
+        c.j1 = j1;
+        c.j2 = j2;
+    }
+    return c;
+}
+
+// Runs up to max_bodies bodies of the chain on bit matrices while its delta
+// stays dense (gdl_fixpoint_dense); FULL's log, the set and the delta are
+// left as the per-iteration bodies would leave them.  Returns false when no
+// body ran.
+bool run_dense_chain(const Chain &c, Relation *rel, int max_bodies,
+                     std::vector<uint64_t> *new_tuples, std::vector<uint64_t> *raw_tuples) {
+    GHashRelContainer *delta = rel->delta, *full = rel->full;
+    GHashRelContainer *r1 = c.j1->inner_rel->full, *r2 = c.j2->inner_rel->full;
+    new_tuples->clear();
+    raw_tuples->clear();
+    if (max_bodies <= 0 || !delta->keys || delta->tuple_counts == 0 || !r1->keys ||
+        !r1->tuple_counts || !r2->keys || !r2->tuple_counts)
+        return false;
+    const gdl_dense_rule rule{GDL_DENSE_SG, r1->keys, r1->tuple_counts, r2->keys,
+                              r2->tuple_counts};
+    uint64_t *log = full->keys;
+    uint64_t cap = full->capacity, end = rel->current_full_size;
+    const uint64_t *d = delta->keys;
+    uint64_t nd = delta->tuple_counts;
+    constexpr int kSlice = 4096;
+    std::vector<uint64_t> nb(kSlice), rb(kSlice);
+    int left = max_bodies;
+    gdl_status st = GDL_OK;
+    while (left > 0) {
+        int bodies = 0;
+        st = gdl_fixpoint_dense(context(), &rule, &rel->full_set, d, nd, &log, &cap, &end,
+                                std::min(left, kSlice), nb.data(), rb.data(), &bodies);
+        if (st != GDL_OK || bodies == 0) break;
+        new_tuples->insert(new_tuples->end(), nb.begin(), nb.begin() + bodies);
+        raw_tuples->insert(raw_tuples->end(), rb.begin(), rb.begin() + bodies);
+        left -= bodies;
+        nd = nb[bodies - 1];
+        d = log + (end - nd);
+        if (nd == 0 || bodies < std::min(left + bodies, kSlice)) break;  // fixpoint or sparse
+    }
+    if (!new_tuples->empty() || log != full->keys) {
+        full->keys = log;
+        full->capacity = cap;
+        full->tuple_counts = end;
+        full->data_raw_row_size = end;
+        full->sorted = false;
+        rel->current_full_size = end;
+    }
+    check(st, "gdl_fixpoint_dense");
+    if (new_tuples->empty()) return false;
+    container_free(delta);
+    const uint64_t last = new_tuples->back();
+    if (last) {
+        uint64_t *dk = device_alloc_keys(last);
+        check(gdl_memcpy_d2d(context(), dk, log + (end - last), last * sizeof(uint64_t)),
+              "dense delta");
+        container_take(delta, dk, last, last, false);
+    }
+    return true;
+}
+
 }  // namespace
 
 namespace gdlog {
@@ -467,7 +576,44 @@ void LIE::fixpoint_loop() {
                 static_relations.end())
             linear = j;
     }
+    // A two-join LIE of SG's shape runs its bodies on bit matrices while its
+    // delta is dense.
+    const Chain chain = linear ? Chain{} : sg_chain(ra_ops, update_relations, tmp_relations,
+                                                    static_relations);
     while (!finished) {
+        if (chain.j1) {
+            Relation *rel = update_relations[0];
+            long long left = (long long)max_iteration + 1 - iteration_counter;
+            if (!ck_file.empty()) {  // land on the next snapshot
+                if (ck_every > 0) left = std::min<long long>(left, ck_every - iteration_counter % ck_every);
+                if (ck_stop > iteration_counter)
+                    left = std::min<long long>(left, ck_stop - iteration_counter);
+            }
+            std::vector<uint64_t> nn, rr;
+            if (left > 0 &&
+                run_dense_chain(chain, rel, (int)std::min<long long>(left, 1 << 30), &nn, &rr)) {
+                bool fixpoint = false;
+                tuple_size_t full = rel->current_full_size;
+                for (uint64_t v : nn) full -= v;
+                for (size_t b = 0; b < nn.size(); ++b) {
+                    full += nn[b];
+                    trace.push_back({iteration_counter, 0, rr[b], nn[b], full});
+                    if (verbose())
+                        std::cout << "iteration " << iteration_counter << " relation "
+                                  << rel->name << " finish dedup new tuples : " << nn[b]
+                                  << " delta tuple size: " << nn[b] << " full counts " << full
+                                  << std::endl;
+                    iteration_counter++;
+                    if (nn[b] == 0) {
+                        fixpoint = true;
+                        break;
+                    }
+                }
+                boundary(fixpoint);
+                if (fixpoint || iteration_counter > max_iteration) break;
+                continue;
+            }
+        }
         if (linear && iteration_counter > 0) {
             Relation *rel = update_relations[0];
             long long left = (long long)max_iteration + 1 - iteration_counter;

@@ -248,6 +248,13 @@ gdl_status gdl_hashset_reserve(gdl_ctx *ctx, gdl_hashset *hs, uint64_t incoming)
  * Replaces flatten_tuples_raw_data of a FULL version (relation.cu:375-388). */
 gdl_status gdl_hashset_extract(gdl_ctx *ctx, const gdl_hashset *hs, uint64_t *d_out,
                                uint64_t out_cap, uint64_t *n_out);
+/* The members in key order (same contract otherwise).  A bitmap set is read
+ * in bit order, which is key order (two streaming passes: no sort); a hash
+ * table's members are extracted and radix-sorted.  Gives the sorted FULL
+ * that Relation::flush_delta keeps (relation.cu:484-487) when the fixpoint
+ * ends. */
+gdl_status gdl_hashset_extract_sorted(gdl_ctx *ctx, const gdl_hashset *hs, uint64_t *d_out,
+                                      uint64_t out_cap, uint64_t *n_out);
 
 /* ---- column maps and predicates (the build's form of the reference hooks,
  *      tuple_generator_hook / tuple_copy_hook / tuple_predicate,
@@ -330,6 +337,48 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
                                gdl_hashset *hs, uint64_t **d_log, uint64_t *log_cap,
                                uint64_t *delta_begin, uint64_t *delta_end, int max_bodies,
                                uint64_t *h_new, uint64_t *h_raw, int *bodies);
+
+/* Bodies of a recursive rule run on bit matrices.  When FULL is a bitmap set
+ * (key_bits 6..17) the delta can be held as bit rows too (row c0, bit c1),
+ * and a body builds each output row from whole delta rows: coalesced row
+ * streams instead of a random probe per produced tuple.  Two shapes, the
+ * column maps of the reference's TC and SG encodings:
+ *   GDL_DENSE_TC  out(x, z) :- r1(y, x), delta(y, z)         (test/tc.cu:80-149)
+ *   GDL_DENSE_SG  out(x, y) :- r1(a, x), delta(a, b), r2(b, y) (test/sg.cu:95-147;
+ *                 key_bits <= 16), the two joins through a deduplicated tmp.
+ * r1 and r2 are the static relations' keys ((c0 << 32) | c1, any order).
+ * gdl_fixpoint_linear uses the TC shape by itself when its map is TC's. */
+enum { GDL_DENSE_TC = 1, GDL_DENSE_SG = 2 };
+typedef struct gdl_dense_rule {
+    int shape;
+    const uint64_t *r1;
+    uint64_t n1;
+    const uint64_t *r2; /* GDL_DENSE_SG only */
+    uint64_t n2;
+} gdl_dense_rule;
+/* Runs bodies of `rule` from the delta d_delta[0, n_delta) (keys that are
+ * members of full, duplicates allowed): each body's new tuples (the set
+ * difference against full) are added to full and appended to the growable
+ * log *d_log (capacity *log_cap keys, *log_end in use; reallocated from the
+ * context allocator when needed), and become the next delta.  h_new / h_raw
+ * (max_bodies entries) get each body's new and produced ("join result size
+ * (non dedup)") tuple counts; *bodies the bodies run.  The context's dense
+ * mode (gdl_set_dense_mode) decides whether bodies run at all: in mode 1
+ * only while the delta is dense enough for row streams to beat tuple probes
+ * (a body with fewer new tuples than that ends the run; it is counted), in
+ * mode 2 until the fixpoint or max_bodies.  *bodies = 0 when the rule does
+ * not apply: mode 0, a hash-table FULL, key_bits outside the range above, a
+ * static relation with ids outside the domain, or (mode 1) a sparse delta.
+ * Equivalent to *bodies iterations of LIE::fixpoint_loop (lie.cu:26-274)
+ * over the rule's RA operators. */
+gdl_status gdl_fixpoint_dense(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *full,
+                              const uint64_t *d_delta, uint64_t n_delta, uint64_t **d_log,
+                              uint64_t *log_cap, uint64_t *log_end, int max_bodies,
+                              uint64_t *h_new, uint64_t *h_raw, int *bodies);
+/* 0: never run dense bodies; 1 (the default, or the GDLOG_DENSE environment
+ * variable at context creation): when a cost model prefers them; 2: whenever
+ * they apply.  Results are the same in every mode. */
+gdl_status gdl_set_dense_mode(gdl_ctx *ctx, int mode);
 
 /* Batched form of gdl_join_insert: many fixpoint instances share the inner
  * and outer arrays, told apart by the top bits of their keys, and each has
