@@ -303,7 +303,9 @@ def batched(gdl, ctx, n_inst, min_edges=1000, max_edges=50000, window=1000):
                    "p99_ms": round(1e3 * pct(lat_s, 99), 2),
                    "derived_tuples_per_s": round(derived / serial_s, 1)},
         "speedup_vs_serial": round(serial_s / batch_s, 3),
-        "bodies": info["bodies"], "mismatches_vs_serial": len(mism),
+        "bodies": info["bodies"],
+        "solo_instances": sum(1 for r in done.values() if r["slot"] == -1),
+        "mismatches_vs_serial": len(mism),
         "mismatches_vs_oracle": vs_oracle,
         "tuning": dict(tuned["tuning"], calibration_s=tuned["seconds"],
                        default_s=tuned["default_seconds"], trials=tuned["trials"],

@@ -125,7 +125,7 @@ class Tuning(Structure):
 
 class TcbConfig(Structure):
     _fields_ = [("max_slots", c_uint32), ("node_bits", c_uint32), ("mem_budget", c_uint64),
-                ("keep_keys", c_int)]
+                ("keep_keys", c_int), ("solo_tuples", c_uint64)]
 
 
 class TcbResult(Structure):
@@ -1039,12 +1039,17 @@ def tc_estimate(edges, n_nodes, samples=32, seed=0):
 class TcBatch:
     """Many TC fixpoints in one disjoint union (include/gdl_batch.h)."""
 
-    def __init__(self, ctx, max_slots=256, node_bits=16, mem_budget=0, keep_keys=True):
+    def __init__(self, ctx, max_slots=256, node_bits=16, mem_budget=0, keep_keys=True,
+                 solo_tuples=None):
+        """solo_tuples: instances estimated at this many tuples or more run alone at
+        admission (None: the library default; 0: all; ~0: none)."""
         self.ctx = ctx
         cfg = TcbConfig()
         ctx.eng.gdl_tcb_config_default(ctypes.byref(cfg))
         cfg.max_slots, cfg.node_bits, cfg.mem_budget = max_slots, node_bits, mem_budget
         cfg.keep_keys = int(keep_keys)
+        if solo_tuples is not None:
+            cfg.solo_tuples = int(solo_tuples) & 0xffffffffffffffff
         h = c_void_p()
         ctx._check(ctx.eng.gdl_tcb_create(ctx.h, ctypes.byref(cfg), ctypes.byref(h)),
                    "tcb_create")

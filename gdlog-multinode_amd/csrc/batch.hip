@@ -183,7 +183,7 @@ struct gdl_tcb {
     bool segs_dirty = true;
     std::deque<TcbDone> done;
     u64 reserved = 0, budget = 0;
-    u64 bodies = 0, admitted = 0, finished = 0;
+    u64 bodies = 0, admitted = 0, finished = 0, solo = 0;
     double t0 = 0;
 };
 
@@ -316,6 +316,132 @@ static gdl_status retire(gdl_tcb *b, TcbInst *in) {
     return GDL_OK;
 }
 
+static gdl_status hip_ok(gdl_ctx *ctx, hipError_t e) {
+    return e == hipSuccess ? GDL_OK : hip_fail(ctx, e, "batch solo");
+}
+
+// The solo lane: an instance whose closure is estimated at cfg.solo_tuples
+// or more fills the device on its own, so sharing bodies with others gains
+// nothing.  It runs to its fixpoint at admission, on the engine path of a
+// single query: FULL = its distinct rows (a bitmap set when its ids fit 17
+// bits), the first body from the first |FULL| raw rows, then the stream-
+// ordered linear loop (gdl_fixpoint_linear, with bit-matrix bodies while the
+// delta is dense).  Its result and trace are those of a batched instance.
+static gdl_status tcb_solo(gdl_tcb *b, u64 qid, const u64 *h_edges, u64 n, u32 n_nodes,
+                           double admit_ms) {
+    gdl_ctx *ctx = b->ctx;
+    const double t_admit = admit_ms;
+    int kb = 1;
+    while (kb < 32 && (1ull << kb) < (u64)n_nodes) ++kb;
+    gdl_hashset hs{};
+    Scratch scr(ctx);
+    u64 *d_raw, *d_rev, *inner;
+    GDL_TRY(scr.get(n, &d_raw));
+    GDL_TRY(scr.get(n, &d_rev));
+    GDL_TRY(scr.get(n, &inner));
+    GDL_HIP(ctx, hipMemcpyAsync(d_raw, h_edges, n * sizeof(u64), hipMemcpyHostToDevice,
+                                ctx->stream));
+    if (kb <= 17) GDL_TRY(gdl_hashset_create_bitmap(ctx, (u32)kb, &hs));
+    else GDL_TRY(gdl_hashset_create(ctx, n + 1024, &hs));
+    u64 *log = nullptr, cap = 0;
+    std::vector<gdl_trace_rec> trace;
+    gdl_index idx{};
+    gdl_status st = GDL_OK;
+    do {
+        // FULL: the distinct rows, also the log's start
+        cap = std::max<u64>(2 * n, 4096);
+        st = dev_alloc(ctx, cap, &log);
+        if (st != GDL_OK) break;
+        u64 n_full = 0;
+        st = hashset_insert_new(ctx, &hs, d_raw, n, log, &n_full);
+        if (st != GDL_OK) break;
+        hs.size = n_full;
+        // the inner: reversed edges (b, a), sorted and unique, indexed on b
+        st = gdl_project_u64(ctx, d_raw, n, gdl_colmap{2, {GDL_COL_INNER1, GDL_COL_INNER0}},
+                             d_rev);
+        if (st != GDL_OK) break;
+        u64 n_inner = 0;
+        st = sort_unique_keys(ctx, d_rev, n, kb, kb, inner, &n_inner);
+        if (st != GDL_OK) break;
+        st = index_build(ctx, inner, n_inner, 1, &idx);
+        if (st != GDL_OK) break;
+        const gdl_colmap map{2, {GDL_COL_INNER1, GDL_COL_OUTER1}};
+        const gdl_pred none{GDL_PRED_NONE, 0, 0};
+        // body 0: the first |FULL| raw rows as the delta
+        u64 *d_new = nullptr, new_cap = 0, nn = 0, raw = 0;
+        st = gdl_join_insert(ctx, inner, n_inner, &idx, d_raw, n_full, map, none, &hs, &d_new,
+                             &new_cap, &nn, &raw);
+        if (st == GDL_OK) {
+            hs.size = n_full + nn;
+            trace.push_back(gdl_trace_rec{0, 0, nn, nn, hs.size});
+            st = ensure_cap(ctx, &log, &cap, n_full + nn, n_full);
+            if (st == GDL_OK && nn)
+                st = hip_ok(ctx, hipMemcpyAsync(log + n_full, d_new, nn * sizeof(u64),
+                                                hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        if (d_new) ctx->alloc->free(d_new, nullptr);
+        if (st != GDL_OK) break;
+        // the later bodies: the linear loop over the log's last segment
+        u64 d0 = n_full, d1 = n_full + nn;
+        if (nn) {
+            constexpr int kSlice = 4096;
+            std::vector<u64> hn(kSlice), hr(kSlice);
+            for (;;) {
+                int bodies = 0;
+                st = gdl_fixpoint_linear(ctx, inner, n_inner, &idx, map, none, &hs, &log, &cap,
+                                         &d0, &d1, kSlice, hn.data(), hr.data(), &bodies);
+                if (st != GDL_OK) break;
+                u64 full = trace.back().full;
+                for (int k = 0; k < bodies; ++k) {
+                    full += hn[k];
+                    trace.push_back(gdl_trace_rec{(int32_t)trace.size(), 0, hn[k], hn[k], full});
+                }
+                if (bodies == 0 || hn[bodies - 1] == 0) break;
+            }
+            if (st != GDL_OK) break;
+        }
+        TcbDone d;
+        d.r.qid = qid;
+        d.r.n_out = d1;
+        d.r.iterations = (int32_t)trace.size();
+        d.r.slot = -1;
+        d.r.n_trace = trace.size();
+        d.r.admit_ms = t_admit;
+        for (const auto &t : trace) d.r.derived += t.delta;
+        st = gdl_checksum_u64(ctx, log, d1, &d.r.checksum);
+        if (st != GDL_OK) break;
+        if (b->cfg.keep_keys && d1) {
+            u64 *sorted;
+            st = scr.get(d1, &sorted);
+            if (st != GDL_OK) break;
+            u64 got = 0;
+            st = gdl_hashset_extract_sorted(ctx, &hs, sorted, d1, &got);
+            if (st == GDL_OK && got != d1)
+                st = fail(ctx, GDL_ERR_DEVICE, "batch solo: set holds " + std::to_string(got) +
+                                                   " tuples, log " + std::to_string(d1));
+            if (st != GDL_OK) break;
+            d.keys.resize(d1);
+            st = hip_ok(ctx, hipMemcpyAsync(d.keys.data(), sorted, d1 * sizeof(u64),
+                                            hipMemcpyDeviceToHost, ctx->stream));
+            if (st == GDL_OK) st = hip_ok(ctx, hipStreamSynchronize(ctx->stream));
+            if (st != GDL_OK) break;
+            d.r.has_keys = 1;
+        }
+        st = gdl_hashset_check(ctx, &hs);
+        if (st != GDL_OK) break;
+        d.trace = std::move(trace);
+        d.r.done_ms = wall_ms() - b->t0;
+        b->done.push_back(std::move(d));
+        ++b->admitted;
+        ++b->finished;
+        ++b->solo;
+    } while (false);
+    if (idx.slots) gdl_index_free(ctx, &idx);
+    if (log) ctx->alloc->free(log, nullptr);
+    gdl_hashset_free(ctx, &hs);
+    return st;
+}
+
 static gdl_status tcb_admit(gdl_tcb *b, u64 qid, const u64 *h_edges, u64 n, u32 n_nodes,
                             u64 est) {
     gdl_ctx *ctx = b->ctx;
@@ -324,10 +450,12 @@ static gdl_status tcb_admit(gdl_tcb *b, u64 qid, const u64 *h_edges, u64 n, u32 
     for (u64 i = 0; i < n; ++i)
         if ((h_edges[i] >> 32) >= n_nodes || (h_edges[i] & 0xffffffffull) >= n_nodes)
             return fail(ctx, GDL_ERR_INVALID_ARG, "batch: edge id >= n_nodes");
+    if (est == 0) est = gdl_tc_estimate(h_edges, n, n_nodes, 32, qid);
+    if (n && est >= b->cfg.solo_tuples)
+        return tcb_solo(b, qid, h_edges, n, n_nodes, wall_ms() - b->t0);
     u32 slot = 0;
     while (slot < b->cfg.max_slots && b->slots[slot]) ++slot;
     if (slot == b->cfg.max_slots) return GDL_ERR_BUSY;
-    if (est == 0) est = gdl_tc_estimate(h_edges, n, n_nodes, 32, qid);
     const u64 need = footprint_bytes(b, est, n);
     if (b->n_active && b->reserved + need > b->budget) return GDL_ERR_BUSY;
 
@@ -514,6 +642,7 @@ void gdl_tcb_config_default(gdl_tcb_config *cfg) {
     cfg->node_bits = 16;
     cfg->mem_budget = 0;
     cfg->keep_keys = 1;
+    cfg->solo_tuples = 1ull << 23;
 }
 
 uint64_t gdl_tc_estimate(const uint64_t *h_edges, uint64_t n, uint32_t n_nodes, uint32_t samples,

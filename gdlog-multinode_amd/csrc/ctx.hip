@@ -15,7 +15,8 @@ const char *kKernelNames[K_NUM] = {
     "merge_count",  "merge_write",  "idx_count",    "idx_build",     "idx_lookup",
     "join_count",   "join_materialize", "project",  "fill",          "hs_insert_new",
     "hs_rehash",    "join_insert",  "hs_extract",   "seg_hist",      "mask_cols",
-    "linear_body",  "segment_labels", "scatter_by_key", "dense_setup", "dense_body",
+    "linear_body",  "segment_labels", "scatter_by_key", "dense_setup", "dense_expand",
+    "dense_combine", "dense_append",
     "bm_extract_sorted"};
 
 // ---------------------------------------------------------------- allocator

@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "gdl_internal.hpp"
 
@@ -36,14 +37,80 @@ namespace gdl {
 constexpr int DB = 256;  // threads per block: 4 waves
 constexpr int DWV = DB / 64;
 
+__device__ __forceinline__ u64 wave_sum(u64 v) {
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// Position of the j-th set bit (0-based) of w (j < popcount(w)).
+__device__ __forceinline__ int select_bit(u64 w, int j) {
+    int pos = 0;
+    u32 x = (u32)w;
+    int c = __popc(x);
+    if (j >= c) {
+        j -= c;
+        pos = 32;
+        x = (u32)(w >> 32);
+    }
+    c = __popc(x & 0xffffu);
+    if (j >= c) { j -= c; pos += 16; x >>= 16; }
+    c = __popc(x & 0xffu);
+    if (j >= c) { j -= c; pos += 8; x >>= 8; }
+    c = __popc(x & 0xfu);
+    if (j >= c) { j -= c; pos += 4; x >>= 4; }
+    c = __popc(x & 0x3u);
+    if (j >= c) { j -= c; pos += 2; x >>= 2; }
+    if (j >= (int)(x & 1u)) pos += 1;
+    return pos;
+}
+
+// Writes the keys of one word per lane (lane l: word wd, bit i -> key of
+// bit number bitbase + i, (row << 32) | column with kb-bit columns) to
+// out[base ..), in lane then bit order, with every store coalesced: lane t
+// writes positions t, t + 64, ... and finds the word and bit it owns by a
+// binary search over the lanes' exclusive counts.  Returns the word keys'
+// total (the same in every lane).
+__device__ __forceinline__ u64 coop_write(u64 wd, u64 bitbase, int kb, u64 *out, u64 base,
+                                          u64 out_cap) {
+    const int lane = threadIdx.x & 63;
+    const u64 c = (u64)__popcll(wd);
+    const u64 incl = wave_inclusive_scan(c);
+    const u64 ex = incl - c;
+    const u64 total = __shfl(incl, 63, 64);
+    const u64 lo_mask = (1ull << kb) - 1;
+    for (u64 t0 = 0; t0 < total; t0 += 64) {
+        const u64 t = t0 + lane;
+        int lo = 0, hi = 63;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            const u64 em = __shfl(ex, mid, 64);
+            if (em <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const u64 word = __shfl(wd, lo, 64);
+        const u64 bb = __shfl(bitbase, lo, 64);
+        const u64 e0 = __shfl(ex, lo, 64);
+        if (t < total) {
+            const u64 bit = bb + (u64)select_bit(word, (int)(t - e0));
+            if (base + t < out_cap) out[base + t] = ((bit >> kb) << 32) | (bit & lo_mask);
+        }
+    }
+    return total;
+}
+
 // ---------------------------------------------------------------------------
-// Key-ordered extraction.  Tiles of DB * 16 words: pass 1 counts each tile's
-// bits, an exclusive scan of the counts gives each tile's first output
-// position, pass 2 writes each tile's keys in bit order, 256 consecutive words
-// a round with a block-wide scan of their bit counts.  With `minus`, rows whose
-// minus_pop entry is non-zero have minus's bits removed (FULL minus a delta).
-constexpr int XT_ROUNDS = 16;
-constexpr u64 XT_WORDS = (u64)DB * XT_ROUNDS;
+// Key-ordered extraction.  Tiles of up to 1024 words, one per wave (rounds
+// of 64 consecutive words): pass 1 counts each tile's bits, an exclusive scan
+// of the counts gives each tile's first output position, pass 2 writes each
+// tile's keys in bit order with a wave prefix sum per round (consecutive
+// words, consecutive output ranges; no block barrier).  With `minus`, rows
+// whose minus_pop entry is non-zero have minus's bits removed.
+// rounds per tile: 16, fewer when that would leave fewer than 16384 tiles
+static int xt_rounds(u64 n_words) {
+    int r = 16;
+    while (r > 1 && n_words / (64ull * r) < 16384) r >>= 1;
+    return r;
+}
 
 __device__ __forceinline__ u64 xt_word(const u64 *words, const u64 *minus, const u32 *minus_pop,
                                        int rs, u64 i) {
@@ -52,42 +119,36 @@ __device__ __forceinline__ u64 xt_word(const u64 *words, const u64 *minus, const
     return w;
 }
 
-__global__ __launch_bounds__(DB) void k_xt_count(const u64 *words, u64 n_words, const u64 *minus,
+__global__ __launch_bounds__(DB) void k_xt_count(const u64 *words, u64 n_words, u64 n_tiles,
+                                                 int rounds, const u64 *minus,
                                                  const u32 *minus_pop, int rs, u64 *tile_counts) {
-    __shared__ u64 s_w[DWV + 1];
-    const u64 t0 = (u64)blockIdx.x * XT_WORDS;
+    const u64 tile = (u64)blockIdx.x * DWV + (threadIdx.x >> 6);
+    if (tile >= n_tiles) return;  // the same for the whole wave
+    const int lane = threadIdx.x & 63;
+    const u64 t0 = tile * 64ull * rounds;
     u64 c = 0;
-#pragma unroll 4
-    for (int r = 0; r < XT_ROUNDS; ++r) {
-        const u64 i = t0 + (u64)r * DB + threadIdx.x;
+    for (int r = 0; r < rounds; ++r) {
+        const u64 i = t0 + (u64)r * 64 + lane;
         if (i < n_words) c += (u64)__popcll(xt_word(words, minus, minus_pop, rs, i));
     }
-    u64 tot;
-    block_exclusive_scan<DB>(c, s_w, &tot);
-    if (threadIdx.x == 0) tile_counts[blockIdx.x] = tot;
+    c = wave_sum(c);
+    if (lane == 0) tile_counts[tile] = c;
 }
 
-__global__ __launch_bounds__(DB) void k_xt_write(const u64 *words, u64 n_words, const u64 *minus,
+__global__ __launch_bounds__(DB) void k_xt_write(const u64 *words, u64 n_words, u64 n_tiles,
+                                                 int rounds, const u64 *minus,
                                                  const u32 *minus_pop, int rs, int kb,
-                                                 const u64 *tile_offsets, u64 *out, u64 out_cap) {
-    __shared__ u64 s_w[DWV + 1];
-    const u64 t0 = (u64)blockIdx.x * XT_WORDS;
-    const u64 lo_mask = (1ull << kb) - 1;
-    u64 base = tile_offsets[blockIdx.x];
-    for (int r = 0; r < XT_ROUNDS; ++r) {
-        if (t0 + (u64)r * DB >= n_words) break;  // the same for the whole block
-        const u64 i = t0 + (u64)r * DB + threadIdx.x;
-        u64 w = i < n_words ? xt_word(words, minus, minus_pop, rs, i) : 0ull;
-        u64 tot;
-        u64 at = base + block_exclusive_scan<DB>((u64)__popcll(w), s_w, &tot);
-        while (w) {
-            const int b = __ffsll((unsigned long long)w) - 1;
-            w &= w - 1;
-            const u64 bit = i * 64 + (u64)b;
-            if (at < out_cap) out[at] = ((bit >> kb) << 32) | (bit & lo_mask);
-            ++at;
-        }
-        base += tot;
+                                                 const u64 *tile_offsets, u64 *out,
+                                                 u64 out_cap) {
+    const u64 tile = (u64)blockIdx.x * DWV + (threadIdx.x >> 6);
+    if (tile >= n_tiles) return;  // the same for the whole wave
+    const int lane = threadIdx.x & 63;
+    const u64 t0 = tile * 64ull * rounds;
+    u64 base = tile_offsets[tile];
+    for (int r = 0; r < rounds; ++r) {
+        const u64 i = t0 + (u64)r * 64 + lane;
+        const u64 w = i < n_words ? xt_word(words, minus, minus_pop, rs, i) : 0ull;
+        base += coop_write(w, i * 64, kb, out, base, out_cap);
     }
 }
 
@@ -99,15 +160,17 @@ static gdl_status bitmap_extract_sorted(gdl_ctx *ctx, const u64 *words, int kb, 
                                         const u32 *minus_pop, u64 *d_out, u64 out_cap,
                                         u64 *n_out) {
     const u64 nw = bitmap_words(kb);
-    const u64 tiles = (nw + XT_WORDS - 1) / XT_WORDS;
+    const int rounds = xt_rounds(nw);
+    const u64 tiles = (nw + 64ull * rounds - 1) / (64ull * rounds);
+    const unsigned blocks = (unsigned)((tiles + DWV - 1) / DWV);
     const int rs = kb >= 6 ? kb - 6 : 0;
     Scratch scr(ctx);
     u64 *tc;
     GDL_TRY(scr.get(tiles + 1, &tc));
     {
         KTimer kt(ctx, K_BM_EXTRACT, 8.0 * nw);
-        hipLaunchKernelGGL(k_xt_count, dim3((unsigned)tiles), dim3(DB), 0, ctx->stream, words, nw,
-                           minus, minus_pop, rs, tc);
+        hipLaunchKernelGGL(k_xt_count, dim3(blocks), dim3(DB), 0, ctx->stream, words, nw, tiles,
+                           rounds, minus, minus_pop, rs, tc);
         GDL_TRY(check_launch(ctx, "k_xt_count"));
     }
     GDL_TRY(scan_inplace(ctx, tc, tiles));
@@ -119,8 +182,8 @@ static gdl_status bitmap_extract_sorted(gdl_ctx *ctx, const u64 *words, int kb, 
                                                " members, output holds " + std::to_string(out_cap));
     if (total == 0) return GDL_OK;
     KTimer kt(ctx, K_BM_EXTRACT, 8.0 * nw + 8.0 * total);
-    hipLaunchKernelGGL(k_xt_write, dim3((unsigned)tiles), dim3(DB), 0, ctx->stream, words, nw,
-                       minus, minus_pop, rs, kb, tc, d_out, out_cap);
+    hipLaunchKernelGGL(k_xt_write, dim3(blocks), dim3(DB), 0, ctx->stream, words, nw, tiles,
+                       rounds, minus, minus_pop, rs, kb, tc, d_out, out_cap);
     return check_launch(ctx, "k_xt_write");
 }
 
@@ -129,99 +192,101 @@ static gdl_status bitmap_extract_sorted(gdl_ctx *ctx, const u64 *words, int kb, 
 
 // Device state of a run of dense bodies.  The counters come first.
 struct DenseState {
-    unsigned long long cursor;    // log end
-    unsigned long long n_new;     // the running body's new tuples
-    unsigned long long raw;       // its produced tuples
-    unsigned long long next_raw;  // TC: the next body's produced tuples (accumulating)
-    unsigned long long rows_io;   // row reads + writes of the running body
+    unsigned long long cursor;   // log end
+    unsigned long long n_new;    // the running body's new tuples
+    unsigned long long raw;      // its produced tuples
+    unsigned long long rows_io;   // the combine's row reads + writes of the running body
+    unsigned long long exp_io;    // the expansion's row reads + writes
+    unsigned long long rows_new;  // rows that got new tuples
     u64 log_cap;
-    u64 dom_left;      // free bits of the active rows (a bound on new tuples)
     u64 sparse_below;  // a body with fewer new tuples ends the run
-    u64 next_bound;    // TC: a bound on the next body's new tuples
-    u64 append_base;   // SG: the log end before the last body's append
+    u64 append_base;   // the log end before the last body's append
     u32 stop, body, blocks_done, max_bodies;
-    u32 appended;       // SG: bodies whose new tuples are in the log
-    u32 append_blocks;  // SG: blocks of the running append that finished
+    u32 appended;       // bodies whose new tuples are in the log
+    u32 append_blocks;  // blocks of the running append that finished
     u32 bad;            // a delta key outside the domain
     u32 pad_;
 };
-enum : u32 {
-    kDnRun = 0,
-    kDnFixpoint = 1,
-    kDnGrow = 2,
-    kDnLimit = 3,
-    kDnSparse = 4,
-    kDnOverflow = 5
-};
+enum : u32 { kDnRun = 0, kDnFixpoint = 1, kDnGrow = 2, kDnLimit = 3, kDnSparse = 4 };
+// per body: new tuples, produced tuples, combine rows, expansion rows, rows
+// with new tuples
+constexpr int kCounts = 5;
 
 // The rule's static relations as row lists.  pred: r1 grouped by its second
 // column (for row x, the y with r1(y, x)); act: the rows with a non-empty
-// pred list (the only rows a body can write); deg1: r1 tuples per first
-// column (TC's produced count of a delta row); succ: r2 grouped by its first
-// column (SG).
+// pred list (the only rows a body can write); items: the combine's work,
+// (x, first pred, end pred, split index or ~0), rows with more than kSlice
+// preds split into slices (most preds first); per split row the slice count,
+// a finished-slice counter and accumulators (NEW, and SG's T).  SG: exp: the
+// rows that are some row's pred (the rows U is built for); succ: r2 grouped
+// by its first column, and succ_tab its per-row entries; planes: bit-planes
+// of r2's row lengths (bit b of plane j = bit j of |r2 row b|).
+constexpr u32 kSlice = 32;
+
 struct DenseArgs {
     const u64 *pred_off;
     const u32 *pred_idx;
     const u32 *act;
     u64 n_act;
-    const u32 *deg1;
+    const uint4 *items;
+    u64 n_items;
+    const u32 *n_slices;
+    u32 *slice_done;
+    u64 *GU, *GT;
+    const u32 *exp_rows;
+    u64 n_exp;
     const u64 *succ_off;
     const u32 *succ_idx;
+    const uint4 *succ_tab;  // {deg, successors} (deg <= 3) or {deg, offset lo, hi, bit row}
+    const u64 *heavy;       // bit rows of the lists longer than kHeavy
+    const u64 *planes;
+    int n_planes;
     u64 rw;
     u64 *full;  // FULL's bitmap words
     u64 *log;
 };
 
-__device__ __forceinline__ u64 wave_sum(u64 v) {
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
-
 // End of a body (thread 0 of the last block): record its counts and decide
 // whether the next enqueued body runs.
-__device__ void dense_epilogue(DenseState *st, u64 *counts, bool tc) {
+__device__ void dense_epilogue(DenseState *st, u64 *counts) {
     const u64 nn = atomicAdd(&st->n_new, 0ull);
     const u64 raw = atomicAdd(&st->raw, 0ull);
-    const u64 nr = atomicAdd(&st->next_raw, 0ull);
     const u64 io = atomicAdd(&st->rows_io, 0ull);
-    const u64 end = atomicAdd(&st->cursor, 0ull);
+    const u64 eio = atomicAdd(&st->exp_io, 0ull);
+    const u64 rn = atomicAdd(&st->rows_new, 0ull);
     const u32 b = st->body;
-    counts[3 * b] = nn;
-    counts[3 * b + 1] = raw;
-    counts[3 * b + 2] = io;
+    u64 *c = counts + kCounts * b;
+    c[0] = nn;
+    c[1] = raw;
+    c[2] = io;
+    c[3] = eio;
+    c[4] = rn;
     st->n_new = 0;
     st->raw = 0;
-    st->next_raw = 0;
     st->rows_io = 0;
+    st->exp_io = 0;
+    st->rows_new = 0;
     st->blocks_done = 0;
-    st->append_base = end;
-    const u64 left = st->dom_left > nn ? st->dom_left - nn : 0;
-    st->dom_left = left;
-    // the next body's new tuples: at most its produced count (TC) and the
-    // active rows' free bits
-    const u64 bound = tc ? (nr < left ? nr : left) : left;
-    st->next_bound = bound;
+    st->append_base = st->cursor;
     u32 stop = kDnRun;
-    if (tc && end > st->log_cap) stop = kDnOverflow;
-    else if (nn == 0) stop = kDnFixpoint;
+    if (nn == 0) stop = kDnFixpoint;
     else if (b + 1 >= st->max_bodies) stop = kDnLimit;
     else if (nn < st->sparse_below) stop = kDnSparse;
-    else if (tc && end + bound > st->log_cap) stop = kDnGrow;
     st->stop = stop;
-    st->body = b + 1;  // last: an append launched next reads it
+    st->body = b + 1;  // last: the append launched next reads it
 }
 
 // Block totals -> state; the last block runs the epilogue.
-__device__ __forceinline__ void dense_block_end(DenseState *st, u64 *counts, bool tc, u64 nn,
-                                                u64 raw, u64 nr, u64 io) {
+__device__ __forceinline__ void dense_block_end(DenseState *st, u64 *counts, u64 nn, u64 raw,
+                                                u64 io, u64 rn) {
     __shared__ u64 s_v[DWV][4];
     __shared__ bool s_last;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (lane == 0) {
         s_v[wv][0] = nn;
         s_v[wv][1] = raw;
-        s_v[wv][2] = nr;
-        s_v[wv][3] = io;
+        s_v[wv][2] = io;
+        s_v[wv][3] = rn;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -230,56 +295,49 @@ __device__ __forceinline__ void dense_block_end(DenseState *st, u64 *counts, boo
             for (int j = 0; j < 4; ++j) t[j] += s_v[v][j];
         if (t[0]) atomicAdd(&st->n_new, (unsigned long long)t[0]);
         if (t[1]) atomicAdd(&st->raw, (unsigned long long)t[1]);
-        if (t[2]) atomicAdd(&st->next_raw, (unsigned long long)t[2]);
-        if (t[3]) atomicAdd(&st->rows_io, (unsigned long long)t[3]);
+        if (t[2]) atomicAdd(&st->rows_io, (unsigned long long)t[2]);
+        if (t[3]) atomicAdd(&st->rows_new, (unsigned long long)t[3]);
         __threadfence();  // release the counter updates before counting this block finished
         s_last = atomicAdd(&st->blocks_done, 1u) == gridDim.x - 1;
     }
     __syncthreads();
     if (s_last && threadIdx.x == 0) {
         __threadfence();  // acquire: every block's updates precede its blocks_done add
-        dense_epilogue(st, counts, tc);
+        dense_epilogue(st, counts);
     }
 }
 
-// acc[k] |= OR of D_in rows over x's pred list entries whose pop is non-zero;
-// returns (wave-uniform) whether any was; *praw gets the lane's share of the
-// preds' pop sum, *pflag the number of flagged preds.
+// ORs rows src[y] (rw words) into acc over the entries y of flag mask m of
+// the wave's 64 pred entries (lane l holds entry y); two rows in flight at a
+// time (m is the same in every lane).
 template <int K>
-__device__ __forceinline__ bool or_pred_rows(const DenseArgs &a, u32 x, const u32 *pop_in,
-                                             const u64 *D_in, u64 *acc, u64 *praw,
-                                             u64 *pflag) {
+__device__ __forceinline__ void or_rows(const u64 *src, u64 rw, u64 m, u32 y, u64 *acc) {
     const int lane = threadIdx.x & 63;
-    const u64 pb = a.pred_off[x], pe = a.pred_off[x + 1];
-    bool any = false;
-    for (u64 base = pb; base < pe; base += 64) {
-        const u64 j = base + lane;
-        u32 y = 0, p = 0;
-        if (j < pe) {
-            y = a.pred_idx[j];
-            p = pop_in[y];
-        }
-        *praw += p;
-        u64 m = __ballot(p != 0);
-        if (m) any = true;
-        *pflag += (u64)__popcll(m);
-        while (m) {
-            const int l = __ffsll((unsigned long long)m) - 1;
+    while (m) {
+        const int l1 = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        const bool two = m != 0;
+        int l2 = l1;
+        if (two) {
+            l2 = __ffsll((unsigned long long)m) - 1;
             m &= m - 1;
-            const u32 yy = (u32)__shfl((int)y, l, 64);
-            const u64 *row = D_in + (u64)yy * a.rw;
+        }
+        const u64 *r1 = src + (u64)(u32)__shfl((int)y, l1, 64) * rw;
+        const u64 *r2 = src + (u64)(u32)__shfl((int)y, l2, 64) * rw;
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const u64 w = (u64)lane + 64u * k;
-                if (K > 1 || w < a.rw) acc[k] |= row[w];
+        for (int k = 0; k < K; ++k) {
+            const u64 w = (u64)lane + 64u * k;
+            if (K > 1 || w < rw) {
+                const u64 v1 = r1[w];
+                const u64 v2 = two ? r2[w] : 0ull;
+                acc[k] |= v1 | v2;
             }
         }
     }
-    return any;
 }
 
-// nw[k] = src[k] & ~FULL row; FULL |= nw; D_out row = nw.  Returns the
-// lane's new-bit count.
+// nw[k] &= ~FULL row; FULL |= nw; D_out row = nw.  Returns the lane's
+// new-bit count.
 template <int K>
 __device__ __forceinline__ u64 diff_row(const DenseArgs &a, u32 x, u64 *nw, u64 *D_out) {
     const int lane = threadIdx.x & 63;
@@ -306,194 +364,368 @@ __device__ __forceinline__ u64 diff_row(const DenseArgs &a, u32 x, u64 *nw, u64 
     return cnt;
 }
 
-// Appends row x's new bits to the log (positions from one cursor atomic per
-// row).
+// SG's produced count of a tmp row T: the sum of r2's row lengths over T's
+// bits, as sum_j 2^j popcount(T & plane_j).  Returns the lane's share.
 template <int K>
-__device__ __forceinline__ void append_row(const DenseArgs &a, u32 x, const u64 *nw, u64 cnt,
-                                           u64 total, DenseState *st) {
+__device__ __forceinline__ u64 plane_dot(const DenseArgs &a, const u64 *t) {
     const int lane = threadIdx.x & 63;
-    unsigned long long pos = 0;
-    if (lane == 0) pos = atomicAdd(&st->cursor, (unsigned long long)total);
-    pos = __shfl(pos, 0, 64);
-    u64 at = pos + wave_inclusive_scan(cnt) - cnt;
-    const u64 cap = st->log_cap;
+    u64 sum = 0;
+    for (int j = 0; j < a.n_planes; ++j) {
+        const u64 *pl = a.planes + (u64)j * a.rw;
+        u64 c = 0;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        u64 v = nw[k];
-        const u64 w = (u64)lane + 64u * k;
-        while (v) {
-            const int b = __ffsll((unsigned long long)v) - 1;
-            v &= v - 1;
-            if (at < cap) a.log[at] = ((u64)x << 32) | (w * 64 + (u64)b);
-            ++at;
+        for (int k = 0; k < K; ++k) {
+            const u64 w = (u64)lane + 64u * k;
+            if (K > 1 || w < a.rw) c += (u64)__popcll(t[k] & pl[w]);
         }
+        sum += c << j;
     }
+    return sum;
 }
 
-// One TC body: a wave per active row, rows strided over the grid's waves.
-template <int K>
-__global__ __launch_bounds__(DB) void k_dense_tc(DenseArgs a, const u32 *pop_in, const u64 *D_in,
-                                                 u32 *pop_out, u64 *D_out, DenseState *st,
-                                                 u64 *counts) {
-    if (st->stop != kDnRun) return;  // the same for every block
-    const int lane = threadIdx.x & 63;
-    const u64 wave = (u64)blockIdx.x * DWV + (threadIdx.x >> 6);
-    const u64 n_waves = (u64)gridDim.x * DWV;
-    u64 w_new = 0, w_raw = 0, w_next = 0, w_io = 0;
-    for (u64 i = wave; i < a.n_act; i += n_waves) {
-        const u32 x = a.act[i];
-        u64 acc[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) acc[k] = 0;
-        u64 raw = 0, flagged = 0;
-        const bool any = or_pred_rows<K>(a, x, pop_in, D_in, acc, &raw, &flagged);
-        w_raw += raw;
-        u64 total = 0;
-        if (any) {
-            const u64 cnt = diff_row<K>(a, x, acc, D_out);
-            total = wave_sum(cnt);
-            if (total) append_row<K>(a, x, acc, cnt, total, st);
-            w_io += flagged + 2;
-        }
-        if (lane == 0) pop_out[x] = (u32)total;
-        w_new += total;
-        w_next += total * a.deg1[x];
-    }
-    dense_block_end(st, counts, true, w_new, wave_sum(w_raw), w_next, w_io);
+// SG: U[a] = the union of r2's successor lists over the bits b of D[a] (the
+// second join applied to one delta row), for every row a that is some row's
+// pred; upop[a] = |U[a]|.  A wave per row, U built in LDS (rw words per
+// wave).  Each lane expands its words' bits 4 at a time (4 independent entry
+// loads in flight): lists of at most kInline successors are in the entry;
+// longer ones are listed per wave and walked by the whole wave after the
+// round, 64 successors a step (a prefix sum over the listed lengths and a
+// binary search per lane); the lists of more than kHeavy successors are bit
+// rows (Eh), listed too and ORed in by the whole wave, two rows in flight.
+constexpr u32 kInline = 3;   // successor lists this short live in the entry itself
+constexpr u32 kHeavy = 64;   // longer lists are bit rows
+constexpr u32 kHeavyMax = 4096;  // bit rows kept at most
+constexpr u32 kHList = 256;  // heavy bits collected per wave and round
+constexpr u32 kMList = 128;  // walked lists collected per wave and round
+constexpr int kBatch = 4;
+constexpr u32 kNoBit = 0xffffffffu;
+
+__device__ __forceinline__ void lds_set(u64 *row, u32 y) {
+    atomicOr((unsigned long long *)&row[y >> 6], 1ull << (y & 63));
 }
 
-// One SG body: a wave per active row; T in registers, the output row in LDS
-// (dynamic, rw words per wave).  Successor lists longer than kLight go to a
-// per-wave queue that the whole wave then walks.
-constexpr u32 kLight = 16;
-constexpr u32 kQueue = 64;
-
 template <int K>
-__global__ __launch_bounds__(DB) void k_dense_sg(DenseArgs a, const u32 *pop_in, const u64 *D_in,
-                                                 u32 *pop_out, u64 *D_out, DenseState *st,
-                                                 u64 *counts) {
+__global__ __launch_bounds__(DB) void k_dense_expand(DenseArgs a, const u32 *pop_in,
+                                                     const u64 *D_in, u32 *upop, u64 *U,
+                                                     DenseState *st) {
     extern __shared__ u64 s_rows[];
-    __shared__ u32 s_q[DWV][kQueue];
-    __shared__ u32 s_qn[DWV];
+    __shared__ u32 s_h[DWV][kHList];
+    __shared__ u32 s_hn[DWV];
+    __shared__ u64 s_mo[DWV][kMList];      // listed walks: CSR offset
+    __shared__ u32 s_mx[DWV][kMList + 1];  // and the exclusive sums of their lengths
+    __shared__ u32 s_mn[DWV];
+    __shared__ u64 s_io[DWV];
     if (st->stop != kDnRun) return;  // the same for every block
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     u64 *s_new = s_rows + (u64)wv * a.rw;
     const u64 wave = (u64)blockIdx.x * DWV + wv;
     const u64 n_waves = (u64)gridDim.x * DWV;
-    u64 w_new = 0, w_raw = 0, w_io = 0;
-    for (u64 i = wave; i < a.n_act; i += n_waves) {
-        const u32 x = a.act[i];
-        u64 acc[K];
+    u64 io = 0;
+    for (u64 i = wave; i < a.n_exp; i += n_waves) {
+        const u32 r = a.exp_rows[i];
+        if (pop_in[r] == 0) {  // the same for the whole wave
+            if (lane == 0) upop[r] = 0;
+            continue;
+        }
+        u64 v[K], acc[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const u64 w = (u64)lane + 64u * k;
+            v[k] = (K > 1 || w < a.rw) ? D_in[(u64)r * a.rw + w] : 0ull;
+            if (K > 1 || w < a.rw) s_new[w] = 0;
+            acc[k] = 0;
+        }
+        wave_sync_lds();
+        // rounds: every lane expands its light bits; heavy bits are listed
+        // (up to kHList a round) and ORed in by the wave after the round
+        bool more = true;
+        while (more) {
+            if (lane == 0) {
+                s_hn[wv] = 0;
+                s_mn[wv] = 0;
+            }
+            wave_sync_lds();
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const u32 w64 = (u32)(((u64)lane + 64u * k) * 64);
+                while (v[k]) {
+                    u32 bb[kBatch];
+                    u64 rest = v[k];
+#pragma unroll
+                    for (int j = 0; j < kBatch; ++j) {
+                        bb[j] = kNoBit;
+                        if (rest) {
+                            bb[j] = w64 + (u32)(__ffsll((unsigned long long)rest) - 1);
+                            rest &= rest - 1;
+                        }
+                    }
+                    uint4 e[kBatch];
+#pragma unroll
+                    for (int j = 0; j < kBatch; ++j)
+                        e[j] = bb[j] != kNoBit ? a.succ_tab[bb[j]] : make_uint4(0, 0, 0, 0);
+                    bool full = false;
+#pragma unroll
+                    for (int j = 0; j < kBatch; ++j) {
+                        if (bb[j] == kNoBit || full) continue;
+                        const u32 deg = e[j].x;
+                        if (deg <= kInline) {
+                            if (deg > 0) lds_set(s_new, e[j].y);
+                            if (deg > 1) lds_set(s_new, e[j].z);
+                            if (deg > 2) lds_set(s_new, e[j].w);
+                        } else if (e[j].w != kNoBit) {  // a bit row: listed for the wave
+                            const u32 q = atomicAdd(&s_hn[wv], 1u);
+                            if (q >= kHList) {
+                                full = true;  // this bit and the rest wait for the next round
+                                continue;
+                            }
+                            s_h[wv][q] = e[j].w;
+                        } else {  // a walk: listed for the wave
+                            const u32 q = atomicAdd(&s_mn[wv], 1u);
+                            if (q >= kMList) {
+                                full = true;
+                                continue;
+                            }
+                            s_mo[wv][q] = (u64)e[j].y | ((u64)e[j].z << 32);
+                            s_mx[wv][q] = deg;
+                        }
+                        v[k] &= ~(1ull << (bb[j] - w64));  // done with this bit
+                    }
+                    if (full) break;
+                }
+            }
+            wave_sync_lds();
+            // the listed walks, flattened: 64 successors a step
+            const u32 nm = min(s_mn[wv], kMList);
+            if (nm) {
+                u32 c0 = lane < (int)nm ? s_mx[wv][lane] : 0u;
+                u32 c1 = lane + 64 < (int)nm ? s_mx[wv][lane + 64] : 0u;
+                const u64 i0 = wave_inclusive_scan(c0);
+                const u64 tot0 = __shfl(i0, 63, 64);
+                const u64 i1 = wave_inclusive_scan(c1) + tot0;
+                const u64 tot = __shfl(i1, 63, 64);
+                wave_sync_lds();
+                if (lane < (int)nm) s_mx[wv][lane] = (u32)(i0 - c0);
+                if (lane + 64 < (int)nm) s_mx[wv][lane + 64] = (u32)(i1 - c1);
+                if (lane == 0) s_mx[wv][nm] = (u32)tot;
+                wave_sync_lds();
+                for (u64 t0 = 0; t0 < tot; t0 += 128) {
+                    u32 y[2];
+#pragma unroll
+                    for (int z = 0; z < 2; ++z) {
+                        const u64 t = t0 + 64u * z + lane;
+                        y[z] = kNoBit;
+                        if (t < tot) {
+                            u32 lo = 0, hi = nm - 1;  // the last list starting at or before t
+                            while (lo < hi) {
+                                const u32 mid = (lo + hi + 1) >> 1;
+                                if (s_mx[wv][mid] <= t) lo = mid;
+                                else hi = mid - 1;
+                            }
+                            y[z] = a.succ_idx[s_mo[wv][lo] + (t - s_mx[wv][lo])];
+                        }
+                    }
+#pragma unroll
+                    for (int z = 0; z < 2; ++z)
+                        if (y[z] != kNoBit) lds_set(s_new, y[z]);
+                }
+            }
+            const u32 nh = min(s_hn[wv], kHList);
+            for (u32 h = 0; h < nh; h += 2) {
+                const u64 *r1 = a.heavy + (u64)s_h[wv][h] * a.rw;
+                const bool two = h + 1 < nh;
+                const u64 *r2 = a.heavy + (u64)s_h[wv][two ? h + 1 : h] * a.rw;
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const u64 w = (u64)lane + 64u * k;
+                    if (K > 1 || w < a.rw) acc[k] |= r1[w] | (two ? r2[w] : 0ull);
+                }
+            }
+            u64 left = 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) left |= v[k];
+            more = __ballot(left != 0) != 0;
+        }
+        wave_sync_lds();
+        u64 cnt = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const u64 w = (u64)lane + 64u * k;
+            if (K > 1 || w < a.rw) {
+                const u64 u = s_new[w] | acc[k];
+                U[(u64)r * a.rw + w] = u;
+                cnt += (u64)__popcll(u);
+            }
+        }
+        cnt = wave_sum(cnt);
+        if (lane == 0) upop[r] = (u32)cnt;
+        io += 2;
+        wave_sync_lds();  // s_new and the list are rewritten for the next row
+    }
+    if (lane == 0) s_io[wv] = io;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u64 t = 0;
+        for (int v2 = 0; v2 < DWV; ++v2) t += s_io[v2];
+        if (t) atomicAdd(&st->exp_io, (unsigned long long)t);
+    }
+}
+
+// One body's output rows, a wave per work item (row x, a slice of its pred
+// list), items strided over the grid's waves.  TC: NEW = OR of D[y] over
+// x's preds.  SG: T = OR of D[a] (the tmp row, for the produced count) and
+// NEW = OR of U[a] over x's preds.  A row with one item takes the set
+// difference at once; the slices of a split row OR into the row's
+// accumulators and the last slice to finish takes it.
+template <int K, bool SG>
+__global__ __launch_bounds__(DB) void k_dense_combine(DenseArgs a, const u32 *pop_in,
+                                                      const u64 *D_in, const u32 *upop,
+                                                      const u64 *U, u32 *pop_out, u64 *D_out,
+                                                      DenseState *st, u64 *counts) {
+    if (st->stop != kDnRun) return;  // the same for every block
+    const int lane = threadIdx.x & 63;
+    const u64 wave = (u64)blockIdx.x * DWV + (threadIdx.x >> 6);
+    const u64 n_waves = (u64)gridDim.x * DWV;
+    u64 w_new = 0, w_raw = 0, w_io = 0, w_rn = 0;
+    for (u64 i = wave; i < a.n_items; i += n_waves) {
+        const uint4 it = a.items[i];
+        const u32 x = it.x, g = it.w;
+        const u64 pb = a.pred_off[x] + it.y, pe = a.pred_off[x] + it.z;
+        u64 acc[K], acct[SG ? K : 1];
 #pragma unroll
         for (int k = 0; k < K; ++k) acc[k] = 0;
-        u64 unused = 0, flagged = 0;
-        const bool any = or_pred_rows<K>(a, x, pop_in, D_in, acc, &unused, &flagged);
-        u64 total = 0;
+#pragma unroll
+        for (int k = 0; k < (SG ? K : 1); ++k) acct[k] = 0;
+        u64 raw = 0, flagged = 0;
+        bool any = false;
+        for (u64 base = pb; base < pe; base += 64) {
+            const u64 j = base + lane;
+            u32 y = 0, p = 0, q = 0;
+            if (j < pe) {
+                y = a.pred_idx[j];
+                p = pop_in[y];
+                if (SG && p) q = upop[y];
+            }
+            if (!SG) raw += p;
+            const u64 m = __ballot(p != 0);
+            if (m) any = true;
+            flagged += (u64)__popcll(m);
+            if constexpr (SG) {
+                or_rows<K>(D_in, a.rw, m, y, acct);
+                const u64 mu = __ballot(q != 0);
+                flagged += (u64)__popcll(mu);
+                or_rows<K>(U, a.rw, mu, y, acc);
+            } else {
+                or_rows<K>(D_in, a.rw, m, y, acc);
+            }
+        }
+        if (g == 0xffffffffu) {  // the whole row
+            u64 total = 0;
+            if (any) {
+                if constexpr (SG) raw += plane_dot<K>(a, acct);
+                total = wave_sum(diff_row<K>(a, x, acc, D_out));
+                w_io += flagged + 2;
+            }
+            if (lane == 0) pop_out[x] = (u32)total;
+            w_new += total;
+            w_rn += total ? 1 : 0;
+            w_raw += raw;
+            continue;
+        }
+        // a slice of a split row
+        u64 *gu = a.GU + (u64)g * a.rw;
+        u64 *gt = SG ? a.GT + (u64)g * a.rw : nullptr;
         if (any) {
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const u64 w = (u64)lane + 64u * k;
-                if (K > 1 || w < a.rw) s_new[w] = 0;
-            }
-            if (lane == 0) s_qn[wv] = 0;
-            wave_sync_lds();
-            u64 raw = 0;
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                u64 bits = acc[k];
-                const u64 w = (u64)lane + 64u * k;
-                while (bits) {
-                    const int bit = __ffsll((unsigned long long)bits) - 1;
-                    bits &= bits - 1;
-                    const u32 b = (u32)(w * 64 + (u64)bit);
-                    const u64 so = a.succ_off[b], se = a.succ_off[b + 1];
-                    raw += se - so;
-                    if (se - so > kLight) {
-                        const u32 q = atomicAdd(&s_qn[wv], 1u);
-                        if (q < kQueue) {
-                            s_q[wv][q] = b;
-                            continue;
-                        }
-                    }
-                    for (u64 j = so; j < se; ++j) {
-                        const u32 y = a.succ_idx[j];
-                        atomicOr((unsigned long long *)&s_new[y >> 6], 1ull << (y & 63));
-                    }
+                if (K > 1 || w < a.rw) {
+                    if (acc[k]) atomicOr((unsigned long long *)&gu[w], (unsigned long long)acc[k]);
+                    if constexpr (SG)
+                        if (acct[k]) atomicOr((unsigned long long *)&gt[w], (unsigned long long)acct[k]);
                 }
             }
-            wave_sync_lds();
-            const u32 nq = min(s_qn[wv], kQueue);
-            for (u32 q = 0; q < nq; ++q) {
-                const u32 b = s_q[wv][q];
-                const u64 so = a.succ_off[b], se = a.succ_off[b + 1];
-                for (u64 j = so + lane; j < se; j += 64) {
-                    const u32 y = a.succ_idx[j];
-                    atomicOr((unsigned long long *)&s_new[y >> 6], 1ull << (y & 63));
-                }
-            }
-            wave_sync_lds();
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const u64 w = (u64)lane + 64u * k;
-                acc[k] = (K > 1 || w < a.rw) ? s_new[w] : 0ull;
-            }
-            wave_sync_lds();  // s_new is rewritten for the next row
-            w_raw += raw;
-            const u64 cnt = diff_row<K>(a, x, acc, D_out);
-            total = wave_sum(cnt);
-            w_io += flagged + 2;
+            w_io += flagged;
         }
-        if (lane == 0) pop_out[x] = (u32)total;
+        if (!SG) w_raw += raw;
+        __threadfence();  // release this slice's ORs before counting it finished
+        u32 old = 0;
+        if (lane == 0) old = atomicAdd(&a.slice_done[g], 1u);
+        old = (u32)__shfl((int)old, 0, 64);
+        if (old + 1 != a.n_slices[g]) continue;  // the same for the whole wave
+        __threadfence();  // acquire: every slice's ORs precede its count
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const u64 w = (u64)lane + 64u * k;
+            acc[k] = 0;
+            if (K > 1 || w < a.rw) {
+                acc[k] = __hip_atomic_load(&gu[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&gu[w], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if constexpr (SG) {
+                    acct[k] = __hip_atomic_load(&gt[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&gt[w], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        }
+        if constexpr (SG) w_raw += plane_dot<K>(a, acct);
+        const u64 total = wave_sum(diff_row<K>(a, x, acc, D_out));
+        if (lane == 0) {
+            pop_out[x] = (u32)total;
+            a.slice_done[g] = 0;
+        }
         w_new += total;
+        w_rn += total ? 1 : 0;
+        w_io += 2;
     }
-    dense_block_end(st, counts, false, w_new, wave_sum(w_raw), 0, w_io);
+    dense_block_end(st, counts, w_new, wave_sum(w_raw), w_io, w_rn);
 }
 
-// SG: appends the last body's new tuples (the D rows with a non-zero pop) to
-// the log once their count is known, or asks the host to grow the log first.
+// Appends the last body's new tuples (the D rows with a non-zero pop) to the
+// log once their count is known, or asks the host to grow the log first.
+// Each block takes a contiguous range of the active rows and each wave a
+// contiguous part of it: one cursor atomic per block, then positions from a
+// block scan of the waves' sums and a running sum over the wave's rows.
 // Nothing to do when the body launched before it did not run (appended ==
 // body).  Every block reads the same state: only the last block to finish
 // updates `appended`, after all of them have read it.
 template <int K>
 __global__ __launch_bounds__(DB) void k_dense_append(DenseArgs a, const u32 *pop, const u64 *D,
                                                      DenseState *st, const u64 *counts) {
+    __shared__ u64 s_w[DWV + 1];
+    __shared__ unsigned long long s_base;
     __shared__ bool s_last;
     const u32 body = st->body;
     if (st->appended >= body) return;
-    const u64 nn = counts[3 * (body - 1)];
-    const u64 base = st->append_base;
-    if (nn > st->log_cap || base > st->log_cap - nn) {
+    const u64 nn = counts[kCounts * (body - 1)];
+    const u64 base0 = st->append_base;
+    if (nn > st->log_cap || base0 > st->log_cap - nn) {
         if (blockIdx.x == 0 && threadIdx.x == 0 && st->stop == kDnRun) st->stop = kDnGrow;
         return;
     }
-    const int lane = threadIdx.x & 63;
-    const u64 wave = (u64)blockIdx.x * DWV + (threadIdx.x >> 6);
-    const u64 n_waves = (u64)gridDim.x * DWV;
-    for (u64 i = wave; i < a.n_act; i += n_waves) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const u64 per_block = (a.n_act + gridDim.x - 1) / gridDim.x;
+    const u64 r0 = min(a.n_act, (u64)blockIdx.x * per_block);
+    const u64 r1 = min(a.n_act, r0 + per_block);
+    const u64 per_wave = (r1 - r0 + DWV - 1) / DWV;
+    const u64 w0 = min(r1, r0 + (u64)wv * per_wave), w1 = min(r1, w0 + per_wave);
+    u64 sum = 0;
+    for (u64 i = w0 + lane; i < w1; i += 64) sum += pop[a.act[i]];
+    sum = wave_sum(sum);
+    u64 tot;
+    const u64 ex = block_exclusive_scan<DB>(lane == 0 ? sum : 0ull, s_w, &tot);
+    if (threadIdx.x == 0) s_base = tot ? atomicAdd(&st->cursor, (unsigned long long)tot) : 0ull;
+    __syncthreads();
+    u64 at_row = s_base + (u64)__shfl((long long)ex, 0, 64);
+    const int kb = 6 + __ffsll((unsigned long long)a.rw) - 1;
+    for (u64 i = w0; i < w1; ++i) {
         const u32 x = a.act[i];
-        if (pop[x] == 0) continue;  // the same for the whole wave
-        u64 v[K];
-        u64 cnt = 0;
+        const u32 c = pop[x];  // the same for the whole wave
+        if (c == 0) continue;
+        // the row's words in order, 64 a round: its keys come out sorted
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const u64 w = (u64)lane + 64u * k;
-            v[k] = (K > 1 || w < a.rw) ? D[(u64)x * a.rw + w] : 0ull;
-            cnt += (u64)__popcll(v[k]);
-        }
-        const u64 total = wave_sum(cnt);
-        unsigned long long pos = 0;
-        if (lane == 0) pos = atomicAdd(&st->cursor, (unsigned long long)total);
-        pos = __shfl(pos, 0, 64);
-        u64 at = pos + wave_inclusive_scan(cnt) - cnt;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            u64 bits = v[k];
-            const u64 w = (u64)lane + 64u * k;
-            while (bits) {
-                const int b = __ffsll((unsigned long long)bits) - 1;
-                bits &= bits - 1;
-                a.log[at++] = ((u64)x << 32) | (w * 64 + (u64)b);
-            }
+            const u64 v = (K > 1 || w < a.rw) ? D[(u64)x * a.rw + w] : 0ull;
+            at_row += coop_write(v, ((u64)x << kb) + w * 64, kb, a.log, at_row, ~0ull);
         }
     }
     __syncthreads();
@@ -552,22 +784,53 @@ __global__ __launch_bounds__(DB) void k_dense_act(const u64 *off, u64 rows, u32 
     }
 }
 
-// Delta keys -> pop counts (duplicates counted) and TC's first produced count.
-__global__ __launch_bounds__(DB) void k_dense_mark(const u64 *keys, u64 n, int kb, u32 *pop,
-                                                   const u32 *deg1, DenseState *st) {
-    u64 nr = 0;
-    for (u64 i = (u64)blockIdx.x * DB + threadIdx.x; i < n; i += (u64)gridDim.x * DB) {
-        const u64 k = keys[i];
-        const u64 c0 = k >> 32, c1 = k & 0xffffffffull;
-        if ((c0 >> kb) || (c1 >> kb)) {
-            atomicOr(&st->bad, 1u);
-            continue;
+// Bit-planes of r2's row lengths: bit b of plane j is bit j of off[b+1] -
+// off[b] (a wave per 64-bit word: lane t holds row 64 w + t, one ballot per
+// plane).
+__global__ __launch_bounds__(DB) void k_dense_planes(const u64 *off, u64 rw, int n_planes,
+                                                     u64 *planes) {
+    const int lane = threadIdx.x & 63;
+    const u64 wave = (u64)blockIdx.x * DWV + (threadIdx.x >> 6);
+    const u64 n_waves = (u64)gridDim.x * DWV;
+    for (u64 w = wave; w < rw; w += n_waves) {
+        const u64 b = w * 64 + lane;
+        const u64 d = off[b + 1] - off[b];
+        for (int j = 0; j < n_planes; ++j) {
+            const u64 m = __ballot((d >> j) & 1ull);
+            if (lane == 0) planes[(u64)j * rw + w] = m;
         }
-        atomicAdd(&pop[c0], 1u);
-        if (deg1) nr += deg1[c0];
     }
-    nr = wave_sum(nr);
-    if ((threadIdx.x & 63) == 0 && nr) atomicAdd(&st->next_raw, (unsigned long long)nr);
+}
+
+// Delta keys -> per-row counts (duplicates counted).
+__global__ __launch_bounds__(DB) void k_dense_mark(const u64 *keys, u64 n, int kb, u32 *pop,
+                                                   DenseState *st) {
+    // one add per run of equal rows within a wave's 64 keys (delta keys
+    // usually come sorted: a row's keys would all add to one counter)
+    const int lane = threadIdx.x & 63;
+    const u64 wave = (u64)blockIdx.x * DWV + (threadIdx.x >> 6);
+    const u64 n_waves = (u64)gridDim.x * DWV;
+    for (u64 base = wave * 64; base < n; base += n_waves * 64) {
+        const u64 i = base + lane;
+        bool valid = false;
+        u64 row = ~0ull - (u64)lane;  // distinct per lane when not a key
+        if (i < n) {
+            const u64 k = keys[i];
+            const u64 c0 = k >> 32, c1 = k & 0xffffffffull;
+            if ((c0 >> kb) || (c1 >> kb)) {
+                atomicOr(&st->bad, 1u);
+            } else {
+                valid = true;
+                row = c0;
+            }
+        }
+        const u64 prev = __shfl_up(row, 1, 64);
+        const bool head = lane == 0 || row != prev;
+        const u64 heads = __ballot(head);
+        const u64 above = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
+        const int next = above ? __ffsll((unsigned long long)above) - 1 : 64;
+        if (head && valid) atomicAdd(&pop[row], (u32)(next - lane));
+    }
 }
 
 __global__ __launch_bounds__(DB) void k_dense_zero_rows(const u32 *pop, u64 rows, u64 rw, u64 *D) {
@@ -583,6 +846,40 @@ __global__ __launch_bounds__(DB) void k_dense_set_bits(const u64 *keys, u64 n, i
         const u64 c0 = k >> 32, c1 = k & 0xffffffffull;
         if ((c0 >> kb) || (c1 >> kb)) continue;
         atomicOr((unsigned long long *)&D[c0 * rw + (c1 >> 6)], 1ull << (c1 & 63));
+    }
+}
+
+// SG's successor entries: one 16-byte load gives a row's successors when it
+// has at most kInline of them, else its length, CSR offset and bit-row index
+// (hidx[b], kNoBit when the list is walked).
+__global__ __launch_bounds__(DB) void k_succ_tab(const u64 *off, const u32 *idx, const u32 *hidx,
+                                                 u64 rows, uint4 *tab) {
+    for (u64 b = (u64)blockIdx.x * DB + threadIdx.x; b < rows; b += (u64)gridDim.x * DB) {
+        const u64 so = off[b], deg = off[b + 1] - so;
+        uint4 e = make_uint4((u32)deg, 0, 0, 0);
+        if (deg <= kInline) {
+            if (deg > 0) e.y = idx[so];
+            if (deg > 1) e.z = idx[so + 1];
+            if (deg > 2) e.w = idx[so + 2];
+        } else {
+            e.y = (u32)so;
+            e.z = (u32)(so >> 32);
+            e.w = hidx[b];
+        }
+        tab[b] = e;
+    }
+}
+
+// Bit rows of the heavy lists: row h gets the successors of heavy[h].
+__global__ __launch_bounds__(DB) void k_heavy_rows(const u64 *off, const u32 *idx,
+                                                   const u32 *heavy_b, u64 n_heavy, u64 rw,
+                                                   u64 *rows_out) {
+    for (u64 h = blockIdx.x; h < n_heavy; h += gridDim.x) {
+        const u32 b = heavy_b[h];
+        for (u64 t = off[b] + threadIdx.x; t < off[b + 1]; t += DB) {
+            const u32 y = idx[t];
+            atomicOr((unsigned long long *)&rows_out[h * rw + (y >> 6)], 1ull << (y & 63));
+        }
     }
 }
 
@@ -680,26 +977,50 @@ static bool dense_rule_ok(const gdl_hashset *hs, int shape) {
     return shape == GDL_DENSE_TC || shape == GDL_DENSE_SG;
 }
 
+// The buffers of one body: delta in (D, pop), out, and SG's U.
+struct BodyBufs {
+    const u32 *pop_in;
+    const u64 *D_in;
+    u32 *pop_out;
+    u64 *D_out;
+    u32 *upop;
+    u64 *U;
+};
+
 template <int K>
-static gdl_status launch_body(gdl_ctx *ctx, int shape, const DenseArgs &a, const u32 *pop_in,
-                              const u64 *D_in, u32 *pop_out, u64 *D_out, DenseState *st,
-                              u64 *counts, unsigned grid) {
+static gdl_status launch_body(gdl_ctx *ctx, int shape, const DenseArgs &a, const BodyBufs &bb,
+                              DenseState *st, u64 *counts, unsigned grid, unsigned grid_exp) {
     if (shape == GDL_DENSE_TC) {
-        hipLaunchKernelGGL(k_dense_tc<K>, dim3(grid), dim3(DB), 0, ctx->stream, a, pop_in, D_in,
-                           pop_out, D_out, st, counts);
-        return check_launch(ctx, "k_dense_tc");
+        KTimer kt(ctx, K_DENSE_COMBINE, 0.0);
+        hipLaunchKernelGGL((k_dense_combine<K, false>), dim3(grid), dim3(DB), 0, ctx->stream, a,
+                           bb.pop_in, bb.D_in, (const u32 *)nullptr, (const u64 *)nullptr,
+                           bb.pop_out, bb.D_out, st, counts);
+        GDL_TRY(check_launch(ctx, "k_dense_combine"));
+    } else if constexpr (K <= 16) {  // SG rows of at most 2^16 bits (LDS)
+        {
+            KTimer kt(ctx, K_DENSE_EXPAND, 0.0);
+            const size_t lds = (size_t)DWV * a.rw * sizeof(u64);
+            hipLaunchKernelGGL(k_dense_expand<K>, dim3(grid_exp), dim3(DB), lds, ctx->stream, a,
+                               bb.pop_in, bb.D_in, bb.upop, bb.U, st);
+            GDL_TRY(check_launch(ctx, "k_dense_expand"));
+        }
+        KTimer kt(ctx, K_DENSE_COMBINE, 0.0);
+        hipLaunchKernelGGL((k_dense_combine<K, true>), dim3(grid), dim3(DB), 0, ctx->stream, a,
+                           bb.pop_in, bb.D_in, (const u32 *)bb.upop, (const u64 *)bb.U,
+                           bb.pop_out, bb.D_out, st, counts);
+        GDL_TRY(check_launch(ctx, "k_dense_combine"));
+    } else {
+        return fail(ctx, GDL_ERR_UNSUPPORTED, "dense SG bodies: ids of at most 16 bits");
     }
-    const size_t lds = (size_t)DWV * a.rw * sizeof(u64);
-    hipLaunchKernelGGL(k_dense_sg<K>, dim3(grid), dim3(DB), lds, ctx->stream, a, pop_in, D_in,
-                       pop_out, D_out, st, counts);
-    GDL_TRY(check_launch(ctx, "k_dense_sg"));
-    hipLaunchKernelGGL(k_dense_append<K>, dim3(grid), dim3(DB), 0, ctx->stream, a, pop_out, D_out,
-                       st, counts);
+    KTimer kt(ctx, K_DENSE_APPEND, 0.0);
+    hipLaunchKernelGGL(k_dense_append<K>, dim3(grid), dim3(DB), 0, ctx->stream, a, bb.pop_out,
+                       bb.D_out, st, counts);
     return check_launch(ctx, "k_dense_append");
 }
 
 static gdl_status launch_append(gdl_ctx *ctx, int kb, const DenseArgs &a, const u32 *pop,
                                 const u64 *D, DenseState *st, const u64 *counts, unsigned grid) {
+    KTimer kt(ctx, K_DENSE_APPEND, 0.0);
 #define GDL_DENSE_APPEND(K)                                                                     \
     hipLaunchKernelGGL(k_dense_append<K>, dim3(grid), dim3(DB), 0, ctx->stream, a, pop, D, st, \
                        counts);                                                                 \
@@ -716,15 +1037,15 @@ static gdl_status launch_append(gdl_ctx *ctx, int kb, const DenseArgs &a, const 
 }
 
 static gdl_status dispatch_body(gdl_ctx *ctx, int shape, int kb, const DenseArgs &a,
-                                const u32 *pop_in, const u64 *D_in, u32 *pop_out, u64 *D_out,
-                                DenseState *st, u64 *counts, unsigned grid) {
+                                const BodyBufs &bb, DenseState *st, u64 *counts, unsigned grid,
+                                unsigned grid_exp) {
     switch (kb) {
-    case 17: return launch_body<32>(ctx, shape, a, pop_in, D_in, pop_out, D_out, st, counts, grid);
-    case 16: return launch_body<16>(ctx, shape, a, pop_in, D_in, pop_out, D_out, st, counts, grid);
-    case 15: return launch_body<8>(ctx, shape, a, pop_in, D_in, pop_out, D_out, st, counts, grid);
-    case 14: return launch_body<4>(ctx, shape, a, pop_in, D_in, pop_out, D_out, st, counts, grid);
-    case 13: return launch_body<2>(ctx, shape, a, pop_in, D_in, pop_out, D_out, st, counts, grid);
-    default: return launch_body<1>(ctx, shape, a, pop_in, D_in, pop_out, D_out, st, counts, grid);
+    case 17: return launch_body<32>(ctx, shape, a, bb, st, counts, grid, grid_exp);
+    case 16: return launch_body<16>(ctx, shape, a, bb, st, counts, grid, grid_exp);
+    case 15: return launch_body<8>(ctx, shape, a, bb, st, counts, grid, grid_exp);
+    case 14: return launch_body<4>(ctx, shape, a, bb, st, counts, grid, grid_exp);
+    case 13: return launch_body<2>(ctx, shape, a, bb, st, counts, grid, grid_exp);
+    default: return launch_body<1>(ctx, shape, a, bb, st, counts, grid, grid_exp);
     }
 }
 
@@ -796,25 +1117,53 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
                        rows, act, (unsigned long long *)n_act_d);
     GDL_TRY(check_launch(ctx, "k_dense_act"));
     a.act = act;
-    if (shape == GDL_DENSE_TC) {
-        u32 *deg1;
-        GDL_TRY(mem.get(rows, &deg1));
-        hipLaunchKernelGGL(k_fill_u32, dim3(dgrid(ctx, rows)), dim3(DB), 0, ctx->stream, deg1,
-                           rows, 0u);
-        GDL_TRY(check_launch(ctx, "k_fill_u32"));
-        if (rule->n1) {
-            hipLaunchKernelGGL(k_dense_count_col, dim3(dgrid(ctx, rule->n1)), dim3(DB), 0,
-                               ctx->stream, rule->r1, rule->n1, 0, kb, deg1, bad);
-            GDL_TRY(check_launch(ctx, "k_dense_count_col"));
-        }
-        a.deg1 = deg1;
-    } else {
+    std::vector<u64> h_soff;  // SG: r2's row offsets on the host
+    if (shape == GDL_DENSE_SG) {
         u64 *succ_off;
         u32 *succ_idx;
         GDL_TRY(group_by_col(ctx, mem, rule->r2, rule->n2, 0, kb, rows, cnt, &succ_off, &succ_idx,
                              bad));
         a.succ_off = succ_off;
         a.succ_idx = succ_idx;
+        h_soff.resize(rows + 1);
+        GDL_HIP(ctx, hipMemcpyAsync(h_soff.data(), succ_off, (rows + 1) * sizeof(u64),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        // the longest lists (more than kHeavy successors) become bit rows
+        std::vector<u32> hb;
+        for (u64 r = 0; r < rows; ++r)
+            if (h_soff[r + 1] - h_soff[r] > kHeavy) hb.push_back((u32)r);
+        if (hb.size() > kHeavyMax) {
+            std::sort(hb.begin(), hb.end(), [&](u32 p, u32 q) {
+                return h_soff[p + 1] - h_soff[p] > h_soff[q + 1] - h_soff[q];
+            });
+            hb.resize(kHeavyMax);
+        }
+        std::vector<u32> h_hidx(rows, kNoBit);
+        for (u32 h = 0; h < (u32)hb.size(); ++h) h_hidx[hb[h]] = h;
+        u32 *hidx, *heavy_b;
+        u64 *heavy;
+        GDL_TRY(mem.get(rows, &hidx));
+        GDL_TRY(mem.get(std::max<size_t>(hb.size(), 1), &heavy_b));
+        GDL_TRY(mem.get(std::max<u64>(hb.size() * rw, 1), &heavy));
+        GDL_HIP(ctx, hipMemcpyAsync(hidx, h_hidx.data(), rows * sizeof(u32), hipMemcpyHostToDevice,
+                                    ctx->stream));
+        if (!hb.empty()) {
+            GDL_HIP(ctx, hipMemcpyAsync(heavy_b, hb.data(), hb.size() * sizeof(u32),
+                                        hipMemcpyHostToDevice, ctx->stream));
+            GDL_HIP(ctx, hipMemsetAsync(heavy, 0, hb.size() * rw * sizeof(u64), ctx->stream));
+            hipLaunchKernelGGL(k_heavy_rows, dim3((unsigned)std::min<size_t>(hb.size(), 4096)),
+                               dim3(DB), 0, ctx->stream, succ_off, succ_idx, heavy_b,
+                               (u64)hb.size(), rw, heavy);
+            GDL_TRY(check_launch(ctx, "k_heavy_rows"));
+        }
+        a.heavy = heavy;
+        uint4 *tab;
+        GDL_TRY(mem.get(rows, &tab));
+        hipLaunchKernelGGL(k_succ_tab, dim3(dgrid(ctx, rows)), dim3(DB), 0, ctx->stream,
+                           succ_off, succ_idx, hidx, rows, tab);
+        GDL_TRY(check_launch(ctx, "k_succ_tab"));
+        a.succ_tab = tab;
     }
     u64 setup[2] = {0, 0};
     GDL_HIP(ctx, hipMemcpyAsync(&setup[0], n_act_d, sizeof(u64), hipMemcpyDeviceToHost,
@@ -825,6 +1174,105 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
     // ids beyond the set's domain in a static relation: the tuple-at-a-time
     // bodies handle (and report) them
     if (setup[1] || a.n_act == 0) return GDL_OK;
+
+    // work items: the active rows, most preds first, split into slices of at
+    // most kSlice preds (on the host: a few hundred thousand rows at most)
+    std::vector<u64> h_off(rows + 1);
+    std::vector<u32> h_act(a.n_act);
+    GDL_HIP(ctx, hipMemcpyAsync(h_off.data(), pred_off, (rows + 1) * sizeof(u64),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    GDL_HIP(ctx, hipMemcpyAsync(h_act.data(), act, a.n_act * sizeof(u32), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::sort(h_act.begin(), h_act.end(), [&](u32 p, u32 q) {
+        const u64 cp = h_off[p + 1] - h_off[p], cq = h_off[q + 1] - h_off[q];
+        return cp != cq ? cp > cq : p < q;
+    });
+    std::vector<uint4> h_items;
+    std::vector<u32> h_slices;
+    h_items.reserve(a.n_act);
+    for (u32 x : h_act) {
+        const u64 c = h_off[x + 1] - h_off[x];
+        if (c <= kSlice) {
+            h_items.push_back(make_uint4(x, 0, (u32)c, 0xffffffffu));
+            continue;
+        }
+        const u32 g = (u32)h_slices.size();
+        const u32 ns = (u32)((c + kSlice - 1) / kSlice);
+        h_slices.push_back(ns);
+        for (u32 j = 0; j < ns; ++j)
+            h_items.push_back(make_uint4(x, j * kSlice, (u32)std::min<u64>(c, (u64)(j + 1) * kSlice), g));
+    }
+    uint4 *items;
+    GDL_TRY(mem.get(h_items.size(), &items));
+    GDL_HIP(ctx, hipMemcpyAsync(act, h_act.data(), a.n_act * sizeof(u32), hipMemcpyHostToDevice,
+                                ctx->stream));
+    GDL_HIP(ctx, hipMemcpyAsync(items, h_items.data(), h_items.size() * sizeof(uint4),
+                                hipMemcpyHostToDevice, ctx->stream));
+    a.items = items;
+    a.n_items = h_items.size();
+    if (!h_slices.empty()) {
+        const u64 ns = h_slices.size();
+        u32 *n_slices, *done;
+        u64 *gu, *gt = nullptr;
+        GDL_TRY(mem.get(ns, &n_slices));
+        GDL_TRY(mem.get(ns, &done));
+        GDL_TRY(mem.get(ns * rw, &gu));
+        GDL_HIP(ctx, hipMemcpyAsync(n_slices, h_slices.data(), ns * sizeof(u32),
+                                    hipMemcpyHostToDevice, ctx->stream));
+        GDL_HIP(ctx, hipMemsetAsync(done, 0, ns * sizeof(u32), ctx->stream));
+        GDL_HIP(ctx, hipMemsetAsync(gu, 0, ns * rw * sizeof(u64), ctx->stream));
+        if (shape == GDL_DENSE_SG) {
+            GDL_TRY(mem.get(ns * rw, &gt));
+            GDL_HIP(ctx, hipMemsetAsync(gt, 0, ns * rw * sizeof(u64), ctx->stream));
+        }
+        a.n_slices = n_slices;
+        a.slice_done = done;
+        a.GU = gu;
+        a.GT = gt;
+    }
+    u32 *upop = nullptr;
+    u64 *U = nullptr;
+    unsigned grid_exp = 1;
+    if (shape == GDL_DENSE_SG) {
+        // U is built for the rows that are some row's pred; the planes of
+        // r2's row lengths
+        std::vector<u32> h_idx(rule->n1);
+        GDL_HIP(ctx, hipMemcpyAsync(h_idx.data(), pred_idx, rule->n1 * sizeof(u32),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        std::vector<uint8_t> used(rows, 0);
+        for (u32 y : h_idx) used[y] = 1;
+        std::vector<u32> h_exp;
+        for (u64 r = 0; r < rows; ++r)
+            if (used[r]) h_exp.push_back((u32)r);
+        u64 max_deg = 0;
+        for (u64 r = 0; r < rows; ++r) max_deg = std::max(max_deg, h_soff[r + 1] - h_soff[r]);
+        int n_planes = 0;
+        while (n_planes < 64 && (max_deg >> n_planes)) ++n_planes;
+        if (n_planes > 24)
+            return fail(ctx, GDL_ERR_UNSUPPORTED, "dense SG bodies: a row of 2^24 or more");
+        u32 *exp_rows;
+        GDL_TRY(mem.get(std::max<u64>(h_exp.size(), 1), &exp_rows));
+        if (!h_exp.empty())
+            GDL_HIP(ctx, hipMemcpyAsync(exp_rows, h_exp.data(), h_exp.size() * sizeof(u32),
+                                        hipMemcpyHostToDevice, ctx->stream));
+        a.exp_rows = exp_rows;
+        a.n_exp = h_exp.size();
+        u64 *planes;
+        GDL_TRY(mem.get(std::max<u64>((u64)n_planes * rw, 1), &planes));
+        if (n_planes) {
+            hipLaunchKernelGGL(k_dense_planes, dim3(dgrid(ctx, rw * 64)), dim3(DB), 0,
+                               ctx->stream, a.succ_off, rw, n_planes, planes);
+            GDL_TRY(check_launch(ctx, "k_dense_planes"));
+        }
+        a.planes = planes;
+        a.n_planes = n_planes;
+        GDL_TRY(mem.get(rows * rw, &U));
+        GDL_TRY(mem.get(rows, &upop));
+        grid_exp = (unsigned)std::max<u64>(1, std::min<u64>((a.n_exp + DWV - 1) / DWV,
+                                                            grid_cap(ctx)));
+    }
 
     // the delta as bit rows (buffer 0)
     u64 *D[2];
@@ -841,14 +1289,13 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
     constexpr int kChunk = 64;
     u64 *st_words, *counts;
     GDL_TRY(scr.get((sizeof(DenseState) + 7) / 8, &st_words));
-    GDL_TRY(scr.get(3 * kChunk, &counts));
+    GDL_TRY(scr.get(kCounts * kChunk, &counts));
     DenseState *st = (DenseState *)st_words;
     GDL_HIP(ctx, hipMemsetAsync(st, 0, sizeof(DenseState), ctx->stream));
     {
         KTimer kt(ctx, K_DENSE_SETUP, 24.0 * n_delta + 8.0 * rows * rw);
         hipLaunchKernelGGL(k_dense_mark, dim3(dgrid(ctx, n_delta)), dim3(DB), 0, ctx->stream,
-                           d_delta, n_delta, kb, pop[0],
-                           shape == GDL_DENSE_TC ? a.deg1 : (const u32 *)nullptr, st);
+                           d_delta, n_delta, kb, pop[0], st);
         GDL_TRY(check_launch(ctx, "k_dense_mark"));
         hipLaunchKernelGGL(k_dense_zero_rows, dim3(dgrid(ctx, rows * rw)), dim3(DB), 0,
                            ctx->stream, pop[0], rows, rw, D[0]);
@@ -864,47 +1311,50 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
         return fail(ctx, GDL_ERR_INVALID_ARG, "dense bodies: a delta key outside the set's domain");
 
     u64 end = *log_end;
-    // new tuples land in active rows only
-    u64 dom_left = a.n_act * rows;
-    u64 bound = std::min<u64>(r.next_raw, dom_left);
+    // room for bodies about twice the delta (a larger one grows the log first)
+    GDL_TRY(grow_log(ctx, d_log, log_cap, end, end + std::max<u64>(2 * n_delta, 4096)));
     const unsigned grid =
-        (unsigned)std::max<u64>(1, std::min<u64>((a.n_act + DWV - 1) / DWV, grid_cap(ctx)));
+        (unsigned)std::max<u64>(1, std::min<u64>((a.n_items + DWV - 1) / DWV, grid_cap(ctx)));
     int done = 0, ramp = 4;
+    u64 in_delta = n_delta;  // the delta entering the next body
     while (done < max_bodies) {
-        if (shape == GDL_DENSE_TC)  // room for the next body's new tuples
-            GDL_TRY(grow_log(ctx, d_log, log_cap, end, end + std::max<u64>(bound, 4096)));
         a.log = *d_log;
         const int k = std::min({ramp, kChunk, max_bodies - done});
         DenseState s{};
         s.cursor = end;
         s.log_cap = *log_cap;
-        s.dom_left = dom_left;
         s.sparse_below = sparse_below;
         s.max_bodies = (u32)k;
         GDL_HIP(ctx, hipMemcpyAsync(st, &s, sizeof(s), hipMemcpyHostToDevice, ctx->stream));
         for (int i = 0; i < k; ++i) {
             const int p = (done + i) & 1;
-            KTimer kt(ctx, K_DENSE_BODY, 0.0);
-            GDL_TRY(dispatch_body(ctx, shape, kb, a, pop[p], D[p], pop[p ^ 1], D[p ^ 1], st,
-                                  counts, grid));
+            const BodyBufs bb{pop[p], D[p], pop[p ^ 1], D[p ^ 1], upop, U};
+            GDL_TRY(dispatch_body(ctx, shape, kb, a, bb, st, counts, grid, grid_exp));
         }
-        u64 c[3 * kChunk];
+        u64 c[kCounts * kChunk];
         GDL_HIP(ctx, hipMemcpyAsync(&r, st, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
         GDL_HIP(ctx, hipMemcpyAsync(c, counts, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
         GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
         GDL_TRY(check_device_error(ctx));
-        if (r.stop == kDnOverflow)
-            return fail(ctx, GDL_ERR_OVERFLOW, "dense bodies: a body outgrew its log bound");
         for (u32 b = 0; b < r.body; ++b, ++done) {
-            h_new[done] = c[3 * b];
-            h_raw[done] = c[3 * b + 1];
-            ctx->stats[K_DENSE_BODY].alg_bytes +=
-                8.0 * (double)rw * (double)c[3 * b + 2] + 16.0 * (double)c[3 * b];
-            hs->size += c[3 * b];
+            const u64 *cb = c + kCounts * b;
+            h_new[done] = cb[0];
+            h_raw[done] = cb[1];
+            const double row = 8.0 * (double)rw;
+            // combine: the pred rows read (D, and SG's U), FULL's row read,
+            // the new delta row written, new words set in FULL
+            ctx->stats[K_DENSE_COMBINE].alg_bytes += row * (double)cb[2] + 8.0 * (double)cb[0];
+            // expansion: the delta row read, U's row written, one 16-byte
+            // entry per delta bit
+            ctx->stats[K_DENSE_EXPAND].alg_bytes += row * (double)cb[3] + 16.0 * (double)in_delta;
+            // append: the rows with new tuples read, their keys written
+            ctx->stats[K_DENSE_APPEND].alg_bytes += row * (double)cb[4] + 8.0 * (double)cb[0];
+            in_delta = cb[0];
+            hs->size += cb[0];
         }
-        if (shape == GDL_DENSE_SG && r.appended < r.body) {
-            // the last body's append found the log full: grow it and append
-            const u64 nn = c[3 * (r.body - 1)];
+        if (r.appended < r.body) {
+            // the last body's append found the log full: grow it, then append
+            const u64 nn = c[kCounts * (r.body - 1)];
             GDL_TRY(grow_log(ctx, d_log, log_cap, r.cursor, r.cursor + nn));
             a.log = *d_log;
             const u32 stop = r.stop;
@@ -918,10 +1368,10 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
             if (r.appended != r.body)
                 return fail(ctx, GDL_ERR_OVERFLOW, "dense bodies: the log did not grow");
             r.stop = stop == kDnGrow ? kDnLimit : stop;
+            // the next chunk's bodies are likely as large: room for 2x more
+            GDL_TRY(grow_log(ctx, d_log, log_cap, r.cursor, r.cursor + 2 * nn));
         }
         end = r.cursor;
-        dom_left = r.dom_left;
-        bound = r.next_bound;
         if (r.body == 0 || r.stop == kDnFixpoint || r.stop == kDnSparse) break;
         ramp = (int)r.body == k ? std::min(2 * ramp, kChunk) : std::max(4, (int)r.body);
     }

@@ -80,7 +80,9 @@ enum KernelId {
     K_LABEL,
     K_SCATTER,
     K_DENSE_SETUP,
-    K_DENSE_BODY,
+    K_DENSE_EXPAND,
+    K_DENSE_COMBINE,
+    K_DENSE_APPEND,
     K_BM_EXTRACT,
     K_NUM
 };

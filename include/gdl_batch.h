@@ -40,6 +40,9 @@ typedef struct gdl_tcb_config {
                             free at create) */
     int keep_keys;       /* 1: a finished instance's sorted tuples are copied to the
                             host; 0: only its size and checksum */
+    uint64_t solo_tuples; /* an instance whose |TC| estimate is at least this runs
+                             alone, to its fixpoint, when admitted (default 2^23;
+                             ~0: never) */
 } gdl_tcb_config;
 
 typedef struct gdl_tcb_result {
@@ -70,7 +73,11 @@ uint64_t gdl_tc_estimate(const uint64_t *h_edges, uint64_t n, uint32_t n_nodes, 
  * (0: gdl_tc_estimate with 32 samples).  GDL_ERR_BUSY when no slot is free
  * or the estimate does not fit the budget next to the instances admitted
  * (an instance is always admitted into an empty batch).  The instance's first
- * body is the batch's next one. */
+ * body is the batch's next one.  An instance estimated at cfg.solo_tuples or
+ * more (one that fills the device by itself) instead runs to its fixpoint
+ * here, on the single-query engine path (the linear loop, with bit-matrix
+ * bodies while its delta is dense); its result (slot -1) is then ready to
+ * poll, with the same tuples, checksum and trace. */
 gdl_status gdl_tcb_admit(gdl_tcb *b, uint64_t qid, const uint64_t *h_edges, uint64_t n,
                          uint32_t n_nodes, uint64_t est_tuples);
 

@@ -143,6 +143,31 @@ def test_batch_matches_alone_runs_with_slot_reuse(ctx):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("solo", [0, 20_000])
+def test_solo_lane_matches_the_oracle(ctx, solo):
+    # solo 0: every instance runs alone at admission; 20k: the larger ones do
+    inst = small_instances(14)
+    inst.append((14, read_edges("data_7035.txt"), 6105))  # repeated rows: raw-prefix delta
+    inst.append((15, gdl.pack([1], [2]), 3))
+    inst.append((16, gdl.pack([0, 1, 2], [1, 2, 0]), 3))  # a cycle: body 0 adds tuples
+    with gdl.TcBatch(ctx, max_slots=5, solo_tuples=solo) as b:
+        done = b.run(inst)
+        info = b.info()
+    assert info["finished"] == len(inst)
+    slots = [done[q]["slot"] for q, _, _ in inst]
+    assert (min(slots) == max(slots) == -1) if solo == 0 else (-1 in slots and max(slots) >= 0)
+    for qid, edges, n in inst:
+        r = done[qid]
+        want = oracle.tc(edges)
+        assert r["n_out"] == want["n_out"], qid
+        assert np.array_equal(r["keys"], want["out"]), qid
+        assert r["iterations"] == want["iterations"], qid
+        assert delta_full(r["trace"]) == delta_full(want["trace"]), qid
+        assert r["derived"] == want["derived"], qid
+        assert r["checksum"] == gdl.tuple_checksum(want["out"]), qid
+
+
+@pytest.mark.gpu
 def test_batch_alone_vs_mix_on_the_gpu(ctx):
     inst = small_instances(12, 1000, 20000, seed0=0)
     with gdl.TcBatch(ctx, keep_keys=False) as b:
