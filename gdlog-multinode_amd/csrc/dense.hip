@@ -105,10 +105,12 @@ __device__ __forceinline__ u64 coop_write(u64 wd, u64 bitbase, int kb, u64 *out,
 // tile's keys in bit order with a wave prefix sum per round (consecutive
 // words, consecutive output ranges; no block barrier).  With `minus`, rows
 // whose minus_pop entry is non-zero have minus's bits removed.
-// rounds per tile: 16, fewer when that would leave fewer than 16384 tiles
+// rounds per tile: 16, fewer when that would leave fewer than 2^18 tiles
 static int xt_rounds(u64 n_words) {
+    // small tiles: a clustered bitmap has tiles far denser than the rest,
+    // and many small ones let the scheduler spread them
     int r = 16;
-    while (r > 1 && n_words / (64ull * r) < 16384) r >>= 1;
+    while (r > 1 && n_words / (64ull * r) < (1ull << 18)) r >>= 1;
     return r;
 }
 
@@ -408,36 +410,36 @@ template <int K>
 __global__ __launch_bounds__(DB) void k_dense_expand(DenseArgs a, const u32 *pop_in,
                                                      const u64 *D_in, u32 *upop, u64 *U,
                                                      DenseState *st) {
-    extern __shared__ u64 s_rows[];
+    // a block per row: thread t expands the bits of words t, t + DB, ...
+    // into the block's LDS row; each wave lists its own walks and bit rows
+    constexpr int KB = K >= 4 ? K / 4 : 1;  // words per thread
+    extern __shared__ u64 s_new[];
     __shared__ u32 s_h[DWV][kHList];
     __shared__ u32 s_hn[DWV];
     __shared__ u64 s_mo[DWV][kMList];      // listed walks: CSR offset
     __shared__ u32 s_mx[DWV][kMList + 1];  // and the exclusive sums of their lengths
     __shared__ u32 s_mn[DWV];
-    __shared__ u64 s_io[DWV];
+    __shared__ u64 s_cnt[DWV + 1];
     if (st->stop != kDnRun) return;  // the same for every block
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    u64 *s_new = s_rows + (u64)wv * a.rw;
-    const u64 wave = (u64)blockIdx.x * DWV + wv;
-    const u64 n_waves = (u64)gridDim.x * DWV;
     u64 io = 0;
-    for (u64 i = wave; i < a.n_exp; i += n_waves) {
+    for (u64 i = blockIdx.x; i < a.n_exp; i += gridDim.x) {
         const u32 r = a.exp_rows[i];
-        if (pop_in[r] == 0) {  // the same for the whole wave
-            if (lane == 0) upop[r] = 0;
+        if (pop_in[r] == 0) {  // the same for the whole block
+            if (threadIdx.x == 0) upop[r] = 0;
             continue;
         }
-        u64 v[K], acc[K];
+        u64 v[KB];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const u64 w = (u64)lane + 64u * k;
-            v[k] = (K > 1 || w < a.rw) ? D_in[(u64)r * a.rw + w] : 0ull;
-            if (K > 1 || w < a.rw) s_new[w] = 0;
-            acc[k] = 0;
+        for (int k = 0; k < KB; ++k) {
+            const u64 w = (u64)threadIdx.x + (u64)DB * k;
+            v[k] = w < a.rw ? D_in[(u64)r * a.rw + w] : 0ull;
         }
-        wave_sync_lds();
-        // rounds: every lane expands its light bits; heavy bits are listed
-        // (up to kHList a round) and ORed in by the wave after the round
+        for (u64 w = threadIdx.x; w < a.rw; w += DB) s_new[w] = 0;
+        __syncthreads();
+        // rounds (per wave): every lane expands its light bits; walks and
+        // bit rows are listed (up to the lists' sizes a round) and done by
+        // the whole wave after the round
         bool more = true;
         while (more) {
             if (lane == 0) {
@@ -446,8 +448,8 @@ __global__ __launch_bounds__(DB) void k_dense_expand(DenseArgs a, const u32 *pop
             }
             wave_sync_lds();
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const u32 w64 = (u32)(((u64)lane + 64u * k) * 64);
+            for (int k = 0; k < KB; ++k) {
+                const u32 w64 = (u32)(((u64)threadIdx.x + (u64)DB * k) * 64);
                 while (v[k]) {
                     u32 bb[kBatch];
                     u64 rest = v[k];
@@ -497,8 +499,8 @@ __global__ __launch_bounds__(DB) void k_dense_expand(DenseArgs a, const u32 *pop
             // the listed walks, flattened: 64 successors a step
             const u32 nm = min(s_mn[wv], kMList);
             if (nm) {
-                u32 c0 = lane < (int)nm ? s_mx[wv][lane] : 0u;
-                u32 c1 = lane + 64 < (int)nm ? s_mx[wv][lane + 64] : 0u;
+                const u32 c0 = lane < (int)nm ? s_mx[wv][lane] : 0u;
+                const u32 c1 = lane + 64 < (int)nm ? s_mx[wv][lane + 64] : 0u;
                 const u64 i0 = wave_inclusive_scan(c0);
                 const u64 tot0 = __shfl(i0, 63, 64);
                 const u64 i1 = wave_inclusive_scan(c1) + tot0;
@@ -506,7 +508,6 @@ __global__ __launch_bounds__(DB) void k_dense_expand(DenseArgs a, const u32 *pop
                 wave_sync_lds();
                 if (lane < (int)nm) s_mx[wv][lane] = (u32)(i0 - c0);
                 if (lane + 64 < (int)nm) s_mx[wv][lane + 64] = (u32)(i1 - c1);
-                if (lane == 0) s_mx[wv][nm] = (u32)tot;
                 wave_sync_lds();
                 for (u64 t0 = 0; t0 < tot; t0 += 128) {
                     u32 y[2];
@@ -529,45 +530,35 @@ __global__ __launch_bounds__(DB) void k_dense_expand(DenseArgs a, const u32 *pop
                         if (y[z] != kNoBit) lds_set(s_new, y[z]);
                 }
             }
+            // the listed bit rows, ORed into the whole LDS row
             const u32 nh = min(s_hn[wv], kHList);
             for (u32 h = 0; h < nh; h += 2) {
                 const u64 *r1 = a.heavy + (u64)s_h[wv][h] * a.rw;
                 const bool two = h + 1 < nh;
                 const u64 *r2 = a.heavy + (u64)s_h[wv][two ? h + 1 : h] * a.rw;
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    const u64 w = (u64)lane + 64u * k;
-                    if (K > 1 || w < a.rw) acc[k] |= r1[w] | (two ? r2[w] : 0ull);
+                for (u64 w = lane; w < a.rw; w += 64) {
+                    const u64 x = r1[w] | (two ? r2[w] : 0ull);
+                    if (x) atomicOr((unsigned long long *)&s_new[w], (unsigned long long)x);
                 }
             }
             u64 left = 0;
 #pragma unroll
-            for (int k = 0; k < K; ++k) left |= v[k];
+            for (int k = 0; k < KB; ++k) left |= v[k];
             more = __ballot(left != 0) != 0;
         }
-        wave_sync_lds();
+        __syncthreads();
         u64 cnt = 0;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const u64 w = (u64)lane + 64u * k;
-            if (K > 1 || w < a.rw) {
-                const u64 u = s_new[w] | acc[k];
-                U[(u64)r * a.rw + w] = u;
-                cnt += (u64)__popcll(u);
-            }
+        for (u64 w = threadIdx.x; w < a.rw; w += DB) {
+            const u64 u = s_new[w];
+            U[(u64)r * a.rw + w] = u;
+            cnt += (u64)__popcll(u);
         }
-        cnt = wave_sum(cnt);
-        if (lane == 0) upop[r] = (u32)cnt;
+        u64 tot;
+        block_exclusive_scan<DB>(cnt, s_cnt, &tot);  // ends with a barrier: s_new is free again
+        if (threadIdx.x == 0) upop[r] = (u32)tot;
         io += 2;
-        wave_sync_lds();  // s_new and the list are rewritten for the next row
     }
-    if (lane == 0) s_io[wv] = io;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        u64 t = 0;
-        for (int v2 = 0; v2 < DWV; ++v2) t += s_io[v2];
-        if (t) atomicAdd(&st->exp_io, (unsigned long long)t);
-    }
+    if (threadIdx.x == 0 && io) atomicAdd(&st->exp_io, (unsigned long long)io);
 }
 
 // One body's output rows, a wave per work item (row x, a slice of its pred
@@ -681,9 +672,10 @@ __global__ __launch_bounds__(DB) void k_dense_combine(DenseArgs a, const u32 *po
 
 // Appends the last body's new tuples (the D rows with a non-zero pop) to the
 // log once their count is known, or asks the host to grow the log first.
-// Each block takes a contiguous range of the active rows and each wave a
-// contiguous part of it: one cursor atomic per block, then positions from a
-// block scan of the waves' sums and a running sum over the wave's rows.
+// Work items are 64-word chunks of the active rows (a dense row is spread
+// over waves); each block takes a contiguous range of them and each wave a
+// contiguous part of it: a counting pass, one cursor atomic per block, then
+// positions from a block scan of the waves' sums and a running sum.
 // Nothing to do when the body launched before it did not run (appended ==
 // body).  Every block reads the same state: only the last block to finish
 // updates `appended`, after all of them have read it.
@@ -701,32 +693,36 @@ __global__ __launch_bounds__(DB) void k_dense_append(DenseArgs a, const u32 *pop
         if (blockIdx.x == 0 && threadIdx.x == 0 && st->stop == kDnRun) st->stop = kDnGrow;
         return;
     }
+    // items: (active row, chunk of 64 words), in row order; a block takes a
+    // contiguous range of them, each wave a contiguous part of it
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const u64 per_block = (a.n_act + gridDim.x - 1) / gridDim.x;
-    const u64 r0 = min(a.n_act, (u64)blockIdx.x * per_block);
-    const u64 r1 = min(a.n_act, r0 + per_block);
+    const int kb = 6 + __ffsll((unsigned long long)a.rw) - 1;
+    const u64 cpr = a.rw >= 64 ? a.rw / 64 : 1;  // chunks per row
+    const u64 n_items = a.n_act * cpr;
+    const u64 per_block = (n_items + gridDim.x - 1) / gridDim.x;
+    const u64 r0 = min(n_items, (u64)blockIdx.x * per_block);
+    const u64 r1 = min(n_items, r0 + per_block);
     const u64 per_wave = (r1 - r0 + DWV - 1) / DWV;
     const u64 w0 = min(r1, r0 + (u64)wv * per_wave), w1 = min(r1, w0 + per_wave);
     u64 sum = 0;
-    for (u64 i = w0 + lane; i < w1; i += 64) sum += pop[a.act[i]];
+    for (u64 i = w0; i < w1; ++i) {
+        const u32 x = a.act[i / cpr];
+        if (pop[x] == 0) continue;  // the same for the whole wave
+        const u64 w = (i % cpr) * 64 + lane;
+        if (w < a.rw) sum += (u64)__popcll(D[(u64)x * a.rw + w]);
+    }
     sum = wave_sum(sum);
     u64 tot;
     const u64 ex = block_exclusive_scan<DB>(lane == 0 ? sum : 0ull, s_w, &tot);
     if (threadIdx.x == 0) s_base = tot ? atomicAdd(&st->cursor, (unsigned long long)tot) : 0ull;
     __syncthreads();
-    u64 at_row = s_base + (u64)__shfl((long long)ex, 0, 64);
-    const int kb = 6 + __ffsll((unsigned long long)a.rw) - 1;
+    u64 at = s_base + (u64)__shfl((long long)ex, 0, 64);
     for (u64 i = w0; i < w1; ++i) {
-        const u32 x = a.act[i];
-        const u32 c = pop[x];  // the same for the whole wave
-        if (c == 0) continue;
-        // the row's words in order, 64 a round: its keys come out sorted
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const u64 w = (u64)lane + 64u * k;
-            const u64 v = (K > 1 || w < a.rw) ? D[(u64)x * a.rw + w] : 0ull;
-            at_row += coop_write(v, ((u64)x << kb) + w * 64, kb, a.log, at_row, ~0ull);
-        }
+        const u32 x = a.act[i / cpr];
+        if (pop[x] == 0) continue;
+        const u64 w = (i % cpr) * 64 + lane;
+        const u64 v = w < a.rw ? D[(u64)x * a.rw + w] : 0ull;
+        at += coop_write(v, ((u64)x << kb) + w * 64, kb, a.log, at, ~0ull);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -999,7 +995,7 @@ static gdl_status launch_body(gdl_ctx *ctx, int shape, const DenseArgs &a, const
     } else if constexpr (K <= 16) {  // SG rows of at most 2^16 bits (LDS)
         {
             KTimer kt(ctx, K_DENSE_EXPAND, 0.0);
-            const size_t lds = (size_t)DWV * a.rw * sizeof(u64);
+            const size_t lds = (size_t)a.rw * sizeof(u64);
             hipLaunchKernelGGL(k_dense_expand<K>, dim3(grid_exp), dim3(DB), lds, ctx->stream, a,
                                bb.pop_in, bb.D_in, bb.upop, bb.U, st);
             GDL_TRY(check_launch(ctx, "k_dense_expand"));
@@ -1270,8 +1266,7 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
         a.n_planes = n_planes;
         GDL_TRY(mem.get(rows * rw, &U));
         GDL_TRY(mem.get(rows, &upop));
-        grid_exp = (unsigned)std::max<u64>(1, std::min<u64>((a.n_exp + DWV - 1) / DWV,
-                                                            grid_cap(ctx)));
+        grid_exp = (unsigned)std::max<u64>(1, std::min<u64>(a.n_exp, grid_cap(ctx)));
     }
 
     // the delta as bit rows (buffer 0)

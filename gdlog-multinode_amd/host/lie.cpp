@@ -580,7 +580,33 @@ void LIE::fixpoint_loop() {
     // delta is dense.
     const Chain chain = linear ? Chain{} : sg_chain(ra_ops, update_relations, tmp_relations,
                                                     static_relations);
+    // An LIE whose operators are all joins of two static relations computes at
+    // iteration 1 exactly what it computed at iteration 0 (the same newt, all
+    // of it already in FULL): iteration 1 is recorded from iteration 0 -- the
+    // same newt counts, no new tuple, the fixpoint -- without running it.
+    bool static_joins = !ra_ops.empty();
+    for (const auto &op : ra_ops) {
+        const auto *j = std::get_if<RelationalJoin>(&op);
+        static_joins = static_joins && j &&
+                       std::find(static_relations.begin(), static_relations.end(),
+                                 j->inner_rel) != static_relations.end() &&
+                       std::find(static_relations.begin(), static_relations.end(),
+                                 j->outer_rel) != static_relations.end();
+    }
     while (!finished) {
+        if (static_joins && iteration_counter == 1 && trace.size() >= update_relations.size()) {
+            const size_t first = trace.size() - update_relations.size();
+            for (size_t r = 0; r < update_relations.size(); ++r) {
+                const IterationRecord t0 = trace[first + r];
+                trace.push_back({1, t0.relation, t0.newt, 0, update_relations[r]->current_full_size});
+                if (verbose())
+                    std::cout << "iteration 1 relation " << update_relations[r]->name
+                              << " no new tuple added" << std::endl;
+            }
+            iteration_counter = 2;
+            boundary(true);
+            break;
+        }
         if (chain.j1) {
             Relation *rel = update_relations[0];
             long long left = (long long)max_iteration + 1 - iteration_counter;
