@@ -402,8 +402,10 @@ constexpr u32 kMList = 128;  // walked lists collected per wave and round
 constexpr int kBatch = 4;
 constexpr u32 kNoBit = 0xffffffffu;
 
+// Sets bit y of an LDS bit row: a 32-bit OR (a 64-bit one spans two banks,
+// so lanes setting random bits collide on banks about twice as often).
 __device__ __forceinline__ void lds_set(u64 *row, u32 y) {
-    atomicOr((unsigned long long *)&row[y >> 6], 1ull << (y & 63));
+    atomicOr(reinterpret_cast<u32 *>(row) + (y >> 5), 1u << (y & 31));
 }
 
 template <int K>
@@ -1306,8 +1308,10 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
         return fail(ctx, GDL_ERR_INVALID_ARG, "dense bodies: a delta key outside the set's domain");
 
     u64 end = *log_end;
-    // room for bodies about twice the delta (a larger one grows the log first)
-    GDL_TRY(grow_log(ctx, d_log, log_cap, end, end + std::max<u64>(2 * n_delta, 4096)));
+    // room for the next bodies at about 4x the delta (a dense delta tends to
+    // grow; a body that does not fit grows the log first, at the cost of a host
+    // wait and a copy of the log)
+    GDL_TRY(grow_log(ctx, d_log, log_cap, end, end + std::max<u64>(4 * n_delta, 4096)));
     const unsigned grid =
         (unsigned)std::max<u64>(1, std::min<u64>((a.n_items + DWV - 1) / DWV, grid_cap(ctx)));
     int done = 0, ramp = 4;
