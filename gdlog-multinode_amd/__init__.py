@@ -125,7 +125,7 @@ class Tuning(Structure):
 
 class TcbConfig(Structure):
     _fields_ = [("max_slots", c_uint32), ("node_bits", c_uint32), ("mem_budget", c_uint64),
-                ("keep_keys", c_int), ("solo_tuples", c_uint64)]
+                ("keep_keys", c_int), ("solo_tuples", c_uint64), ("solo_threads", c_uint32)]
 
 
 class TcbResult(Structure):
@@ -1040,7 +1040,7 @@ class TcBatch:
     """Many TC fixpoints in one disjoint union (include/gdl_batch.h)."""
 
     def __init__(self, ctx, max_slots=256, node_bits=16, mem_budget=0, keep_keys=True,
-                 solo_tuples=None):
+                 solo_tuples=None, solo_threads=None):
         """solo_tuples: instances estimated at this many tuples or more run alone at
         admission (None: the library default; 0: all; ~0: none)."""
         self.ctx = ctx
@@ -1050,6 +1050,8 @@ class TcBatch:
         cfg.keep_keys = int(keep_keys)
         if solo_tuples is not None:
             cfg.solo_tuples = int(solo_tuples) & 0xffffffffffffffff
+        if solo_threads is not None:
+            cfg.solo_threads = int(solo_threads)
         h = c_void_p()
         ctx._check(ctx.eng.gdl_tcb_create(ctx.h, ctypes.byref(cfg), ctypes.byref(h)),
                    "tcb_create")

@@ -15,9 +15,13 @@
 // histogram and two counters come back to the host.
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
 
 #include "gdl_batch.h"
 #include "gdl_internal.hpp"
@@ -181,10 +185,28 @@ struct gdl_tcb {
     gdl_hs_seg *d_segs = nullptr, *h_segs = nullptr;  // h_segs: pinned
     u64 *d_hist = nullptr, *h_hist = nullptr;          // h_hist: pinned
     bool segs_dirty = true;
-    std::deque<TcbDone> done;
+    std::deque<TcbDone> done;  // guarded by mu
     u64 reserved = 0, budget = 0;
-    u64 bodies = 0, admitted = 0, finished = 0, solo = 0;
+    u64 bodies = 0, admitted = 0, finished = 0;  // finished: guarded by mu
     double t0 = 0;
+    // the solo lane: a thread with a context (and stream) of its own runs the
+    // queued instances one after another, beside the batch's bodies
+    struct SoloJob {
+        u64 qid;
+        std::vector<u64> edges;
+        u32 n_nodes;
+        double admit_ms;
+    };
+    std::mutex mu;
+    std::condition_variable cv;        // a job queued, or stop
+    std::condition_variable cv_done;   // a result ready
+    std::deque<SoloJob> solo_q;        // guarded by mu
+    u32 solo_running = 0;              // guarded by mu
+    gdl_status solo_status = GDL_OK;   // guarded by mu: the first failure
+    std::string solo_error;            // guarded by mu
+    bool stop = false;                 // guarded by mu
+    std::vector<gdl_ctx *> solo_ctx;
+    std::vector<std::thread> solo_threads;
 };
 
 namespace gdl {
@@ -309,10 +331,14 @@ static gdl_status retire(gdl_tcb *b, TcbInst *in) {
     b->segs_dirty = true;
     b->reserved -= std::min(b->reserved, in->reserved);
     d.r.done_ms = wall_ms() - b->t0;
-    b->done.push_back(std::move(d));
+    {
+        std::lock_guard<std::mutex> lock(b->mu);
+        b->done.push_back(std::move(d));
+        ++b->finished;
+    }
+    b->cv_done.notify_all();
     b->slots[in->slot].reset();
     --b->n_active;
-    ++b->finished;
     return GDL_OK;
 }
 
@@ -327,10 +353,8 @@ static gdl_status hip_ok(gdl_ctx *ctx, hipError_t e) {
 // bits), the first body from the first |FULL| raw rows, then the stream-
 // ordered linear loop (gdl_fixpoint_linear, with bit-matrix bodies while the
 // delta is dense).  Its result and trace are those of a batched instance.
-static gdl_status tcb_solo(gdl_tcb *b, u64 qid, const u64 *h_edges, u64 n, u32 n_nodes,
-                           double admit_ms) {
-    gdl_ctx *ctx = b->ctx;
-    const double t_admit = admit_ms;
+static gdl_status solo_run(gdl_ctx *ctx, u64 qid, const u64 *h_edges, u64 n, u32 n_nodes,
+                           int keep_keys, double t_admit, double t0, TcbDone *out) {
     int kb = 1;
     while (kb < 32 && (1ull << kb) < (u64)n_nodes) ++kb;
     gdl_hashset hs{};
@@ -410,7 +434,7 @@ static gdl_status tcb_solo(gdl_tcb *b, u64 qid, const u64 *h_edges, u64 n, u32 n
         for (const auto &t : trace) d.r.derived += t.delta;
         st = gdl_checksum_u64(ctx, log, d1, &d.r.checksum);
         if (st != GDL_OK) break;
-        if (b->cfg.keep_keys && d1) {
+        if (keep_keys && d1) {
             u64 *sorted;
             st = scr.get(d1, &sorted);
             if (st != GDL_OK) break;
@@ -430,16 +454,73 @@ static gdl_status tcb_solo(gdl_tcb *b, u64 qid, const u64 *h_edges, u64 n, u32 n
         st = gdl_hashset_check(ctx, &hs);
         if (st != GDL_OK) break;
         d.trace = std::move(trace);
-        d.r.done_ms = wall_ms() - b->t0;
-        b->done.push_back(std::move(d));
-        ++b->admitted;
-        ++b->finished;
-        ++b->solo;
+        d.r.done_ms = wall_ms() - t0;
+        *out = std::move(d);
     } while (false);
     if (idx.slots) gdl_index_free(ctx, &idx);
     if (log) ctx->alloc->free(log, nullptr);
     gdl_hashset_free(ctx, &hs);
     return st;
+}
+
+// The solo lane's thread: its own context on the batch's device (the
+// batch's tuning), jobs in admission order, results into the batch's done
+// queue.  The first failure stops the lane; gdl_tcb_step reports it.
+static void solo_main(gdl_tcb *b, gdl_ctx *ctx) {
+    (void)hipSetDevice(ctx->device);  // this thread's calls go to the batch's device
+    for (;;) {
+        gdl_tcb::SoloJob job;
+        {
+            std::unique_lock<std::mutex> lock(b->mu);
+            b->cv.wait(lock, [&] { return b->stop || !b->solo_q.empty(); });
+            if (b->stop && b->solo_q.empty()) return;
+            job = std::move(b->solo_q.front());
+            b->solo_q.pop_front();
+        }
+        TcbDone d;
+        gdl_status st = solo_run(ctx, job.qid, job.edges.data(), job.edges.size(), job.n_nodes,
+                                 b->cfg.keep_keys, job.admit_ms, b->t0, &d);
+        {
+            std::lock_guard<std::mutex> lock(b->mu);
+            --b->solo_running;
+            if (st == GDL_OK) {
+                b->done.push_back(std::move(d));
+                ++b->finished;
+            } else if (b->solo_status == GDL_OK) {
+                b->solo_status = st;
+                b->solo_error = ctx->last_error;
+            }
+        }
+        b->cv_done.notify_all();
+    }
+}
+
+static gdl_status solo_enqueue(gdl_tcb *b, u64 qid, const u64 *h_edges, u64 n, u32 n_nodes) {
+    if (b->solo_threads.empty()) {
+        // the lane's threads, each with a context of its own on the batch's
+        // device (its tuning and dense mode): two big instances at once fill
+        // the device through each other's host waits and latency-bound bodies
+        gdl_config cfg = b->ctx->cfg;
+        cfg.mem_cap_bytes = 0;
+        for (u32 t = 0; t < b->cfg.solo_threads; ++t) {
+            gdl_ctx *c = nullptr;
+            GDL_TRY(gdl_ctx_create(&cfg, &c));
+            c->tune = b->ctx->tune;
+            c->dense_mode = b->ctx->dense_mode;
+            b->solo_ctx.push_back(c);
+        }
+        for (gdl_ctx *c : b->solo_ctx) b->solo_threads.emplace_back(solo_main, b, c);
+    }
+    gdl_tcb::SoloJob job{qid, std::vector<u64>(h_edges, h_edges + n), n_nodes,
+                         wall_ms() - b->t0};
+    {
+        std::lock_guard<std::mutex> lock(b->mu);
+        b->solo_q.push_back(std::move(job));
+        ++b->solo_running;
+        ++b->admitted;
+    }
+    b->cv.notify_one();
+    return GDL_OK;
 }
 
 static gdl_status tcb_admit(gdl_tcb *b, u64 qid, const u64 *h_edges, u64 n, u32 n_nodes,
@@ -450,9 +531,12 @@ static gdl_status tcb_admit(gdl_tcb *b, u64 qid, const u64 *h_edges, u64 n, u32 
     for (u64 i = 0; i < n; ++i)
         if ((h_edges[i] >> 32) >= n_nodes || (h_edges[i] & 0xffffffffull) >= n_nodes)
             return fail(ctx, GDL_ERR_INVALID_ARG, "batch: edge id >= n_nodes");
-    if (est == 0) est = gdl_tc_estimate(h_edges, n, n_nodes, 32, qid);
-    if (n && est >= b->cfg.solo_tuples)
-        return tcb_solo(b, qid, h_edges, n, n_nodes, wall_ms() - b->t0);
+    // 8 sampled sources: admission and the solo decision need the order of
+    // magnitude; each sample is a breadth-first search on the host (32 of them
+    // cost the 256-instance workload about as much as its small instances' GPU
+    // work)
+    if (est == 0) est = gdl_tc_estimate(h_edges, n, n_nodes, 8, qid);
+    if (n && est >= b->cfg.solo_tuples) return solo_enqueue(b, qid, h_edges, n, n_nodes);
     u32 slot = 0;
     while (slot < b->cfg.max_slots && b->slots[slot]) ++slot;
     if (slot == b->cfg.max_slots) return GDL_ERR_BUSY;
@@ -515,6 +599,19 @@ static gdl_status tcb_admit(gdl_tcb *b, u64 qid, const u64 *h_edges, u64 n, u32 
 
 static gdl_status tcb_step(gdl_tcb *b, u32 *n_active_out) {
     gdl_ctx *ctx = b->ctx;
+    {
+        std::unique_lock<std::mutex> lock(b->mu);
+        if (b->solo_status != GDL_OK)
+            return fail(ctx, b->solo_status, "batch solo lane: " + b->solo_error);
+        // nothing to run here: wait (at most 1 ms) for a solo result instead
+        // of returning to a caller that polls in a loop
+        if (b->n_active == 0 && b->solo_running > 0 && b->done.empty())
+            b->cv_done.wait_for(lock, std::chrono::milliseconds(1));
+    }
+    if (b->n_active == 0) {
+        *n_active_out = 0;
+        return GDL_OK;
+    }
     if (b->e_dirty) GDL_TRY(rebuild_edges(b));
     // every instance's worst case: |delta| x longest run, at most the pairs
     // not yet derived; when that is loose, the exact produced count per
@@ -643,6 +740,7 @@ void gdl_tcb_config_default(gdl_tcb_config *cfg) {
     cfg->mem_budget = 0;
     cfg->keep_keys = 1;
     cfg->solo_tuples = 1ull << 23;
+    cfg->solo_threads = 2;
 }
 
 uint64_t gdl_tc_estimate(const uint64_t *h_edges, uint64_t n, uint32_t n_nodes, uint32_t samples,
@@ -696,8 +794,9 @@ gdl_status gdl_tcb_create(gdl_ctx *ctx, const gdl_tcb_config *cfg_in, gdl_tcb **
     if (cfg_in) cfg = *cfg_in;
     else gdl_tcb_config_default(&cfg);
     if (cfg.max_slots == 0 || cfg.max_slots > kMaxSegs || cfg.node_bits < 1 ||
-        cfg.node_bits > 30)
-        return fail(ctx, GDL_ERR_INVALID_ARG, "batch: max_slots 1..256, node_bits 1..30");
+        cfg.node_bits > 30 || cfg.solo_threads < 1 || cfg.solo_threads > 8)
+        return fail(ctx, GDL_ERR_INVALID_ARG,
+                    "batch: max_slots 1..256, node_bits 1..30, solo_threads 1..8");
     // slot ids must fit the bits above node_bits, and the all-ones slot is
     // kept free so no tuple key is ~0 (the hash sets' empty marker)
     const u64 slot_room = 1ull << (32 - cfg.node_bits);
@@ -733,6 +832,15 @@ gdl_status gdl_tcb_create(gdl_ctx *ctx, const gdl_tcb_config *cfg_in, gdl_tcb **
 gdl_status gdl_tcb_destroy(gdl_tcb *b) {
     if (!b) return GDL_ERR_INVALID_ARG;
     gdl_ctx *ctx = b->ctx;
+    if (!b->solo_threads.empty()) {
+        {
+            std::lock_guard<std::mutex> lock(b->mu);
+            b->stop = true;
+        }
+        b->cv.notify_all();
+        for (auto &t : b->solo_threads) t.join();  // the queued solo instances finish first
+    }
+    for (gdl_ctx *c : b->solo_ctx) gdl_ctx_destroy(c);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &p : b->slots) {
         if (!p) continue;
@@ -764,13 +872,16 @@ gdl_status gdl_tcb_step(gdl_tcb *b, uint32_t *n_active) {
 }
 
 int gdl_tcb_poll(gdl_tcb *b, gdl_tcb_result *out) {
-    if (!b || !out || b->done.empty()) return 0;
+    if (!b || !out) return 0;
+    std::lock_guard<std::mutex> lock(b->mu);
+    if (b->done.empty()) return 0;
     *out = b->done.front().r;
     return 1;
 }
 
 gdl_status gdl_tcb_fetch(gdl_tcb *b, uint64_t qid, uint64_t *h_keys, gdl_trace_rec *h_trace) {
     if (!b) return GDL_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lock(b->mu);
     for (auto it = b->done.begin(); it != b->done.end(); ++it) {
         if (it->r.qid != qid) continue;
         if (h_keys && !it->keys.empty())
@@ -786,6 +897,7 @@ gdl_status gdl_tcb_fetch(gdl_tcb *b, uint64_t qid, uint64_t *h_keys, gdl_trace_r
 gdl_status gdl_tcb_info(gdl_tcb *b, uint64_t *bodies, uint64_t *admitted, uint64_t *finished,
                         uint64_t *reserved_bytes, uint64_t *budget_bytes) {
     if (!b) return GDL_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lock(b->mu);
     if (bodies) *bodies = b->bodies;
     if (admitted) *admitted = b->admitted;
     if (finished) *finished = b->finished;

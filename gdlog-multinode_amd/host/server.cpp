@@ -90,7 +90,7 @@ uint64_t estimate_tuples(int kind, const uint64_t *a, uint64_t na, const uint64_
     const uint64_t ids = std::max(max_id(a, na), nb ? max_id(b, nb) : 0) + 1;
     *n_nodes = (uint32_t)std::min<uint64_t>(ids, 0xffffffffull);
     const double nn = (double)ids * (double)ids;
-    if (kind == GDL_QUERY_TC) return gdl_tc_estimate(a, na, (uint32_t)ids, 32, 0x5eed);
+    if (kind == GDL_QUERY_TC) return gdl_tc_estimate(a, na, (uint32_t)ids, 8, 0x5eed);
     if (kind == GDL_QUERY_SG) {
         // generation pairs of the first rule (children of one parent), grown
         std::map<uint64_t, uint64_t> deg;
@@ -312,6 +312,7 @@ struct gdl_server {
             bc.node_bits = cfg.node_bits;
             bc.mem_budget = ~0ull >> 1;  // the server accounts for memory
             bc.keep_keys = cfg.keep_keys;
+            bc.solo_tuples = ~0ull;  // large queries go to the server's workers
             st = gdl_tcb_create(ctx, &bc, &b);
         }
         {
@@ -377,6 +378,7 @@ struct gdl_server {
                 bc.node_bits = cfg.node_bits;
                 bc.mem_budget = ~0ull >> 1;
                 bc.keep_keys = cfg.keep_keys;
+                bc.solo_tuples = ~0ull;
                 if (gdl_tcb_create(ctx, &bc, &b) != GDL_OK) break;
                 continue;
             }

@@ -41,8 +41,9 @@ typedef struct gdl_tcb_config {
     int keep_keys;       /* 1: a finished instance's sorted tuples are copied to the
                             host; 0: only its size and checksum */
     uint64_t solo_tuples; /* an instance whose |TC| estimate is at least this runs
-                             alone, to its fixpoint, when admitted (default 2^23;
-                             ~0: never) */
+                             in the solo lane (default 2^23; ~0: never) */
+    uint32_t solo_threads; /* the solo lane's threads, each with a context and
+                              stream of its own (1..8, default 2) */
 } gdl_tcb_config;
 
 typedef struct gdl_tcb_result {
@@ -70,19 +71,24 @@ uint64_t gdl_tc_estimate(const uint64_t *h_edges, uint64_t n, uint32_t n_nodes, 
 
 /* Admits an instance: n raw edge rows (host, packed keys, ids < n_nodes, any
  * order, repeats allowed).  est_tuples: the footprint estimate of its |TC|
- * (0: gdl_tc_estimate with 32 samples).  GDL_ERR_BUSY when no slot is free
+ * (0: gdl_tc_estimate with 8 samples).  GDL_ERR_BUSY when no slot is free
  * or the estimate does not fit the budget next to the instances admitted
  * (an instance is always admitted into an empty batch).  The instance's first
  * body is the batch's next one.  An instance estimated at cfg.solo_tuples or
- * more (one that fills the device by itself) instead runs to its fixpoint
- * here, on the single-query engine path (the linear loop, with bit-matrix
- * bodies while its delta is dense); its result (slot -1) is then ready to
- * poll, with the same tuples, checksum and trace. */
+ * more (one that fills the device by itself) instead joins the solo lane:
+ * threads of the batch with a context (and stream) of their own run such
+ * instances, beside the batch's bodies and each other, on the single-query
+ * engine path (the linear loop, with bit-matrix bodies while the delta is
+ * dense).  Their results (slot -1) come to gdl_tcb_poll with the same tuples,
+ * checksum and trace; a failure there is reported by the next gdl_tcb_step.
+ * gdl_tcb_destroy waits for the lane's queued instances. */
 gdl_status gdl_tcb_admit(gdl_tcb *b, uint64_t qid, const uint64_t *h_edges, uint64_t n,
                          uint32_t n_nodes, uint64_t est_tuples);
 
 /* Runs one body over every admitted instance and retires those whose delta
- * came out empty.  *n_active: instances still running after it. */
+ * came out empty.  *n_active: instances still running after it (the solo
+ * lane's not counted; with none running here and solo instances pending, it
+ * waits up to 1 ms for a solo result). */
 gdl_status gdl_tcb_step(gdl_tcb *b, uint32_t *n_active);
 
 /* Oldest finished, unfetched instance: 1 and *out filled, or 0. */
