@@ -720,7 +720,7 @@ def main():
                     help="instances of the batched-TC measurement (0: skip it)")
     ap.add_argument("--served-requests", type=int, default=1000,
                     help="requests of the served-mix measurement (0: skip it)")
-    ap.add_argument("--served-workers", type=int, default=2,
+    ap.add_argument("--served-workers", type=int, default=4,
                     help="program workers of the query server beside its TC batch")
     ap.add_argument("--datalog-reps", type=int, default=3,
                     help="timed runs of the Datalog front-end and path-length "
