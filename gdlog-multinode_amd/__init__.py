@@ -612,7 +612,8 @@ class Context:
 
     def set_dense_mode(self, mode):
         """0: no bit-matrix bodies; 1: when the cost model prefers them (default);
-        2: whenever they apply."""
+        2: whenever they apply; 3 / 4: as 2, with SG's second join always / never
+        expanded by columns (two bit-matrix transposes around a row OR)."""
         self._check(self.eng.gdl_set_dense_mode(self.h, int(mode)), "set_dense_mode")
 
     def hashset_extract_sorted(self, hs):

@@ -17,7 +17,7 @@ const char *kKernelNames[K_NUM] = {
     "hs_rehash",    "join_insert",  "hs_extract",   "seg_hist",      "mask_cols",
     "linear_body",  "segment_labels", "scatter_by_key", "dense_setup", "dense_expand",
     "dense_combine", "dense_append",
-    "bm_extract_sorted"};
+    "bm_extract_sorted", "dense_transpose", "dense_or_rows"};
 
 // ---------------------------------------------------------------- allocator
 
@@ -137,6 +137,53 @@ gdl_status read_u64(gdl_ctx *ctx, const u64 *d, u64 *h, int count) {
     return GDL_OK;
 }
 
+Staging::~Staging() {
+    if (used_) (void)hipStreamSynchronize(ctx_->stream);
+}
+
+gdl_status Staging::sync() {
+    GDL_HIP(ctx_, hipStreamSynchronize(ctx_->stream));
+    used_ = 0;
+    return GDL_OK;
+}
+
+gdl_status Staging::room(size_t n, char **at) {
+    const size_t a = (n + 255) & ~(size_t)255;
+    if (used_ + a > ctx_->h_stage_cap) {
+        GDL_TRY(sync());  // uploads in flight read the buffer
+        if (a > ctx_->h_stage_cap) {
+            const size_t cap = std::max({a, 2 * ctx_->h_stage_cap, (size_t)1 << 20});
+            if (ctx_->h_stage) (void)hipHostFree(ctx_->h_stage);
+            ctx_->h_stage = nullptr;
+            ctx_->h_stage_cap = 0;
+            GDL_HIP(ctx_, hipHostMalloc((void **)&ctx_->h_stage, cap, hipHostMallocDefault));
+            ctx_->h_stage_cap = cap;
+        }
+    }
+    *at = ctx_->h_stage + used_;
+    used_ += a;
+    return GDL_OK;
+}
+
+gdl_status Staging::up(void *d_dst, const void *h_src, size_t n) {
+    if (!n) return GDL_OK;
+    char *at;
+    GDL_TRY(room(n, &at));
+    memcpy(at, h_src, n);
+    GDL_HIP(ctx_, hipMemcpyAsync(d_dst, at, n, hipMemcpyHostToDevice, ctx_->stream));
+    return GDL_OK;
+}
+
+gdl_status Staging::down(void *h_dst, const void *d_src, size_t n) {
+    if (!n) return GDL_OK;
+    char *at;
+    GDL_TRY(room(n, &at));
+    GDL_HIP(ctx_, hipMemcpyAsync(at, d_src, n, hipMemcpyDeviceToHost, ctx_->stream));
+    GDL_TRY(sync());
+    memcpy(h_dst, at, n);
+    return GDL_OK;
+}
+
 gdl_status check_device_error(gdl_ctx *ctx) {
     u32 err = 0;
     GDL_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, ctx->d_err, sizeof(u32), hipMemcpyDeviceToHost,
@@ -251,7 +298,7 @@ gdl_status gdl_ctx_create(const gdl_config *cfg_in, gdl_ctx **out) {
     ctx->stats_on = cfg.collect_stats != 0;
     if (const char *e = getenv("GDLOG_DENSE")) {
         const int m = atoi(e);
-        ctx->dense_mode = m < 0 ? 0 : (m > 2 ? 2 : m);
+        ctx->dense_mode = m < 0 ? 0 : (m > 4 ? 4 : m);
     }
     ctx->alloc = new Allocator(cfg.device);
     ctx->alloc->set_cap(cfg.mem_cap_bytes);
@@ -286,6 +333,7 @@ gdl_status gdl_ctx_destroy(gdl_ctx *ctx) {
     delete ctx->alloc;
     if (ctx->d_err) (void)hipFree(ctx->d_err);
     if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
+    if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return GDL_OK;

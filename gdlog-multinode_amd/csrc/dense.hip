@@ -64,37 +64,33 @@ __device__ __forceinline__ int select_bit(u64 w, int j) {
     return pos;
 }
 
-// Writes the keys of one word per lane (lane l: word wd, bit i -> key of
-// bit number bitbase + i, (row << 32) | column with kb-bit columns) to
-// out[base ..), in lane then bit order, with every store coalesced: lane t
-// writes positions t, t + 64, ... and finds the word and bit it owns by a
-// binary search over the lanes' exclusive counts.  Returns the word keys'
-// total (the same in every lane).
-__device__ __forceinline__ u64 coop_write(u64 wd, u64 bitbase, int kb, u64 *out, u64 base,
-                                          u64 out_cap) {
+// Writes the keys of one round of 64 consecutive words, one per lane (lane
+// l: word wd, bit i -> key of bit number bitbase + 64 l + i, (row << 32) |
+// column with kb-bit columns) to out[base ..), in bit order.  Each lane puts
+// the round-local numbers of its bits in the wave's LDS list (buf, 4096
+// entries) from its exclusive count on; the wave then stores the keys with
+// consecutive lanes on consecutive positions.  Returns the round's total
+// (the same in every lane).
+__device__ __forceinline__ u64 stage_write(u64 wd, u64 bitbase, int kb, u64 *out, u64 base,
+                                           u64 out_cap, uint16_t *buf) {
     const int lane = threadIdx.x & 63;
     const u64 c = (u64)__popcll(wd);
     const u64 incl = wave_inclusive_scan(c);
-    const u64 ex = incl - c;
     const u64 total = __shfl(incl, 63, 64);
-    const u64 lo_mask = (1ull << kb) - 1;
-    for (u64 t0 = 0; t0 < total; t0 += 64) {
-        const u64 t = t0 + lane;
-        int lo = 0, hi = 63;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            const u64 em = __shfl(ex, mid, 64);
-            if (em <= t) lo = mid;
-            else hi = mid - 1;
-        }
-        const u64 word = __shfl(wd, lo, 64);
-        const u64 bb = __shfl(bitbase, lo, 64);
-        const u64 e0 = __shfl(ex, lo, 64);
-        if (t < total) {
-            const u64 bit = bb + (u64)select_bit(word, (int)(t - e0));
-            if (base + t < out_cap) out[base + t] = ((bit >> kb) << 32) | (bit & lo_mask);
-        }
+    if (total == 0) return 0;
+    u32 at = (u32)(incl - c);
+    const u32 l64 = (u32)lane << 6;
+    while (wd) {
+        buf[at++] = (uint16_t)(l64 | (u32)(__ffsll((unsigned long long)wd) - 1));
+        wd &= wd - 1;
     }
+    wave_sync_lds();
+    const u64 lo_mask = (1ull << kb) - 1;
+    for (u64 t = lane; t < total; t += 64) {
+        const u64 bit = bitbase + buf[t];
+        if (base + t < out_cap) out[base + t] = ((bit >> kb) << 32) | (bit & lo_mask);
+    }
+    wave_sync_lds();  // the list is rewritten by the next round
     return total;
 }
 
@@ -105,12 +101,12 @@ __device__ __forceinline__ u64 coop_write(u64 wd, u64 bitbase, int kb, u64 *out,
 // tile's keys in bit order with a wave prefix sum per round (consecutive
 // words, consecutive output ranges; no block barrier).  With `minus`, rows
 // whose minus_pop entry is non-zero have minus's bits removed.
-// rounds per tile: 16, fewer when that would leave fewer than 2^18 tiles
+// rounds per tile: 16, fewer when that would leave fewer than 2^16 tiles
 static int xt_rounds(u64 n_words) {
     // small tiles: a clustered bitmap has tiles far denser than the rest,
     // and many small ones let the scheduler spread them
     int r = 16;
-    while (r > 1 && n_words / (64ull * r) < (1ull << 18)) r >>= 1;
+    while (r > 1 && n_words / (64ull * r) < (1ull << 16)) r >>= 1;
     return r;
 }
 
@@ -142,15 +138,16 @@ __global__ __launch_bounds__(DB) void k_xt_write(const u64 *words, u64 n_words, 
                                                  const u32 *minus_pop, int rs, int kb,
                                                  const u64 *tile_offsets, u64 *out,
                                                  u64 out_cap) {
+    __shared__ uint16_t s_buf[DWV][4096];
     const u64 tile = (u64)blockIdx.x * DWV + (threadIdx.x >> 6);
     if (tile >= n_tiles) return;  // the same for the whole wave
     const int lane = threadIdx.x & 63;
     const u64 t0 = tile * 64ull * rounds;
     u64 base = tile_offsets[tile];
     for (int r = 0; r < rounds; ++r) {
-        const u64 i = t0 + (u64)r * 64 + lane;
+        const u64 r0 = t0 + (u64)r * 64, i = r0 + lane;
         const u64 w = i < n_words ? xt_word(words, minus, minus_pop, rs, i) : 0ull;
-        base += coop_write(w, i * 64, kb, out, base, out_cap);
+        base += stage_write(w, r0 * 64, kb, out, base, out_cap, s_buf[threadIdx.x >> 6]);
     }
 }
 
@@ -200,6 +197,9 @@ struct DenseState {
     unsigned long long rows_io;   // the combine's row reads + writes of the running body
     unsigned long long exp_io;    // the expansion's row reads + writes
     unsigned long long rows_new;  // rows that got new tuples
+    unsigned long long tr_io;     // SG column expansion: 64-byte segments the transposes moved
+    unsigned long long or_io;     // and the row reads + writes of its row OR
+    u64 col_above;     // SG: a body whose delta has at least this many tuples expands by columns
     u64 log_cap;
     u64 sparse_below;  // a body with fewer new tuples ends the run
     u64 append_base;   // the log end before the last body's append
@@ -207,22 +207,21 @@ struct DenseState {
     u32 appended;       // bodies whose new tuples are in the log
     u32 append_blocks;  // blocks of the running append that finished
     u32 bad;            // a delta key outside the domain
-    u32 pad_;
+    u32 col;            // SG: the running body expands by columns
 };
 enum : u32 { kDnRun = 0, kDnFixpoint = 1, kDnGrow = 2, kDnLimit = 3, kDnSparse = 4 };
 // per body: new tuples, produced tuples, combine rows, expansion rows, rows
-// with new tuples
-constexpr int kCounts = 5;
+// with new tuples, column expansion: transposed segments, OR rows
+constexpr int kCounts = 7;
 
 // The rule's static relations as row lists.  pred: r1 grouped by its second
 // column (for row x, the y with r1(y, x)); act: the rows with a non-empty
 // pred list (the only rows a body can write); items: the combine's work,
 // (x, first pred, end pred, split index or ~0), rows with more than kSlice
 // preds split into slices (most preds first); per split row the slice count,
-// a finished-slice counter and accumulators (NEW, and SG's T).  SG: exp: the
-// rows that are some row's pred (the rows U is built for); succ: r2 grouped
-// by its first column, and succ_tab its per-row entries; planes: bit-planes
-// of r2's row lengths (bit b of plane j = bit j of |r2 row b|).
+// a finished-slice counter and accumulators (NEW, and SG's T).  SG: succ: r2
+// grouped by its first column, and succ_tab its per-row entries; planes:
+// bit-planes of r2's row lengths (bit b of plane j = bit j of |r2 row b|).
 constexpr u32 kSlice = 32;
 
 struct DenseArgs {
@@ -235,8 +234,7 @@ struct DenseArgs {
     const u32 *n_slices;
     u32 *slice_done;
     u64 *GU, *GT;
-    const u32 *exp_rows;
-    u64 n_exp;
+    u64 n_exp;  // SG: the rows U is built for (0 .. n_exp - 1, those with a delta row)
     const u64 *succ_off;
     const u32 *succ_idx;
     const uint4 *succ_tab;  // {deg, successors} (deg <= 3) or {deg, offset lo, hi, bit row}
@@ -256,6 +254,8 @@ __device__ void dense_epilogue(DenseState *st, u64 *counts) {
     const u64 io = atomicAdd(&st->rows_io, 0ull);
     const u64 eio = atomicAdd(&st->exp_io, 0ull);
     const u64 rn = atomicAdd(&st->rows_new, 0ull);
+    const u64 tio = atomicAdd(&st->tr_io, 0ull);
+    const u64 oio = atomicAdd(&st->or_io, 0ull);
     const u32 b = st->body;
     u64 *c = counts + kCounts * b;
     c[0] = nn;
@@ -263,11 +263,16 @@ __device__ void dense_epilogue(DenseState *st, u64 *counts) {
     c[2] = io;
     c[3] = eio;
     c[4] = rn;
+    c[5] = tio;
+    c[6] = oio;
     st->n_new = 0;
     st->raw = 0;
     st->rows_io = 0;
     st->exp_io = 0;
     st->rows_new = 0;
+    st->tr_io = 0;
+    st->or_io = 0;
+    st->col = nn >= st->col_above ? 1u : 0u;  // how the next body expands
     st->blocks_done = 0;
     st->append_base = st->cursor;
     u32 stop = kDnRun;
@@ -422,11 +427,11 @@ __global__ __launch_bounds__(DB) void k_dense_expand(DenseArgs a, const u32 *pop
     __shared__ u32 s_mx[DWV][kMList + 1];  // and the exclusive sums of their lengths
     __shared__ u32 s_mn[DWV];
     __shared__ u64 s_cnt[DWV + 1];
-    if (st->stop != kDnRun) return;  // the same for every block
+    if (st->stop != kDnRun || st->col) return;  // the same for every block
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     u64 io = 0;
     for (u64 i = blockIdx.x; i < a.n_exp; i += gridDim.x) {
-        const u32 r = a.exp_rows[i];
+        const u32 r = (u32)i;
         if (pop_in[r] == 0) {  // the same for the whole block
             if (threadIdx.x == 0) upop[r] = 0;
             continue;
@@ -687,6 +692,7 @@ __global__ __launch_bounds__(DB) void k_dense_append(DenseArgs a, const u32 *pop
     __shared__ u64 s_w[DWV + 1];
     __shared__ unsigned long long s_base;
     __shared__ bool s_last;
+    __shared__ uint16_t s_buf[DWV][4096];
     const u32 body = st->body;
     if (st->appended >= body) return;
     const u64 nn = counts[kCounts * (body - 1)];
@@ -722,9 +728,9 @@ __global__ __launch_bounds__(DB) void k_dense_append(DenseArgs a, const u32 *pop
     for (u64 i = w0; i < w1; ++i) {
         const u32 x = a.act[i / cpr];
         if (pop[x] == 0) continue;
-        const u64 w = (i % cpr) * 64 + lane;
+        const u64 w0 = (i % cpr) * 64, w = w0 + lane;
         const u64 v = w < a.rw ? D[(u64)x * a.rw + w] : 0ull;
-        at += coop_write(v, ((u64)x << kb) + w * 64, kb, a.log, at, ~0ull);
+        at += stage_write(v, ((u64)x << kb) + w0 * 64, kb, a.log, at, ~0ull, s_buf[wv]);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -736,6 +742,205 @@ __global__ __launch_bounds__(DB) void k_dense_append(DenseArgs a, const u32 *pop
         __threadfence();
         st->append_blocks = 0;
         st->appended = body;
+    }
+}
+
+// ---- SG's column expansion.  U = D R2 (R2 the bit matrix of r2) is also
+// (R2^T D^T)^T: transpose D, OR the rows of D^T over each column's r2 pred
+// list, transpose back.  That is a fixed few passes over the bit matrix plus
+// |r2| row reads, whatever the delta's size, where the expansion above
+// costs a successor-table probe per delta bit and an LDS OR per successor:
+// large deltas expand by columns (DenseState::col, set by the previous
+// body's epilogue from its new-tuple count), small ones bit by bit.
+
+__device__ __forceinline__ u64 shfl_xor_u64(u64 v, int m) {
+    const int lo = __shfl_xor((int)(u32)v, m, 64);
+    const int hi = __shfl_xor((int)(u32)(v >> 32), m, 64);
+    return ((u64)(u32)hi << 32) | (u64)(u32)lo;
+}
+
+// Transposes the 64 x 64 bit block held a row per lane (bit c of lane r =
+// element (r, c)): afterwards lane c holds column c (bit r = element (r, c)).
+// Six block swaps (the recursive transpose of Hacker's Delight, 7-3): at
+// step s the elements whose row and column differ in bit s trade places.
+__device__ __forceinline__ u64 transpose64(u64 v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const int s = 32 >> i;
+        const u64 m = i == 0   ? 0x00000000ffffffffull
+                      : i == 1 ? 0x0000ffff0000ffffull
+                      : i == 2 ? 0x00ff00ff00ff00ffull
+                      : i == 3 ? 0x0f0f0f0f0f0f0f0full
+                      : i == 4 ? 0x3333333333333333ull
+                               : 0x5555555555555555ull;  // the columns with bit s clear
+        const u64 x = shfl_xor_u64(v, s);
+        v = (lane & s) ? (((x >> s) & m) | (v & ~m)) : ((v & m) | ((x & m) << s));
+    }
+    return v;
+}
+
+constexpr int TB = 512;  // transpose block: 8 waves
+constexpr int kTW = 8;   // a tile is 64 kTW rows x kTW words (one 64-bit block per wave and word)
+
+// dst = src^T for the 2^kb x 2^kb bit matrix src (rows of rw words), a tile
+// per block (grid (rw / kTW)^2).  Wave w loads rows 64 w .. 64 w + 63 of the
+// tile, 64 contiguous bytes a lane, and transposes its kTW bit blocks in
+// registers; LDS regroups them so that each output row's kTW words (64
+// contiguous bytes) are stored by one lane.  Rows with src_pop 0 read as
+// zero (their words may be stale).  dst_pop[r] += the bits stored in row r
+// (dst_pop starts at zero); zero_pop[0, n_zero) is cleared on the way.
+__global__ __launch_bounds__(TB) void k_bt_transpose(const u64 *src, const u32 *src_pop, u64 rw,
+                                                     u64 *dst, u32 *dst_pop, u32 *zero_pop,
+                                                     u64 n_zero, DenseState *st) {
+    __shared__ u64 s_t[kTW][64][kTW + 1];
+    __shared__ u32 s_live[TB / 64];
+    if (st->stop != kDnRun || !st->col) return;  // the same for every block
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (u64 i = (u64)blockIdx.x * TB + threadIdx.x; i < n_zero; i += (u64)gridDim.x * TB)
+        zero_pop[i] = 0;
+    const u64 tpd = rw / kTW;
+    const u64 ti = blockIdx.x / tpd, tj = blockIdx.x % tpd;
+    const u64 row = ti * (64 * kTW) + (u64)wv * 64 + lane;
+    u64 v[kTW];
+    const bool live = src_pop == nullptr || src_pop[row] != 0;
+    if (live) {
+        const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(src + row * rw + tj * kTW);
+#pragma unroll
+        for (int j = 0; j < kTW / 2; ++j) {
+            const ulonglong2 q = p[j];
+            v[2 * j] = q.x;
+            v[2 * j + 1] = q.y;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kTW; ++j) v[j] = 0;
+    }
+    const u64 lm = __ballot(live);
+    if (lane == 0) s_live[wv] = (u32)__popcll(lm);
+#pragma unroll
+    for (int j = 0; j < kTW; ++j)
+        s_t[j][lane][wv] = __ballot(v[j] != 0) ? transpose64(v[j]) : 0ull;
+    __syncthreads();
+    // wave w stores bit block w of every row tile: output rows 64 w + lane
+    const u64 orow = tj * (64 * kTW) + (u64)wv * 64 + lane;
+    u64 o[kTW];
+    u32 c = 0;
+#pragma unroll
+    for (int t = 0; t < kTW; ++t) {
+        o[t] = s_t[wv][lane][t];
+        c += (u32)__popcll(o[t]);
+    }
+    ulonglong2 *q = reinterpret_cast<ulonglong2 *>(dst + orow * rw + ti * kTW);
+#pragma unroll
+    for (int j = 0; j < kTW / 2; ++j) q[j] = make_ulonglong2(o[2 * j], o[2 * j + 1]);
+    if (dst_pop && c) atomicAdd(&dst_pop[orow], c);
+    if (threadIdx.x == 0) {
+        u64 segs = TB;  // the stored ones
+        for (int w = 0; w < TB / 64; ++w) segs += s_live[w];
+        atomicAdd(&st->tr_io, (unsigned long long)segs);
+    }
+}
+
+// One static relation's rows as OR work: row x's sources are idx[off[x],
+// off[x + 1]); items as the combine's (x, first, end, split index or ~0),
+// with the split rows' slice counts, finished-slice counters and
+// accumulators G.
+struct OrRows {
+    const u64 *off;
+    const u32 *idx;
+    const uint4 *items;
+    u64 n_items;
+    const u32 *n_slices;
+    u32 *slice_done;
+    u64 *G;
+};
+
+// dst[x] = OR of src[b] over x's sources b (sources with src_pop 0 skipped),
+// dst_pop[x] = |dst[x]|, a wave per item (rows written only when some source
+// is not empty: a reader skips rows with pop 0).
+template <int K>
+__global__ __launch_bounds__(DB) void k_dense_or_rows(OrRows o, u64 rw, const u32 *src_pop,
+                                                      const u64 *src, u32 *dst_pop, u64 *dst,
+                                                      DenseState *st) {
+    __shared__ u64 s_io[DWV];
+    if (st->stop != kDnRun || !st->col) return;  // the same for every block
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const u64 wave = (u64)blockIdx.x * DWV + wv;
+    const u64 n_waves = (u64)gridDim.x * DWV;
+    u64 w_io = 0;
+    for (u64 i = wave; i < o.n_items; i += n_waves) {
+        const uint4 it = o.items[i];
+        const u32 x = it.x, g = it.w;
+        const u64 pb = o.off[x] + it.y, pe = o.off[x] + it.z;
+        u64 acc[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] = 0;
+        bool any = false;
+        for (u64 base = pb; base < pe; base += 64) {
+            const u64 j = base + lane;
+            u32 y = 0, p = 0;
+            if (j < pe) {
+                y = o.idx[j];
+                p = src_pop[y];
+            }
+            const u64 m = __ballot(p != 0);
+            if (m) any = true;
+            w_io += (u64)__popcll(m);
+            or_rows<K>(src, rw, m, y, acc);
+        }
+        u64 *drow = dst + (u64)x * rw;
+        if (g != 0xffffffffu) {  // a slice of a split row
+            u64 *gr = o.G + (u64)g * rw;
+            if (any) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const u64 w = (u64)lane + 64u * k;
+                    if ((K > 1 || w < rw) && acc[k])
+                        atomicOr((unsigned long long *)&gr[w], (unsigned long long)acc[k]);
+                }
+            }
+            __threadfence();  // release this slice's ORs before counting it finished
+            u32 old = 0;
+            if (lane == 0) old = atomicAdd(&o.slice_done[g], 1u);
+            old = (u32)__shfl((int)old, 0, 64);
+            if (old + 1 != o.n_slices[g]) continue;  // the same for the whole wave
+            __threadfence();  // acquire: every slice's ORs precede its count
+            any = false;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const u64 w = (u64)lane + 64u * k;
+                acc[k] = 0;
+                if (K > 1 || w < rw) {
+                    acc[k] = __hip_atomic_load(&gr[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&gr[w], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                if (acc[k]) any = true;
+            }
+            any = __ballot(any) != 0;
+            if (lane == 0) o.slice_done[g] = 0;
+        }
+        u64 c = 0;
+        if (any) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const u64 w = (u64)lane + 64u * k;
+                if (K > 1 || w < rw) {
+                    drow[w] = acc[k];
+                    c += (u64)__popcll(acc[k]);
+                }
+            }
+            w_io += 1;
+        }
+        c = wave_sum(c);
+        if (lane == 0) dst_pop[x] = (u32)c;
+    }
+    if (lane == 0) s_io[wv] = w_io;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u64 t = 0;
+        for (int w = 0; w < DWV; ++w) t += s_io[w];
+        if (t) atomicAdd(&st->or_io, (unsigned long long)t);
     }
 }
 
@@ -969,6 +1174,25 @@ static u64 dense_threshold(int shape, int kb, u64 n1, u64 n2) {
     return (u64)std::max(1.0, t);
 }
 
+// SG: the smallest delta that expands by columns.  Expanding by columns
+// moves about (4 rows + |r2|) rows of rw words (two transposes, the row OR);
+// expanding bit by bit costs about col_bytes_per_bit() bytes of streaming
+// time per delta bit (a successor-table probe, LDS ORs per successor).
+static double col_bytes_per_bit() {
+    const char *e = std::getenv("GDLOG_DENSE_COL_BYTES_PER_BIT");
+    const double v = e ? std::atof(e) : 0.0;
+    return v > 0.0 ? v : 1024.0;
+}
+
+static u64 col_threshold(int mode, int kb, u64 n2) {
+    if (mode == 4 || kb < 9) return ~0ull;  // a transpose tile is 512 rows x 8 words
+    if (mode == 3) return 0;
+    if (kb < 13) return ~0ull;  // small matrices: the fixed passes do not pay
+    const double rw = (double)(1ull << (kb - 6));
+    const double bytes = (4.0 * (double)(1ull << kb) + (double)n2) * rw * 8.0;
+    return (u64)std::max(1.0, bytes / col_bytes_per_bit());
+}
+
 static bool dense_rule_ok(const gdl_hashset *hs, int shape) {
     if (!hs || !hs->slots || hs->key_bits < 6 || hs->key_bits > 17) return false;
     if (shape == GDL_DENSE_SG && hs->key_bits > 16) return false;  // LDS rows of <= 8 KB
@@ -985,9 +1209,43 @@ struct BodyBufs {
     u64 *U;
 };
 
+// SG's column expansion: r2 grouped by its second column as OR work, the
+// OR's output V (= U^T) with its pops, the pops of D^T (stored in U's
+// buffer: U is built after D^T is no longer needed).
+struct ColBufs {
+    bool on;
+    OrRows o;
+    u64 *V;
+    u32 *vpop, *dtpop;
+    u64 rows;
+    unsigned grid_or, grid_tr;
+};
+
+template <int K>
+static gdl_status launch_col(gdl_ctx *ctx, const DenseArgs &a, const BodyBufs &bb,
+                             const ColBufs &cb, DenseState *st) {
+    {
+        KTimer kt(ctx, K_DENSE_TRANSPOSE, 0.0);  // D^T into U's buffer
+        hipLaunchKernelGGL(k_bt_transpose, dim3(cb.grid_tr), dim3(TB), 0, ctx->stream, bb.D_in,
+                           bb.pop_in, a.rw, bb.U, cb.dtpop, bb.upop, cb.rows, st);
+        GDL_TRY(check_launch(ctx, "k_bt_transpose"));
+    }
+    {
+        KTimer kt(ctx, K_DENSE_ORROWS, 0.0);  // V = R2^T D^T
+        hipLaunchKernelGGL(k_dense_or_rows<K>, dim3(cb.grid_or), dim3(DB), 0, ctx->stream, cb.o,
+                           a.rw, (const u32 *)cb.dtpop, (const u64 *)bb.U, cb.vpop, cb.V, st);
+        GDL_TRY(check_launch(ctx, "k_dense_or_rows"));
+    }
+    KTimer kt(ctx, K_DENSE_TRANSPOSE, 0.0);  // U = V^T
+    hipLaunchKernelGGL(k_bt_transpose, dim3(cb.grid_tr), dim3(TB), 0, ctx->stream, cb.V,
+                       cb.vpop, a.rw, bb.U, bb.upop, cb.dtpop, cb.rows, st);
+    return check_launch(ctx, "k_bt_transpose");
+}
+
 template <int K>
 static gdl_status launch_body(gdl_ctx *ctx, int shape, const DenseArgs &a, const BodyBufs &bb,
-                              DenseState *st, u64 *counts, unsigned grid, unsigned grid_exp) {
+                              const ColBufs &cb, DenseState *st, u64 *counts, unsigned grid,
+                              unsigned grid_exp) {
     if (shape == GDL_DENSE_TC) {
         KTimer kt(ctx, K_DENSE_COMBINE, 0.0);
         hipLaunchKernelGGL((k_dense_combine<K, false>), dim3(grid), dim3(DB), 0, ctx->stream, a,
@@ -995,6 +1253,7 @@ static gdl_status launch_body(gdl_ctx *ctx, int shape, const DenseArgs &a, const
                            bb.pop_out, bb.D_out, st, counts);
         GDL_TRY(check_launch(ctx, "k_dense_combine"));
     } else if constexpr (K <= 16) {  // SG rows of at most 2^16 bits (LDS)
+        if (cb.on) GDL_TRY(launch_col<K>(ctx, a, bb, cb, st));
         {
             KTimer kt(ctx, K_DENSE_EXPAND, 0.0);
             const size_t lds = (size_t)a.rw * sizeof(u64);
@@ -1035,15 +1294,62 @@ static gdl_status launch_append(gdl_ctx *ctx, int kb, const DenseArgs &a, const 
 }
 
 static gdl_status dispatch_body(gdl_ctx *ctx, int shape, int kb, const DenseArgs &a,
-                                const BodyBufs &bb, DenseState *st, u64 *counts, unsigned grid,
-                                unsigned grid_exp) {
+                                const BodyBufs &bb, const ColBufs &cb, DenseState *st,
+                                u64 *counts, unsigned grid, unsigned grid_exp) {
     switch (kb) {
-    case 17: return launch_body<32>(ctx, shape, a, bb, st, counts, grid, grid_exp);
-    case 16: return launch_body<16>(ctx, shape, a, bb, st, counts, grid, grid_exp);
-    case 15: return launch_body<8>(ctx, shape, a, bb, st, counts, grid, grid_exp);
-    case 14: return launch_body<4>(ctx, shape, a, bb, st, counts, grid, grid_exp);
-    case 13: return launch_body<2>(ctx, shape, a, bb, st, counts, grid, grid_exp);
-    default: return launch_body<1>(ctx, shape, a, bb, st, counts, grid, grid_exp);
+    case 17: return launch_body<32>(ctx, shape, a, bb, cb, st, counts, grid, grid_exp);
+    case 16: return launch_body<16>(ctx, shape, a, bb, cb, st, counts, grid, grid_exp);
+    case 15: return launch_body<8>(ctx, shape, a, bb, cb, st, counts, grid, grid_exp);
+    case 14: return launch_body<4>(ctx, shape, a, bb, cb, st, counts, grid, grid_exp);
+    case 13: return launch_body<2>(ctx, shape, a, bb, cb, st, counts, grid, grid_exp);
+    default: return launch_body<1>(ctx, shape, a, bb, cb, st, counts, grid, grid_exp);
+    }
+}
+
+// The set-up's host copies of row lists, kept per thread between calls: a
+// fresh vector of a few hundred KB faults in its pages on every call.
+struct DenseHostScratch {
+    std::vector<u64> soff, off, off2;
+    std::vector<u32> hb, hidx, act, slices, act2, slices2;
+    std::vector<uint4> items, items2;
+};
+
+static DenseHostScratch &dense_host_scratch() {
+    static thread_local DenseHostScratch h;
+    return h;
+}
+
+// Work items of rows grouped in CSR form (h_off): the rows with a non-empty
+// list, longest first by the bit length of their list length (rows in order
+// within one length class: a linear bucket pass, where a comparison sort of
+// a few ten thousand rows takes hundreds of microseconds), those of more
+// than kSlice entries split into slices; slices gets each split row's slice
+// count.  act is reordered likewise.
+static void build_items(const std::vector<u64> &h_off, std::vector<u32> &act,
+                        std::vector<uint4> &items, std::vector<u32> &slices) {
+    auto cls = [&](u32 x) { return 64 - __builtin_clzll((unsigned long long)(h_off[x + 1] - h_off[x]) | 1ull); };
+    u64 start[66] = {};
+    for (u32 x : act) ++start[65 - cls(x)];
+    for (int c = 1; c < 66; ++c) start[c] += start[c - 1];
+    static thread_local std::vector<u32> sorted;
+    sorted.resize(act.size());
+    for (u32 x : act) sorted[start[64 - cls(x)]++] = x;
+    act.swap(sorted);
+    items.clear();
+    slices.clear();
+    items.reserve(act.size());
+    for (u32 x : act) {
+        const u64 c = h_off[x + 1] - h_off[x];
+        if (c <= kSlice) {
+            items.push_back(make_uint4(x, 0, (u32)c, 0xffffffffu));
+            continue;
+        }
+        const u32 g = (u32)slices.size();
+        const u32 ns = (u32)((c + kSlice - 1) / kSlice);
+        slices.push_back(ns);
+        for (u32 j = 0; j < ns; ++j)
+            items.push_back(
+                make_uint4(x, j * kSlice, (u32)std::min<u64>(c, (u64)(j + 1) * kSlice), g));
     }
 }
 
@@ -1069,7 +1375,7 @@ static gdl_status grow_log(gdl_ctx *ctx, u64 **d_log, u64 *log_cap, u64 keep, u6
 // whenever the rule and the set allow it).
 u64 dense_threshold_for(const gdl_ctx *ctx, const gdl_hashset *hs, int shape, u64 n1, u64 n2) {
     if (ctx->dense_mode == 0 || !dense_rule_ok(hs, shape)) return ~0ull;
-    if (ctx->dense_mode == 2) return 0;
+    if (ctx->dense_mode >= 2) return 0;
     return dense_threshold(shape, (int)hs->key_bits, n1, n2);
 }
 
@@ -1084,11 +1390,16 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
     const u64 threshold = dense_threshold_for(ctx, hs, shape, rule->n1, rule->n2);
     if (max_bodies <= 0 || n_delta == 0 || threshold == ~0ull || n_delta < threshold)
         return GDL_OK;
-    const u64 sparse_below = threshold / 2;
+    // TC's tuple-at-a-time bodies run stream-ordered too, so a TC run hands a
+    // delta that turned sparse back to them; SG's run one iteration per host
+    // round trip (hundreds of microseconds), against a few short launches for
+    // a dense body on a small delta: an SG run goes on to the fixpoint
+    const u64 sparse_below = shape == GDL_DENSE_SG ? 0 : threshold / 2;
     const int kb = (int)hs->key_bits;
     const u64 rows = 1ull << kb, rw = 1ull << (kb - 6);
     DevBufs mem(ctx);
     Scratch scr(ctx);
+    Staging stg(ctx);  // the set-up's host copies
     u32 *cnt, *bad;
     GDL_TRY(scr.get(rows, &cnt));
     GDL_TRY(scr.get(1, &bad));
@@ -1115,7 +1426,8 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
                        rows, act, (unsigned long long *)n_act_d);
     GDL_TRY(check_launch(ctx, "k_dense_act"));
     a.act = act;
-    std::vector<u64> h_soff;  // SG: r2's row offsets on the host
+    DenseHostScratch &H = dense_host_scratch();
+    std::vector<u64> &h_soff = H.soff;  // SG: r2's row offsets on the host
     if (shape == GDL_DENSE_SG) {
         u64 *succ_off;
         u32 *succ_idx;
@@ -1124,11 +1436,10 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
         a.succ_off = succ_off;
         a.succ_idx = succ_idx;
         h_soff.resize(rows + 1);
-        GDL_HIP(ctx, hipMemcpyAsync(h_soff.data(), succ_off, (rows + 1) * sizeof(u64),
-                                    hipMemcpyDeviceToHost, ctx->stream));
-        GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        GDL_TRY(stg.down(h_soff.data(), succ_off, (rows + 1) * sizeof(u64)));
         // the longest lists (more than kHeavy successors) become bit rows
-        std::vector<u32> hb;
+        std::vector<u32> &hb = H.hb;
+        hb.clear();
         for (u64 r = 0; r < rows; ++r)
             if (h_soff[r + 1] - h_soff[r] > kHeavy) hb.push_back((u32)r);
         if (hb.size() > kHeavyMax) {
@@ -1137,18 +1448,17 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
             });
             hb.resize(kHeavyMax);
         }
-        std::vector<u32> h_hidx(rows, kNoBit);
+        std::vector<u32> &h_hidx = H.hidx;
+        h_hidx.assign(rows, kNoBit);
         for (u32 h = 0; h < (u32)hb.size(); ++h) h_hidx[hb[h]] = h;
         u32 *hidx, *heavy_b;
         u64 *heavy;
         GDL_TRY(mem.get(rows, &hidx));
         GDL_TRY(mem.get(std::max<size_t>(hb.size(), 1), &heavy_b));
         GDL_TRY(mem.get(std::max<u64>(hb.size() * rw, 1), &heavy));
-        GDL_HIP(ctx, hipMemcpyAsync(hidx, h_hidx.data(), rows * sizeof(u32), hipMemcpyHostToDevice,
-                                    ctx->stream));
+        GDL_TRY(stg.up(hidx, h_hidx.data(), rows * sizeof(u32)));
         if (!hb.empty()) {
-            GDL_HIP(ctx, hipMemcpyAsync(heavy_b, hb.data(), hb.size() * sizeof(u32),
-                                        hipMemcpyHostToDevice, ctx->stream));
+            GDL_TRY(stg.up(heavy_b, hb.data(), hb.size() * sizeof(u32)));
             GDL_HIP(ctx, hipMemsetAsync(heavy, 0, hb.size() * rw * sizeof(u64), ctx->stream));
             hipLaunchKernelGGL(k_heavy_rows, dim3((unsigned)std::min<size_t>(hb.size(), 4096)),
                                dim3(DB), 0, ctx->stream, succ_off, succ_idx, heavy_b,
@@ -1164,10 +1474,10 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
         a.succ_tab = tab;
     }
     u64 setup[2] = {0, 0};
-    GDL_HIP(ctx, hipMemcpyAsync(&setup[0], n_act_d, sizeof(u64), hipMemcpyDeviceToHost,
-                                ctx->stream));
-    GDL_HIP(ctx, hipMemcpyAsync(&setup[1], bad, sizeof(u32), hipMemcpyDeviceToHost, ctx->stream));
-    GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    u32 h_bad = 0;
+    GDL_TRY(stg.down(&setup[0], n_act_d, sizeof(u64)));
+    GDL_TRY(stg.down(&h_bad, bad, sizeof(u32)));
+    setup[1] = h_bad;
     a.n_act = setup[0];
     // ids beyond the set's domain in a static relation: the tuple-at-a-time
     // bodies handle (and report) them
@@ -1175,38 +1485,19 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
 
     // work items: the active rows, most preds first, split into slices of at
     // most kSlice preds (on the host: a few hundred thousand rows at most)
-    std::vector<u64> h_off(rows + 1);
-    std::vector<u32> h_act(a.n_act);
-    GDL_HIP(ctx, hipMemcpyAsync(h_off.data(), pred_off, (rows + 1) * sizeof(u64),
-                                hipMemcpyDeviceToHost, ctx->stream));
-    GDL_HIP(ctx, hipMemcpyAsync(h_act.data(), act, a.n_act * sizeof(u32), hipMemcpyDeviceToHost,
-                                ctx->stream));
-    GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    std::sort(h_act.begin(), h_act.end(), [&](u32 p, u32 q) {
-        const u64 cp = h_off[p + 1] - h_off[p], cq = h_off[q + 1] - h_off[q];
-        return cp != cq ? cp > cq : p < q;
-    });
-    std::vector<uint4> h_items;
-    std::vector<u32> h_slices;
-    h_items.reserve(a.n_act);
-    for (u32 x : h_act) {
-        const u64 c = h_off[x + 1] - h_off[x];
-        if (c <= kSlice) {
-            h_items.push_back(make_uint4(x, 0, (u32)c, 0xffffffffu));
-            continue;
-        }
-        const u32 g = (u32)h_slices.size();
-        const u32 ns = (u32)((c + kSlice - 1) / kSlice);
-        h_slices.push_back(ns);
-        for (u32 j = 0; j < ns; ++j)
-            h_items.push_back(make_uint4(x, j * kSlice, (u32)std::min<u64>(c, (u64)(j + 1) * kSlice), g));
-    }
+    std::vector<u64> &h_off = H.off;
+    std::vector<u32> &h_act = H.act;
+    h_off.resize(rows + 1);
+    h_act.resize(a.n_act);
+    GDL_TRY(stg.down(h_off.data(), pred_off, (rows + 1) * sizeof(u64)));
+    GDL_TRY(stg.down(h_act.data(), act, a.n_act * sizeof(u32)));
+    std::vector<uint4> &h_items = H.items;
+    std::vector<u32> &h_slices = H.slices;
+    build_items(h_off, h_act, h_items, h_slices);
     uint4 *items;
     GDL_TRY(mem.get(h_items.size(), &items));
-    GDL_HIP(ctx, hipMemcpyAsync(act, h_act.data(), a.n_act * sizeof(u32), hipMemcpyHostToDevice,
-                                ctx->stream));
-    GDL_HIP(ctx, hipMemcpyAsync(items, h_items.data(), h_items.size() * sizeof(uint4),
-                                hipMemcpyHostToDevice, ctx->stream));
+    GDL_TRY(stg.up(act, h_act.data(), a.n_act * sizeof(u32)));
+    GDL_TRY(stg.up(items, h_items.data(), h_items.size() * sizeof(uint4)));
     a.items = items;
     a.n_items = h_items.size();
     if (!h_slices.empty()) {
@@ -1216,8 +1507,7 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
         GDL_TRY(mem.get(ns, &n_slices));
         GDL_TRY(mem.get(ns, &done));
         GDL_TRY(mem.get(ns * rw, &gu));
-        GDL_HIP(ctx, hipMemcpyAsync(n_slices, h_slices.data(), ns * sizeof(u32),
-                                    hipMemcpyHostToDevice, ctx->stream));
+        GDL_TRY(stg.up(n_slices, h_slices.data(), ns * sizeof(u32)));
         GDL_HIP(ctx, hipMemsetAsync(done, 0, ns * sizeof(u32), ctx->stream));
         GDL_HIP(ctx, hipMemsetAsync(gu, 0, ns * rw * sizeof(u64), ctx->stream));
         if (shape == GDL_DENSE_SG) {
@@ -1233,30 +1523,16 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
     u64 *U = nullptr;
     unsigned grid_exp = 1;
     if (shape == GDL_DENSE_SG) {
-        // U is built for the rows that are some row's pred; the planes of
-        // r2's row lengths
-        std::vector<u32> h_idx(rule->n1);
-        GDL_HIP(ctx, hipMemcpyAsync(h_idx.data(), pred_idx, rule->n1 * sizeof(u32),
-                                    hipMemcpyDeviceToHost, ctx->stream));
-        GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        std::vector<uint8_t> used(rows, 0);
-        for (u32 y : h_idx) used[y] = 1;
-        std::vector<u32> h_exp;
-        for (u64 r = 0; r < rows; ++r)
-            if (used[r]) h_exp.push_back((u32)r);
+        // U is built for every delta row (a row that is no row's pred is
+        // built and never read: cheaper than listing the pred rows on the
+        // host); the planes of r2's row lengths
+        a.n_exp = rows;
         u64 max_deg = 0;
         for (u64 r = 0; r < rows; ++r) max_deg = std::max(max_deg, h_soff[r + 1] - h_soff[r]);
         int n_planes = 0;
         while (n_planes < 64 && (max_deg >> n_planes)) ++n_planes;
         if (n_planes > 24)
             return fail(ctx, GDL_ERR_UNSUPPORTED, "dense SG bodies: a row of 2^24 or more");
-        u32 *exp_rows;
-        GDL_TRY(mem.get(std::max<u64>(h_exp.size(), 1), &exp_rows));
-        if (!h_exp.empty())
-            GDL_HIP(ctx, hipMemcpyAsync(exp_rows, h_exp.data(), h_exp.size() * sizeof(u32),
-                                        hipMemcpyHostToDevice, ctx->stream));
-        a.exp_rows = exp_rows;
-        a.n_exp = h_exp.size();
         u64 *planes;
         GDL_TRY(mem.get(std::max<u64>((u64)n_planes * rw, 1), &planes));
         if (n_planes) {
@@ -1269,6 +1545,61 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
         GDL_TRY(mem.get(rows * rw, &U));
         GDL_TRY(mem.get(rows, &upop));
         grid_exp = (unsigned)std::max<u64>(1, std::min<u64>(a.n_exp, grid_cap(ctx)));
+    }
+    // SG's column expansion: r2 grouped by its second column, as OR work
+    ColBufs col{};
+    const u64 col_above =
+        shape == GDL_DENSE_SG ? col_threshold(ctx->dense_mode, kb, rule->n2) : ~0ull;
+    if (col_above != ~0ull) {
+        u64 *off2;
+        u32 *idx2;
+        GDL_TRY(group_by_col(ctx, mem, rule->r2, rule->n2, 1, kb, rows, cnt, &off2, &idx2, bad));
+        std::vector<u64> &h_off2 = H.off2;
+        h_off2.resize(rows + 1);
+        GDL_TRY(stg.down(h_off2.data(), off2, (rows + 1) * sizeof(u64)));
+        std::vector<u32> &h_act2 = H.act2;
+        h_act2.clear();
+        for (u64 r = 0; r < rows; ++r)
+            if (h_off2[r + 1] > h_off2[r]) h_act2.push_back((u32)r);
+        std::vector<uint4> &h_items2 = H.items2;
+        std::vector<u32> &h_slices2 = H.slices2;
+        build_items(h_off2, h_act2, h_items2, h_slices2);
+        col.on = !h_items2.empty();
+        col.o.off = off2;
+        col.o.idx = idx2;
+        col.rows = rows;
+        if (col.on) {
+            uint4 *items2;
+            GDL_TRY(mem.get(h_items2.size(), &items2));
+            GDL_TRY(stg.up(items2, h_items2.data(), h_items2.size() * sizeof(uint4)));
+            col.o.items = items2;
+            col.o.n_items = h_items2.size();
+            if (!h_slices2.empty()) {
+                const u64 ns = h_slices2.size();
+                u32 *n_slices, *done;
+                u64 *g;
+                GDL_TRY(mem.get(ns, &n_slices));
+                GDL_TRY(mem.get(ns, &done));
+                GDL_TRY(mem.get(ns * rw, &g));
+                GDL_TRY(stg.up(n_slices, h_slices2.data(), ns * sizeof(u32)));
+                GDL_HIP(ctx, hipMemsetAsync(done, 0, ns * sizeof(u32), ctx->stream));
+                GDL_HIP(ctx, hipMemsetAsync(g, 0, ns * rw * sizeof(u64), ctx->stream));
+                col.o.n_slices = n_slices;
+                col.o.slice_done = done;
+                col.o.G = g;
+            }
+            GDL_TRY(mem.get(rows * rw, &col.V));
+            GDL_TRY(mem.get(rows, &col.vpop));
+            GDL_TRY(mem.get(rows, &col.dtpop));
+            for (u32 *p : {col.vpop, col.dtpop}) {
+                hipLaunchKernelGGL(k_fill_u32, dim3(dgrid(ctx, rows)), dim3(DB), 0, ctx->stream,
+                                   p, rows, 0u);
+                GDL_TRY(check_launch(ctx, "k_fill_u32"));
+            }
+            col.grid_or = (unsigned)std::max<u64>(
+                1, std::min<u64>((col.o.n_items + DWV - 1) / DWV, grid_cap(ctx)));
+            col.grid_tr = (unsigned)((rw / kTW) * (rw / kTW));
+        }
     }
 
     // the delta as bit rows (buffer 0)
@@ -1302,8 +1633,7 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
         GDL_TRY(check_launch(ctx, "k_dense_set_bits"));
     }
     DenseState r{};
-    GDL_HIP(ctx, hipMemcpyAsync(&r, st, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
-    GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    GDL_TRY(stg.down(&r, st, sizeof(r)));
     if (r.bad)
         return fail(ctx, GDL_ERR_INVALID_ARG, "dense bodies: a delta key outside the set's domain");
 
@@ -1324,16 +1654,24 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
         s.log_cap = *log_cap;
         s.sparse_below = sparse_below;
         s.max_bodies = (u32)k;
-        GDL_HIP(ctx, hipMemcpyAsync(st, &s, sizeof(s), hipMemcpyHostToDevice, ctx->stream));
+        s.col_above = col.on ? col_above : ~0ull;
+        s.col = in_delta >= s.col_above ? 1u : 0u;
+        GDL_TRY(stg.up(st, &s, sizeof(s)));
         for (int i = 0; i < k; ++i) {
             const int p = (done + i) & 1;
             const BodyBufs bb{pop[p], D[p], pop[p ^ 1], D[p ^ 1], upop, U};
-            GDL_TRY(dispatch_body(ctx, shape, kb, a, bb, st, counts, grid, grid_exp));
+            GDL_TRY(dispatch_body(ctx, shape, kb, a, bb, col, st, counts, grid, grid_exp));
+            if (done + i == 0) {
+                // the first delta's pops: a later combine writes the pops of
+                // the rows with preds only, so the rest must read as empty
+                hipLaunchKernelGGL(k_fill_u32, dim3(dgrid(ctx, rows)), dim3(DB), 0, ctx->stream,
+                                   pop[0], rows, 0u);
+                GDL_TRY(check_launch(ctx, "k_fill_u32"));
+            }
         }
         u64 c[kCounts * kChunk];
-        GDL_HIP(ctx, hipMemcpyAsync(&r, st, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
-        GDL_HIP(ctx, hipMemcpyAsync(c, counts, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
-        GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        GDL_TRY(stg.down(&r, st, sizeof(r)));
+        GDL_TRY(stg.down(c, counts, sizeof(c)));
         GDL_TRY(check_device_error(ctx));
         for (u32 b = 0; b < r.body; ++b, ++done) {
             const u64 *cb = c + kCounts * b;
@@ -1343,9 +1681,14 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
             // combine: the pred rows read (D, and SG's U), FULL's row read,
             // the new delta row written, new words set in FULL
             ctx->stats[K_DENSE_COMBINE].alg_bytes += row * (double)cb[2] + 8.0 * (double)cb[0];
-            // expansion: the delta row read, U's row written, one 16-byte
-            // entry per delta bit
-            ctx->stats[K_DENSE_EXPAND].alg_bytes += row * (double)cb[3] + 16.0 * (double)in_delta;
+            // expansion bit by bit: the delta row read, U's row written, one
+            // 16-byte entry per delta bit; by columns: the 64-byte segments
+            // the transposes read and wrote, the OR's row reads and writes
+            if (cb[5] == 0)
+                ctx->stats[K_DENSE_EXPAND].alg_bytes +=
+                    row * (double)cb[3] + 16.0 * (double)in_delta;
+            ctx->stats[K_DENSE_TRANSPOSE].alg_bytes += 64.0 * (double)cb[5];
+            ctx->stats[K_DENSE_ORROWS].alg_bytes += row * (double)cb[6];
             // append: the rows with new tuples read, their keys written
             ctx->stats[K_DENSE_APPEND].alg_bytes += row * (double)cb[4] + 8.0 * (double)cb[0];
             in_delta = cb[0];
@@ -1359,10 +1702,9 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
             const u32 stop = r.stop;
             r.stop = kDnRun;
             r.log_cap = *log_cap;
-            GDL_HIP(ctx, hipMemcpyAsync(st, &r, sizeof(r), hipMemcpyHostToDevice, ctx->stream));
+            GDL_TRY(stg.up(st, &r, sizeof(r)));
             GDL_TRY(launch_append(ctx, kb, a, pop[done & 1], D[done & 1], st, counts, grid));
-            GDL_HIP(ctx, hipMemcpyAsync(&r, st, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
-            GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            GDL_TRY(stg.down(&r, st, sizeof(r)));
             GDL_TRY(check_device_error(ctx));
             if (r.appended != r.body)
                 return fail(ctx, GDL_ERR_OVERFLOW, "dense bodies: the log did not grow");
@@ -1407,7 +1749,7 @@ gdl_status gdl_hashset_extract_sorted(gdl_ctx *ctx, const gdl_hashset *hs, uint6
 }
 
 gdl_status gdl_set_dense_mode(gdl_ctx *ctx, int mode) {
-    if (!ctx || mode < 0 || mode > 2) return GDL_ERR_INVALID_ARG;
+    if (!ctx || mode < 0 || mode > 4) return GDL_ERR_INVALID_ARG;
     ctx->dense_mode = mode;
     return GDL_OK;
 }

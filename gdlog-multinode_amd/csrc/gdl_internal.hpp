@@ -84,6 +84,8 @@ enum KernelId {
     K_DENSE_COMBINE,
     K_DENSE_APPEND,
     K_BM_EXTRACT,
+    K_DENSE_TRANSPOSE,
+    K_DENSE_ORROWS,
     K_NUM
 };
 extern const char *kKernelNames[K_NUM];
@@ -107,6 +109,8 @@ struct gdl_ctx {
     gdl_config cfg{};
     gdl::Allocator *alloc = nullptr;
     uint64_t *h_pinned = nullptr;  // pinned staging for scalars read back
+    char *h_stage = nullptr;       // pinned staging for bulk copies (gdl::Staging)
+    size_t h_stage_cap = 0;
     uint32_t *d_err = nullptr;     // kernel-side error word
     int cu_count = 0;
     gdl_tuning tune{GDL_DEFAULT_LOAD_FACTOR, 8, 8, 256};  // load_factor mirrors cfg.load_factor
@@ -209,6 +213,30 @@ gdl_status read_u64(gdl_ctx *ctx, const u64 *d, u64 *h, int count = 1);
 gdl_status noalloc_setup(gdl_ctx *ctx);
 gdl_status check_launch(gdl_ctx *ctx, const char *what);
 gdl_status check_device_error(gdl_ctx *ctx);
+
+// Bulk host <-> device copies through the context's pinned staging buffer: a
+// copy between the device and pageable host memory makes the runtime pin or
+// stage that memory on every call (hundreds of microseconds for a few
+// hundred KB).  up() copies into the buffer and sends it asynchronously on
+// the context's stream (the caller's host data may change as soon as it
+// returns); down() waits for the stream.  The buffer is reused after each
+// wait; the destructor waits for uploads still in flight.  One thread per
+// context, as for every call on it.
+class Staging {
+  public:
+    explicit Staging(gdl_ctx *ctx) : ctx_(ctx) {}
+    ~Staging();
+    Staging(const Staging &) = delete;
+    Staging &operator=(const Staging &) = delete;
+    gdl_status up(void *d_dst, const void *h_src, size_t n);
+    gdl_status down(void *h_dst, const void *d_src, size_t n);
+    gdl_status sync();
+
+  private:
+    gdl_status room(size_t n, char **at);
+    gdl_ctx *ctx_;
+    size_t used_ = 0;
+};
 
 // ---- internal entry points shared between translation units
 gdl_status exclusive_scan_u32_to_u64(gdl_ctx *ctx, const u32 *d_in, u64 n, u64 *d_out,

@@ -826,13 +826,14 @@ static gdl_status join_insert_impl(gdl_ctx *ctx, const u64 *d_inner, u64 n_inner
     }
     if (bound == 0) return GDL_OK;
     // The wave-flattened kernel is balanced inside a wave but not across
-    // waves: a wave whose 64 outer tuples hit a hub walks the whole hub run.
-    // Number the outputs and spread them over the grid instead when one run
-    // is long against the expected work per wave.
+    // waves: a wave whose 64 outer tuples hit a hub walks the whole hub run,
+    // one step (64 outputs) per hub tuple.  Number the outputs and spread
+    // them over the grid instead when that is long against the expected
+    // steps per wave.
     const u64 waves = std::min<u64>((n_outer + fb - 1) / fb, cap_blocks) * (fb / 64);
     const u64 avg_run = std::max<u64>(1, n_inner / std::max<u64>(1, inner_idx->n_distinct));
     const u64 est_total = counted ? bound : n_outer * avg_run;
-    const bool balanced = mr > kHubRun && mr * 4 > est_total / waves;
+    const bool balanced = mr > kHubRun && mr * 16 > est_total / waves;
     if (balanced && !counted) {
         GDL_TRY(lb.get(n_outer, &counts));
         GDL_TRY(lb.get(n_outer, &begins));
@@ -1069,7 +1070,8 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
         h.delta_limit = g >= grid_cap(ctx) ? ~0ull : (u64)g * LB * 8;
         // a delta that turns dense ends the chunk too (bit-matrix bodies next)
         if (dense_at != ~0ull) h.delta_limit = std::min<u64>(h.delta_limit, dense_at - 1);
-        GDL_HIP(ctx, hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
+        Staging stg(ctx);
+        GDL_TRY(stg.up(st, &h, sizeof(h)));
         for (int i = 0; i < k; ++i) {
             KTimer kt(ctx, K_LINEAR, 0.0);
             hipLaunchKernelGGL(k_linear_body<LB>, dim3(g), dim3(LB), 0, ctx->stream,
@@ -1081,9 +1083,8 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
         }
         LinState r{};
         u64 c[2 * kChunk];
-        GDL_HIP(ctx, hipMemcpyAsync(&r, st, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
-        GDL_HIP(ctx, hipMemcpyAsync(c, counts, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
-        GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        GDL_TRY(stg.down(&r, st, sizeof(r)));
+        GDL_TRY(stg.down(c, counts, sizeof(c)));
         GDL_TRY(check_device_error(ctx));
         if (r.stop == kLinOverflow)
             return fail(ctx, GDL_ERR_OVERFLOW, "linear fixpoint: a body outgrew its log bound");

@@ -259,7 +259,9 @@ gdl_status exclusive_scan_u64(gdl_ctx *ctx, const u64 *d_in, u64 n, u64 *d_out) 
 }
 
 gdl_status scan_inplace(gdl_ctx *ctx, u64 *v, u64 n) {
-    if (n <= 262144) return scan_top(ctx, v, n, nullptr);
+    // one block up to 2^15 entries (a few microseconds); a 1024-thread block
+    // over 2^18 of them takes ~150 us, the three-pass scan a few launches
+    if (n <= 32768) return scan_top(ctx, v, n, nullptr);
     Scratch scr(ctx);
     u64 *tmp;
     GDL_TRY(scr.get(n + 1, &tmp));

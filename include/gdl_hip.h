@@ -364,9 +364,10 @@ typedef struct gdl_dense_rule {
  * (max_bodies entries) get each body's new and produced ("join result size
  * (non dedup)") tuple counts; *bodies the bodies run.  The context's dense
  * mode (gdl_set_dense_mode) decides whether bodies run at all: in mode 1
- * only while the delta is dense enough for row streams to beat tuple probes
- * (a body with fewer new tuples than that ends the run; it is counted), in
- * mode 2 until the fixpoint or max_bodies.  *bodies = 0 when the rule does
+ * from a delta dense enough for row streams to beat tuple probes, and for
+ * GDL_DENSE_TC only while it stays so (a body with fewer new tuples than that
+ * ends the run; it is counted), GDL_DENSE_SG until the fixpoint or
+ * max_bodies; in mode 2 until the fixpoint or max_bodies.  *bodies = 0 when the rule does
  * not apply: mode 0, a hash-table FULL, key_bits outside the range above, a
  * static relation with ids outside the domain, or (mode 1) a sparse delta.
  * Equivalent to *bodies iterations of LIE::fixpoint_loop (lie.cu:26-274)
@@ -377,7 +378,12 @@ gdl_status gdl_fixpoint_dense(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hash
                               uint64_t *h_new, uint64_t *h_raw, int *bodies);
 /* 0: never run dense bodies; 1 (the default, or the GDLOG_DENSE environment
  * variable at context creation): when a cost model prefers them; 2: whenever
- * they apply.  Results are the same in every mode. */
+ * they apply.  An SG body builds U = delta x r2 either bit by bit (a
+ * successor-list probe per delta bit) or by columns (U^T = r2^T x delta^T:
+ * two bit-matrix transposes around a row OR over r2's pred lists, a fixed
+ * cost for domains of 2^9 ids or more); a cost model picks per body, or mode
+ * 3 (dense whenever they apply, always by columns) and mode 4 (likewise,
+ * never by columns) force one.  Results are the same in every mode. */
 gdl_status gdl_set_dense_mode(gdl_ctx *ctx, int mode);
 
 /* Batched form of gdl_join_insert: many fixpoint instances share the inner

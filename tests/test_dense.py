@@ -196,20 +196,69 @@ def test_dense_tc_bodies(ctx, name):
 
 
 @pytest.mark.gpu
+def test_dense_tc_bodies_delta_rows_without_preds(ctx):
+    # delta rows that no r1 tuple points at: the first body reads them, no
+    # later body writes them, so the third body (which reads the first
+    # body's buffers again) must see them empty
+    e = np.unique(GRAPHS["er_1000"]())
+    src = set((e >> np.uint64(32)).tolist())
+    lonely = np.array([v for v in range(1024) if v not in src][:40], dtype=np.uint64)
+    extra = (lonely << np.uint64(32)) | np.arange(lonely.size, dtype=np.uint64)
+    delta = np.unique(np.concatenate([e, extra]))
+    r1 = reverse(e)
+    want, fs = tc_bodies(r1, delta, delta, max_bodies=4)
+    ctx.set_dense_mode(2)
+    try:
+        got = ctx.fixpoint_dense(gdl.DENSE_TC, 10, delta, delta, r1, max_bodies=4)
+    finally:
+        ctx.set_dense_mode(1)
+    _check_run(got, want, delta, fs)
+
+
+# dense mode 2: the cost model picks how each SG body expands U = D R2;
+# 3: always by columns (two bit-matrix transposes around a row OR over r2's
+# pred lists, for domains of 2^9 ids or more); 4: always bit by bit
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [2, 3, 4])
 @pytest.mark.parametrize("name", ["er_300", "er_1000", "rmat_10", "hpc_talk"])
-def test_dense_sg_bodies(ctx, name):
+def test_dense_sg_bodies(ctx, name, mode):
     e = np.unique(GRAPHS[name]())
     kb = max(6, int(max(int((e >> np.uint64(32)).max()), int((e & M32).max())) + 1).bit_length())
     init = np.unique(np.array(
         [key(x, y) for p, xs in _by_first(e).items() for x in xs for y in xs if x != y],
         dtype=np.uint64))
     want, fs = sg_bodies(e, e, init, init)
-    ctx.set_dense_mode(2)
+    ctx.set_dense_mode(mode)
     try:
         got = ctx.fixpoint_dense(gdl.DENSE_SG, kb, init, init, e, e)
     finally:
         ctx.set_dense_mode(1)
     _check_run(got, want, init, fs)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [3, 4])
+@pytest.mark.parametrize("kb", [9, 11, 13])
+def test_dense_sg_bodies_two_relations(ctx, kb, mode):
+    # r1 != r2 and a delta that is not symmetric: the column expansion's two
+    # transposes must not lean on SG's symmetry; r2 has hub columns (a row
+    # with more than 32 preds is OR-ed in slices)
+    rng = np.random.default_rng(kb)
+    n = 1 << kb
+    r1 = np.unique(gdl.pack(rng.integers(0, n, 2 * n), rng.integers(0, n, 2 * n)))
+    hub = rng.integers(0, n, 8)
+    r2 = np.unique(np.concatenate([
+        gdl.pack(rng.integers(0, n, 2 * n), rng.integers(0, n, 2 * n)),
+        gdl.pack(rng.integers(0, n, 400), np.repeat(hub, 50)),
+        gdl.pack(np.repeat(hub[:2], 70), rng.integers(0, n, 140))]))
+    delta = np.unique(gdl.pack(rng.integers(0, n, 2 * n), rng.integers(0, n, 2 * n)))
+    want, fs = sg_bodies(r1, r2, delta, delta, max_bodies=3)
+    ctx.set_dense_mode(mode)
+    try:
+        got = ctx.fixpoint_dense(gdl.DENSE_SG, kb, delta, delta, r1, r2, max_bodies=3)
+    finally:
+        ctx.set_dense_mode(1)
+    _check_run(got, want, delta, fs)
 
 
 @pytest.mark.gpu
@@ -270,7 +319,7 @@ def test_programs_same_with_dense_bodies_off_auto_forced(ctx, name):
     e = make()
     run = ctx.run_tc if kind == "tc" else ctx.run_sg
     res = []
-    for mode in (0, 1, 2):
+    for mode in (0, 1, 2, 3, 4):
         ctx.set_dense_mode(mode)
         try:
             res.append(run(e))
