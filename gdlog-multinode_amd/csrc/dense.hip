@@ -1695,9 +1695,11 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
             hs->size += cb[0];
         }
         if (r.appended < r.body) {
-            // the last body's append found the log full: grow it, then append
+            // the last body's append found the log full: grow it, with room for
+            // two more bodies as large (a dense delta tends to stay large for
+            // a few bodies), then append
             const u64 nn = c[kCounts * (r.body - 1)];
-            GDL_TRY(grow_log(ctx, d_log, log_cap, r.cursor, r.cursor + nn));
+            GDL_TRY(grow_log(ctx, d_log, log_cap, r.cursor, r.cursor + 3 * nn));
             a.log = *d_log;
             const u32 stop = r.stop;
             r.stop = kDnRun;
@@ -1709,8 +1711,6 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
             if (r.appended != r.body)
                 return fail(ctx, GDL_ERR_OVERFLOW, "dense bodies: the log did not grow");
             r.stop = stop == kDnGrow ? kDnLimit : stop;
-            // the next chunk's bodies are likely as large: room for 2x more
-            GDL_TRY(grow_log(ctx, d_log, log_cap, r.cursor, r.cursor + 2 * nn));
         }
         end = r.cursor;
         if (r.body == 0 || r.stop == kDnFixpoint || r.stop == kDnSparse) break;

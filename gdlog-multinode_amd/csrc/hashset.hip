@@ -17,29 +17,104 @@ namespace gdl {
 constexpr u64 kEmpty = ~0ull;
 constexpr int HB = 256;
 
-// flag word at slots[cap]: 1 when the key ~0 is a member
+__device__ __forceinline__ u64 shfl64(u64 v, int src) {
+    const int lo = __shfl((int)(u32)v, src, 64), hi = __shfl((int)(u32)(v >> 32), src, 64);
+    return ((u64)(u32)hi << 32) | (u64)(u32)lo;
+}
+
+// Bitmap insert of one key per lane (valid lanes), aggregated over the wave:
+// a run of consecutive lanes on one word ORs its bits with one atomic (read
+// first: no atomic when they are all set), whose old value the run's lanes
+// share; a lane whose bit an earlier lane of its run also sets is not new.
+// Sorted keys put up to 64 lanes on one word, whose atomics would otherwise
+// serialise.  Returns whether the lane's key is new.
+__device__ __forceinline__ bool bm_insert_wave(u64 *words, int kb, bool valid, u64 k) {
+    const int lane = threadIdx.x & 63;
+    const bool dom = valid && bm_in_domain(k, kb);
+    if (valid && !dom) atomicOr((unsigned long long *)&words[bm_flag_word(kb)], 1ull);
+    const u64 b = dom ? bm_bit(k, kb) : 0;
+    const u64 wi = dom ? (b >> 6) : ~0ull;
+    const u64 m = dom ? (1ull << (b & 63)) : 0ull;
+    const u64 pwi = shfl64(wi, lane > 0 ? lane - 1 : 0);
+    const bool head = lane == 0 || pwi != wi;
+    const u64 heads = __ballot(head);
+    const u64 le = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+    const int rid = __popcll(heads & le);  // run number, 1-based
+    // within runs: the OR of the lanes from this one on (the head's is the
+    // run's) and of the lanes up to this one
+    u64 acc = m, pre = m;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int up = lane + d < 64 ? lane + d : lane, dn = lane >= d ? lane - d : lane;
+        const u64 x = shfl64(acc, up), y = shfl64(pre, dn);
+        const int ru = __shfl(rid, up, 64), rd = __shfl(rid, dn, 64);
+        if (lane + d < 64 && ru == rid) acc |= x;
+        if (lane >= d && rd == rid) pre |= y;
+    }
+    const u64 pp = shfl64(pre, lane > 0 ? lane - 1 : 0);
+    const bool dup = dom && !head && (pp & m) != 0;
+    u64 old = 0;
+    if (head && dom) {
+        old = words[wi];
+        if ((old & acc) != acc)
+            old = atomicOr((unsigned long long *)&words[wi], (unsigned long long)acc);
+    }
+    const int h = 63 - __clzll((long long)(heads & le));  // this lane's run head
+    old = shfl64(old, h);
+    return dom && !dup && !(old & m);
+}
+
+// flag word at slots[cap]: 1 when the key ~0 is a member.  A block takes
+// kHsPer * HB consecutive keys a round (thread t keys t, t + HB, ...) and
+// reserves the round's output with one atomic: one counter atomic per wave,
+// all on one address, serialised at a few ns each.
+constexpr int kHsPer = 4;
 __global__ __launch_bounds__(HB) void k_hs_insert_new(u64 *slots, u64 mask, const u64 *keys, u64 n,
                                                       u64 *out, unsigned long long *counter) {
+    __shared__ u32 s_cnt[HB / 64 + 1];
+    __shared__ unsigned long long s_base;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const u64 lt = lane_mask_lt();
-    const u64 stride = (u64)gridDim.x * HB;
-    // every lane of a wave runs the same number of rounds (ballots below)
-    for (u64 base = (u64)blockIdx.x * HB; base < n; base += stride) {
-        const u64 i = base + threadIdx.x;
-        bool fresh = false;
-        u64 k = 0;
-        if (i < n) {
-            k = keys[i];
-            fresh = hs_insert_key(slots, mask, k);
+    const u64 per_round = (u64)HB * kHsPer;
+    // every thread of a block runs the same rounds (barriers below)
+    for (u64 base = (u64)blockIdx.x * per_round; base < n; base += (u64)gridDim.x * per_round) {
+        u64 k[kHsPer], bal[kHsPer];
+        bool fresh[kHsPer];
+        u32 c = 0;
+#pragma unroll
+        for (int j = 0; j < kHsPer; ++j) {
+            const u64 i = base + (u64)j * HB + threadIdx.x;
+            k[j] = 0;
+            fresh[j] = false;
+            if (mask & kHsBitmap) {
+                if (i < n) k[j] = keys[i];
+                fresh[j] = bm_insert_wave(slots, (int)(mask & 63), i < n, k[j]);
+            } else if (i < n) {
+                k[j] = keys[i];
+                fresh[j] = hs_insert_key(slots, mask, k[j]);
+            }
+            bal[j] = __ballot(fresh[j]);
+            c += (u32)__popcll(bal[j]);
         }
-        const u64 bal = __ballot(fresh);
-        if (bal) {
-            unsigned long long pos = 0;
-            const int leader = __ffsll((unsigned long long)bal) - 1;
-            if ((int)(threadIdx.x & 63) == leader)
-                pos = atomicAdd(counter, (unsigned long long)__popcll(bal));
-            pos = __shfl(pos, leader, 64);
-            if (fresh) out[pos + __popcll(bal & lt)] = k;
+        if (lane == 0) s_cnt[wv] = c;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            u32 t = 0;
+            for (int w = 0; w < HB / 64; ++w) {
+                const u32 x = s_cnt[w];
+                s_cnt[w] = t;
+                t += x;
+            }
+            s_base = t ? atomicAdd(counter, (unsigned long long)t) : 0ull;
         }
+        __syncthreads();
+        u64 pos = s_base + s_cnt[wv];
+#pragma unroll
+        for (int j = 0; j < kHsPer; ++j) {
+            if (fresh[j]) out[pos + __popcll(bal[j] & lt)] = k[j];
+            pos += (u64)__popcll(bal[j]);
+        }
+        __syncthreads();  // s_cnt and s_base are rewritten next round
     }
 }
 
@@ -202,7 +277,8 @@ gdl_status hashset_insert_new(gdl_ctx *ctx, gdl_hashset *hs, const u64 *keys, u6
     GDL_HIP(ctx, hipMemsetAsync(counter, 0, sizeof(u64), ctx->stream));
     {
         KTimer kt(ctx, K_HS_INSERT, 24.0 * n);
-        hipLaunchKernelGGL(k_hs_insert_new, dim3(hs_grid(ctx, n)), dim3(HB), 0, ctx->stream,
+        hipLaunchKernelGGL(k_hs_insert_new, dim3(hs_grid(ctx, (n + kHsPer - 1) / kHsPer)),
+                           dim3(HB), 0, ctx->stream,
                            (u64 *)hs->slots, hs_arg(hs), keys, n, d_new,
                            (unsigned long long *)counter);
         GDL_TRY(check_launch(ctx, "k_hs_insert_new"));

@@ -207,13 +207,18 @@ def test_project_and_filter(ctx):
     assert np.array_equal(f, kk[(kk >> np.uint64(32)) != (kk & M32)])
 
 
+# sorted keys put many lanes of a wave on one bitmap word (and repeats on
+# neighbouring lanes): the bitmap insert aggregates them per word
+@pytest.mark.parametrize("order", ["random", "sorted"])
 @pytest.mark.parametrize("set_kind", ["hash", "bitmap"])
-def test_hashset_insert_new_is_set_difference(ctx, set_kind):
+def test_hashset_insert_new_is_set_difference(ctx, set_kind, order):
     hs = ctx.hashset_create(0) if set_kind == "hash" else ctx.hashset_create_bitmap(10)
     seen = set()
     try:
         for batch in range(6):  # the hash table grows and rehashes across batches
             k = rand_keys(150_000 * (batch + 1), 10, 10)  # heavy repeats
+            if order == "sorted":
+                k = np.sort(k)
             if batch == 2 and set_kind == "hash":
                 k = np.concatenate([k, np.array([0xffffffffffffffff] * 3, dtype=np.uint64)])
             new = ctx.hashset_insert_new(hs, k)
