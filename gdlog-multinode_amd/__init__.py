@@ -108,10 +108,11 @@ class ProgBuffers(Structure):
 
 class DenseRule(Structure):
     _fields_ = [("shape", c_int), ("r1", c_void_p), ("n1", c_uint64), ("r2", c_void_p),
-                ("n2", c_uint64)]
+                ("n2", c_uint64), ("flags", c_uint32), ("pad_", c_uint32)]
 
 
 DENSE_TC, DENSE_SG = 1, 2
+DENSE_LOG_LAST = 1
 
 
 class HsSeg(Structure):
@@ -627,7 +628,8 @@ class Context:
         o.free()
         return r
 
-    def fixpoint_dense(self, shape, key_bits, full, delta, r1, r2=None, max_bodies=1 << 20):
+    def fixpoint_dense(self, shape, key_bits, full, delta, r1, r2=None, max_bodies=1 << 20,
+                       flags=0):
         """gdl_fixpoint_dense from host arrays: FULL (a set; also the log's start), the
         delta, the static relations.  Returns dict(new, raw, log, members) with the
         per-body counts, the log (FULL then each body's new tuples) and the bitmap set's
@@ -656,7 +658,7 @@ class Context:
             if d2 is not None:
                 bufs.append(d2)
             rule = DenseRule(shape, d1.ptr, d1.n, d2.ptr if d2 is not None else None,
-                             d2.n if d2 is not None else 0)
+                             d2.n if d2 is not None else 0, flags, 0)
             hn = np.zeros(max_bodies, dtype=np.uint64)
             hr = np.zeros(max_bodies, dtype=np.uint64)
             nb = c_int()

@@ -1245,7 +1245,7 @@ static gdl_status launch_col(gdl_ctx *ctx, const DenseArgs &a, const BodyBufs &b
 template <int K>
 static gdl_status launch_body(gdl_ctx *ctx, int shape, const DenseArgs &a, const BodyBufs &bb,
                               const ColBufs &cb, DenseState *st, u64 *counts, unsigned grid,
-                              unsigned grid_exp) {
+                              unsigned grid_exp, bool append) {
     if (shape == GDL_DENSE_TC) {
         KTimer kt(ctx, K_DENSE_COMBINE, 0.0);
         hipLaunchKernelGGL((k_dense_combine<K, false>), dim3(grid), dim3(DB), 0, ctx->stream, a,
@@ -1269,6 +1269,7 @@ static gdl_status launch_body(gdl_ctx *ctx, int shape, const DenseArgs &a, const
     } else {
         return fail(ctx, GDL_ERR_UNSUPPORTED, "dense SG bodies: ids of at most 16 bits");
     }
+    if (!append) return GDL_OK;
     KTimer kt(ctx, K_DENSE_APPEND, 0.0);
     hipLaunchKernelGGL(k_dense_append<K>, dim3(grid), dim3(DB), 0, ctx->stream, a, bb.pop_out,
                        bb.D_out, st, counts);
@@ -1295,14 +1296,14 @@ static gdl_status launch_append(gdl_ctx *ctx, int kb, const DenseArgs &a, const 
 
 static gdl_status dispatch_body(gdl_ctx *ctx, int shape, int kb, const DenseArgs &a,
                                 const BodyBufs &bb, const ColBufs &cb, DenseState *st,
-                                u64 *counts, unsigned grid, unsigned grid_exp) {
+                                u64 *counts, unsigned grid, unsigned grid_exp, bool append) {
     switch (kb) {
-    case 17: return launch_body<32>(ctx, shape, a, bb, cb, st, counts, grid, grid_exp);
-    case 16: return launch_body<16>(ctx, shape, a, bb, cb, st, counts, grid, grid_exp);
-    case 15: return launch_body<8>(ctx, shape, a, bb, cb, st, counts, grid, grid_exp);
-    case 14: return launch_body<4>(ctx, shape, a, bb, cb, st, counts, grid, grid_exp);
-    case 13: return launch_body<2>(ctx, shape, a, bb, cb, st, counts, grid, grid_exp);
-    default: return launch_body<1>(ctx, shape, a, bb, cb, st, counts, grid, grid_exp);
+    case 17: return launch_body<32>(ctx, shape, a, bb, cb, st, counts, grid, grid_exp, append);
+    case 16: return launch_body<16>(ctx, shape, a, bb, cb, st, counts, grid, grid_exp, append);
+    case 15: return launch_body<8>(ctx, shape, a, bb, cb, st, counts, grid, grid_exp, append);
+    case 14: return launch_body<4>(ctx, shape, a, bb, cb, st, counts, grid, grid_exp, append);
+    case 13: return launch_body<2>(ctx, shape, a, bb, cb, st, counts, grid, grid_exp, append);
+    default: return launch_body<1>(ctx, shape, a, bb, cb, st, counts, grid, grid_exp, append);
     }
 }
 
@@ -1638,6 +1639,7 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
         return fail(ctx, GDL_ERR_INVALID_ARG, "dense bodies: a delta key outside the set's domain");
 
     u64 end = *log_end;
+    const bool log_last = (rule->flags & GDL_DENSE_LOG_LAST) != 0;
     // room for the next bodies at about 4x the delta (a dense delta tends to
     // grow; a body that does not fit grows the log first, at the cost of a host
     // wait and a copy of the log)
@@ -1646,6 +1648,7 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
         (unsigned)std::max<u64>(1, std::min<u64>((a.n_items + DWV - 1) / DWV, grid_cap(ctx)));
     int done = 0, ramp = 4;
     u64 in_delta = n_delta;  // the delta entering the next body
+    u64 last_rows = 0;       // rows with new tuples in the last body
     while (done < max_bodies) {
         a.log = *d_log;
         const int k = std::min({ramp, kChunk, max_bodies - done});
@@ -1660,7 +1663,8 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
         for (int i = 0; i < k; ++i) {
             const int p = (done + i) & 1;
             const BodyBufs bb{pop[p], D[p], pop[p ^ 1], D[p ^ 1], upop, U};
-            GDL_TRY(dispatch_body(ctx, shape, kb, a, bb, col, st, counts, grid, grid_exp));
+            GDL_TRY(dispatch_body(ctx, shape, kb, a, bb, col, st, counts, grid, grid_exp,
+                                  !log_last));
             if (done + i == 0) {
                 // the first delta's pops: a later combine writes the pops of
                 // the rows with preds only, so the rest must read as empty
@@ -1690,11 +1694,13 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
             ctx->stats[K_DENSE_TRANSPOSE].alg_bytes += 64.0 * (double)cb[5];
             ctx->stats[K_DENSE_ORROWS].alg_bytes += row * (double)cb[6];
             // append: the rows with new tuples read, their keys written
-            ctx->stats[K_DENSE_APPEND].alg_bytes += row * (double)cb[4] + 8.0 * (double)cb[0];
+            if (!log_last)
+                ctx->stats[K_DENSE_APPEND].alg_bytes += row * (double)cb[4] + 8.0 * (double)cb[0];
             in_delta = cb[0];
+            last_rows = cb[4];
             hs->size += cb[0];
         }
-        if (r.appended < r.body) {
+        if (!log_last && r.appended < r.body) {
             // the last body's append found the log full: grow it, with room for
             // two more bodies as large (a dense delta tends to stay large for
             // a few bodies), then append
@@ -1715,6 +1721,29 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
         end = r.cursor;
         if (r.body == 0 || r.stop == kDnFixpoint || r.stop == kDnSparse) break;
         ramp = (int)r.body == k ? std::min(2 * ramp, kChunk) : std::max(4, (int)r.body);
+    }
+    if (log_last && done > 0 && h_new[done - 1] > 0) {
+        // the last body's new tuples (its D rows) go to the log: the next delta
+        const u64 nn = h_new[done - 1];
+        GDL_TRY(grow_log(ctx, d_log, log_cap, end, end + nn));
+        a.log = *d_log;
+        DenseState s{};
+        s.cursor = end;
+        s.append_base = end;
+        s.log_cap = *log_cap;
+        s.body = 1;
+        s.max_bodies = 1;
+        s.col_above = ~0ull;
+        GDL_TRY(stg.up(st, &s, sizeof(s)));
+        GDL_TRY(stg.up(counts, &nn, sizeof(nn)));
+        GDL_TRY(launch_append(ctx, kb, a, pop[done & 1], D[done & 1], st, counts, grid));
+        GDL_TRY(stg.down(&r, st, sizeof(r)));
+        GDL_TRY(check_device_error(ctx));
+        if (r.appended != 1 || r.cursor != end + nn)
+            return fail(ctx, GDL_ERR_DEVICE, "dense bodies: the last body's append failed");
+        end = r.cursor;
+        const double row = 8.0 * (double)rw;
+        ctx->stats[K_DENSE_APPEND].alg_bytes += row * (double)last_rows + 8.0 * (double)nn;
     }
     *bodies = done;
     *log_end = end;

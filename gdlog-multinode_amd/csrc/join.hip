@@ -1008,7 +1008,7 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
     const bool tc_map = map.arity == 2 && map.src[0] == GDL_COL_INNER1 &&
                         map.src[1] == GDL_COL_OUTER1 && pred.op == GDL_PRED_NONE &&
                         inner_idx->prefix_cols == 1;
-    const gdl_dense_rule rule{GDL_DENSE_TC, d_inner, n_inner, nullptr, 0};
+    const gdl_dense_rule rule{GDL_DENSE_TC, d_inner, n_inner, nullptr, 0, 0, 0};
     u64 dense_at = tc_map ? dense_threshold_for(ctx, hs, GDL_DENSE_TC, n_inner, 0) : ~0ull;
     int done = 0, ramp = 4;
     while (done < max_bodies) {

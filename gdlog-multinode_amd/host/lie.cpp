@@ -177,6 +177,21 @@ void enter_append_mode(Relation *rel, bool hooks) {
     rel->append_mode = true;
 }
 
+// FULL's log rebuilt from its set (in key order) after bit-matrix bodies that
+// logged only their last body's tuples.
+void materialize_log(Relation *rel) {
+    const tuple_size_t n = rel->current_full_size;
+    uint64_t *keys = n ? device_alloc_keys(n) : nullptr;
+    uint64_t got = 0;
+    gdl_status st = n ? gdl_hashset_extract_sorted(context(), &rel->full_set, keys, n, &got)
+                      : GDL_OK;
+    if (st == GDL_OK && got != n) st = GDL_ERR_DEVICE;
+    if (st != GDL_OK) device_free(keys);
+    check(st, "FULL's log from its set");
+    container_take(rel->full, keys, n, n, true);
+    rel->log_stale = false;
+}
+
 // FULL back to one sorted array (unless keep_log: FULL stays the append log,
 // its iterations' deltas in order); drops the hash set.
 void leave_append_mode(Relation *rel, bool keep_log) {
@@ -186,7 +201,7 @@ void leave_append_mode(Relation *rel, bool keep_log) {
         uint64_t *sorted = device_alloc_keys(n);
         const int kb = (int)rel->full_set.key_bits;
         gdl_status st;
-        if (kb && (1ull << (2 * kb)) / 8 <= 32ull * n) {
+        if (kb && (rel->log_stale || (1ull << (2 * kb)) / 8 <= 32ull * n)) {
             // a bitmap set is FULL in key order: two streaming passes over it
             // instead of a sort of the log (when the bitmap is not much larger
             // than the keys)
@@ -206,6 +221,9 @@ void leave_append_mode(Relation *rel, bool keep_log) {
         check(st, "sorted FULL");
         container_take(rel->full, sorted, n, n, true);
     }
+    if (rel->log_stale && n && !rel->full->sorted)
+        throw Error(GDL_ERR_DEVICE, rel->name + ": FULL's log is stale");
+    rel->log_stale = false;
     check(gdl_hashset_check(ctx, &rel->full_set), "gdl_hashset_check");
     check(gdl_hashset_free(ctx, &rel->full_set), "gdl_hashset_free");
     device_free(rel->pend_keys);
@@ -435,11 +453,12 @@ Chain sg_chain(std::vector<ra_op> &ops, const std::vector<Relation *> &updates,
     return c;
 }
 
-// Runs up to max_bodies bodies of the chain on bit matrices while its delta
-// stays dense (gdl_fixpoint_dense); FULL's log, the set and the delta are
-// left as the per-iteration bodies would leave them.  Returns false when no
-// body ran.
-bool run_dense_chain(const Chain &c, Relation *rel, int max_bodies,
+// Runs up to max_bodies bodies of the chain on bit matrices
+// (gdl_fixpoint_dense); the set and the delta are left as the per-iteration
+// bodies would leave them, and FULL's log too unless log_last: then the log
+// gets only the last body's tuples (the delta) and is marked stale (the set
+// holds FULL).  Returns false when no body ran.
+bool run_dense_chain(const Chain &c, Relation *rel, int max_bodies, bool log_last,
                      std::vector<uint64_t> *new_tuples, std::vector<uint64_t> *raw_tuples) {
     GHashRelContainer *delta = rel->delta, *full = rel->full;
     GHashRelContainer *r1 = c.j1->inner_rel->full, *r2 = c.j2->inner_rel->full;
@@ -449,9 +468,11 @@ bool run_dense_chain(const Chain &c, Relation *rel, int max_bodies,
         !r1->tuple_counts || !r2->keys || !r2->tuple_counts)
         return false;
     const gdl_dense_rule rule{GDL_DENSE_SG, r1->keys, r1->tuple_counts, r2->keys,
-                              r2->tuple_counts};
+                              r2->tuple_counts, log_last ? (uint32_t)GDL_DENSE_LOG_LAST : 0u, 0};
     uint64_t *log = full->keys;
-    uint64_t cap = full->capacity, end = rel->current_full_size;
+    // the log's length: FULL's count, or less when it is stale already
+    uint64_t cap = full->capacity, end = rel->log_stale ? full->tuple_counts : rel->current_full_size;
+    const uint64_t start = rel->current_full_size;
     const uint64_t *d = delta->keys;
     uint64_t nd = delta->tuple_counts;
     constexpr int kSlice = 4096;
@@ -470,13 +491,16 @@ bool run_dense_chain(const Chain &c, Relation *rel, int max_bodies,
         d = log + (end - nd);
         if (nd == 0 || bodies < std::min(left + bodies, kSlice)) break;  // fixpoint or sparse
     }
+    uint64_t added = 0;
+    for (uint64_t v : *new_tuples) added += v;
     if (!new_tuples->empty() || log != full->keys) {
         full->keys = log;
         full->capacity = cap;
         full->tuple_counts = end;
         full->data_raw_row_size = end;
         full->sorted = false;
-        rel->current_full_size = end;
+        rel->current_full_size = start + added;
+        rel->log_stale = end != rel->current_full_size;
     }
     check(st, "gdl_fixpoint_dense");
     if (new_tuples->empty()) return false;
@@ -616,8 +640,12 @@ void LIE::fixpoint_loop() {
                     left = std::min<long long>(left, ck_stop - iteration_counter);
             }
             std::vector<uint64_t> nn, rr;
-            if (left > 0 &&
-                run_dense_chain(chain, rel, (int)std::min<long long>(left, 1 << 30), &nn, &rr)) {
+            // without snapshots or a kept log, FULL's log is read only at the
+            // end of the fixpoint, where the set gives FULL: the bodies log
+            // only their last body's tuples (the delta)
+            const bool log_last = ck_file.empty() && !keep_full_log;
+            if (left > 0 && run_dense_chain(chain, rel, (int)std::min<long long>(left, 1 << 30),
+                                            log_last, &nn, &rr)) {
                 bool fixpoint = false;
                 tuple_size_t full = rel->current_full_size;
                 for (uint64_t v : nn) full -= v;
@@ -637,6 +665,7 @@ void LIE::fixpoint_loop() {
                 }
                 boundary(fixpoint);
                 if (fixpoint || iteration_counter > max_iteration) break;
+                if (rel->log_stale) materialize_log(rel);  // tuple-at-a-time bodies next
                 continue;
             }
         }

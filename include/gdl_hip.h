@@ -349,12 +349,18 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
  * r1 and r2 are the static relations' keys ((c0 << 32) | c1, any order).
  * gdl_fixpoint_linear uses the TC shape by itself when its map is TC's. */
 enum { GDL_DENSE_TC = 1, GDL_DENSE_SG = 2 };
+/* flags: GDL_DENSE_LOG_LAST appends only the last body's new tuples to the
+ * log (the next delta); the other bodies' tuples are in full only, for a
+ * caller that reads FULL from the set afterwards. */
+enum { GDL_DENSE_LOG_LAST = 1 };
 typedef struct gdl_dense_rule {
     int shape;
     const uint64_t *r1;
     uint64_t n1;
     const uint64_t *r2; /* GDL_DENSE_SG only */
     uint64_t n2;
+    uint32_t flags;
+    uint32_t pad_;
 } gdl_dense_rule;
 /* Runs bodies of `rule` from the delta d_delta[0, n_delta) (keys that are
  * members of full, duplicates allowed): each body's new tuples (the set

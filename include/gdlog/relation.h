@@ -105,6 +105,11 @@ struct Relation {
     gdl_hashset tmp_set{};
     uint64_t *pend_keys = nullptr;
     uint64_t pend_cap = 0, pend_n = 0, pend_raw = 0;
+    // Append mode after bit-matrix bodies that logged only their last body's
+    // tuples: FULL's log holds part of FULL (current_full_size counts all of
+    // it) and full_set all of it; the end of the fixpoint reads FULL from the
+    // set.
+    bool log_stale = false;
 
     // Merges DELTA into FULL (relation.cu:405-521).
     void flush_delta(int grid_size, int block_size, float *detail_time);

@@ -237,6 +237,39 @@ def test_dense_sg_bodies(ctx, name, mode):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("shape", ["tc", "sg"])
+def test_dense_log_last_logs_the_last_body_only(ctx, shape):
+    # GDL_DENSE_LOG_LAST: the same bodies and set; the log is FULL's start
+    # plus the last body's new tuples (the next delta)
+    e = np.unique(GRAPHS["er_1000"]())
+    if shape == "tc":
+        full0 = e
+        run = lambda k: tc_bodies(reverse(e), e, e, max_bodies=k)  # noqa: E731
+        args = (gdl.DENSE_TC, 10, e, e, reverse(e))
+    else:
+        full0 = np.unique(np.array(
+            [key(x, y) for p, xs in _by_first(e).items() for x in xs for y in xs if x != y],
+            dtype=np.uint64))
+        run = lambda k: sg_bodies(e, e, full0, full0, max_bodies=k)  # noqa: E731
+        args = (gdl.DENSE_SG, 10, full0, full0, e, e)
+    want, fs = run(5)
+    _, fs_before_last = run(len(want) - 1)
+    ctx.set_dense_mode(2)
+    try:
+        got = ctx.fixpoint_dense(*args, max_bodies=5, flags=gdl.DENSE_LOG_LAST)
+    finally:
+        ctx.set_dense_mode(1)
+    assert list(zip(got["new"], got["raw"])) == want
+    assert got["members"].tolist() == sorted(fs)
+    log = got["log"]
+    last = want[-1][0]
+    assert log.size == full0.size + last
+    assert np.array_equal(np.sort(log[:full0.size]), np.sort(full0))
+    tail = log[full0.size:].tolist()
+    assert len(set(tail)) == len(tail) and set(tail) == fs - fs_before_last
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("mode", [3, 4])
 @pytest.mark.parametrize("kb", [9, 11, 13])
 def test_dense_sg_bodies_two_relations(ctx, kb, mode):
