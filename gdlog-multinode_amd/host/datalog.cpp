@@ -950,16 +950,7 @@ constexpr int kGrid = 32 * 256, kBlock = 512;  // carried through the API only
 // The raw rows become the sorted FULL: the fixpoint seeds delta from the
 // first |FULL| raw rows, which must then be every tuple once.
 void raw_from_full(Relation *r) {
-    device_free(r->raw_keys);
-    r->raw_keys = nullptr;
-    r->raw_rows = 0;
-    const tuple_size_t n = r->full->tuple_counts;
-    if (n) {
-        r->raw_keys = device_alloc_keys(n);
-        check(gdl_memcpy_d2d(context(), r->raw_keys, r->full->keys, n * sizeof(uint64_t)),
-              "raw rows from FULL");
-        r->raw_rows = n;
-    }
+    raw_rows_from_full(r);  // FULL's keys until FULL changes
     r->raw_sorted = true;
 }
 

@@ -37,7 +37,7 @@ void seed_delta(Relation *rel) {
     const tuple_size_t n = std::min(rel->full->tuple_counts, rel->raw_rows);
     if (n == 0) return;
     uint64_t *k = device_alloc_keys(n);
-    check(gdl_memcpy_d2d(context(), k, rel->raw_keys, n * sizeof(uint64_t)), "seed delta");
+    check(gdl_memcpy_d2d(context(), k, raw_rows_ptr(rel), n * sizeof(uint64_t)), "seed delta");
     container_take(rel->delta, k, n, n, rel->raw_sorted);
 }
 
@@ -405,6 +405,7 @@ bool read_snapshot(LIE &lie, const std::string &path, int *iteration, bool *fini
         if (name != rel->name || arity != rel->arity)
             throw Error(GDL_ERR_INVALID_ARG, "checkpoint " + path + ": relation " + rel->name +
                                                  " is not the snapshot's " + name);
+        materialize_raw(rel);  // FULL is replaced below
         uint64_t *full = get_keys(f, nf);
         container_take(rel->full, full, nf, nf, full_sorted);
         rel->current_full_size = nf;
@@ -567,6 +568,9 @@ void LIE::fixpoint_loop() {
             rel->current_full_size = rel->full->tuple_counts;
             seed_delta(rel);
         }
+        // the raw rows outlive this fixpoint's changes to FULL unless its
+        // end reloads them
+        if (!reload_full_flag) materialize_raw(rel);
         if (auto_mode && !full_is_inner(ra_ops, rel) && !needs_sorted_full(ra_ops, rel)) {
             if (resumed && rel->full->sorted && rel->delta->keys && rel->delta->tuple_counts)
                 log_with_delta_tail(rel, keep_full_log);
@@ -846,17 +850,7 @@ void LIE::fixpoint_loop() {
             // lie.cu:219-254: full reloaded from the merged tuples; its rows
             // (data_raw) become the sorted full.
             rel->full->tuple_counts = rel->current_full_size;
-            device_free(rel->raw_keys);
-            rel->raw_keys = nullptr;
-            rel->raw_rows = 0;
-            const tuple_size_t n = rel->current_full_size;
-            if (n) {
-                rel->raw_keys = device_alloc_keys(n);
-                check(gdl_memcpy_d2d(ctx, rel->raw_keys, rel->full->keys, n * sizeof(uint64_t)),
-                      "reload full");
-                rel->raw_rows = n;
-            }
-            rel->raw_sorted = rel->full->sorted;
+            raw_rows_from_full(rel);  // the rows are FULL's keys until FULL changes
             container_free(rel->delta);
             container_free(rel->newt);
         }

@@ -296,6 +296,7 @@ void init_relation(Relation *target, const std::string &name, int arity,
     target->raw_keys = nullptr;
     target->raw_rows = 0;
     target->raw_sorted = false;
+    target->raw_is_full = false;
     target->layout = choose_layout(target, nullptr, 0);
 }
 
@@ -421,13 +422,41 @@ void free_relation(Relation *target) {
     device_free(target->raw_keys);
     target->raw_keys = nullptr;
     target->raw_rows = 0;
+    target->raw_is_full = false;
 }
+
+
+namespace gdlog {
+void raw_rows_from_full(Relation *r) {
+    device_free(r->raw_keys);
+    r->raw_keys = nullptr;
+    r->raw_rows = r->full->keys ? r->full->tuple_counts : 0;
+    r->raw_sorted = r->full->sorted;
+    r->raw_is_full = r->raw_rows > 0;
+}
+
+void materialize_raw(Relation *r) {
+    if (!r->raw_is_full) return;
+    const tuple_size_t n = r->raw_rows;
+    uint64_t *k = device_alloc_keys(n);
+    gdl_status st = gdl_memcpy_d2d(context(), k, r->full->keys, n * sizeof(uint64_t));
+    if (st != GDL_OK) device_free(k);
+    check(st, "raw rows from FULL");
+    r->raw_keys = k;
+    r->raw_is_full = false;
+}
+
+const uint64_t *raw_rows_ptr(const Relation *r) {
+    return r->raw_is_full ? r->full->keys : r->raw_keys;
+}
+}  // namespace gdlog
 
 void Relation::flush_delta(int grid_size, int block_size, float *detail_time) {
     (void)grid_size;
     (void)block_size;
     (void)detail_time;
     if (delta->tuple_counts == 0) return;
+    materialize_raw(this);  // FULL changes below
     container_normalize(delta);
     const tuple_size_t nf = current_full_size, nd = delta->tuple_counts;
     uint64_t *merged = device_alloc_keys(nf + nd);

@@ -84,6 +84,10 @@ struct Relation {
     uint64_t *raw_keys = nullptr;
     tuple_size_t raw_rows = 0;
     bool raw_sorted = false;
+    // The raw rows are FULL's keys (raw_keys unused): set where the rows are
+    // reloaded from FULL (lie.cu:219-254) instead of copying them; whatever
+    // is about to change FULL first makes the copy (materialize_raw).
+    bool raw_is_full = false;
 
     MonotonicOrder monotonic_order = MonotonicOrder::DESC;
 
@@ -187,6 +191,12 @@ const gdl_index *container_index(GHashRelContainer *c, int prefix_cols);
 void container_drop_index(GHashRelContainer *c);
 uint64_t *device_alloc_keys(tuple_size_t n);
 void device_free(void *p);
+// The raw rows become FULL's keys (see Relation::raw_is_full).
+void raw_rows_from_full(Relation *r);
+// Gives the raw rows their own copy when they are FULL's keys.
+void materialize_raw(Relation *r);
+// The raw rows (FULL's keys when raw_is_full).
+const uint64_t *raw_rows_ptr(const Relation *r);
 // Sorted duplicate-free copy of `n` keys (does not consume `d_in`).
 tuple_size_t sort_unique(const uint64_t *d_in, tuple_size_t n, uint64_t **d_out);
 
