@@ -654,8 +654,9 @@ class Context:
                 log, cap = p, c_uint64(int(df.n))
             dd, d1 = self.to_device(delta), self.to_device(r1)
             bufs += [dd, d1]
-            d2 = self.to_device(r2) if r2 is not None else None
-            if d2 is not None:
+            # the same array for r1 and r2 passes one device buffer (one relation)
+            d2 = d1 if r2 is r1 else (self.to_device(r2) if r2 is not None else None)
+            if d2 is not None and d2 is not d1:
                 bufs.append(d2)
             rule = DenseRule(shape, d1.ptr, d1.n, d2.ptr if d2 is not None else None,
                              d2.n if d2 is not None else 0, flags, 0)

@@ -208,6 +208,8 @@ struct DenseState {
     u32 append_blocks;  // blocks of the running append that finished
     u32 bad;            // a delta key outside the domain
     u32 col;            // SG: the running body expands by columns
+    u32 asym;           // set-up: FULL or the first delta differs from its transpose
+    u32 pad_;
 };
 enum : u32 { kDnRun = 0, kDnFixpoint = 1, kDnGrow = 2, kDnLimit = 3, kDnSparse = 4 };
 // per body: new tuples, produced tuples, combine rows, expansion rows, rows
@@ -241,6 +243,12 @@ struct DenseArgs {
     const u64 *heavy;       // bit rows of the lists longer than kHeavy
     const u64 *planes;
     int n_planes;
+    // SG with r1 = r2 and a symmetric start (see dense_run): every delta is
+    // symmetric, D^T = D, so the column expansion skips its first transpose
+    // and its row OR's output V is the tmp T (the combine reads T[x] = V[x])
+    int sym;
+    const u64 *V;
+    const u32 *vpop;
     u64 rw;
     u64 *full;  // FULL's bitmap words
     u64 *log;
@@ -388,6 +396,21 @@ __device__ __forceinline__ u64 plane_dot(const DenseArgs &a, const u64 *t) {
         sum += c << j;
     }
     return sum;
+}
+
+// SG, a symmetric run's column bodies: the tmp row T[x] is V[x], the row
+// OR's output (rows with vpop 0 are empty).  plane_dot of it.
+template <int K>
+__device__ __forceinline__ u64 v_dot(const DenseArgs &a, u32 x) {
+    const int lane = threadIdx.x & 63;
+    const bool live = a.vpop[x] != 0;  // the same for the whole wave
+    u64 t[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const u64 w = (u64)lane + 64u * k;
+        t[k] = (live && (K > 1 || w < a.rw)) ? a.V[(u64)x * a.rw + w] : 0ull;
+    }
+    return live ? plane_dot<K>(a, t) : 0ull;
 }
 
 // SG: U[a] = the union of r2's successor lists over the bits b of D[a] (the
@@ -580,6 +603,9 @@ __global__ __launch_bounds__(DB) void k_dense_combine(DenseArgs a, const u32 *po
                                                       const u64 *U, u32 *pop_out, u64 *D_out,
                                                       DenseState *st, u64 *counts) {
     if (st->stop != kDnRun) return;  // the same for every block
+    // T = V in a symmetric run's column bodies (read before any block can
+    // finish: the epilogue sets the next body's flag)
+    const bool vt = SG && a.sym && st->col != 0;
     const int lane = threadIdx.x & 63;
     const u64 wave = (u64)blockIdx.x * DWV + (threadIdx.x >> 6);
     const u64 n_waves = (u64)gridDim.x * DWV;
@@ -606,9 +632,9 @@ __global__ __launch_bounds__(DB) void k_dense_combine(DenseArgs a, const u32 *po
             if (!SG) raw += p;
             const u64 m = __ballot(p != 0);
             if (m) any = true;
-            flagged += (u64)__popcll(m);
+            if (!vt) flagged += (u64)__popcll(m);
             if constexpr (SG) {
-                or_rows<K>(D_in, a.rw, m, y, acct);
+                if (!vt) or_rows<K>(D_in, a.rw, m, y, acct);
                 const u64 mu = __ballot(q != 0);
                 flagged += (u64)__popcll(mu);
                 or_rows<K>(U, a.rw, mu, y, acc);
@@ -619,9 +645,9 @@ __global__ __launch_bounds__(DB) void k_dense_combine(DenseArgs a, const u32 *po
         if (g == 0xffffffffu) {  // the whole row
             u64 total = 0;
             if (any) {
-                if constexpr (SG) raw += plane_dot<K>(a, acct);
+                if constexpr (SG) raw += vt ? v_dot<K>(a, x) : plane_dot<K>(a, acct);
                 total = wave_sum(diff_row<K>(a, x, acc, D_out));
-                w_io += flagged + 2;
+                w_io += flagged + (vt ? 3 : 2);
             }
             if (lane == 0) pop_out[x] = (u32)total;
             w_new += total;
@@ -639,7 +665,8 @@ __global__ __launch_bounds__(DB) void k_dense_combine(DenseArgs a, const u32 *po
                 if (K > 1 || w < a.rw) {
                     if (acc[k]) atomicOr((unsigned long long *)&gu[w], (unsigned long long)acc[k]);
                     if constexpr (SG)
-                        if (acct[k]) atomicOr((unsigned long long *)&gt[w], (unsigned long long)acct[k]);
+                        if (!vt && acct[k])
+                            atomicOr((unsigned long long *)&gt[w], (unsigned long long)acct[k]);
                 }
             }
             w_io += flagged;
@@ -659,13 +686,18 @@ __global__ __launch_bounds__(DB) void k_dense_combine(DenseArgs a, const u32 *po
                 acc[k] = __hip_atomic_load(&gu[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 __hip_atomic_store(&gu[w], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if constexpr (SG) {
-                    acct[k] = __hip_atomic_load(&gt[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    __hip_atomic_store(&gt[w], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (!vt) {
+                        acct[k] = __hip_atomic_load(&gt[w], __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_store(&gt[w], 0ull, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    }
                 }
             }
         }
-        if constexpr (SG) w_raw += plane_dot<K>(a, acct);
+        if constexpr (SG) w_raw += vt ? v_dot<K>(a, x) : plane_dot<K>(a, acct);
         const u64 total = wave_sum(diff_row<K>(a, x, acc, D_out));
+        if (vt) w_io += 1;
         if (lane == 0) {
             pop_out[x] = (u32)total;
             a.slice_done[g] = 0;
@@ -842,6 +874,42 @@ __global__ __launch_bounds__(TB) void k_bt_transpose(const u64 *src, const u32 *
     }
 }
 
+// Sets *asym when the bit matrix M (rows of rw words; with pop, rows whose
+// pop is 0 read as zero) differs from its transpose: each tile is
+// transposed as k_bt_transpose does and compared with the tile it would be
+// stored over.  Grid (rw / kTW)^2.
+__global__ __launch_bounds__(TB) void k_bt_asym(const u64 *M, const u32 *pop, u64 rw, u32 *asym) {
+    __shared__ u64 s_t[kTW][64][kTW + 1];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const u64 tpd = rw / kTW;
+    const u64 ti = blockIdx.x / tpd, tj = blockIdx.x % tpd;
+    auto load = [&](u64 row, u64 w0, u64 *v) {
+        if (pop == nullptr || pop[row] != 0) {
+            const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(M + row * rw + w0);
+#pragma unroll
+            for (int j = 0; j < kTW / 2; ++j) {
+                const ulonglong2 q = p[j];
+                v[2 * j] = q.x;
+                v[2 * j + 1] = q.y;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kTW; ++j) v[j] = 0;
+        }
+    };
+    u64 v[kTW];
+    load(ti * (64 * kTW) + (u64)wv * 64 + lane, tj * kTW, v);
+#pragma unroll
+    for (int j = 0; j < kTW; ++j)
+        s_t[j][lane][wv] = __ballot(v[j] != 0) ? transpose64(v[j]) : 0ull;
+    __syncthreads();
+    load(tj * (64 * kTW) + (u64)wv * 64 + lane, ti * kTW, v);
+    bool diff = false;
+#pragma unroll
+    for (int t = 0; t < kTW; ++t) diff |= s_t[wv][lane][t] != v[t];
+    if (__ballot(diff) && lane == 0) atomicOr(asym, 1u);
+}
+
 // One static relation's rows as OR work: row x's sources are idx[off[x],
 // off[x + 1]); items as the combine's (x, first, end, split index or ~0),
 // with the split rows' slice counts, finished-slice counters and
@@ -862,10 +930,13 @@ struct OrRows {
 template <int K>
 __global__ __launch_bounds__(DB) void k_dense_or_rows(OrRows o, u64 rw, const u32 *src_pop,
                                                       const u64 *src, u32 *dst_pop, u64 *dst,
+                                                      u32 *zero_pop, u64 n_zero,
                                                       DenseState *st) {
     __shared__ u64 s_io[DWV];
     if (st->stop != kDnRun || !st->col) return;  // the same for every block
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (u64 i = (u64)blockIdx.x * DB + threadIdx.x; i < n_zero; i += (u64)gridDim.x * DB)
+        zero_pop[i] = 0;
     const u64 wave = (u64)blockIdx.x * DWV + wv;
     const u64 n_waves = (u64)gridDim.x * DWV;
     u64 w_io = 0;
@@ -1224,16 +1295,21 @@ struct ColBufs {
 template <int K>
 static gdl_status launch_col(gdl_ctx *ctx, const DenseArgs &a, const BodyBufs &bb,
                              const ColBufs &cb, DenseState *st) {
-    {
+    if (!a.sym) {
         KTimer kt(ctx, K_DENSE_TRANSPOSE, 0.0);  // D^T into U's buffer
         hipLaunchKernelGGL(k_bt_transpose, dim3(cb.grid_tr), dim3(TB), 0, ctx->stream, bb.D_in,
                            bb.pop_in, a.rw, bb.U, cb.dtpop, bb.upop, cb.rows, st);
         GDL_TRY(check_launch(ctx, "k_bt_transpose"));
     }
     {
-        KTimer kt(ctx, K_DENSE_ORROWS, 0.0);  // V = R2^T D^T
+        // V = R2^T D^T (D^T = D when the run is symmetric: then it clears U's
+        // pops, which the transpose below accumulates)
+        KTimer kt(ctx, K_DENSE_ORROWS, 0.0);
+        const u32 *src_pop = a.sym ? bb.pop_in : (const u32 *)cb.dtpop;
+        const u64 *src = a.sym ? bb.D_in : (const u64 *)bb.U;
         hipLaunchKernelGGL(k_dense_or_rows<K>, dim3(cb.grid_or), dim3(DB), 0, ctx->stream, cb.o,
-                           a.rw, (const u32 *)cb.dtpop, (const u64 *)bb.U, cb.vpop, cb.V, st);
+                           a.rw, src_pop, src, cb.vpop, cb.V, a.sym ? bb.upop : (u32 *)nullptr,
+                           a.sym ? cb.rows : (u64)0, st);
         GDL_TRY(check_launch(ctx, "k_dense_or_rows"));
     }
     KTimer kt(ctx, K_DENSE_TRANSPOSE, 0.0);  // U = V^T
@@ -1633,10 +1709,28 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
                            d_delta, n_delta, kb, rw, D[0]);
         GDL_TRY(check_launch(ctx, "k_dense_set_bits"));
     }
+    // SG with r1 = r2 (one relation, as in the reference's program): when FULL
+    // and the first delta are symmetric, every later delta is too (NEW =
+    // R1^T D R1 minus FULL is symmetric when D and FULL are, and FULL grows
+    // by NEW), so the column expansion needs no transpose of D
+    const bool sym_rule = shape == GDL_DENSE_SG && col.on && rule->r1 == rule->r2 &&
+                          rule->n1 == rule->n2;
+    if (sym_rule) {
+        KTimer kt(ctx, K_DENSE_SETUP, 4.0 * 8.0 * rows * rw);
+        hipLaunchKernelGGL(k_bt_asym, dim3(col.grid_tr), dim3(TB), 0, ctx->stream,
+                           (const u64 *)hs->slots, (const u32 *)nullptr, rw, &st->asym);
+        GDL_TRY(check_launch(ctx, "k_bt_asym"));
+        hipLaunchKernelGGL(k_bt_asym, dim3(col.grid_tr), dim3(TB), 0, ctx->stream,
+                           (const u64 *)D[0], (const u32 *)pop[0], rw, &st->asym);
+        GDL_TRY(check_launch(ctx, "k_bt_asym"));
+    }
     DenseState r{};
     GDL_TRY(stg.down(&r, st, sizeof(r)));
     if (r.bad)
         return fail(ctx, GDL_ERR_INVALID_ARG, "dense bodies: a delta key outside the set's domain");
+    a.sym = sym_rule && r.asym == 0 ? 1 : 0;
+    a.V = col.V;
+    a.vpop = col.vpop;
 
     u64 end = *log_end;
     const bool log_last = (rule->flags & GDL_DENSE_LOG_LAST) != 0;

@@ -33,14 +33,6 @@ __device__ void join_01_11(tuple_type inner, tuple_type outer, tuple_type output
 }
 __device__ tuple_generator_hook join_01_11_device = join_01_11;
 
-template <typename T>
-static T host_hook(const T &device_symbol) {
-    T h;
-    (void)hipMemcpyFromSymbol(&h, (const void *)&device_symbol, sizeof(T), 0,
-                              hipMemcpyDeviceToHost);
-    return h;
-}
-
 void cspa_program(const EdgeInput &assign, const EdgeInput &dereference, int grid_size,
                   int block_size, ProgramOutput *out) {
     double t0 = now_ms();

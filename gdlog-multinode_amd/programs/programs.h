@@ -1,10 +1,35 @@
 // programs.h -- shared input/output plumbing of the TC, SG and CSPA programs.
 #pragma once
 
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "gdlog/lie.h"
+
+// The host copy of a __device__ hook pointer (a tuple generator, predicate or
+// copy hook), read with hipMemcpyFromSymbol once per symbol and device: the
+// reference's programs read it on every run, and each read waits for the
+// whole device.
+template <typename T>
+T host_hook(const T &device_symbol) {
+    static std::mutex mu;
+    static std::map<std::pair<const void *, int>, T> cache;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const std::pair<const void *, int> key((const void *)&device_symbol, dev);
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    T h{};
+    if (hipMemcpyFromSymbol(&h, (const void *)&device_symbol, sizeof(T), 0,
+                            hipMemcpyDeviceToHost) != hipSuccess)
+        return h;  // not cached: the operator's hook check reports the null hook
+    cache.emplace(key, h);
+    return h;
+}
 
 // Edge input of a program: host rows of two 8-byte columns (the reference's
 // get_relation_from_file layout) or device-resident packed keys.

@@ -32,12 +32,8 @@ void sg_program(const EdgeInput &graph, int grid_size, int block_size, ProgramOu
     LIE init_scc(grid_size, block_size);
     init_scc.add_relations(edge_2__1_2, true);
     init_scc.add_relations(sg_2__1_2, false);
-    tuple_generator_hook reorder_path_host;
-    (void)hipMemcpyFromSymbol(&reorder_path_host, (const void *)&reorder_path_device,
-                              sizeof(tuple_generator_hook), 0, hipMemcpyDeviceToHost);
-    tuple_predicate tuple_pred_eq_11_host;
-    (void)hipMemcpyFromSymbol(&tuple_pred_eq_11_host, (const void *)&tuple_pred_eq_11_device,
-                              sizeof(tuple_predicate), 0, hipMemcpyDeviceToHost);
+    tuple_generator_hook reorder_path_host = host_hook(reorder_path_device);
+    tuple_predicate tuple_pred_eq_11_host = host_hook(tuple_pred_eq_11_device);
     init_scc.add_ra(RelationalJoin(edge_2__1_2, FULL, edge_2__1_2, FULL, sg_2__1_2,
                                    reorder_path_host, tuple_pred_eq_11_host, LEFT, grid_size,
                                    block_size, join_detail));
@@ -51,9 +47,7 @@ void sg_program(const EdgeInput &graph, int grid_size, int block_size, ProgramOu
     sg_lie.add_relations(edge_2__1_2, true);
     sg_lie.add_relations(sg_2__1_2, false);
     sg_lie.add_tmp_relation(tmp);
-    tuple_generator_hook reorder_path1_host;
-    (void)hipMemcpyFromSymbol(&reorder_path1_host, (const void *)&reorder_path1_device,
-                              sizeof(tuple_generator_hook), 0, hipMemcpyDeviceToHost);
+    tuple_generator_hook reorder_path1_host = host_hook(reorder_path1_device);
     sg_lie.add_ra(RelationalJoin(edge_2__1_2, FULL, sg_2__1_2, DELTA, tmp, reorder_path1_host,
                                  nullptr, LEFT, grid_size, block_size, join_detail));
     sg_lie.add_ra(RelationalJoin(edge_2__1_2, FULL, tmp, NEWT, sg_2__1_2, reorder_path1_host,

@@ -27,9 +27,7 @@ void tc_program(const EdgeInput &graph, int max_iteration, int grid_size, int bl
     tc_scc.add_relations(edge_2__2_1, true);
     tc_scc.add_relations(path_2__1_2, false);
     float join_detail[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    tuple_generator_hook reorder_path_host;
-    (void)hipMemcpyFromSymbol(&reorder_path_host, (const void *)&reorder_path_device,
-                              sizeof(tuple_generator_hook), 0, hipMemcpyDeviceToHost);
+    tuple_generator_hook reorder_path_host = host_hook(reorder_path_device);
     tc_scc.add_ra(RelationalJoin(edge_2__2_1, FULL, path_2__1_2, DELTA, path_2__1_2,
                                  reorder_path_host, nullptr, LEFT, grid_size, block_size,
                                  join_detail));

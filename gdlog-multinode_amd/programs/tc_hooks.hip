@@ -33,13 +33,6 @@ __device__ tuple_predicate not_into_42_device = not_into_42;
 __device__ tuple_generator_hook shift_gen_device = shift_gen;
 __device__ tuple_copy_hook scale_copy_device = scale_copy;
 
-template <typename T>
-T host_hook(const T &sym) {
-    T h = nullptr;
-    (void)hipMemcpyFromSymbol(&h, (const void *)&sym, sizeof(T), 0, hipMemcpyDeviceToHost);
-    return h;
-}
-
 }  // namespace
 
 void tc_hooks_program(const EdgeInput &graph, int variant, int grid_size, int block_size,

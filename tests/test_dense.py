@@ -270,12 +270,14 @@ def test_dense_log_last_logs_the_last_body_only(ctx, shape):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("same", [False, True])
 @pytest.mark.parametrize("mode", [3, 4])
 @pytest.mark.parametrize("kb", [9, 11, 13])
-def test_dense_sg_bodies_two_relations(ctx, kb, mode):
-    # r1 != r2 and a delta that is not symmetric: the column expansion's two
-    # transposes must not lean on SG's symmetry; r2 has hub columns (a row
-    # with more than 32 preds is OR-ed in slices)
+def test_dense_sg_bodies_two_relations(ctx, kb, mode, same):
+    # a delta that is not symmetric, with r1 != r2 or with one relation for
+    # both (whose symmetric shortcut must then be declined): the column
+    # expansion's transposes must not lean on SG's symmetry; r2 has hub
+    # columns (a row with more than 32 preds is OR-ed in slices)
     rng = np.random.default_rng(kb)
     n = 1 << kb
     r1 = np.unique(gdl.pack(rng.integers(0, n, 2 * n), rng.integers(0, n, 2 * n)))
@@ -284,6 +286,8 @@ def test_dense_sg_bodies_two_relations(ctx, kb, mode):
         gdl.pack(rng.integers(0, n, 2 * n), rng.integers(0, n, 2 * n)),
         gdl.pack(rng.integers(0, n, 400), np.repeat(hub, 50)),
         gdl.pack(np.repeat(hub[:2], 70), rng.integers(0, n, 140))]))
+    if same:
+        r1 = r2
     delta = np.unique(gdl.pack(rng.integers(0, n, 2 * n), rng.integers(0, n, 2 * n)))
     want, fs = sg_bodies(r1, r2, delta, delta, max_bodies=3)
     ctx.set_dense_mode(mode)
