@@ -17,7 +17,7 @@ const char *kKernelNames[K_NUM] = {
     "hs_rehash",    "join_insert",  "hs_extract",   "seg_hist",      "mask_cols",
     "linear_body",  "segment_labels", "scatter_by_key", "dense_setup", "dense_expand",
     "dense_combine", "dense_append",
-    "bm_extract_sorted", "dense_transpose", "dense_or_rows"};
+    "bm_extract_sorted", "dense_transpose", "dense_or_rows", "join_dense"};
 
 // ---------------------------------------------------------------- allocator
 
@@ -296,6 +296,8 @@ gdl_status gdl_ctx_create(const gdl_config *cfg_in, gdl_ctx **out) {
     ctx->cfg = cfg;
     ctx->tune.load_factor = cfg.load_factor;
     ctx->stats_on = cfg.collect_stats != 0;
+    if (const char *e = getenv("GDLOG_VERBOSE"))
+        if (*e && *e != '0') ctx->cfg.verbose = 1;  // engine notes on stderr too
     if (const char *e = getenv("GDLOG_DENSE")) {
         const int m = atoi(e);
         ctx->dense_mode = m < 0 ? 0 : (m > 4 ? 4 : m);

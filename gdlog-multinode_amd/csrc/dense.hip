@@ -821,13 +821,14 @@ constexpr int kTW = 8;   // a tile is 64 kTW rows x kTW words (one 64-bit block 
 // registers; LDS regroups them so that each output row's kTW words (64
 // contiguous bytes) are stored by one lane.  Rows with src_pop 0 read as
 // zero (their words may be stale).  dst_pop[r] += the bits stored in row r
-// (dst_pop starts at zero); zero_pop[0, n_zero) is cleared on the way.
+// (dst_pop starts at zero); zero_pop[0, n_zero) is cleared on the way.  With
+// st, it runs only in a column body of a running dense loop.
 __global__ __launch_bounds__(TB) void k_bt_transpose(const u64 *src, const u32 *src_pop, u64 rw,
                                                      u64 *dst, u32 *dst_pop, u32 *zero_pop,
                                                      u64 n_zero, DenseState *st) {
     __shared__ u64 s_t[kTW][64][kTW + 1];
     __shared__ u32 s_live[TB / 64];
-    if (st->stop != kDnRun || !st->col) return;  // the same for every block
+    if (st && (st->stop != kDnRun || !st->col)) return;  // the same for every block
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (u64 i = (u64)blockIdx.x * TB + threadIdx.x; i < n_zero; i += (u64)gridDim.x * TB)
         zero_pop[i] = 0;
@@ -867,7 +868,7 @@ __global__ __launch_bounds__(TB) void k_bt_transpose(const u64 *src, const u32 *
 #pragma unroll
     for (int j = 0; j < kTW / 2; ++j) q[j] = make_ulonglong2(o[2 * j], o[2 * j + 1]);
     if (dst_pop && c) atomicAdd(&dst_pop[orow], c);
-    if (threadIdx.x == 0) {
+    if (st && threadIdx.x == 0) {
         u64 segs = TB;  // the stored ones
         for (int w = 0; w < TB / 64; ++w) segs += s_live[w];
         atomicAdd(&st->tr_io, (unsigned long long)segs);
@@ -1167,6 +1168,261 @@ static unsigned dgrid(gdl_ctx *ctx, u64 n) {
     const u64 g = (n + DB - 1) / DB;
     return (unsigned)std::max<u64>(1, std::min<u64>(g, grid_cap(ctx)));
 }
+
+__global__ __launch_bounds__(DB) void k_fill_u64(u64 *p, u64 n, u64 v) {
+    for (u64 i = (u64)blockIdx.x * DB + threadIdx.x; i < n; i += (u64)gridDim.x * DB) p[i] = v;
+}
+
+// ---- joins on bit matrices.  A column-map join of two binary relations on
+// their column 0 into a bitmap set, when the id domain is small: with M the
+// bit matrix of one side, every tuple (b, d) of the other ORs row M[b] into
+// row d of the result R (out(a, c) :- outer(b, a), inner(b, c) is R = O^T I:
+// M = I, the outer tuples scatter; with the columns swapped, M = O and the
+// inner tuples scatter).  NEW = R minus the set; the set takes NEW; NEW's
+// keys are appended in key order.  The join's produced count is the count
+// pass's exact one.
+
+__global__ __launch_bounds__(DB) void k_dj_set(const u64 *keys, u64 n, u64 rw, u64 *M) {
+    for (u64 i = (u64)blockIdx.x * DB + threadIdx.x; i < n; i += (u64)gridDim.x * DB) {
+        const u64 k = keys[i];
+        const u64 r = k >> 32, c = k & 0xffffffffull;
+        atomicOr((unsigned long long *)&M[r * rw + (c >> 6)], 1ull << (c & 63));
+    }
+}
+
+// R[d] |= M[b] for every tuple (b, d) of keys; L = min(64, rw) lanes a tuple.
+__global__ __launch_bounds__(DB) void k_dj_scatter(const u64 *keys, u64 n, u64 rw, const u64 *M,
+                                                   u64 *R) {
+    const int lane = threadIdx.x & 63;
+    const int L = rw >= 64 ? 64 : (int)rw;
+    const int per = 64 / L;
+    const u64 wave = (u64)blockIdx.x * DWV + (threadIdx.x >> 6);
+    const u64 n_waves = (u64)gridDim.x * DWV;
+    for (u64 t0 = wave * per; t0 < n; t0 += n_waves * per) {
+        const u64 t = t0 + (u64)(lane / L);
+        if (t >= n) continue;
+        const u64 k = keys[t];
+        const u64 *src = M + (k >> 32) * rw;
+        u64 *dst = R + (k & 0xffffffffull) * rw;
+        for (u64 w = (u64)(lane % L); w < rw; w += L) {
+            const u64 v = src[w];
+            if (v) atomicOr((unsigned long long *)&dst[w], (unsigned long long)v);
+        }
+    }
+}
+
+// The same two steps for a matrix that fits in LDS (a small id domain, where
+// global atomics from every tuple would pile up on a few thousand words):
+// block g builds its own copy of the matrix in LDS from its contiguous share
+// of the tuples (SCATTER: row M[b] ORed into row d per tuple (b, d); else bit
+// c of row r per tuple (r, c)) and writes it to part[g]; k_dj_fold ORs the
+// copies.
+template <bool SCATTER>
+__global__ __launch_bounds__(DB) void k_dj_lds(const u64 *keys, u64 n, u64 rw, const u64 *M,
+                                               u64 *part, u64 words) {
+    extern __shared__ u64 s_m[];
+    for (u64 i = threadIdx.x; i < words; i += DB) s_m[i] = 0;
+    __syncthreads();
+    const u64 chunk = (n + gridDim.x - 1) / gridDim.x;
+    const u64 t0 = (u64)blockIdx.x * chunk, t1 = min(n, t0 + chunk);
+    if constexpr (SCATTER) {
+        const u32 L = rw >= 64 ? 64u : (u32)rw;
+        for (u64 t = t0 + threadIdx.x / L; t < t1; t += DB / L) {
+            const u64 k = keys[t];
+            const u64 *src = M + (k >> 32) * rw;
+            u64 *dst = s_m + (k & 0xffffffffull) * rw;
+            for (u64 w = threadIdx.x % L; w < rw; w += L) {
+                const u64 v = src[w];
+                if (v) atomicOr((unsigned long long *)&dst[w], (unsigned long long)v);
+            }
+        }
+    } else {
+        for (u64 t = t0 + threadIdx.x; t < t1; t += DB) {
+            const u64 k = keys[t];
+            const u64 r = k >> 32, c = k & 0xffffffffull;
+            atomicOr((unsigned long long *)&s_m[r * rw + (c >> 6)], 1ull << (c & 63));
+        }
+    }
+    __syncthreads();
+    u64 *out = part + (u64)blockIdx.x * words;
+    for (u64 i = threadIdx.x; i < words; i += DB) out[i] = s_m[i];
+}
+
+// out[i] |= OR of the G copies' word i (out starts at zero): kDjFold threads a
+// word, each ORing every kDjFold-th copy, one atomic each.
+constexpr u32 kDjFold = 16;
+__global__ __launch_bounds__(DB) void k_dj_fold(const u64 *part, u32 G, u64 words, u64 *out) {
+    const u64 n = words * kDjFold;
+    for (u64 t = (u64)blockIdx.x * DB + threadIdx.x; t < n; t += (u64)gridDim.x * DB) {
+        const u64 i = t % words;
+        u64 v = 0;
+        for (u32 g = (u32)(t / words); g < G; g += kDjFold) v |= part[(u64)g * words + i];
+        if (v) atomicOr((unsigned long long *)&out[i], (unsigned long long)v);
+    }
+}
+
+// R := R & ~S, S |= R (one thread a word).
+__global__ __launch_bounds__(DB) void k_dj_diff(u64 *R, u64 *S, u64 n_words) {
+    for (u64 i = (u64)blockIdx.x * DB + threadIdx.x; i < n_words; i += (u64)gridDim.x * DB) {
+        const u64 r = R[i];
+        if (!r) continue;
+        const u64 s = S[i];
+        const u64 nw = r & ~s;
+        R[i] = nw;
+        if (nw) S[i] = s | nw;
+    }
+}
+
+constexpr int kDenseJoinMaxBits = 13;  // matrices of at most 8 MiB
+constexpr u64 kDjLds = 64 * 1024;      // a matrix this small is built in LDS copies
+
+bool dense_join_pays(const gdl_ctx *ctx, const gdl_hashset *hs, gdl_colmap map, gdl_pred pred,
+                     int prefix_cols, u64 n_inner, u64 n_outer, u64 produced) {
+    const int kb = (int)hs->key_bits;
+    // a bitmap set over a small domain that holds every loaded id, a one-column
+    // join, a map of two non-key columns, no predicate
+    if (kb < 6 || kb > kDenseJoinMaxBits || ctx->id_bits <= 0 || ctx->id_bits > kb) return false;
+    if (ctx->dense_mode == 0 || prefix_cols != 1 || map.arity != 2 || pred.op != GDL_PRED_NONE)
+        return false;
+    const bool o_i = map.src[0] == GDL_COL_OUTER1 && map.src[1] == GDL_COL_INNER1;
+    const bool i_o = map.src[0] == GDL_COL_INNER1 && map.src[1] == GDL_COL_OUTER1;
+    if (!o_i && !i_o) return false;
+    if (ctx->dense_mode >= 2) return true;
+    const u64 rows = 1ull << kb, rw = 1ull << (kb - 6), words = rows * rw;
+    // the scattered side: the smaller one where the result can be transposed
+    const u64 n_scatter = kb >= 9 ? std::min(n_outer, n_inner) : (o_i ? n_outer : n_inner);
+    // bytes moved: the matrices cleared, built, differenced and read (~6 passes)
+    // and a row read + OR per scattered tuple, against ~16 bytes per produced
+    // tuple one by one (inner read, set probe), with a margin for the passes'
+    // launches and host wait
+    const double dense_bytes = 8.0 * (6.0 * (double)words + (double)n_scatter * (double)rw);
+    return 16.0 * (double)produced > 4.0 * dense_bytes + 4.0e6;
+}
+
+bool dense_join_insert(gdl_ctx *ctx, const u64 *d_inner, u64 n_inner, const u64 *d_outer,
+                       u64 n_outer, gdl_colmap map, gdl_pred pred, int prefix_cols,
+                       gdl_hashset *hs, u64 produced, u64 **d_new, u64 *new_cap, u64 *n_new,
+                       u64 *n_raw, gdl_status *status) {
+    if (!dense_join_pays(ctx, hs, map, pred, prefix_cols, n_inner, n_outer, produced))
+        return false;
+    const int kb = (int)hs->key_bits;
+    const bool o_i = map.src[0] == GDL_COL_OUTER1;
+    const u64 rows = 1ull << kb, rw = 1ull << (kb - 6), words = rows * rw;
+    // scatter over the outer tuples with M = I (row a of O^T I per outer tuple
+    // (b, a)) or over the inner ones with M = O (row c of I^T O): whichever side
+    // is smaller, transposing when that gives the map's transpose
+    const bool scatter_outer = kb >= 9 ? n_outer <= n_inner : o_i;
+    const bool transpose = scatter_outer != o_i;
+    const u64 n_scatter = scatter_outer ? n_outer : n_inner;
+    auto run = [&]() -> gdl_status {
+        Scratch scr(ctx);
+        u64 *M, *R;
+        GDL_TRY(scr.get(words, &M));
+        GDL_TRY(scr.get(words, &R));
+        const unsigned gw = dgrid(ctx, words);
+        hipLaunchKernelGGL(k_fill_u64, dim3(gw), dim3(DB), 0, ctx->stream, M, words, 0ull);
+        GDL_TRY(check_launch(ctx, "k_fill_u64"));
+        hipLaunchKernelGGL(k_fill_u64, dim3(gw), dim3(DB), 0, ctx->stream, R, words, 0ull);
+        GDL_TRY(check_launch(ctx, "k_fill_u64"));
+        const u64 *mk = scatter_outer ? d_inner : d_outer;
+        const u64 nm = scatter_outer ? n_inner : n_outer;
+        const u64 *sk = scatter_outer ? d_outer : d_inner;
+        {
+            KTimer kt(ctx, K_JOIN_DENSE, 8.0 * (3.0 * words + nm + n_scatter) +
+                                             8.0 * (double)n_scatter * (double)rw);
+            if (words * sizeof(u64) <= kDjLds) {
+                // block-private copies in LDS, folded
+                const u64 cus = (u64)std::max(ctx->cu_count, 1);
+                const u32 G = (u32)std::max<u64>(
+                    1, std::min<u64>(cus, (std::max(nm, n_scatter) + 4095) / 4096));
+                u64 *part;
+                GDL_TRY(scr.get((u64)G * words, &part));
+                const size_t lds = words * sizeof(u64);
+                const unsigned gf = dgrid(ctx, words * kDjFold);
+                if (nm) {
+                    hipLaunchKernelGGL(k_dj_lds<false>, dim3(G), dim3(DB), lds, ctx->stream, mk,
+                                       nm, rw, (const u64 *)nullptr, part, words);
+                    GDL_TRY(check_launch(ctx, "k_dj_lds"));
+                    hipLaunchKernelGGL(k_dj_fold, dim3(gf), dim3(DB), 0, ctx->stream,
+                                       (const u64 *)part, G, words, M);
+                    GDL_TRY(check_launch(ctx, "k_dj_fold"));
+                }
+                hipLaunchKernelGGL(k_dj_lds<true>, dim3(G), dim3(DB), lds, ctx->stream, sk,
+                                   n_scatter, rw, (const u64 *)M, part, words);
+                GDL_TRY(check_launch(ctx, "k_dj_lds"));
+                hipLaunchKernelGGL(k_dj_fold, dim3(gf), dim3(DB), 0, ctx->stream,
+                                   (const u64 *)part, G, words, R);
+                GDL_TRY(check_launch(ctx, "k_dj_fold"));
+            } else {
+                if (nm) {
+                    hipLaunchKernelGGL(k_dj_set, dim3(dgrid(ctx, nm)), dim3(DB), 0, ctx->stream,
+                                       mk, nm, rw, M);
+                    GDL_TRY(check_launch(ctx, "k_dj_set"));
+                }
+                const u64 per = rw >= 64 ? 1 : 64 / rw;
+                const u64 sg = std::max<u64>(
+                    1, std::min<u64>((n_scatter + per * DWV - 1) / (per * DWV), grid_cap(ctx)));
+                hipLaunchKernelGGL(k_dj_scatter, dim3((unsigned)sg), dim3(DB), 0, ctx->stream, sk,
+                                   n_scatter, rw, (const u64 *)M, R);
+                GDL_TRY(check_launch(ctx, "k_dj_scatter"));
+            }
+            if (transpose) {  // R^T into M (no longer needed), then swap the names
+                hipLaunchKernelGGL(k_bt_transpose, dim3((unsigned)((rw / kTW) * (rw / kTW))),
+                                   dim3(TB), 0, ctx->stream, (const u64 *)R, (const u32 *)nullptr,
+                                   rw, M, (u32 *)nullptr, (u32 *)nullptr, (u64)0,
+                                   (DenseState *)nullptr);
+                GDL_TRY(check_launch(ctx, "k_bt_transpose"));
+                std::swap(M, R);
+            }
+            hipLaunchKernelGGL(k_dj_diff, dim3(gw), dim3(DB), 0, ctx->stream, R, (u64 *)hs->slots,
+                               words);
+            GDL_TRY(check_launch(ctx, "k_dj_diff"));
+        }
+        // NEW's keys, in key order, after the keys already in *d_new
+        const int rounds = xt_rounds(words);
+        const u64 tiles = (words + 64ull * rounds - 1) / (64ull * rounds);
+        const unsigned blocks = (unsigned)((tiles + DWV - 1) / DWV);
+        u64 *tc;
+        GDL_TRY(scr.get(tiles + 1, &tc));
+        {
+            KTimer kt(ctx, K_BM_EXTRACT, 8.0 * words);
+            hipLaunchKernelGGL(k_xt_count, dim3(blocks), dim3(DB), 0, ctx->stream, (const u64 *)R,
+                               words, tiles, rounds, (const u64 *)nullptr, (const u32 *)nullptr,
+                               0, tc);
+            GDL_TRY(check_launch(ctx, "k_xt_count"));
+        }
+        GDL_TRY(scan_inplace(ctx, tc, tiles));
+        u64 total = 0;
+        GDL_TRY(read_u64(ctx, tc + tiles, &total));
+        if (*n_new + total > *new_cap || !*d_new) {
+            const u64 cap = *n_new + total + (*n_new + total) / 4 + 1024;
+            void *p = nullptr;
+            std::string err;
+            gdl_status st = ctx->alloc->alloc(cap * sizeof(u64), &p, &err);
+            if (st != GDL_OK) return fail(ctx, st, err);
+            if (*n_new)
+                GDL_HIP(ctx, hipMemcpyAsync(p, *d_new, *n_new * sizeof(u64),
+                                            hipMemcpyDeviceToDevice, ctx->stream));
+            if (*d_new) ctx->alloc->free(*d_new, nullptr);
+            *d_new = (u64 *)p;
+            *new_cap = cap;
+        }
+        if (total) {
+            KTimer kt(ctx, K_BM_EXTRACT, 8.0 * words + 8.0 * total);
+            hipLaunchKernelGGL(k_xt_write, dim3(blocks), dim3(DB), 0, ctx->stream, (const u64 *)R,
+                               words, tiles, rounds, (const u64 *)nullptr, (const u32 *)nullptr,
+                               0, kb, (const u64 *)tc, *d_new + *n_new, total);
+            GDL_TRY(check_launch(ctx, "k_xt_write"));
+        }
+        hs->size += total;
+        *n_new += total;
+        *n_raw = produced;
+        return GDL_OK;
+    };
+    *status = run();
+    return true;
+}
+
 
 // Device memory owned by the context allocator for the lifetime of a run.
 struct DevBufs {

@@ -86,6 +86,7 @@ enum KernelId {
     K_BM_EXTRACT,
     K_DENSE_TRANSPOSE,
     K_DENSE_ORROWS,
+    K_JOIN_DENSE,
     K_NUM
 };
 extern const char *kKernelNames[K_NUM];
@@ -267,6 +268,19 @@ constexpr u32 kMaxSegs = 256;
 // context's dense mode runs them (~0: never), and the run itself
 // (gdl_fixpoint_dense's contract).
 u64 dense_threshold_for(const gdl_ctx *ctx, const gdl_hashset *hs, int shape, u64 n1, u64 n2);
+// Whether a join into the bitmap set hs producing `produced` tuples would run
+// on bit matrices (the conditions of dense_join_insert).
+bool dense_join_pays(const gdl_ctx *ctx, const gdl_hashset *hs, gdl_colmap map, gdl_pred pred,
+                     int prefix_cols, u64 n_inner, u64 n_outer, u64 produced);
+// A join whose exact produced count is `produced`, into the bitmap set hs,
+// run on bit matrices when that is much cheaper than producing its tuples
+// one by one: returns false (nothing done) when the join does not qualify.
+// Otherwise *status is the outcome, with gdl_join_insert's results (new keys
+// appended to *d_new in key order, *n_new, *n_raw = produced).
+bool dense_join_insert(gdl_ctx *ctx, const u64 *d_inner, u64 n_inner, const u64 *d_outer,
+                       u64 n_outer, gdl_colmap map, gdl_pred pred, int prefix_cols,
+                       gdl_hashset *hs, u64 produced, u64 **d_new, u64 *new_cap, u64 *n_new,
+                       u64 *n_raw, gdl_status *status);
 gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
                      const u64 *d_delta, u64 n_delta, u64 **d_log, u64 *log_cap, u64 *log_end,
                      int max_bodies, u64 *h_new, u64 *h_raw, int *bodies);

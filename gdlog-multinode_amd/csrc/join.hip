@@ -178,7 +178,7 @@ __global__ __launch_bounds__(JB) void k_join_materialize(const u64 *inner, const
 // keys are staged per wave in LDS and flushed with one atomic per kStage keys
 // (counters[0] is the output cursor); counters[1] counts produced tuples.
 constexpr u32 kStage = 256;
-constexpr u64 kHubRun = 256;  // longer inner runs use the load-balanced variant
+constexpr u64 kHubRun = 32;  // longer inner runs may use the load-balanced variant
 
 // Writes stay below out_cap; a cursor past it tells the host the caller's
 // output bound was wrong.
@@ -813,7 +813,11 @@ static gdl_status join_insert_impl(gdl_ctx *ctx, const u64 *d_inner, u64 n_inner
     // the caller has sized both already (the output capacity is checked after
     // the kernel), so only the load-balanced kernel below counts, for its
     // offsets
-    const bool counted = !SEG && bound > loose_limit;
+    // (and a join that would run on bit matrices if its bound were its count)
+    bool counted = !SEG && bound > loose_limit;
+    if constexpr (!SEG)
+        counted = counted || dense_join_pays(ctx, hs, map, pred, inner_idx->prefix_cols, n_inner,
+                                             n_outer, bound);
     Scratch lb(ctx);
     u32 *counts = nullptr;
     u64 *begins = nullptr, *offsets = nullptr;
@@ -825,6 +829,14 @@ static gdl_status join_insert_impl(gdl_ctx *ctx, const u64 *d_inner, u64 n_inner
         GDL_TRY(exclusive_scan_u32_to_u64(ctx, counts, n_outer, offsets, &bound));
     }
     if (bound == 0) return GDL_OK;
+    if constexpr (!SEG) {
+        // many tuples per new one over a small id domain: bit matrices
+        gdl_status ds = GDL_OK;
+        if (counted && dense_join_insert(ctx, d_inner, n_inner, d_outer, n_outer, map, pred,
+                                         inner_idx->prefix_cols, hs, bound, d_new, new_cap,
+                                         n_new, n_raw, &ds))
+            return ds;
+    }
     // The wave-flattened kernel is balanced inside a wave but not across
     // waves: a wave whose 64 outer tuples hit a hub walks the whole hub run,
     // one step (64 outputs) per hub tuple.  Number the outputs and spread
@@ -833,7 +845,17 @@ static gdl_status join_insert_impl(gdl_ctx *ctx, const u64 *d_inner, u64 n_inner
     const u64 waves = std::min<u64>((n_outer + fb - 1) / fb, cap_blocks) * (fb / 64);
     const u64 avg_run = std::max<u64>(1, n_inner / std::max<u64>(1, inner_idx->n_distinct));
     const u64 est_total = counted ? bound : n_outer * avg_run;
-    const bool balanced = mr > kHubRun && mr * 16 > est_total / waves;
+    // (or when there are too few waves for the work: each would walk many
+    // 64-output steps, each a chain of dependent loads)
+    const bool balanced =
+        mr > kHubRun && (mr * 16 > est_total / waves || est_total / (waves * 64) > 32);
+    if (ctx->cfg.verbose)
+        fprintf(stderr,
+                "gdl: join_insert outer %llu inner %llu max_run %llu produced %s%llu set %s%d -> %s\n",
+                (unsigned long long)n_outer, (unsigned long long)n_inner, (unsigned long long)mr,
+                counted ? "" : "<= ", (unsigned long long)bound,
+                hs && hs->key_bits ? "bitmap " : "hash ", hs ? (int)hs->key_bits : 0,
+                balanced ? "balanced" : "wave-flattened");
     if (balanced && !counted) {
         GDL_TRY(lb.get(n_outer, &counts));
         GDL_TRY(lb.get(n_outer, &begins));

@@ -346,6 +346,9 @@ PROGRAMS = {
     "tc_tree": ("tc", lambda: gdl.gen_tree(7, 20000, 300)),
     "sg_rmat_12": ("sg", lambda: gdl.gen_rmat(1, 12, 6000)),
     "sg_er_500": ("sg", lambda: gdl.gen_er(2, 500, 1100)),
+    # joins of many tuples per new one over a small id domain run on bit
+    # matrices (modes 2-4 force them on every join into a bitmap set)
+    "cspa_zipf_150": ("cspa", lambda: gdl.gen_cspa(5, 150, 1.1, 200, 100)),
 }
 
 
@@ -354,7 +357,10 @@ PROGRAMS = {
 def test_programs_same_with_dense_bodies_off_auto_forced(ctx, name):
     kind, make = PROGRAMS[name]
     e = make()
-    run = ctx.run_tc if kind == "tc" else ctx.run_sg
+    if kind == "cspa":
+        run = lambda inp: ctx.run_cspa(*inp, out_cap=2_000_000, extra_cap=2_000_000)  # noqa: E731
+    else:
+        run = ctx.run_tc if kind == "tc" else ctx.run_sg
     res = []
     for mode in (0, 1, 2, 3, 4):
         ctx.set_dense_mode(mode)
@@ -367,7 +373,11 @@ def test_programs_same_with_dense_bodies_off_auto_forced(ctx, name):
         assert r["iterations"] == res[0]["iterations"]
         assert r["trace"] == res[0]["trace"]  # produced counts included
         assert r["trace_init"] == res[0]["trace_init"]
-    want = oracle.tc(e) if kind == "tc" else oracle.sg(e)
+    want = {"tc": oracle.tc, "sg": oracle.sg}[kind](e) if kind != "cspa" else oracle.cspa(*e)
     assert np.array_equal(res[2]["out"], want["out"])
+    if kind == "cspa":
+        for r in res:
+            assert r["sizes"] == want["sizes"]
+            assert all(np.array_equal(g, w) for g, w in zip(r["extra"], want["extra"]))
     assert [(t[0], t[3], t[4]) for t in res[2]["trace"]] == \
         [(t[0], t[3], t[4]) for t in want["trace"]]
