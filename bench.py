@@ -314,7 +314,7 @@ def batched(gdl, ctx, n_inst, min_edges=1000, max_edges=50000, window=1000):
 
 
 SERVED_MIX = (("tc_er_2000_5000", 0.6), ("tc_tree_200k", 0.1), ("sg_rmat15_40k", 0.1),
-              ("cspa_zipf_500", 0.2))
+              ("cspa_stated_1000", 0.2))
 
 
 SERVED_LOADS = (0.5, 1.0, 1.5, 2.5, 4.0)  # offered load, x the one-at-a-time capacity
@@ -323,7 +323,8 @@ SERVED_SLO_MS = 1000.0                   # the p99 latency both sides are held t
 
 def served(gdl, ctx, n_req, seed=7, workers=1, loads=SERVED_LOADS, slo_ms=SERVED_SLO_MS):
     """Served mix: queries drawn from SERVED_MIX (TC / SG at the sizes of the
-    single-query workloads, CSPA at 500 ids), offered as Poisson arrivals at
+    single-query workloads, CSPA at the stated input ratio (3 assign and 1.5
+    dereference facts per id) on 1,000 ids), offered as Poisson arrivals at
     several loads (multiples of the one-at-a-time engine's capacity).  At
     every load both sides run for real on the same arrivals: the query server
     (continuous batching, gdl_serve.h) and a one-at-a-time loop that takes the
@@ -347,7 +348,7 @@ def served(gdl, ctx, n_req, seed=7, workers=1, loads=SERVED_LOADS, slo_ms=SERVED
         elif k == 2:
             reqs.append((i, gdl.QUERY_SG, gdl.gen_rmat(s, 15, 40000), None))
         else:
-            a, d = gdl.gen_cspa(s, 500, 1.1, 750, 375)
+            a, d = gdl.gen_cspa(s, 1000, 1.1, 3000, 1500)
             reqs.append((i, gdl.QUERY_CSPA, a, d))
     run = {gdl.QUERY_TC: lambda a, b: ctx.run_tc(a, want_keys=False, want_trace=False),
            gdl.QUERY_SG: lambda a, b: ctx.run_sg(a, want_keys=False, want_trace=False),

@@ -1182,11 +1182,42 @@ __global__ __launch_bounds__(DB) void k_fill_u64(u64 *p, u64 n, u64 v) {
 // keys are appended in key order.  The join's produced count is the count
 // pass's exact one.
 
+// Bit c of row r of M per key (r, c).  Lanes on one word (a run of
+// neighbouring lanes: a sorted relation puts up to 64 keys of a wave on a
+// few words) OR their bits together first, and the run's first lane makes
+// the one atomic.
 __global__ __launch_bounds__(DB) void k_dj_set(const u64 *keys, u64 n, u64 rw, u64 *M) {
-    for (u64 i = (u64)blockIdx.x * DB + threadIdx.x; i < n; i += (u64)gridDim.x * DB) {
-        const u64 k = keys[i];
-        const u64 r = k >> 32, c = k & 0xffffffffull;
-        atomicOr((unsigned long long *)&M[r * rw + (c >> 6)], 1ull << (c & 63));
+    const int lane = threadIdx.x & 63;
+    const u64 stride = (u64)gridDim.x * DB;
+    // every lane of a wave runs the same rounds (shuffles below)
+    for (u64 base = (u64)blockIdx.x * DB + (threadIdx.x & ~63u); base < n; base += stride) {
+        const u64 i = base + lane;
+        u64 wi = ~0ull, m = 0;
+        if (i < n) {
+            const u64 k = keys[i];
+            const u64 r = k >> 32, c = k & 0xffffffffull;
+            wi = r * rw + (c >> 6);
+            m = 1ull << (c & 63);
+        }
+        const int prev = lane > 0 ? lane - 1 : 0;
+        const u64 pwi = ((u64)(u32)__shfl((int)(wi >> 32), prev, 64) << 32) |
+                        (u64)(u32)__shfl((int)(u32)wi, prev, 64);
+        const bool head = lane == 0 || pwi != wi;
+        const u64 heads = __ballot(head);
+        const u64 le = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+        const int rid = __popcll(heads & le);
+        u64 acc = m;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int up = lane + d < 64 ? lane + d : lane;
+            // every lane shuffles (a lane outside the branch would not)
+            const u64 x = ((u64)(u32)__shfl((int)(acc >> 32), up, 64) << 32) |
+                          (u64)(u32)__shfl((int)(u32)acc, up, 64);
+            const int ru = __shfl(rid, up, 64);
+            if (lane + d < 64 && ru == rid) acc |= x;
+        }
+        if (head && wi != ~0ull && acc)
+            atomicOr((unsigned long long *)&M[wi], (unsigned long long)acc);
     }
 }
 
