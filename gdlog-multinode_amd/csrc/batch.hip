@@ -91,8 +91,15 @@ __global__ __launch_bounds__(BB) void k_hs_checksum(const u64 *slots, u64 cap, u
 __global__ __launch_bounds__(BB) void k_checksum(const u64 *keys, u64 n, unsigned long long *sum) {
     __shared__ u64 s_part[BB / 64];
     u64 acc = 0;
-    for (u64 i = (u64)blockIdx.x * BB + threadIdx.x; i < n; i += (u64)gridDim.x * BB)
-        acc += mix64(keys[i]);
+    const u64 stride = (u64)gridDim.x * BB;
+    u64 i = (u64)blockIdx.x * BB + threadIdx.x;
+    // four independent loads in flight per lane, then the remainder
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const u64 a = keys[i], b = keys[i + stride], c = keys[i + 2 * stride],
+                  d = keys[i + 3 * stride];
+        acc += (mix64(a) + mix64(b)) + (mix64(c) + mix64(d));
+    }
+    for (; i < n; i += stride) acc += mix64(keys[i]);
     for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
     if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = acc;
     __syncthreads();

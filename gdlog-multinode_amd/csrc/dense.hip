@@ -117,38 +117,86 @@ __device__ __forceinline__ u64 xt_word(const u64 *words, const u64 *minus, const
     return w;
 }
 
+// R rounds per tile: every round's words are loaded before the first is
+// processed (R loads in flight per lane).
+template <int R>
 __global__ __launch_bounds__(DB) void k_xt_count(const u64 *words, u64 n_words, u64 n_tiles,
-                                                 int rounds, const u64 *minus,
-                                                 const u32 *minus_pop, int rs, u64 *tile_counts) {
+                                                 const u64 *minus, const u32 *minus_pop, int rs,
+                                                 u64 *tile_counts) {
     const u64 tile = (u64)blockIdx.x * DWV + (threadIdx.x >> 6);
     if (tile >= n_tiles) return;  // the same for the whole wave
     const int lane = threadIdx.x & 63;
-    const u64 t0 = tile * 64ull * rounds;
-    u64 c = 0;
-    for (int r = 0; r < rounds; ++r) {
+    const u64 t0 = tile * 64ull * R;
+    u64 w[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
         const u64 i = t0 + (u64)r * 64 + lane;
-        if (i < n_words) c += (u64)__popcll(xt_word(words, minus, minus_pop, rs, i));
+        w[r] = i < n_words ? xt_word(words, minus, minus_pop, rs, i) : 0ull;
     }
+    u64 c = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) c += (u64)__popcll(w[r]);
     c = wave_sum(c);
     if (lane == 0) tile_counts[tile] = c;
 }
 
+template <int R>
 __global__ __launch_bounds__(DB) void k_xt_write(const u64 *words, u64 n_words, u64 n_tiles,
-                                                 int rounds, const u64 *minus,
-                                                 const u32 *minus_pop, int rs, int kb,
-                                                 const u64 *tile_offsets, u64 *out,
+                                                 const u64 *minus, const u32 *minus_pop, int rs,
+                                                 int kb, const u64 *tile_offsets, u64 *out,
                                                  u64 out_cap) {
     __shared__ uint16_t s_buf[DWV][4096];
     const u64 tile = (u64)blockIdx.x * DWV + (threadIdx.x >> 6);
     if (tile >= n_tiles) return;  // the same for the whole wave
     const int lane = threadIdx.x & 63;
-    const u64 t0 = tile * 64ull * rounds;
-    u64 base = tile_offsets[tile];
-    for (int r = 0; r < rounds; ++r) {
-        const u64 r0 = t0 + (u64)r * 64, i = r0 + lane;
-        const u64 w = i < n_words ? xt_word(words, minus, minus_pop, rs, i) : 0ull;
-        base += stage_write(w, r0 * 64, kb, out, base, out_cap, s_buf[threadIdx.x >> 6]);
+    const u64 t0 = tile * 64ull * R;
+    u64 w[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const u64 i = t0 + (u64)r * 64 + lane;
+        w[r] = i < n_words ? xt_word(words, minus, minus_pop, rs, i) : 0ull;
     }
+    u64 base = tile_offsets[tile];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        base += stage_write(w[r], (t0 + (u64)r * 64) * 64, kb, out, base, out_cap,
+                            s_buf[threadIdx.x >> 6]);
+}
+
+// Pass 1 and pass 2 of the extraction over `tiles` tiles of 64 * rounds words.
+static gdl_status launch_xt_count(gdl_ctx *ctx, int rounds, const u64 *words, u64 nw, u64 tiles,
+                                  const u64 *minus, const u32 *minus_pop, int rs, u64 *tc) {
+    const unsigned blocks = (unsigned)((tiles + DWV - 1) / DWV);
+#define GDL_XT_COUNT(R)                                                                        \
+    hipLaunchKernelGGL(k_xt_count<R>, dim3(blocks), dim3(DB), 0, ctx->stream, words, nw, tiles, \
+                       minus, minus_pop, rs, tc)
+    switch (rounds) {
+    case 1: GDL_XT_COUNT(1); break;
+    case 2: GDL_XT_COUNT(2); break;
+    case 4: GDL_XT_COUNT(4); break;
+    case 8: GDL_XT_COUNT(8); break;
+    default: GDL_XT_COUNT(16); break;
+    }
+#undef GDL_XT_COUNT
+    return check_launch(ctx, "k_xt_count");
+}
+
+static gdl_status launch_xt_write(gdl_ctx *ctx, int rounds, const u64 *words, u64 nw, u64 tiles,
+                                  const u64 *minus, const u32 *minus_pop, int rs, int kb,
+                                  const u64 *tc, u64 *out, u64 out_cap) {
+    const unsigned blocks = (unsigned)((tiles + DWV - 1) / DWV);
+#define GDL_XT_WRITE(R)                                                                        \
+    hipLaunchKernelGGL(k_xt_write<R>, dim3(blocks), dim3(DB), 0, ctx->stream, words, nw, tiles, \
+                       minus, minus_pop, rs, kb, tc, out, out_cap)
+    switch (rounds) {
+    case 1: GDL_XT_WRITE(1); break;
+    case 2: GDL_XT_WRITE(2); break;
+    case 4: GDL_XT_WRITE(4); break;
+    case 8: GDL_XT_WRITE(8); break;
+    default: GDL_XT_WRITE(16); break;
+    }
+#undef GDL_XT_WRITE
+    return check_launch(ctx, "k_xt_write");
 }
 
 static u64 bitmap_words(int kb) { return kb >= 3 ? (1ull << (2 * kb - 6)) : 1ull; }
@@ -161,16 +209,13 @@ static gdl_status bitmap_extract_sorted(gdl_ctx *ctx, const u64 *words, int kb, 
     const u64 nw = bitmap_words(kb);
     const int rounds = xt_rounds(nw);
     const u64 tiles = (nw + 64ull * rounds - 1) / (64ull * rounds);
-    const unsigned blocks = (unsigned)((tiles + DWV - 1) / DWV);
     const int rs = kb >= 6 ? kb - 6 : 0;
     Scratch scr(ctx);
     u64 *tc;
     GDL_TRY(scr.get(tiles + 1, &tc));
     {
         KTimer kt(ctx, K_BM_EXTRACT, 8.0 * nw);
-        hipLaunchKernelGGL(k_xt_count, dim3(blocks), dim3(DB), 0, ctx->stream, words, nw, tiles,
-                           rounds, minus, minus_pop, rs, tc);
-        GDL_TRY(check_launch(ctx, "k_xt_count"));
+        GDL_TRY(launch_xt_count(ctx, rounds, words, nw, tiles, minus, minus_pop, rs, tc));
     }
     GDL_TRY(scan_inplace(ctx, tc, tiles));
     u64 total = 0;
@@ -181,9 +226,8 @@ static gdl_status bitmap_extract_sorted(gdl_ctx *ctx, const u64 *words, int kb, 
                                                " members, output holds " + std::to_string(out_cap));
     if (total == 0) return GDL_OK;
     KTimer kt(ctx, K_BM_EXTRACT, 8.0 * nw + 8.0 * total);
-    hipLaunchKernelGGL(k_xt_write, dim3(blocks), dim3(DB), 0, ctx->stream, words, nw, tiles,
-                       rounds, minus, minus_pop, rs, kb, tc, d_out, out_cap);
-    return check_launch(ctx, "k_xt_write");
+    return launch_xt_write(ctx, rounds, words, nw, tiles, minus, minus_pop, rs, kb, tc, d_out,
+                           out_cap);
 }
 
 // ---------------------------------------------------------------------------
@@ -199,6 +243,7 @@ struct DenseState {
     unsigned long long rows_new;  // rows that got new tuples
     unsigned long long tr_io;     // SG column expansion: 64-byte segments the transposes moved
     unsigned long long or_io;     // and the row reads + writes of its row OR
+    unsigned long long d0_bits;   // set-up: distinct tuples of the first delta
     u64 col_above;     // SG: a body whose delta has at least this many tuples expands by columns
     u64 log_cap;
     u64 sparse_below;  // a body with fewer new tuples ends the run
@@ -209,7 +254,7 @@ struct DenseState {
     u32 bad;            // a delta key outside the domain
     u32 col;            // SG: the running body expands by columns
     u32 asym;           // set-up: FULL or the first delta differs from its transpose
-    u32 pad_;
+    u32 d0_out;         // set-up: the first delta has a tuple FULL lacks
 };
 enum : u32 { kDnRun = 0, kDnFixpoint = 1, kDnGrow = 2, kDnLimit = 3, kDnSparse = 4 };
 // per body: new tuples, produced tuples, combine rows, expansion rows, rows
@@ -878,12 +923,20 @@ __global__ __launch_bounds__(TB) void k_bt_transpose(const u64 *src, const u32 *
 // Sets *asym when the bit matrix M (rows of rw words; with pop, rows whose
 // pop is 0 read as zero) differs from its transpose: each tile is
 // transposed as k_bt_transpose does and compared with the tile it would be
-// stored over.  Grid (rw / kTW)^2.
-__global__ __launch_bounds__(TB) void k_bt_asym(const u64 *M, const u32 *pop, u64 rw, u32 *asym) {
+// stored over.  Grid (rw / kTW)^2.  Returns at once when `same` records
+// that M has exactly the same_n tuples of a bitmap it lies within.
+__global__ __launch_bounds__(TB) void k_bt_asym(const u64 *M, const u32 *pop, u64 rw,
+                                                u32 *asym, const DenseState *same,
+                                                u64 same_n) {
     __shared__ u64 s_t[kTW][64][kTW + 1];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const u64 tpd = rw / kTW;
     const u64 ti = blockIdx.x / tpd, tj = blockIdx.x % tpd;
+    // tile (ti, tj) against (tj, ti) is the same test as (tj, ti) against
+    // (ti, tj): the upper triangle suffices
+    if (ti > tj) return;
+    // M holds exactly the tuples of a matrix checked already
+    if (same && same->d0_bits == same_n && same->d0_out == 0) return;
     auto load = [&](u64 row, u64 w0, u64 *v) {
         if (pop == nullptr || pop[row] != 0) {
             const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(M + row * rw + w0);
@@ -1114,13 +1167,49 @@ __global__ __launch_bounds__(DB) void k_dense_zero_rows(const u32 *pop, u64 rows
         if (pop[i / rw]) D[i] = 0ull;
 }
 
+// Sets the delta's bits in D.  Lanes of a wave with keys in the same word
+// (delta keys usually come sorted) OR their bits together first and the
+// first lane of each run of equal words does the atomic.  With `full` (a
+// bitmap of the same layout), counts the bits that were not yet set into
+// st->d0_bits and flags st->d0_out when a bit is not in `full`.
 __global__ __launch_bounds__(DB) void k_dense_set_bits(const u64 *keys, u64 n, int kb, u64 rw,
-                                                       u64 *D) {
-    for (u64 i = (u64)blockIdx.x * DB + threadIdx.x; i < n; i += (u64)gridDim.x * DB) {
-        const u64 k = keys[i];
-        const u64 c0 = k >> 32, c1 = k & 0xffffffffull;
-        if ((c0 >> kb) || (c1 >> kb)) continue;
-        atomicOr((unsigned long long *)&D[c0 * rw + (c1 >> 6)], 1ull << (c1 & 63));
+                                                       u64 *D, const u64 *full,
+                                                       DenseState *st) {
+    const int lane = threadIdx.x & 63;
+    const u64 wave = (u64)blockIdx.x * DWV + (threadIdx.x >> 6);
+    const u64 n_waves = (u64)gridDim.x * DWV;
+    u64 fresh = 0;
+    bool out = false;
+    for (u64 base = wave * 64; base < n; base += n_waves * 64) {
+        const u64 i = base + lane;
+        u64 wi = ~0ull - (u64)lane;  // distinct per lane when not a key
+        u64 bits = 0;
+        if (i < n) {
+            const u64 k = keys[i];
+            const u64 c0 = k >> 32, c1 = k & 0xffffffffull;
+            if (!(c0 >> kb) && !(c1 >> kb)) {
+                wi = c0 * rw + (c1 >> 6);
+                bits = 1ull << (c1 & 63);
+            }
+        }
+        // suffix OR over the lanes after this one that hold the same word
+        for (int d = 1; d < 64; d <<= 1) {
+            const u64 ow = __shfl_down(wi, d, 64), ob = __shfl_down(bits, d, 64);
+            if (lane + d < 64 && ow == wi) bits |= ob;
+        }
+        const u64 pw = __shfl_up(wi, 1, 64);
+        if (bits && (lane == 0 || pw != wi)) {
+            const u64 old = atomicOr((unsigned long long *)&D[wi], bits);
+            fresh += (u64)__popcll(bits & ~old);
+            if (full && (bits & ~full[wi])) out = true;
+        }
+    }
+    if (!full) return;
+    fresh = wave_sum(fresh);
+    const bool any_out = __ballot(out) != 0;
+    if (lane == 0) {
+        if (fresh) atomicAdd(&st->d0_bits, (unsigned long long)fresh);
+        if (any_out) atomicOr(&st->d0_out, 1u);
     }
 }
 
@@ -1412,15 +1501,12 @@ bool dense_join_insert(gdl_ctx *ctx, const u64 *d_inner, u64 n_inner, const u64 
         // NEW's keys, in key order, after the keys already in *d_new
         const int rounds = xt_rounds(words);
         const u64 tiles = (words + 64ull * rounds - 1) / (64ull * rounds);
-        const unsigned blocks = (unsigned)((tiles + DWV - 1) / DWV);
         u64 *tc;
         GDL_TRY(scr.get(tiles + 1, &tc));
         {
             KTimer kt(ctx, K_BM_EXTRACT, 8.0 * words);
-            hipLaunchKernelGGL(k_xt_count, dim3(blocks), dim3(DB), 0, ctx->stream, (const u64 *)R,
-                               words, tiles, rounds, (const u64 *)nullptr, (const u32 *)nullptr,
-                               0, tc);
-            GDL_TRY(check_launch(ctx, "k_xt_count"));
+            GDL_TRY(launch_xt_count(ctx, rounds, (const u64 *)R, words, tiles, nullptr, nullptr, 0,
+                                    tc));
         }
         GDL_TRY(scan_inplace(ctx, tc, tiles));
         u64 total = 0;
@@ -1440,10 +1526,8 @@ bool dense_join_insert(gdl_ctx *ctx, const u64 *d_inner, u64 n_inner, const u64 
         }
         if (total) {
             KTimer kt(ctx, K_BM_EXTRACT, 8.0 * words + 8.0 * total);
-            hipLaunchKernelGGL(k_xt_write, dim3(blocks), dim3(DB), 0, ctx->stream, (const u64 *)R,
-                               words, tiles, rounds, (const u64 *)nullptr, (const u32 *)nullptr,
-                               0, kb, (const u64 *)tc, *d_new + *n_new, total);
-            GDL_TRY(check_launch(ctx, "k_xt_write"));
+            GDL_TRY(launch_xt_write(ctx, rounds, (const u64 *)R, words, tiles, nullptr, nullptr,
+                                    0, kb, (const u64 *)tc, *d_new + *n_new, total));
         }
         hs->size += total;
         *n_new += total;
@@ -1984,6 +2068,12 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
     GDL_TRY(scr.get(kCounts * kChunk, &counts));
     DenseState *st = (DenseState *)st_words;
     GDL_HIP(ctx, hipMemsetAsync(st, 0, sizeof(DenseState), ctx->stream));
+    // SG with r1 = r2 (one relation, as in the reference's program): when FULL
+    // and the first delta are symmetric, every later delta is too (NEW =
+    // R1^T D R1 minus FULL is symmetric when D and FULL are, and FULL grows
+    // by NEW), so the column expansion needs no transpose of D
+    const bool sym_rule = shape == GDL_DENSE_SG && col.on && rule->r1 == rule->r2 &&
+                          rule->n1 == rule->n2;
     {
         KTimer kt(ctx, K_DENSE_SETUP, 24.0 * n_delta + 8.0 * rows * rw);
         hipLaunchKernelGGL(k_dense_mark, dim3(dgrid(ctx, n_delta)), dim3(DB), 0, ctx->stream,
@@ -1993,22 +2083,22 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
                            ctx->stream, pop[0], rows, rw, D[0]);
         GDL_TRY(check_launch(ctx, "k_dense_zero_rows"));
         hipLaunchKernelGGL(k_dense_set_bits, dim3(dgrid(ctx, n_delta)), dim3(DB), 0, ctx->stream,
-                           d_delta, n_delta, kb, rw, D[0]);
+                           d_delta, n_delta, kb, rw, D[0],
+                           sym_rule ? (const u64 *)hs->slots : (const u64 *)nullptr, st);
         GDL_TRY(check_launch(ctx, "k_dense_set_bits"));
     }
-    // SG with r1 = r2 (one relation, as in the reference's program): when FULL
-    // and the first delta are symmetric, every later delta is too (NEW =
-    // R1^T D R1 minus FULL is symmetric when D and FULL are, and FULL grows
-    // by NEW), so the column expansion needs no transpose of D
-    const bool sym_rule = shape == GDL_DENSE_SG && col.on && rule->r1 == rule->r2 &&
-                          rule->n1 == rule->n2;
     if (sym_rule) {
-        KTimer kt(ctx, K_DENSE_SETUP, 4.0 * 8.0 * rows * rw);
+        // FULL's words read once; the first delta's check, when it runs,
+        // is not counted
+        KTimer kt(ctx, K_DENSE_SETUP, 8.0 * rows * rw);
         hipLaunchKernelGGL(k_bt_asym, dim3(col.grid_tr), dim3(TB), 0, ctx->stream,
-                           (const u64 *)hs->slots, (const u32 *)nullptr, rw, &st->asym);
+                           (const u64 *)hs->slots, (const u32 *)nullptr, rw, &st->asym,
+                           (const DenseState *)nullptr, (u64)0);
         GDL_TRY(check_launch(ctx, "k_bt_asym"));
+        // skipped on the device when the first delta is all of FULL
         hipLaunchKernelGGL(k_bt_asym, dim3(col.grid_tr), dim3(TB), 0, ctx->stream,
-                           (const u64 *)D[0], (const u32 *)pop[0], rw, &st->asym);
+                           (const u64 *)D[0], (const u32 *)pop[0], rw, &st->asym,
+                           (const DenseState *)st, (u64)hs->size);
         GDL_TRY(check_launch(ctx, "k_bt_asym"));
     }
     DenseState r{};
