@@ -143,7 +143,31 @@ Staging::~Staging() {
 
 gdl_status Staging::sync() {
     GDL_HIP(ctx_, hipStreamSynchronize(ctx_->stream));
+    land_pending();
     used_ = 0;
+    return GDL_OK;
+}
+
+void Staging::land_pending() {
+    for (const Pending &p : pending_) memcpy(p.h_dst, ctx_->h_stage + p.at, p.n);
+    pending_.clear();
+}
+
+gdl_status Staging::later(void *h_dst, const void *d_src, size_t n) {
+    if (!n) return GDL_OK;
+    char *at;
+    GDL_TRY(room(n, &at));
+    GDL_HIP(ctx_, hipMemcpyAsync(at, d_src, n, hipMemcpyDeviceToHost, ctx_->stream));
+    pending_.push_back({h_dst, (size_t)(at - ctx_->h_stage), n});
+    return GDL_OK;
+}
+
+gdl_status Staging::land() { return sync(); }
+
+gdl_status Staging::land_at(hipEvent_t ev) {
+    // work enqueued after the event may still read the buffer: it stays in use
+    GDL_HIP(ctx_, hipEventSynchronize(ev));
+    land_pending();
     return GDL_OK;
 }
 
@@ -190,6 +214,10 @@ gdl_status check_device_error(gdl_ctx *ctx) {
                                 ctx->stream));
     GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
     memcpy(&err, ctx->h_pinned, sizeof(u32));
+    return device_error_status(ctx, err);
+}
+
+gdl_status device_error_status(gdl_ctx *ctx, u32 err) {
     if (err) {
         u32 zero = 0;
         (void)hipMemcpy(ctx->d_err, &zero, sizeof(u32), hipMemcpyHostToDevice);

@@ -1762,6 +1762,17 @@ struct DenseHostScratch {
     std::vector<uint4> items, items2;
 };
 
+// An event the host waits on, destroyed with the scope.
+struct HostEvent {
+    hipEvent_t e = nullptr;
+    HostEvent() = default;
+    HostEvent(const HostEvent &) = delete;
+    HostEvent &operator=(const HostEvent &) = delete;
+    ~HostEvent() {
+        if (e) (void)hipEventDestroy(e);
+    }
+};
+
 static DenseHostScratch &dense_host_scratch() {
     static thread_local DenseHostScratch h;
     return h;
@@ -1874,17 +1885,111 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
                        rows, act, (unsigned long long *)n_act_d);
     GDL_TRY(check_launch(ctx, "k_dense_act"));
     a.act = act;
-    DenseHostScratch &H = dense_host_scratch();
-    std::vector<u64> &h_soff = H.soff;  // SG: r2's row offsets on the host
+    u64 *succ_off = nullptr;
+    u32 *succ_idx = nullptr;
     if (shape == GDL_DENSE_SG) {
-        u64 *succ_off;
-        u32 *succ_idx;
         GDL_TRY(group_by_col(ctx, mem, rule->r2, rule->n2, 0, kb, rows, cnt, &succ_off, &succ_idx,
                              bad));
         a.succ_off = succ_off;
         a.succ_idx = succ_idx;
+    }
+    // SG's column expansion: r2 grouped by its second column, as OR work
+    ColBufs col{};
+    const u64 col_above =
+        shape == GDL_DENSE_SG ? col_threshold(ctx->dense_mode, kb, rule->n2) : ~0ull;
+    u64 *off2 = nullptr;
+    u32 *idx2 = nullptr;
+    if (col_above != ~0ull)
+        GDL_TRY(group_by_col(ctx, mem, rule->r2, rule->n2, 1, kb, rows, cnt, &off2, &idx2, bad));
+
+    // the row offsets and the active rows to the host in one wait, which the
+    // first delta's bit rows (enqueued next) run behind
+    DenseHostScratch &H = dense_host_scratch();
+    std::vector<u64> &h_soff = H.soff;  // SG: r2's row offsets on the host
+    std::vector<u64> &h_off = H.off;
+    std::vector<u64> &h_off2 = H.off2;
+    std::vector<u32> &h_act = H.act;
+    u64 n_act = 0;
+    u32 h_bad = 0;
+    h_off.resize(rows + 1);
+    h_act.resize(rows);
+    GDL_TRY(stg.later(&n_act, n_act_d, sizeof(u64)));
+    GDL_TRY(stg.later(&h_bad, bad, sizeof(u32)));
+    GDL_TRY(stg.later(h_off.data(), pred_off, (rows + 1) * sizeof(u64)));
+    GDL_TRY(stg.later(h_act.data(), act, rows * sizeof(u32)));
+    if (succ_off) {
         h_soff.resize(rows + 1);
-        GDL_TRY(stg.down(h_soff.data(), succ_off, (rows + 1) * sizeof(u64)));
+        GDL_TRY(stg.later(h_soff.data(), succ_off, (rows + 1) * sizeof(u64)));
+    }
+    if (off2) {
+        h_off2.resize(rows + 1);
+        GDL_TRY(stg.later(h_off2.data(), off2, (rows + 1) * sizeof(u64)));
+    }
+    HostEvent grouped;
+    GDL_HIP(ctx, hipEventCreateWithFlags(&grouped.e, hipEventDisableTiming));
+    GDL_HIP(ctx, hipEventRecord(grouped.e, ctx->stream));
+
+    // the delta as bit rows (buffer 0)
+    u64 *D[2];
+    u32 *pop[2];
+    GDL_TRY(mem.get(rows * rw, &D[0]));
+    GDL_TRY(mem.get(rows * rw, &D[1]));
+    GDL_TRY(mem.get(rows, &pop[0]));
+    GDL_TRY(mem.get(rows, &pop[1]));
+    for (int p = 0; p < 2; ++p) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(dgrid(ctx, rows)), dim3(DB), 0, ctx->stream, pop[p],
+                           rows, 0u);
+        GDL_TRY(check_launch(ctx, "k_fill_u32"));
+    }
+    constexpr int kChunk = 64;
+    u64 *st_words, *counts;
+    GDL_TRY(scr.get((sizeof(DenseState) + 7) / 8, &st_words));
+    GDL_TRY(scr.get(kCounts * kChunk, &counts));
+    DenseState *st = (DenseState *)st_words;
+    GDL_HIP(ctx, hipMemsetAsync(st, 0, sizeof(DenseState), ctx->stream));
+    // SG with r1 = r2 (one relation, as in the reference's program): when FULL
+    // and the first delta are symmetric, every later delta is too (NEW =
+    // R1^T D R1 minus FULL is symmetric when D and FULL are, and FULL grows
+    // by NEW), so the column expansion needs no transpose of D.  Checked
+    // whenever the column expansion may run.
+    const bool sym_cand = col_above != ~0ull && rule->r1 == rule->r2 && rule->n1 == rule->n2;
+    {
+        KTimer kt(ctx, K_DENSE_SETUP, 24.0 * n_delta + 8.0 * rows * rw);
+        hipLaunchKernelGGL(k_dense_mark, dim3(dgrid(ctx, n_delta)), dim3(DB), 0, ctx->stream,
+                           d_delta, n_delta, kb, pop[0], st);
+        GDL_TRY(check_launch(ctx, "k_dense_mark"));
+        hipLaunchKernelGGL(k_dense_zero_rows, dim3(dgrid(ctx, rows * rw)), dim3(DB), 0,
+                           ctx->stream, pop[0], rows, rw, D[0]);
+        GDL_TRY(check_launch(ctx, "k_dense_zero_rows"));
+        hipLaunchKernelGGL(k_dense_set_bits, dim3(dgrid(ctx, n_delta)), dim3(DB), 0, ctx->stream,
+                           d_delta, n_delta, kb, rw, D[0],
+                           sym_cand ? (const u64 *)hs->slots : (const u64 *)nullptr, st);
+        GDL_TRY(check_launch(ctx, "k_dense_set_bits"));
+    }
+    if (sym_cand) {
+        // FULL's words read once; the first delta's check, when it runs,
+        // is not counted
+        KTimer kt(ctx, K_DENSE_SETUP, 8.0 * rows * rw);
+        const unsigned grid_tr = (unsigned)((rw / kTW) * (rw / kTW));
+        hipLaunchKernelGGL(k_bt_asym, dim3(grid_tr), dim3(TB), 0, ctx->stream,
+                           (const u64 *)hs->slots, (const u32 *)nullptr, rw, &st->asym,
+                           (const DenseState *)nullptr, (u64)0);
+        GDL_TRY(check_launch(ctx, "k_bt_asym"));
+        // skipped on the device when the first delta is all of FULL
+        hipLaunchKernelGGL(k_bt_asym, dim3(grid_tr), dim3(TB), 0, ctx->stream,
+                           (const u64 *)D[0], (const u32 *)pop[0], rw, &st->asym,
+                           (const DenseState *)st, (u64)hs->size);
+        GDL_TRY(check_launch(ctx, "k_bt_asym"));
+    }
+
+    GDL_TRY(stg.land_at(grouped.e));
+    a.n_act = n_act;
+    // ids beyond the set's domain in a static relation: the tuple-at-a-time
+    // bodies handle (and report) them
+    if (h_bad || a.n_act == 0) return GDL_OK;
+    h_act.resize(a.n_act);
+
+    if (shape == GDL_DENSE_SG) {
         // the longest lists (more than kHeavy successors) become bit rows
         std::vector<u32> &hb = H.hb;
         hb.clear();
@@ -1921,24 +2026,9 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
         GDL_TRY(check_launch(ctx, "k_succ_tab"));
         a.succ_tab = tab;
     }
-    u64 setup[2] = {0, 0};
-    u32 h_bad = 0;
-    GDL_TRY(stg.down(&setup[0], n_act_d, sizeof(u64)));
-    GDL_TRY(stg.down(&h_bad, bad, sizeof(u32)));
-    setup[1] = h_bad;
-    a.n_act = setup[0];
-    // ids beyond the set's domain in a static relation: the tuple-at-a-time
-    // bodies handle (and report) them
-    if (setup[1] || a.n_act == 0) return GDL_OK;
 
     // work items: the active rows, most preds first, split into slices of at
     // most kSlice preds (on the host: a few hundred thousand rows at most)
-    std::vector<u64> &h_off = H.off;
-    std::vector<u32> &h_act = H.act;
-    h_off.resize(rows + 1);
-    h_act.resize(a.n_act);
-    GDL_TRY(stg.down(h_off.data(), pred_off, (rows + 1) * sizeof(u64)));
-    GDL_TRY(stg.down(h_act.data(), act, a.n_act * sizeof(u32)));
     std::vector<uint4> &h_items = H.items;
     std::vector<u32> &h_slices = H.slices;
     build_items(h_off, h_act, h_items, h_slices);
@@ -1994,17 +2084,7 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
         GDL_TRY(mem.get(rows, &upop));
         grid_exp = (unsigned)std::max<u64>(1, std::min<u64>(a.n_exp, grid_cap(ctx)));
     }
-    // SG's column expansion: r2 grouped by its second column, as OR work
-    ColBufs col{};
-    const u64 col_above =
-        shape == GDL_DENSE_SG ? col_threshold(ctx->dense_mode, kb, rule->n2) : ~0ull;
-    if (col_above != ~0ull) {
-        u64 *off2;
-        u32 *idx2;
-        GDL_TRY(group_by_col(ctx, mem, rule->r2, rule->n2, 1, kb, rows, cnt, &off2, &idx2, bad));
-        std::vector<u64> &h_off2 = H.off2;
-        h_off2.resize(rows + 1);
-        GDL_TRY(stg.down(h_off2.data(), off2, (rows + 1) * sizeof(u64)));
+    if (off2) {
         std::vector<u32> &h_act2 = H.act2;
         h_act2.clear();
         for (u64 r = 0; r < rows; ++r)
@@ -2050,57 +2130,7 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
         }
     }
 
-    // the delta as bit rows (buffer 0)
-    u64 *D[2];
-    u32 *pop[2];
-    GDL_TRY(mem.get(rows * rw, &D[0]));
-    GDL_TRY(mem.get(rows * rw, &D[1]));
-    GDL_TRY(mem.get(rows, &pop[0]));
-    GDL_TRY(mem.get(rows, &pop[1]));
-    for (int p = 0; p < 2; ++p) {
-        hipLaunchKernelGGL(k_fill_u32, dim3(dgrid(ctx, rows)), dim3(DB), 0, ctx->stream, pop[p],
-                           rows, 0u);
-        GDL_TRY(check_launch(ctx, "k_fill_u32"));
-    }
-    constexpr int kChunk = 64;
-    u64 *st_words, *counts;
-    GDL_TRY(scr.get((sizeof(DenseState) + 7) / 8, &st_words));
-    GDL_TRY(scr.get(kCounts * kChunk, &counts));
-    DenseState *st = (DenseState *)st_words;
-    GDL_HIP(ctx, hipMemsetAsync(st, 0, sizeof(DenseState), ctx->stream));
-    // SG with r1 = r2 (one relation, as in the reference's program): when FULL
-    // and the first delta are symmetric, every later delta is too (NEW =
-    // R1^T D R1 minus FULL is symmetric when D and FULL are, and FULL grows
-    // by NEW), so the column expansion needs no transpose of D
-    const bool sym_rule = shape == GDL_DENSE_SG && col.on && rule->r1 == rule->r2 &&
-                          rule->n1 == rule->n2;
-    {
-        KTimer kt(ctx, K_DENSE_SETUP, 24.0 * n_delta + 8.0 * rows * rw);
-        hipLaunchKernelGGL(k_dense_mark, dim3(dgrid(ctx, n_delta)), dim3(DB), 0, ctx->stream,
-                           d_delta, n_delta, kb, pop[0], st);
-        GDL_TRY(check_launch(ctx, "k_dense_mark"));
-        hipLaunchKernelGGL(k_dense_zero_rows, dim3(dgrid(ctx, rows * rw)), dim3(DB), 0,
-                           ctx->stream, pop[0], rows, rw, D[0]);
-        GDL_TRY(check_launch(ctx, "k_dense_zero_rows"));
-        hipLaunchKernelGGL(k_dense_set_bits, dim3(dgrid(ctx, n_delta)), dim3(DB), 0, ctx->stream,
-                           d_delta, n_delta, kb, rw, D[0],
-                           sym_rule ? (const u64 *)hs->slots : (const u64 *)nullptr, st);
-        GDL_TRY(check_launch(ctx, "k_dense_set_bits"));
-    }
-    if (sym_rule) {
-        // FULL's words read once; the first delta's check, when it runs,
-        // is not counted
-        KTimer kt(ctx, K_DENSE_SETUP, 8.0 * rows * rw);
-        hipLaunchKernelGGL(k_bt_asym, dim3(col.grid_tr), dim3(TB), 0, ctx->stream,
-                           (const u64 *)hs->slots, (const u32 *)nullptr, rw, &st->asym,
-                           (const DenseState *)nullptr, (u64)0);
-        GDL_TRY(check_launch(ctx, "k_bt_asym"));
-        // skipped on the device when the first delta is all of FULL
-        hipLaunchKernelGGL(k_bt_asym, dim3(col.grid_tr), dim3(TB), 0, ctx->stream,
-                           (const u64 *)D[0], (const u32 *)pop[0], rw, &st->asym,
-                           (const DenseState *)st, (u64)hs->size);
-        GDL_TRY(check_launch(ctx, "k_bt_asym"));
-    }
+    const bool sym_rule = sym_cand && col.on;
     DenseState r{};
     GDL_TRY(stg.down(&r, st, sizeof(r)));
     if (r.bad)
@@ -2145,9 +2175,12 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
             }
         }
         u64 c[kCounts * kChunk];
-        GDL_TRY(stg.down(&r, st, sizeof(r)));
-        GDL_TRY(stg.down(c, counts, sizeof(c)));
-        GDL_TRY(check_device_error(ctx));
+        u32 dev_err = 0;
+        GDL_TRY(stg.later(&r, st, sizeof(r)));
+        GDL_TRY(stg.later(c, counts, sizeof(c)));
+        GDL_TRY(stg.later(&dev_err, ctx->d_err, sizeof(u32)));
+        GDL_TRY(stg.land());
+        GDL_TRY(device_error_status(ctx, dev_err));
         for (u32 b = 0; b < r.body; ++b, ++done) {
             const u64 *cb = c + kCounts * b;
             h_new[done] = cb[0];
@@ -2183,8 +2216,10 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
             r.log_cap = *log_cap;
             GDL_TRY(stg.up(st, &r, sizeof(r)));
             GDL_TRY(launch_append(ctx, kb, a, pop[done & 1], D[done & 1], st, counts, grid));
-            GDL_TRY(stg.down(&r, st, sizeof(r)));
-            GDL_TRY(check_device_error(ctx));
+            GDL_TRY(stg.later(&r, st, sizeof(r)));
+            GDL_TRY(stg.later(&dev_err, ctx->d_err, sizeof(u32)));
+            GDL_TRY(stg.land());
+            GDL_TRY(device_error_status(ctx, dev_err));
             if (r.appended != r.body)
                 return fail(ctx, GDL_ERR_OVERFLOW, "dense bodies: the log did not grow");
             r.stop = stop == kDnGrow ? kDnLimit : stop;
@@ -2208,8 +2243,11 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
         GDL_TRY(stg.up(st, &s, sizeof(s)));
         GDL_TRY(stg.up(counts, &nn, sizeof(nn)));
         GDL_TRY(launch_append(ctx, kb, a, pop[done & 1], D[done & 1], st, counts, grid));
-        GDL_TRY(stg.down(&r, st, sizeof(r)));
-        GDL_TRY(check_device_error(ctx));
+        u32 dev_err = 0;
+        GDL_TRY(stg.later(&r, st, sizeof(r)));
+        GDL_TRY(stg.later(&dev_err, ctx->d_err, sizeof(u32)));
+        GDL_TRY(stg.land());
+        GDL_TRY(device_error_status(ctx, dev_err));
         if (r.appended != 1 || r.cursor != end + nn)
             return fail(ctx, GDL_ERR_DEVICE, "dense bodies: the last body's append failed");
         end = r.cursor;

@@ -214,6 +214,8 @@ gdl_status read_u64(gdl_ctx *ctx, const u64 *d, u64 *h, int count = 1);
 gdl_status noalloc_setup(gdl_ctx *ctx);
 gdl_status check_launch(gdl_ctx *ctx, const char *what);
 gdl_status check_device_error(gdl_ctx *ctx);
+// The status for a device error word read with other values (resets it).
+gdl_status device_error_status(gdl_ctx *ctx, u32 err);
 
 // Bulk host <-> device copies through the context's pinned staging buffer: a
 // copy between the device and pageable host memory makes the runtime pin or
@@ -232,11 +234,24 @@ class Staging {
     gdl_status up(void *d_dst, const void *h_src, size_t n);
     gdl_status down(void *h_dst, const void *d_src, size_t n);
     gdl_status sync();
+    // A copy to the host that lands in h_dst at the next land(), land_at()
+    // or sync(): several reads for one wait.
+    gdl_status later(void *h_dst, const void *d_src, size_t n);
+    gdl_status land();
+    // Waits for `ev` only (recorded after the later() copies): the stream
+    // keeps running the work enqueued after it.
+    gdl_status land_at(hipEvent_t ev);
 
   private:
+    struct Pending {
+        void *h_dst;
+        size_t at, n;
+    };
     gdl_status room(size_t n, char **at);
+    void land_pending();
     gdl_ctx *ctx_;
     size_t used_ = 0;
+    std::vector<Pending> pending_;
 };
 
 // ---- internal entry points shared between translation units
