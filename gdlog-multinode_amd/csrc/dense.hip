@@ -243,7 +243,6 @@ struct DenseState {
     unsigned long long rows_new;  // rows that got new tuples
     unsigned long long tr_io;     // SG column expansion: 64-byte segments the transposes moved
     unsigned long long or_io;     // and the row reads + writes of its row OR
-    unsigned long long d0_bits;   // set-up: distinct tuples of the first delta
     u64 col_above;     // SG: a body whose delta has at least this many tuples expands by columns
     u64 log_cap;
     u64 sparse_below;  // a body with fewer new tuples ends the run
@@ -255,6 +254,7 @@ struct DenseState {
     u32 col;            // SG: the running body expands by columns
     u32 asym;           // set-up: FULL or the first delta differs from its transpose
     u32 d0_out;         // set-up: the first delta has a tuple FULL lacks
+    u32 d0_loose;       // set-up: its keys are not strictly increasing
 };
 enum : u32 { kDnRun = 0, kDnFixpoint = 1, kDnGrow = 2, kDnLimit = 3, kDnSparse = 4 };
 // per body: new tuples, produced tuples, combine rows, expansion rows, rows
@@ -923,11 +923,11 @@ __global__ __launch_bounds__(TB) void k_bt_transpose(const u64 *src, const u32 *
 // Sets *asym when the bit matrix M (rows of rw words; with pop, rows whose
 // pop is 0 read as zero) differs from its transpose: each tile is
 // transposed as k_bt_transpose does and compared with the tile it would be
-// stored over.  Grid (rw / kTW)^2.  Returns at once when `same` records
-// that M has exactly the same_n tuples of a bitmap it lies within.
+// stored over.  Grid (rw / kTW)^2.  Returns at once when `same` (given
+// only when M was set from as many keys as the checked bitmap has tuples)
+// records that those keys were distinct and all in the checked bitmap.
 __global__ __launch_bounds__(TB) void k_bt_asym(const u64 *M, const u32 *pop, u64 rw,
-                                                u32 *asym, const DenseState *same,
-                                                u64 same_n) {
+                                                u32 *asym, const DenseState *same) {
     __shared__ u64 s_t[kTW][64][kTW + 1];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const u64 tpd = rw / kTW;
@@ -935,8 +935,10 @@ __global__ __launch_bounds__(TB) void k_bt_asym(const u64 *M, const u32 *pop, u6
     // tile (ti, tj) against (tj, ti) is the same test as (tj, ti) against
     // (ti, tj): the upper triangle suffices
     if (ti > tj) return;
-    // M holds exactly the tuples of a matrix checked already
-    if (same && same->d0_bits == same_n && same->d0_out == 0) return;
+    // M holds exactly the tuples of a matrix checked already: as many
+    // strictly increasing (so distinct) keys as that matrix has tuples, all
+    // of them in it
+    if (same && same->d0_loose == 0 && same->d0_out == 0) return;
     auto load = [&](u64 row, u64 w0, u64 *v) {
         if (pop == nullptr || pop[row] != 0) {
             const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(M + row * rw + w0);
@@ -1167,49 +1169,40 @@ __global__ __launch_bounds__(DB) void k_dense_zero_rows(const u32 *pop, u64 rows
         if (pop[i / rw]) D[i] = 0ull;
 }
 
-// Sets the delta's bits in D.  Lanes of a wave with keys in the same word
-// (delta keys usually come sorted) OR their bits together first and the
-// first lane of each run of equal words does the atomic.  With `full` (a
-// bitmap of the same layout), counts the bits that were not yet set into
-// st->d0_bits and flags st->d0_out when a bit is not in `full`.
+// Sets the delta's bits in D (atomics whose result is not needed: nothing
+// waits on them).  With `full` (a bitmap of the same layout), flags
+// st->d0_out when a key is not in `full` and st->d0_loose when the keys are
+// not strictly increasing (then they may repeat).
 __global__ __launch_bounds__(DB) void k_dense_set_bits(const u64 *keys, u64 n, int kb, u64 rw,
                                                        u64 *D, const u64 *full,
                                                        DenseState *st) {
     const int lane = threadIdx.x & 63;
     const u64 wave = (u64)blockIdx.x * DWV + (threadIdx.x >> 6);
     const u64 n_waves = (u64)gridDim.x * DWV;
-    u64 fresh = 0;
-    bool out = false;
+    bool out = false, loose = false;
     for (u64 base = wave * 64; base < n; base += n_waves * 64) {
         const u64 i = base + lane;
-        u64 wi = ~0ull - (u64)lane;  // distinct per lane when not a key
-        u64 bits = 0;
+        u64 k = 0;
         if (i < n) {
-            const u64 k = keys[i];
+            k = keys[i];
             const u64 c0 = k >> 32, c1 = k & 0xffffffffull;
             if (!(c0 >> kb) && !(c1 >> kb)) {
-                wi = c0 * rw + (c1 >> 6);
-                bits = 1ull << (c1 & 63);
+                const u64 wi = c0 * rw + (c1 >> 6), bit = 1ull << (c1 & 63);
+                atomicOr((unsigned long long *)&D[wi], bit);
+                if (full && !(full[wi] & bit)) out = true;
             }
         }
-        // suffix OR over the lanes after this one that hold the same word
-        for (int d = 1; d < 64; d <<= 1) {
-            const u64 ow = __shfl_down(wi, d, 64), ob = __shfl_down(bits, d, 64);
-            if (lane + d < 64 && ow == wi) bits |= ob;
-        }
-        const u64 pw = __shfl_up(wi, 1, 64);
-        if (bits && (lane == 0 || pw != wi)) {
-            const u64 old = atomicOr((unsigned long long *)&D[wi], bits);
-            fresh += (u64)__popcll(bits & ~old);
-            if (full && (bits & ~full[wi])) out = true;
+        if (full) {
+            u64 prev = __shfl_up(k, 1, 64);
+            if (lane == 0 && i > 0 && i < n) prev = keys[i - 1];
+            if (i > 0 && i < n && k <= prev) loose = true;
         }
     }
     if (!full) return;
-    fresh = wave_sum(fresh);
-    const bool any_out = __ballot(out) != 0;
+    const bool any_out = __ballot(out) != 0, any_loose = __ballot(loose) != 0;
     if (lane == 0) {
-        if (fresh) atomicAdd(&st->d0_bits, (unsigned long long)fresh);
         if (any_out) atomicOr(&st->d0_out, 1u);
+        if (any_loose) atomicOr(&st->d0_loose, 1u);
     }
 }
 
@@ -1973,12 +1966,13 @@ gdl_status dense_run(gdl_ctx *ctx, const gdl_dense_rule *rule, gdl_hashset *hs,
         const unsigned grid_tr = (unsigned)((rw / kTW) * (rw / kTW));
         hipLaunchKernelGGL(k_bt_asym, dim3(grid_tr), dim3(TB), 0, ctx->stream,
                            (const u64 *)hs->slots, (const u32 *)nullptr, rw, &st->asym,
-                           (const DenseState *)nullptr, (u64)0);
+                           (const DenseState *)nullptr);
         GDL_TRY(check_launch(ctx, "k_bt_asym"));
         // skipped on the device when the first delta is all of FULL
         hipLaunchKernelGGL(k_bt_asym, dim3(grid_tr), dim3(TB), 0, ctx->stream,
                            (const u64 *)D[0], (const u32 *)pop[0], rw, &st->asym,
-                           (const DenseState *)st, (u64)hs->size);
+                           n_delta == hs->size ? (const DenseState *)st
+                                               : (const DenseState *)nullptr);
         GDL_TRY(check_launch(ctx, "k_bt_asym"));
     }
 
