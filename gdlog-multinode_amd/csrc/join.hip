@@ -1105,9 +1105,12 @@ gdl_status gdl_fixpoint_linear(gdl_ctx *ctx, const uint64_t *d_inner, uint64_t n
         }
         LinState r{};
         u64 c[2 * kChunk];
-        GDL_TRY(stg.down(&r, st, sizeof(r)));
-        GDL_TRY(stg.down(c, counts, sizeof(c)));
-        GDL_TRY(check_device_error(ctx));
+        u32 dev_err = 0;
+        GDL_TRY(stg.later(&r, st, sizeof(r)));
+        GDL_TRY(stg.later(c, counts, sizeof(c)));
+        GDL_TRY(stg.later(&dev_err, ctx->d_err, sizeof(u32)));
+        GDL_TRY(stg.land());  // one wait for the state, the counts and the error word
+        GDL_TRY(device_error_status(ctx, dev_err));
         if (r.stop == kLinOverflow)
             return fail(ctx, GDL_ERR_OVERFLOW, "linear fixpoint: a body outgrew its log bound");
         u64 outer_n = nd;
