@@ -680,8 +680,13 @@ static gdl_status tcb_step(gdl_tcb *b, u32 *n_active_out) {
                           b->d_hist));
     GDL_HIP(ctx, hipMemcpyAsync(b->h_hist, b->d_hist, b->cfg.max_slots * sizeof(u64),
                                 hipMemcpyDeviceToHost, ctx->stream));
+    // the device error word in the same wait (both host buffers are pinned)
+    GDL_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, ctx->d_err, sizeof(u32), hipMemcpyDeviceToHost,
+                                ctx->stream));
     GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    GDL_TRY(check_device_error(ctx));
+    u32 dev_err = 0;
+    memcpy(&dev_err, ctx->h_pinned, sizeof(u32));
+    GDL_TRY(device_error_status(ctx, dev_err));
     u64 check = 0;
     for (auto &p : b->slots) {
         if (!p) continue;
