@@ -373,7 +373,7 @@ const char *gdl_last_error(gdl_ctx *ctx) { return ctx ? ctx->last_error.c_str() 
 
 gdl_status gdl_sync(gdl_ctx *ctx) {
     if (!ctx) return GDL_ERR_INVALID_ARG;
-    GDL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // the error word's read-back waits for everything enqueued before it
     return check_device_error(ctx);
 }
 
